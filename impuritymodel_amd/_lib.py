@@ -1,0 +1,92 @@
+"""ctypes binding of libimped.so (declared in include/imped.h).
+
+The library is built in-tree by ``impuritymodel_amd.build_lib``.  There is no
+CPU fallback: loading fails with an ImportError when the library is missing,
+and calls raise when no GPU is present.
+"""
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libimped.so")
+
+IMPED_OK = 0
+IMPED_ERR_INVALID = -1
+IMPED_ERR_HIP = -2
+IMPED_ERR_CAPACITY = -3
+IMPED_ERR_BREAKDOWN = -4
+IMPED_ERR_NOCONV = -5
+
+_p = C.c_void_p
+_i32 = C.c_int32
+_i64 = C.c_int64
+_u64 = C.c_uint64
+_d = C.c_double
+
+# name -> argtypes (all return int)
+SIGNATURES = {
+    "imped_abi_version": [],
+    "imped_set_stream": [_p],
+    "imped_device_synchronize": [],
+    "imped_ops_create": [_i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
+    "imped_ops_destroy": [_p],
+    "imped_vec_create": [_i32, _i32, _p, _p, _p, _p],
+    "imped_vec_destroy": [_p],
+    "imped_vec_offsets": [_p, _p],
+    "imped_vec_get": [_p, _p, _p, _p],
+    "imped_vec_apply": [_p, _p, _i32, _p, _p, _d, _i32, _p],
+    "imped_vec_from_eigen": [_p, _i32, _p, _p, _d, _p],
+    "imped_batch_create": [_p, _i32, _p, _p],
+    "imped_batch_destroy": [_p],
+    "imped_batch_set_seeds": [_p, _p, _p, _p],
+    "imped_batch_seed_from_vec": [_p, _p],
+    "imped_batch_build": [_p, _i32, _d, _i32],
+    "imped_batch_set_csr": [_p, _p, _p, _p, _p],
+    "imped_batch_sizes": [_p, _p, _p, _p],
+    "imped_batch_get_basis": [_p, _p],
+    "imped_batch_get_csr": [_p, _p, _p, _p],
+    "imped_batch_lanczos": [_p, _p, _i32, _i32, _p, _p, _p, _p],
+    "imped_batch_eigen": [_p, _i32, _d, _i32, _u64, _p, _p, _p],
+    "imped_batch_cf": [_p, _p, _p, _p, _p, _d, _p, _i32, _p, _i32, _d, _p, _p],
+}
+
+
+class ImpedError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("libimped error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libimped.so is not built (expected at %s); run python -m impuritymodel_amd.build_lib" % LIB_PATH
+        )
+    lib = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.imped_last_error.argtypes = []
+    lib.imped_last_error.restype = C.c_char_p
+    return lib
+
+
+lib = _load()
+
+
+def check(rc, allow=()):
+    if rc != IMPED_OK and rc not in allow:
+        raise ImpedError(rc, lib.imped_last_error().decode(errors="replace"))
+    return rc
+
+
+def ptr(a):
+    """Data pointer of a numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"], "arrays passed to libimped must be contiguous"
+    return a.ctypes.data_as(C.c_void_p)

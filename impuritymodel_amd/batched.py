@@ -1,0 +1,168 @@
+"""Batched XAS: many impurity models' spectra in the same device launches.
+
+For M models this runs the reference's XAS chain (get_spectra.py:198-202
+ground state, spectra.py:222-238 XAS) with every step batched over models:
+
+1. fixed-basis H of all models' ground bases in one build
+   (get_hamiltonian_matrix, finite.py:1514-1610),
+2. lowest ``nPsiMax`` eigenpairs of all models in one eigensolver launch
+   (eigensystem, finite.py:58-118), energy cut (get_spectra.py:302-303) and
+   eigenvector truncation (finite.py:114-117),
+3. all T_t|psi_i> (3 dipole polarizations x kept eigenstates x models) in
+   one operator application, normalized (spectra.py:786-790),
+4. the BFS closures and CSR Hamiltonians of all chains in one build
+   (expand_basis_and_hamiltonian, finite.py:1806-1894),
+5. all Lanczos chains in one launch (finite.py:1974-1992),
+6. continued fractions, Boltzmann weights and the polarization sum of all
+   models in one launch (spectra.py:677-680, average.py:11-45).
+
+``plain_xas`` computes the same spectrum one model and one chain at a time
+with the same kernels (the unbatched GPU baseline).
+"""
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import opcompile
+from .engine import Batch, Ops, vec_apply
+from .finite import get_basis
+from .models import K_B
+from .operators import getDipoleOperators
+
+POLARIZATIONS = ([1, 0, 0], [0, 1, 0], [0, 0, 1])
+
+
+@dataclass
+class XASResult:
+    omega: np.ndarray
+    spectra: np.ndarray          # [M, n_w] thermally averaged, polarization-summed -Im G
+    energies: list               # per model: kept eigenvalues
+    timings: dict = field(default_factory=dict)
+    chain_dims: np.ndarray = None
+    chain_nnz: np.ndarray = None
+    eig_steps: np.ndarray = None
+    kout: np.ndarray = None
+
+
+def ground_bases(models, W):
+    out = []
+    for m in models:
+        b = get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+        out.append(opcompile.bytes_to_words(list(b), m.n_spin_orbitals, W))
+    return out
+
+
+def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0, energy_cut=10.0, reorth=0,
+              eig_tol=1e-12, eig_max_steps=600, seed=12345, bases=None):
+    """Thermal, polarization-summed XAS of every model (all steps batched)."""
+    t0 = time.perf_counter()
+    tim = {}
+    if omega is None:
+        omega = np.linspace(-25.0, 25.0, 2000)
+    omega = np.asarray(omega, dtype=np.float64)
+    M = len(models)
+    n_orb = max(m.n_spin_orbitals for m in models)
+    W = opcompile.n_words(n_orb)
+    ops_h = Ops([m.hOp for m in models], [m.restrictions for m in models], n_orb, W)
+    dip = getDipoleOperators(models[0].nBaths, POLARIZATIONS)
+    ops_t = Ops([d for m in models for d in dip], [m.restrictions for m in models for _ in dip], n_orb, W)
+    if bases is None:
+        bases = ground_bases(models, W)
+    tim["host_setup"] = time.perf_counter() - t0
+
+    t1 = time.perf_counter()
+    gb = Batch(ops_h, np.arange(M, dtype=np.int32))
+    gb.set_seeds(bases)
+    gb.build(fixed_basis=True, weight_min=1e-12)
+    tim["ground_build"] = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    nev = nPsiMax
+    evals, _, steps, ok = gb.eigen(nev, tol=eig_tol, max_steps=eig_max_steps, seed=seed, want_vectors=False)
+    tim["ground_eigen"] = time.perf_counter() - t1
+
+    t1 = time.perf_counter()
+    lim = energy_cut * K_B * T
+    kept = []
+    for m in range(M):
+        e = evals[m]
+        kept.append([i for i in range(nev) if e[i] - e[0] < lim])
+    psi_chain = np.array([m for m in range(M) for i in kept[m]], np.int32)
+    psi_which = np.array([i for m in range(M) for i in kept[m]], np.int32)
+    psi_index = {}
+    for k, (m, i) in enumerate(zip(psi_chain, psi_which)):
+        psi_index[(int(m), int(i))] = k
+    psis = gb.vec_from_eigen(psi_chain, psi_which, 1e-7)
+    nt = len(dip)
+    out_src, out_model, chain_model, energy, spec_id, zflag = [], [], [], [], [], []
+    for m in range(M):
+        for t in range(nt):
+            for i in kept[m]:
+                out_src.append(psi_index[(m, i)])
+                out_model.append(nt * m + t)
+                chain_model.append(m)
+                energy.append(evals[m][i])
+                spec_id.append(m)
+                zflag.append(1 if t == 0 else 0)
+    seeds = vec_apply(ops_t, psis, out_src, out_model, 1e-7, normalize=True)
+    _, _, norm2 = seeds.get()
+    tim["transition"] = time.perf_counter() - t1
+
+    t1 = time.perf_counter()
+    xb = Batch(ops_h, np.array(chain_model, np.int32))
+    xb.seed_from_vec(seeds)
+    xb.build(fixed_basis=False, weight_min=1e-12)
+    tim["expand_build"] = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    K = int(min(krylovSize, max(int(xb.chain_n.max()), 1)))
+    _, _, kout, status = xb.lanczos(K, reorth=reorth)
+    if np.any(status != 0):
+        raise ValueError("Warning: beta==0, implementation absent!")
+    tim["lanczos"] = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    _, spec = xb.cf(np.array(energy), norm2, np.ones(len(energy), np.int32), np.array(zflag, np.int32), K_B * T,
+                    np.array(spec_id, np.int32), M, omega, delta)
+    tim["cf"] = time.perf_counter() - t1
+    tim["total"] = time.perf_counter() - t0
+    return XASResult(omega, spec, [[evals[m][i] for i in kept[m]] for m in range(M)], tim, xb.chain_n.copy(),
+                     xb.chain_nnz.copy(), steps, kout)
+
+
+def plain_xas(model, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0, energy_cut=10.0, reorth=0,
+              eig_tol=1e-12, eig_max_steps=600, seed=12345, basis=None):
+    """The same spectrum with one launch sequence per chain (unbatched GPU path)."""
+    if omega is None:
+        omega = np.linspace(-25.0, 25.0, 2000)
+    omega = np.asarray(omega, dtype=np.float64)
+    n = model.n_spin_orbitals
+    W = opcompile.n_words(n)
+    ops_h = Ops([model.hOp], [model.restrictions], n, W)
+    dip = getDipoleOperators(model.nBaths, POLARIZATIONS)
+    if basis is None:
+        basis = ground_bases([model], W)[0]
+    gb = Batch(ops_h, [0])
+    gb.set_seeds([basis])
+    gb.build(fixed_basis=True)
+    evals, _, _, _ = gb.eigen(nPsiMax, tol=eig_tol, max_steps=eig_max_steps, seed=seed, want_vectors=False)
+    e = evals[0]
+    kept = [i for i in range(nPsiMax) if e[i] - e[0] < energy_cut * K_B * T]
+    acc = np.zeros(len(omega))
+    Z = 0.0
+    e0 = min(e[i] for i in kept)
+    for i in kept:
+        Z += np.exp(-(e[i] - e0) / (K_B * T))
+    for t, d in enumerate(dip):
+        ops_t = Ops([d], [model.restrictions], n, W)
+        for i in kept:
+            psi = gb.vec_from_eigen([0], [i], 1e-7)
+            seed_vec = vec_apply(ops_t, psi, [0], [0], 1e-7, normalize=True)
+            _, _, norm2 = seed_vec.get()
+            xb = Batch(ops_h, [0])
+            xb.seed_from_vec(seed_vec)
+            xb.build(fixed_basis=False)
+            K = int(min(krylovSize, max(int(xb.chain_n[0]), 1)))
+            xb.lanczos(K, reorth=reorth)
+            _, spec = xb.cf([e[i]], norm2, [1], [1], K_B * T, [0], 1, omega, delta)
+            acc += np.exp(-(e[i] - e0) / (K_B * T)) * spec[0]
+    return acc / Z

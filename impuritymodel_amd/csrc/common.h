@@ -1,0 +1,138 @@
+// Shared definitions of the MI355X exact-diagonalization engine (gfx950).
+//
+// Product states are W packed uint64 words; spin-orbital i lives in word i/64
+// at bit 63 - i%64, so comparing words lexicographically orders states like
+// the reference's big-endian bytes keys (product_state_representation.py:131-161).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#define IMPED_OK 0
+#define IMPED_ERR_INVALID -1
+#define IMPED_ERR_HIP -2
+#define IMPED_ERR_CAPACITY -3
+#define IMPED_ERR_BREAKDOWN -4
+#define IMPED_ERR_NOCONV -5
+
+#define IMPED_MAXW 2
+#define IMPED_WAVE 64
+
+namespace imped {
+
+void set_error(const std::string& msg);
+
+struct HipError {
+  std::string msg;
+};
+
+#define HIP_CHECK(expr)                                                                  \
+  do {                                                                                   \
+    hipError_t e__ = (expr);                                                             \
+    if (e__ != hipSuccess) {                                                             \
+      throw ::imped::HipError{std::string(#expr) + ": " + hipGetErrorString(e__) +       \
+                              " at " + __FILE__ + ":" + std::to_string(__LINE__)};       \
+    }                                                                                    \
+  } while (0)
+
+// Device buffer owned by a handle; grows on demand, never shrinks.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void reserve(size_t nb) {
+    if (nb <= bytes) return;
+    if (p) HIP_CHECK(hipFree(p));
+    p = nullptr;
+    size_t alloc = nb < 256 ? 256 : nb;
+    HIP_CHECK(hipMalloc(&p, alloc));
+    bytes = alloc;
+  }
+  // Grow while keeping the first `keep` bytes.
+  void grow_keep(size_t nb, size_t keep, hipStream_t s) {
+    if (nb <= bytes) return;
+    size_t alloc = bytes * 2 > nb ? bytes * 2 : nb;
+    void* q = nullptr;
+    HIP_CHECK(hipMalloc(&q, alloc));
+    if (p && keep) HIP_CHECK(hipMemcpyAsync(q, p, keep, hipMemcpyDeviceToDevice, s));
+    if (p) {
+      HIP_CHECK(hipStreamSynchronize(s));
+      HIP_CHECK(hipFree(p));
+    }
+    p = q;
+    bytes = alloc;
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+template <class T>
+inline void h2d(DevBuf& b, const T* src, size_t n, hipStream_t s) {
+  b.reserve(n * sizeof(T));
+  if (n) HIP_CHECK(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+template <class T>
+inline void d2h(T* dst, const void* src, size_t n, hipStream_t s) {
+  if (n) HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+inline unsigned blocks_for(int64_t n, int threads) {
+  int64_t b = (n + threads - 1) / threads;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+// ------------------------------------------------------------------ device helpers
+
+struct Key {
+  uint64_t w[IMPED_MAXW];
+};
+
+__device__ __forceinline__ uint32_t hash_key(uint32_t chain, const uint64_t* k, int W) {
+  uint64_t h = 0x9E3779B97F4A7C15ull ^ (uint64_t)chain * 0xD6E8FEB86659FD93ull;
+  for (int w = 0; w < W; ++w) {
+    h ^= k[w] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 31;
+  }
+  h *= 0x94D049BB133111EBull;
+  h ^= h >> 29;
+  return (uint32_t)h;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// |z|^2 evaluated as abs(z)**2 in the reference (finite.py:1509).
+__device__ __forceinline__ double abs2_ref(double2 z) {
+  double a = hypot(z.x, z.y);
+  return a * a;
+}
+
+// 1/(a + ib) by Smith's scaling (robust to over/underflow).
+__device__ __forceinline__ double2 crecip(double2 d) {
+  if (fabs(d.x) >= fabs(d.y)) {
+    double r = d.y / d.x;
+    double den = d.x + d.y * r;
+    return make_double2(1.0 / den, -r / den);
+  } else {
+    double r = d.x / d.y;
+    double den = d.x * r + d.y;
+    return make_double2(r / den, -1.0 / den);
+  }
+}
+
+}  // namespace imped

@@ -1,0 +1,875 @@
+// Host orchestration and the C ABI of the engine (libimped.so).
+//
+// Handles own device memory.  A *batch* is a set of chains, each the
+// Hilbert-space sector of one Green's function (or one model's ground-state
+// sector); every step below runs for all chains of a batch in the same
+// launches.
+//
+//   ops    compiled operator tables of several models        (imped_ops_*)
+//   vec    sparse multi-configurational states on the device (imped_vec_*)
+//   batch  bases + CSR Hamiltonians + Lanczos/eigen results  (imped_batch_*)
+//
+// Reference counterparts: applyOp (impurityModel/ed/finite.py:1301),
+// get_hamiltonian_matrix (finite.py:1514), expand_basis_and_hamiltonian
+// (finite.py:1806), get_tridiagonal_krylov_vectors (finite.py:1897),
+// eigensystem (finite.py:58), the continued fraction of getGreen
+// (spectra.py:677-680) and thermal_average (average.py:11).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "../../include/imped.h"
+#include "kernels.h"
+#include "krylov.h"
+
+namespace imped {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+static hipStream_t g_stream = nullptr;
+
+struct Ops {
+  int W = 1, M = 0, n_orb = 0;
+  DevBuf model_group_off, group_term_off, group_flip, term_masks, term_meta, term_coef, model_restr_off, restr_mask,
+      restr_lohi;
+  OpsView view() const {
+    return OpsView{model_group_off.as<int32_t>(), group_term_off.as<int32_t>(), group_flip.as<uint64_t>(),
+                   term_masks.as<uint64_t>(),     term_meta.as<uint32_t>(),     term_coef.as<double2>(),
+                   model_restr_off.as<int32_t>(), restr_mask.as<uint64_t>(),    restr_lohi.as<int32_t>()};
+  }
+};
+
+// Sparse vectors: vector v holds entries [off[v], off[v+1]).
+struct Vec {
+  int W = 1, V = 0;
+  std::vector<int64_t> h_off;  // V+1
+  DevBuf keys, amps, off, vid, pos, norm2;
+  int64_t total() const { return h_off.empty() ? 0 : h_off.back(); }
+};
+
+struct Batch {
+  Ops* ops = nullptr;
+  int C = 0, W = 1;
+  std::vector<int32_t> h_chain_model;
+  DevBuf chain_model;
+  // seeds (host offsets; device keys/amps)
+  std::vector<int64_t> h_seed_off;
+  DevBuf seed_keys, seed_amps, seed_off_dev;
+  bool have_seed_amps = false;
+  // state pool (global ids in discovery order)
+  int64_t P = 0, Pcap = 0;
+  DevBuf pool_keys, pool_chain, pool_local, pool_coloff, pool_colcnt;
+  // hash
+  uint32_t hcap = 0;
+  DevBuf table;
+  // per-chain sizes
+  DevBuf chain_size, seg_start;
+  std::vector<int64_t> h_n, h_row_base, h_nnz;
+  // column storage (expansion)
+  int64_t E = 0;
+  DevBuf col_keys, col_val;
+  // CSR in chain-local order, rows of chain c at [row_base[c], row_base[c]+n[c])
+  int64_t R = 0, NNZ = 0;
+  DevBuf rowptr, col, val, row_chain, row_base, chain_n;
+  // Lanczos
+  int K = 0;
+  DevBuf alpha, beta, kout, status;
+  // ground states
+  int nev = 0;
+  DevBuf evals, evecs;
+  bool built = false;
+};
+
+// ------------------------------------------------------------------ small kernels
+
+__global__ void set_local_seed_kernel(const int64_t* seed_off, int C, int32_t* pool_local, int32_t* pool_chain,
+                                      int64_t* chain_size) {
+  int c = blockIdx.x;
+  if (c >= C) return;
+  int64_t a = seed_off[c], b = seed_off[c + 1];
+  for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+    pool_local[i] = (int32_t)(i - a);
+    pool_chain[i] = c;
+  }
+  if (threadIdx.x == 0) chain_size[c] = b - a;
+}
+
+__global__ void append_layer_kernel(int W, const int32_t* perm, int64_t n, const uint64_t* cand_keys,
+                                    const int32_t* cand_chain, const int32_t* cand_slot, int64_t P0, uint64_t* pool_keys,
+                                    int32_t* pool_chain, int32_t* table) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t c = perm[i];
+  int64_t g = P0 + i;
+  for (int w = 0; w < W; ++w) pool_keys[g * W + w] = cand_keys[(size_t)c * W + w];
+  pool_chain[g] = cand_chain[c];
+  table[cand_slot[c]] = (int32_t)g;
+}
+
+__global__ void layer_heads_kernel(const int32_t* pool_chain, int64_t P0, int64_t n, int64_t* seg_start) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t c = pool_chain[P0 + i];
+  if (i == 0 || pool_chain[P0 + i - 1] != c) seg_start[c] = i;
+}
+
+__global__ void layer_local_kernel(const int32_t* pool_chain, int64_t P0, int64_t n, const int64_t* seg_start,
+                                   int64_t* chain_size, int32_t* pool_local) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t c = pool_chain[P0 + i];
+  int64_t r = i - seg_start[c];
+  pool_local[P0 + i] = (int32_t)(chain_size[c] + r);
+  bool last = (i == n - 1) || pool_chain[P0 + i + 1] != c;
+  if (last) chain_size[c] += r + 1;
+}
+
+__global__ void record_cols_kernel(const int64_t* offsets, const int32_t* counts, int64_t n, int64_t base,
+                                   int64_t* pool_coloff, int32_t* pool_colcnt, int64_t P0) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  pool_coloff[P0 + i] = base + offsets[i];
+  pool_colcnt[P0 + i] = counts[i];
+}
+
+__global__ void row_order_kernel(const int32_t* pool_chain, const int32_t* pool_local, const int64_t* row_base,
+                                 int64_t P, int32_t* row_gid, int32_t* row_chain) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= P) return;
+  int32_t c = pool_chain[g];
+  int64_t r = row_base[c] + pool_local[g];
+  row_gid[r] = (int32_t)g;
+  row_chain[r] = c;
+}
+
+__global__ void row_counts_kernel(const int32_t* row_gid, const int32_t* pool_colcnt, int64_t R, int32_t* cnt) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < R) cnt[r] = pool_colcnt[row_gid[r]];
+}
+
+// Row r of H is the conjugate of column r: <r|H|t> = conj(<t|H|r>).
+template <int W>
+__global__ void csr_fill_kernel(HashView h, const int32_t* row_gid, const int32_t* row_chain, int64_t R,
+                                const int64_t* pool_coloff, const uint64_t* col_keys, const double2* col_vals,
+                                const int32_t* pool_local, const int64_t* rowptr, int32_t* col, double2* val,
+                                int32_t* err) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  int32_t g = row_gid[r];
+  int32_t c = row_chain[r];
+  int64_t src = pool_coloff[g];
+  int64_t a = rowptr[r], b = rowptr[r + 1];
+  for (int64_t e = a; e < b; ++e) {
+    const uint64_t* k = col_keys + (size_t)(src + (e - a)) * W;
+    uint32_t slot = hash_key((uint32_t)c, k, W) & h.mask;
+    int32_t id = HASH_EMPTY;
+    for (;;) {
+      int32_t cur = h.table[slot];
+      if (cur == HASH_EMPTY) break;
+      if (!(cur & CAND_FLAG) && h.pool_chain[cur] == c) {
+        bool eq = true;
+        for (int w = 0; w < W; ++w) eq = eq && h.pool_keys[(size_t)cur * W + w] == k[w];
+        if (eq) {
+          id = cur;
+          break;
+        }
+      }
+      slot = (slot + 1) & h.mask;
+    }
+    if (id < 0) {
+      atomicExch(err, 1);
+      col[e] = 0;
+      val[e] = make_double2(0.0, 0.0);
+      continue;
+    }
+    col[e] = pool_local[id];
+    val[e] = cconj(col_vals[src + (e - a)]);
+  }
+}
+
+__global__ void keys_by_rows_kernel(int W, const int32_t* row_gid, int64_t R, const uint64_t* pool_keys,
+                                    uint64_t* out) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  for (int w = 0; w < W; ++w) out[r * W + w] = pool_keys[(size_t)row_gid[r] * W + w];
+}
+
+__global__ void chain_of_rows_kernel(const int64_t* row_base, const int64_t* chain_n, int C, int32_t* row_chain) {
+  int c = blockIdx.x;
+  if (c >= C) return;
+  for (int64_t i = threadIdx.x; i < chain_n[c]; i += blockDim.x) row_chain[row_base[c] + i] = c;
+}
+
+// ------------------------------------------------------------------ helpers
+
+static hipStream_t stream() { return g_stream; }
+
+static int bits_for(int64_t n) {
+  int b = 1;
+  while ((int64_t(1) << b) < n) ++b;
+  return b;
+}
+
+static void ensure_hash(Batch& b, int64_t need_entries) {
+  uint64_t want = 1024;
+  while (want < (uint64_t)need_entries * 2) want <<= 1;
+  if (want <= b.hcap) return;
+  if (want > (1ull << 31)) throw HipError{"hash table too large"};
+  b.hcap = (uint32_t)want;
+  b.table.reserve((size_t)b.hcap * sizeof(int32_t));
+  launch_hash_clear(b.table.as<int32_t>(), b.hcap, stream());
+  HashView hv{b.table.as<int32_t>(), b.hcap - 1, b.pool_keys.as<uint64_t>(), b.pool_chain.as<int32_t>()};
+  launch_hash_insert_pool(b.W, hv, 0, b.P, stream());
+}
+
+static void ensure_pool(Batch& b, int64_t need) {
+  if (need <= b.Pcap) return;
+  int64_t cap = std::max<int64_t>(need, b.Pcap * 2);
+  const size_t W = b.W;
+  b.pool_keys.grow_keep(cap * W * 8, b.P * W * 8, stream());
+  b.pool_chain.grow_keep(cap * 4, b.P * 4, stream());
+  b.pool_local.grow_keep(cap * 4, b.P * 4, stream());
+  b.pool_coloff.grow_keep(cap * 8, b.P * 8, stream());
+  b.pool_colcnt.grow_keep(cap * 4, b.P * 4, stream());
+  b.Pcap = cap;
+}
+
+static HashView hash_view(Batch& b) {
+  return HashView{b.table.as<int32_t>(), b.hcap - 1, b.pool_keys.as<uint64_t>(), b.pool_chain.as<int32_t>()};
+}
+
+// Scratch shared by the build steps.
+struct BuildScratch {
+  DevBuf tasks, counts, offsets, cand_keys, cand_chain, cand_slot, cand_cnt, alive, perm, rowgid, err;
+};
+static BuildScratch& bscratch() {
+  static BuildScratch s;
+  return s;
+}
+
+static int64_t make_tasks(Batch& b, int64_t begin, int64_t end) {
+  BuildScratch& sc = bscratch();
+  int64_t cap = (end - begin) / 64 + b.C + 1;
+  sc.tasks.reserve(cap * sizeof(WaveTask));
+  return build_wave_tasks(b.pool_chain.as<int32_t>(), begin, end, b.chain_model.as<int32_t>(), b.C, sc.tasks.p,
+                          cap, stream(), nullptr);
+}
+
+// Apply the Hamiltonian to pool states [f0, f1): count, scan, fill.  In
+// expand mode new targets become candidates; returns the entry count.
+static int64_t apply_layer(Batch& b, int64_t f0, int64_t f1, int mode, double weight_min, int64_t* n_alive) {
+  BuildScratch& sc = bscratch();
+  const int64_t n = f1 - f0;
+  int64_t nt = make_tasks(b, f0, f1);
+  ApplyArgs a{};
+  a.ops = b.ops->view();
+  a.tasks = sc.tasks.as<WaveTask>();
+  a.n_tasks = nt;
+  a.src_keys = b.pool_keys.as<uint64_t>();
+  a.weight_min = weight_min;
+  sc.counts.reserve(n * sizeof(int32_t));
+  // the wave tasks carry absolute source ids; counts/offsets are indexed the
+  // same way through a shifted base pointer
+  a.counts = sc.counts.as<int32_t>() - f0;
+  a.hash = hash_view(b);
+  a.pool_local = b.pool_local.as<int32_t>();
+  launch_apply(b.W, mode, false, a, stream());
+  sc.offsets.reserve((n + 1) * sizeof(int64_t));
+  int64_t E = scan_counts(sc.counts.as<int32_t>(), n, sc.offsets.as<int64_t>(), stream());
+  a.offsets = sc.offsets.as<int64_t>() - f0;
+  if (mode == MODE_EXPAND) {
+    int64_t base = b.E;
+    b.col_keys.grow_keep((base + E) * b.W * 8, base * b.W * 8, stream());
+    b.col_val.grow_keep((base + E) * 16, base * 16, stream());
+    hipLaunchKernelGGL(record_cols_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, stream(), sc.offsets.as<int64_t>(),
+                       sc.counts.as<int32_t>(), n, base, b.pool_coloff.as<int64_t>(), b.pool_colcnt.as<int32_t>(), f0);
+    ensure_hash(b, b.P + E);
+    a.hash = hash_view(b);
+    sc.cand_keys.reserve(std::max<int64_t>(E, 1) * b.W * 8);
+    sc.cand_chain.reserve(std::max<int64_t>(E, 1) * 4);
+    sc.cand_slot.reserve(std::max<int64_t>(E, 1) * 4);
+    sc.alive.reserve(std::max<int64_t>(E, 1) * 4);
+    sc.cand_cnt.reserve(2 * sizeof(int32_t));
+    HIP_CHECK(hipMemsetAsync(sc.cand_cnt.p, 0, 2 * sizeof(int32_t), stream()));
+    a.out_keys = b.col_keys.as<uint64_t>() + base * b.W;
+    a.out_val = b.col_val.as<double2>() + base;
+    a.offsets = sc.offsets.as<int64_t>() - f0;
+    a.cand = CandView{sc.cand_keys.as<uint64_t>(), sc.cand_chain.as<int32_t>(), sc.cand_slot.as<int32_t>(),
+                      sc.cand_cnt.as<int32_t>(),   sc.alive.as<int32_t>(),      sc.cand_cnt.as<int32_t>() + 1,
+                      (int32_t)std::min<int64_t>(E, INT32_MAX)};
+    if (E > 0) launch_apply(b.W, mode, true, a, stream());
+    int32_t cc[2] = {0, 0};
+    HIP_CHECK(hipMemcpyAsync(cc, sc.cand_cnt.p, sizeof(cc), hipMemcpyDeviceToHost, stream()));
+    HIP_CHECK(hipStreamSynchronize(stream()));
+    b.E = base + E;
+    *n_alive = cc[1];
+  } else {
+    // fixed basis: write the CSR directly (rows are pool ids = local ids)
+    b.rowptr.reserve((n + 1) * sizeof(int64_t));
+    HIP_CHECK(hipMemcpyAsync(b.rowptr.p, sc.offsets.p, n * sizeof(int64_t), hipMemcpyDeviceToDevice, stream()));
+    HIP_CHECK(hipMemcpyAsync(b.rowptr.as<int64_t>() + n, &E, sizeof(int64_t), hipMemcpyHostToDevice, stream()));
+    HIP_CHECK(hipStreamSynchronize(stream()));
+    b.col.reserve(std::max<int64_t>(E, 1) * 4);
+    b.val.reserve(std::max<int64_t>(E, 1) * 16);
+    a.out_col = b.col.as<int32_t>();
+    a.out_val = b.val.as<double2>();
+    if (E > 0) launch_apply(b.W, mode, true, a, stream());
+    HIP_CHECK(hipStreamSynchronize(stream()));
+    *n_alive = 0;
+  }
+  return E;
+}
+
+static void finalize_sizes(Batch& b) {
+  b.h_n.assign(b.C, 0);
+  HIP_CHECK(hipMemcpyAsync(b.h_n.data(), b.chain_size.p, b.C * sizeof(int64_t), hipMemcpyDeviceToHost, stream()));
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  b.h_row_base.assign(b.C + 1, 0);
+  for (int c = 0; c < b.C; ++c) b.h_row_base[c + 1] = b.h_row_base[c] + b.h_n[c];
+  b.R = b.h_row_base[b.C];
+  h2d(b.row_base, b.h_row_base.data(), b.C + 1, stream());
+  h2d(b.chain_n, b.h_n.data(), b.C, stream());
+}
+
+static void compute_chain_nnz(Batch& b) {
+  std::vector<int64_t> rp(b.C + 1);
+  b.h_nnz.assign(b.C, 0);
+  for (int c = 0; c <= b.C; ++c) {
+    HIP_CHECK(hipMemcpyAsync(&rp[c], b.rowptr.as<int64_t>() + b.h_row_base[c], sizeof(int64_t),
+                             hipMemcpyDeviceToHost, stream()));
+  }
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  for (int c = 0; c < b.C; ++c) b.h_nnz[c] = rp[c + 1] - rp[c];
+  b.NNZ = rp[b.C];
+}
+
+static void build(Batch& b, int mode, double weight_min, int max_layers) {
+  const hipStream_t s = stream();
+  BuildScratch& sc = bscratch();
+  const int64_t S = b.h_seed_off.back();
+  b.P = 0;
+  b.E = 0;
+  ensure_pool(b, std::max<int64_t>(S, 1024));
+  if (S) HIP_CHECK(hipMemcpyAsync(b.pool_keys.p, b.seed_keys.p, S * b.W * 8, hipMemcpyDeviceToDevice, s));
+  DevBuf& seed_off = b.seed_off_dev;
+  h2d(seed_off, b.h_seed_off.data(), b.C + 1, s);
+  b.chain_size.reserve(b.C * sizeof(int64_t));
+  b.seg_start.reserve(b.C * sizeof(int64_t));
+  if (b.C) {
+    hipLaunchKernelGGL(set_local_seed_kernel, dim3(b.C), dim3(256), 0, s, seed_off.as<int64_t>(), b.C,
+                       b.pool_local.as<int32_t>(), b.pool_chain.as<int32_t>(), b.chain_size.as<int64_t>());
+    HIP_CHECK(hipGetLastError());
+  }
+  b.P = S;
+  b.hcap = 0;
+  ensure_hash(b, std::max<int64_t>(S, 512));
+
+  if (mode == 1) {
+    // Fixed basis (get_hamiltonian_matrix): rows are seeds in order.
+    int64_t na = 0;
+    int64_t E = apply_layer(b, 0, S, MODE_FIXED, weight_min, &na);
+    finalize_sizes(b);
+    b.NNZ = E;
+    b.row_chain.reserve(std::max<int64_t>(b.R, 1) * 4);
+    if (b.C)
+      hipLaunchKernelGGL(chain_of_rows_kernel, dim3(b.C), dim3(256), 0, s, b.row_base.as<int64_t>(),
+                         b.chain_n.as<int64_t>(), b.C, b.row_chain.as<int32_t>());
+    compute_chain_nnz(b);
+    b.built = true;
+    return;
+  }
+
+  int64_t f0 = 0, f1 = S;
+  int layers = 0;
+  while (f1 > f0) {
+    if (max_layers > 0 && layers >= max_layers) break;
+    ++layers;
+    int64_t n_alive = 0;
+    apply_layer(b, f0, f1, MODE_EXPAND, weight_min, &n_alive);
+    if (n_alive == 0) {
+      f0 = f1;
+      break;
+    }
+    // sort the new states by (chain, key) and append them
+    sc.perm.reserve(n_alive * sizeof(int32_t));
+    HIP_CHECK(hipMemcpyAsync(sc.perm.p, sc.alive.p, n_alive * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    sort_chain_keys(b.W, b.ops->n_orb, bits_for(b.C), sc.cand_keys.as<uint64_t>(), sc.cand_chain.as<int32_t>(),
+                    sc.perm.as<int32_t>(), n_alive, s);
+    ensure_pool(b, b.P + n_alive);
+    const int64_t P0 = b.P;
+    hipLaunchKernelGGL(append_layer_kernel, dim3(blocks_for(n_alive, 256)), dim3(256), 0, s, b.W,
+                       sc.perm.as<int32_t>(), n_alive, sc.cand_keys.as<uint64_t>(), sc.cand_chain.as<int32_t>(),
+                       sc.cand_slot.as<int32_t>(), P0, b.pool_keys.as<uint64_t>(), b.pool_chain.as<int32_t>(),
+                       b.table.as<int32_t>());
+    hipLaunchKernelGGL(layer_heads_kernel, dim3(blocks_for(n_alive, 256)), dim3(256), 0, s,
+                       b.pool_chain.as<int32_t>(), P0, n_alive, b.seg_start.as<int64_t>());
+    hipLaunchKernelGGL(layer_local_kernel, dim3(blocks_for(n_alive, 256)), dim3(256), 0, s,
+                       b.pool_chain.as<int32_t>(), P0, n_alive, b.seg_start.as<int64_t>(),
+                       b.chain_size.as<int64_t>(), b.pool_local.as<int32_t>());
+    HIP_CHECK(hipGetLastError());
+    b.P = P0 + n_alive;
+    f0 = f1;
+    f1 = b.P;
+  }
+  // States of an unfinished last layer keep empty columns.
+  if (f1 > f0) {
+    HIP_CHECK(hipMemsetAsync(b.pool_colcnt.as<int32_t>() + f0, 0, (f1 - f0) * 4, s));
+  }
+  finalize_sizes(b);
+  // CSR rows in chain-local order
+  sc.rowgid.reserve(std::max<int64_t>(b.R, 1) * 4);
+  b.row_chain.reserve(std::max<int64_t>(b.R, 1) * 4);
+  hipLaunchKernelGGL(row_order_kernel, dim3(blocks_for(b.P, 256)), dim3(256), 0, s, b.pool_chain.as<int32_t>(),
+                     b.pool_local.as<int32_t>(), b.row_base.as<int64_t>(), b.P, sc.rowgid.as<int32_t>(),
+                     b.row_chain.as<int32_t>());
+  sc.counts.reserve(std::max<int64_t>(b.R, 1) * 4);
+  hipLaunchKernelGGL(row_counts_kernel, dim3(blocks_for(b.R, 256)), dim3(256), 0, s, sc.rowgid.as<int32_t>(),
+                     b.pool_colcnt.as<int32_t>(), b.R, sc.counts.as<int32_t>());
+  b.rowptr.reserve((b.R + 1) * sizeof(int64_t));
+  int64_t nnz = scan_counts(sc.counts.as<int32_t>(), b.R, b.rowptr.as<int64_t>(), s);
+  HIP_CHECK(hipMemcpyAsync(b.rowptr.as<int64_t>() + b.R, &nnz, sizeof(int64_t), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  b.col.reserve(std::max<int64_t>(nnz, 1) * 4);
+  b.val.reserve(std::max<int64_t>(nnz, 1) * 16);
+  sc.err.reserve(4);
+  HIP_CHECK(hipMemsetAsync(sc.err.p, 0, 4, s));
+  if (b.R) {
+    if (b.W == 1)
+      hipLaunchKernelGGL(csr_fill_kernel<1>, dim3(blocks_for(b.R, 128)), dim3(128), 0, s, hash_view(b),
+                         sc.rowgid.as<int32_t>(), b.row_chain.as<int32_t>(), b.R, b.pool_coloff.as<int64_t>(),
+                         b.col_keys.as<uint64_t>(), b.col_val.as<double2>(), b.pool_local.as<int32_t>(),
+                         b.rowptr.as<int64_t>(), b.col.as<int32_t>(), b.val.as<double2>(), sc.err.as<int32_t>());
+    else
+      hipLaunchKernelGGL(csr_fill_kernel<2>, dim3(blocks_for(b.R, 128)), dim3(128), 0, s, hash_view(b),
+                         sc.rowgid.as<int32_t>(), b.row_chain.as<int32_t>(), b.R, b.pool_coloff.as<int64_t>(),
+                         b.col_keys.as<uint64_t>(), b.col_val.as<double2>(), b.pool_local.as<int32_t>(),
+                         b.rowptr.as<int64_t>(), b.col.as<int32_t>(), b.val.as<double2>(), sc.err.as<int32_t>());
+    HIP_CHECK(hipGetLastError());
+  }
+  // basis keys in row order (reuse col_keys tail is not safe; keep a copy in seed_keys)
+  b.seed_keys.reserve(std::max<int64_t>(b.R, 1) * b.W * 8);
+  DevBuf tmp;
+  tmp.reserve(std::max<int64_t>(b.R, 1) * b.W * 8);
+  hipLaunchKernelGGL(keys_by_rows_kernel, dim3(blocks_for(b.R, 256)), dim3(256), 0, s, b.W, sc.rowgid.as<int32_t>(),
+                     b.R, b.pool_keys.as<uint64_t>(), tmp.as<uint64_t>());
+  HIP_CHECK(hipMemcpyAsync(b.pool_keys.p, tmp.p, b.R * b.W * 8, hipMemcpyDeviceToDevice, s));
+  int32_t err = 0;
+  HIP_CHECK(hipMemcpyAsync(&err, sc.err.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (err) throw HipError{"internal error: CSR target missing from basis"};
+  compute_chain_nnz(b);
+  b.built = true;
+}
+
+}  // namespace imped
+
+using namespace imped;
+
+#define API_BEGIN try {
+#define API_END                                  \
+  }                                              \
+  catch (const HipError& e) {                    \
+    set_error(e.msg);                            \
+    return IMPED_ERR_HIP;                        \
+  }                                              \
+  catch (const std::exception& e) {              \
+    set_error(e.what());                         \
+    return IMPED_ERR_INVALID;                    \
+  }                                              \
+  return IMPED_OK;
+
+extern "C" {
+
+const char* imped_last_error(void) { return g_last_error.c_str(); }
+
+int imped_abi_version(void) { return IMPED_ABI_VERSION; }
+
+int imped_set_stream(void* stream) {
+  g_stream = reinterpret_cast<hipStream_t>(stream);
+  return IMPED_OK;
+}
+
+int imped_device_synchronize(void) {
+  API_BEGIN
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int imped_ops_create(int W, int n_orb, int n_models, const int32_t* model_group_off, const int32_t* group_term_off,
+                     const uint64_t* group_flip, const uint64_t* term_masks, const uint32_t* term_meta,
+                     const double* term_coef, const int32_t* model_restr_off, const uint64_t* restr_mask,
+                     const int32_t* restr_lohi, imped_ops* out) {
+  API_BEGIN
+  if (W < 1 || W > IMPED_MAXW || n_models < 1 || n_orb < 1 || n_orb > 64 * W) {
+    set_error("imped_ops_create: invalid W / n_orb / n_models");
+    return IMPED_ERR_INVALID;
+  }
+  auto o = std::make_unique<Ops>();
+  o->W = W;
+  o->M = n_models;
+  o->n_orb = n_orb;
+  const int G = model_group_off[n_models];
+  const int T = group_term_off[G];
+  const int R = model_restr_off[n_models];
+  hipStream_t s = stream();
+  h2d(o->model_group_off, model_group_off, n_models + 1, s);
+  h2d(o->group_term_off, group_term_off, G + 1, s);
+  h2d(o->group_flip, group_flip, (size_t)G * W, s);
+  h2d(o->term_masks, term_masks, (size_t)T * 3 * W, s);
+  h2d(o->term_meta, term_meta, T, s);
+  h2d(o->term_coef, term_coef, (size_t)T * 2, s);
+  h2d(o->model_restr_off, model_restr_off, n_models + 1, s);
+  h2d(o->restr_mask, restr_mask, (size_t)R * W, s);
+  h2d(o->restr_lohi, restr_lohi, (size_t)R * 2, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+  *out = reinterpret_cast<imped_ops>(o.release());
+  API_END
+}
+
+int imped_ops_destroy(imped_ops ops) {
+  delete reinterpret_cast<Ops*>(ops);
+  return IMPED_OK;
+}
+
+int imped_batch_create(imped_ops ops, int n_chains, const int32_t* chain_model, imped_batch* out) {
+  API_BEGIN
+  Ops* o = reinterpret_cast<Ops*>(ops);
+  if (!o || n_chains < 0) {
+    set_error("imped_batch_create: invalid arguments");
+    return IMPED_ERR_INVALID;
+  }
+  for (int c = 0; c < n_chains; ++c) {
+    if (chain_model[c] < 0 || chain_model[c] >= o->M) {
+      set_error("imped_batch_create: chain model out of range");
+      return IMPED_ERR_INVALID;
+    }
+  }
+  auto b = std::make_unique<Batch>();
+  b->ops = o;
+  b->C = n_chains;
+  b->W = o->W;
+  b->h_chain_model.assign(chain_model, chain_model + n_chains);
+  h2d(b->chain_model, chain_model, std::max(n_chains, 1), stream());
+  b->h_seed_off.assign(n_chains + 1, 0);
+  *out = reinterpret_cast<imped_batch>(b.release());
+  API_END
+}
+
+int imped_batch_destroy(imped_batch batch) {
+  delete reinterpret_cast<Batch*>(batch);
+  return IMPED_OK;
+}
+
+int imped_batch_set_seeds(imped_batch batch, const int64_t* seed_off, const uint64_t* seed_keys,
+                          const double* seed_amps) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  for (int c = 0; c < b->C; ++c)
+    if (seed_off[c + 1] < seed_off[c] || seed_off[0] != 0) {
+      set_error("imped_batch_set_seeds: offsets must start at 0 and be non-decreasing");
+      return IMPED_ERR_INVALID;
+    }
+  b->h_seed_off.assign(seed_off, seed_off + b->C + 1);
+  const int64_t S = b->h_seed_off.back();
+  h2d(b->seed_keys, seed_keys, std::max<int64_t>(S, 1) * b->W, stream());
+  b->have_seed_amps = seed_amps != nullptr;
+  if (seed_amps) h2d(b->seed_amps, seed_amps, std::max<int64_t>(S, 1) * 2, stream());
+  b->built = false;
+  API_END
+}
+
+int imped_batch_seed_from_vec(imped_batch batch, imped_vec vec) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  Vec* v = reinterpret_cast<Vec*>(vec);
+  if (v->V != b->C || v->W != b->W) {
+    set_error("imped_batch_seed_from_vec: vector count or word count mismatch");
+    return IMPED_ERR_INVALID;
+  }
+  b->h_seed_off = v->h_off;
+  const int64_t S = v->total();
+  b->seed_keys.reserve(std::max<int64_t>(S, 1) * b->W * 8);
+  b->seed_amps.reserve(std::max<int64_t>(S, 1) * 16);
+  if (S) {
+    HIP_CHECK(hipMemcpyAsync(b->seed_keys.p, v->keys.p, S * b->W * 8, hipMemcpyDeviceToDevice, stream()));
+    HIP_CHECK(hipMemcpyAsync(b->seed_amps.p, v->amps.p, S * 16, hipMemcpyDeviceToDevice, stream()));
+  }
+  b->have_seed_amps = true;
+  b->built = false;
+  API_END
+}
+
+int imped_batch_build(imped_batch batch, int fixed_basis, double weight_min, int max_layers) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  build(*b, fixed_basis ? 1 : 0, weight_min, max_layers);
+  API_END
+}
+
+int imped_batch_set_csr(imped_batch batch, const int64_t* chain_n, const int64_t* rowptr, const int32_t* col,
+                        const double* val) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  b->h_n.assign(chain_n, chain_n + b->C);
+  b->h_row_base.assign(b->C + 1, 0);
+  for (int c = 0; c < b->C; ++c) {
+    if (chain_n[c] < 0) {
+      set_error("imped_batch_set_csr: negative chain size");
+      return IMPED_ERR_INVALID;
+    }
+    b->h_row_base[c + 1] = b->h_row_base[c] + chain_n[c];
+  }
+  b->R = b->h_row_base[b->C];
+  b->NNZ = rowptr[b->R];
+  if (rowptr[0] != 0 || b->NNZ < 0) {
+    set_error("imped_batch_set_csr: rowptr must start at 0");
+    return IMPED_ERR_INVALID;
+  }
+  for (int64_t r = 0; r < b->R; ++r)
+    if (rowptr[r + 1] < rowptr[r]) {
+      set_error("imped_batch_set_csr: rowptr must be non-decreasing");
+      return IMPED_ERR_INVALID;
+    }
+  for (int c = 0; c < b->C; ++c)
+    for (int64_t e = rowptr[b->h_row_base[c]]; e < rowptr[b->h_row_base[c + 1]]; ++e)
+      if (col[e] < 0 || col[e] >= chain_n[c]) {
+        set_error("imped_batch_set_csr: column index out of range");
+        return IMPED_ERR_INVALID;
+      }
+  hipStream_t s = stream();
+  h2d(b->rowptr, rowptr, b->R + 1, s);
+  h2d(b->col, col, std::max<int64_t>(b->NNZ, 1), s);
+  h2d(b->val, val, std::max<int64_t>(b->NNZ, 1) * 2, s);
+  h2d(b->row_base, b->h_row_base.data(), b->C + 1, s);
+  h2d(b->chain_n, b->h_n.data(), std::max(b->C, 1), s);
+  b->h_nnz.assign(b->C, 0);
+  for (int c = 0; c < b->C; ++c) b->h_nnz[c] = rowptr[b->h_row_base[c + 1]] - rowptr[b->h_row_base[c]];
+  HIP_CHECK(hipStreamSynchronize(s));
+  b->built = true;
+  API_END
+}
+
+int imped_batch_sizes(imped_batch batch, int64_t* chain_n, int64_t* chain_nnz, int64_t* totals) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b->built) {
+    set_error("imped_batch_sizes: batch not built");
+    return IMPED_ERR_INVALID;
+  }
+  if (chain_n) std::copy(b->h_n.begin(), b->h_n.end(), chain_n);
+  if (chain_nnz) std::copy(b->h_nnz.begin(), b->h_nnz.end(), chain_nnz);
+  if (totals) {
+    totals[0] = b->R;
+    totals[1] = b->NNZ;
+  }
+  API_END
+}
+
+int imped_batch_get_basis(imped_batch batch, uint64_t* keys) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  d2h(keys, b->pool_keys.p, (size_t)b->R * b->W, stream());
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int imped_batch_get_csr(imped_batch batch, int64_t* rowptr, int32_t* col, double* val) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (rowptr) d2h(rowptr, b->rowptr.p, b->R + 1, stream());
+  if (col) d2h(col, b->col.p, b->NNZ, stream());
+  if (val) d2h(val, b->val.p, b->NNZ * 2, stream());
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int imped_batch_lanczos(imped_batch batch, const double* v0_host, int K, int reorth, double* alpha, double* beta,
+                        int32_t* kout, int32_t* status) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b->built || K < 1) {
+    set_error("imped_batch_lanczos: batch not built or K < 1");
+    return IMPED_ERR_INVALID;
+  }
+  DevBuf v0;
+  const double2* v0d = nullptr;
+  if (v0_host) {
+    h2d(v0, v0_host, std::max<int64_t>(b->R, 1) * 2, stream());
+    v0d = v0.as<double2>();
+  } else {
+    if (!b->have_seed_amps) {
+      set_error("imped_batch_lanczos: no start vector");
+      return IMPED_ERR_INVALID;
+    }
+    // seed amplitudes occupy the first local rows of every chain
+    v0.reserve(std::max<int64_t>(b->R, 1) * 16);
+    HIP_CHECK(hipMemsetAsync(v0.p, 0, std::max<int64_t>(b->R, 1) * 16, stream()));
+    for (int c = 0; c < b->C; ++c) {
+      int64_t ns = b->h_seed_off[c + 1] - b->h_seed_off[c];
+      if (ns)
+        HIP_CHECK(hipMemcpyAsync(v0.as<double2>() + b->h_row_base[c], b->seed_amps.as<double2>() + b->h_seed_off[c],
+                                 ns * 16, hipMemcpyDeviceToDevice, stream()));
+    }
+    v0d = v0.as<double2>();
+  }
+  b->K = K;
+  b->alpha.reserve((size_t)std::max(b->C, 1) * K * 8);
+  b->beta.reserve((size_t)std::max(b->C, 1) * K * 8);
+  b->kout.reserve((size_t)std::max(b->C, 1) * 4);
+  b->status.reserve((size_t)std::max(b->C, 1) * 4);
+  KrylovBatch kb{b->C,
+                 b->row_base.as<int64_t>(),
+                 b->chain_n.as<int64_t>(),
+                 b->rowptr.as<int64_t>(),
+                 b->col.as<int32_t>(),
+                 b->val.as<double2>(),
+                 b->h_n,
+                 b->h_nnz};
+  run_lanczos(kb, v0d, K, reorth, b->alpha.as<double>(), b->beta.as<double>(), b->kout.as<int32_t>(),
+              b->status.as<int32_t>(), stream());
+  if (alpha) d2h(alpha, b->alpha.p, (size_t)b->C * K, stream());
+  if (beta) d2h(beta, b->beta.p, (size_t)b->C * K, stream());
+  if (kout) d2h(kout, b->kout.p, b->C, stream());
+  if (status) d2h(status, b->status.p, b->C, stream());
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int imped_batch_eigen(imped_batch batch, int nev, double tol, int max_steps, uint64_t seed, double* evals,
+                      double* evecs_host, int32_t* steps_out) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b->built || nev < 1) {
+    set_error("imped_batch_eigen: batch not built or nev < 1");
+    return IMPED_ERR_INVALID;
+  }
+  b->nev = nev;
+  b->evals.reserve((size_t)std::max(b->C, 1) * nev * 8);
+  b->evecs.reserve(std::max<int64_t>(b->R, 1) * nev * 16);
+  KrylovBatch kb{b->C,
+                 b->row_base.as<int64_t>(),
+                 b->chain_n.as<int64_t>(),
+                 b->rowptr.as<int64_t>(),
+                 b->col.as<int32_t>(),
+                 b->val.as<double2>(),
+                 b->h_n,
+                 b->h_nnz};
+  DevBuf steps;
+  steps.reserve(std::max(b->C, 1) * 4);
+  int rc = run_eigen(kb, nev, tol, max_steps, seed, b->evals.as<double>(), b->evecs.as<double2>(),
+                     steps.as<int32_t>(), stream());
+  if (evals) d2h(evals, b->evals.p, (size_t)b->C * nev, stream());
+  if (evecs_host) d2h(evecs_host, b->evecs.p, (size_t)b->R * nev * 2, stream());
+  if (steps_out) d2h(steps_out, steps.p, b->C, stream());
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  if (rc != IMPED_OK) {
+    set_error("imped_batch_eigen: not converged within max_steps");
+    return rc;
+  }
+  API_END
+}
+
+int imped_batch_cf(imped_batch batch, const double* energy, const double* norm2, const int32_t* sign,
+                   const int32_t* zflag, double kT, const int32_t* spec_id, int n_spec, const double* omega, int n_w,
+                   double delta, double* g_out, double* spec_out) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (b->K < 1) {
+    set_error("imped_batch_cf: run imped_batch_lanczos first");
+    return IMPED_ERR_INVALID;
+  }
+  if (!(kT > 0.0)) {
+    set_error("imped_batch_cf: kT must be positive");
+    return IMPED_ERR_INVALID;
+  }
+  run_cf(b->C, b->K, b->alpha.as<double>(), b->beta.as<double>(), b->kout.as<int32_t>(), energy, norm2, sign, zflag,
+         kT, spec_id, n_spec, omega, n_w, delta, g_out, spec_out, stream());
+  API_END
+}
+
+// ------------------------------------------------------------------ sparse vectors
+
+int imped_vec_create(int W, int n_vec, const int64_t* off, const uint64_t* keys, const double* amps, imped_vec* out) {
+  API_BEGIN
+  auto v = std::make_unique<Vec>();
+  v->W = W;
+  v->V = n_vec;
+  v->h_off.assign(off, off + n_vec + 1);
+  const int64_t n = v->total();
+  h2d(v->keys, keys, std::max<int64_t>(n, 1) * W, stream());
+  h2d(v->amps, amps, std::max<int64_t>(n, 1) * 2, stream());
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  *out = reinterpret_cast<imped_vec>(v.release());
+  API_END
+}
+
+int imped_vec_destroy(imped_vec vec) {
+  delete reinterpret_cast<Vec*>(vec);
+  return IMPED_OK;
+}
+
+int imped_vec_offsets(imped_vec vec, int64_t* off) {
+  Vec* v = reinterpret_cast<Vec*>(vec);
+  std::copy(v->h_off.begin(), v->h_off.end(), off);
+  return IMPED_OK;
+}
+
+int imped_vec_get(imped_vec vec, uint64_t* keys, double* amps, double* norm2) {
+  API_BEGIN
+  Vec* v = reinterpret_cast<Vec*>(vec);
+  const int64_t n = v->total();
+  if (keys) d2h(keys, v->keys.p, (size_t)n * v->W, stream());
+  if (amps) d2h(amps, v->amps.p, (size_t)n * 2, stream());
+  if (norm2 && v->norm2.p) d2h(norm2, v->norm2.p, v->V, stream());
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int imped_vec_apply(imped_ops ops, imped_vec src, int n_out, const int32_t* out_src, const int32_t* out_model,
+                    double weight_min, int normalize, imped_vec* out) {
+  API_BEGIN
+  Ops* o = reinterpret_cast<Ops*>(ops);
+  Vec* v = reinterpret_cast<Vec*>(src);
+  if (v->W != o->W) {
+    set_error("imped_vec_apply: word count mismatch");
+    return IMPED_ERR_INVALID;
+  }
+  for (int i = 0; i < n_out; ++i)
+    if (out_src[i] < 0 || out_src[i] >= v->V || out_model[i] < 0 || out_model[i] >= o->M) {
+      set_error("imped_vec_apply: output vector mapping out of range");
+      return IMPED_ERR_INVALID;
+    }
+  auto r = std::make_unique<Vec>();
+  r->W = o->W;
+  r->V = n_out;
+  vector_apply(o->view(), v->W, v->h_off, v->keys.as<uint64_t>(), v->amps.as<double2>(), n_out, out_src,
+               out_model, weight_min, normalize != 0, r->h_off, r->keys, r->amps, r->norm2, stream(), o->n_orb);
+  *out = reinterpret_cast<imped_vec>(r.release());
+  API_END
+}
+
+int imped_vec_from_eigen(imped_batch batch, int n_out, const int32_t* chain, const int32_t* which, double weight_min,
+                         imped_vec* out) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (b->nev < 1) {
+    set_error("imped_vec_from_eigen: run imped_batch_eigen first");
+    return IMPED_ERR_INVALID;
+  }
+  for (int i = 0; i < n_out; ++i)
+    if (chain[i] < 0 || chain[i] >= b->C || which[i] < 0 || which[i] >= b->nev) {
+      set_error("imped_vec_from_eigen: index out of range");
+      return IMPED_ERR_INVALID;
+    }
+  auto r = std::make_unique<Vec>();
+  r->W = b->W;
+  r->V = n_out;
+  truncate_eigvecs(b->W, b->pool_keys.as<uint64_t>(), b->evecs.as<double2>(), b->nev, b->h_row_base, b->h_n, n_out,
+                   chain, which, weight_min, r->h_off, r->keys, r->amps, stream());
+  *out = reinterpret_cast<imped_vec>(r.release());
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// Krylov-space kernels: batched Lanczos, ground-state eigensolver,
+// continued fractions with the thermal epilogue, and sparse-vector helpers.
+#pragma once
+
+#include "kernels.h"
+
+namespace imped {
+
+// CSR of all chains of a batch; rows of chain c are [row_base[c], row_base[c]+chain_n[c]),
+// column indices are chain-local.
+struct KrylovBatch {
+  int C;
+  const int64_t* row_base;
+  const int64_t* chain_n;
+  const int64_t* rowptr;
+  const int32_t* col;
+  const double2* val;
+  std::vector<int64_t> h_n, h_nnz;
+};
+
+// Plain (reorth = 0) or fully reorthogonalized (reorth = 1) Lanczos from v0
+// for every chain; alpha/beta are [C][K], kout[c] = min(K, n_c) (or the
+// step of a beta == 0 breakdown, with status[c] = IMPED_ERR_BREAKDOWN).
+void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, double* alpha, double* beta,
+                 int32_t* kout, int32_t* status, hipStream_t s);
+
+// Lowest nev eigenpairs of every chain.  evals [C][nev]; evecs: chain c's
+// block starts at row_base[c]*nev, eigenvector k at offset k*n_c.
+int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_t seed, double* evals,
+              double2* evecs, int32_t* steps, hipStream_t s);
+
+// Continued fractions of all chains on the omega grid plus the Boltzmann
+// average over each spectrum's chains (host arrays are uploaded).
+void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t* kout, const double* energy,
+            const double* norm2, const int32_t* sign, const int32_t* zflag, double kT, const int32_t* spec_id,
+            int n_spec, const double* omega, int n_w, double delta, double* g_out, double* spec_out, hipStream_t s);
+
+// out vector o = op[out_model[o]] applied to source vector out_src[o].
+void vector_apply(OpsView ops, int W, const std::vector<int64_t>& src_off, const uint64_t* src_keys,
+                  const double2* src_amps, int n_out, const int32_t* out_src, const int32_t* out_model,
+                  double weight_min, bool normalize, std::vector<int64_t>& out_off, DevBuf& out_keys,
+                  DevBuf& out_amps, DevBuf& out_norm2, hipStream_t s, int n_orb);
+
+// Eigenvector components with |v_i|^2 >= weight_min, in basis order.
+void truncate_eigvecs(int W, const uint64_t* basis_keys, const double2* evecs, int nev,
+                      const std::vector<int64_t>& row_base, const std::vector<int64_t>& chain_n, int n_out,
+                      const int32_t* chain, const int32_t* which, double weight_min, std::vector<int64_t>& out_off,
+                      DevBuf& out_keys, DevBuf& out_amps, hipStream_t s);
+
+}  // namespace imped

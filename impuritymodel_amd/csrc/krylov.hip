@@ -1,0 +1,633 @@
+// Krylov-space kernels of the engine.
+//
+// * lanczos_kernel: get_tridiagonal_krylov_vectors (impurityModel/ed/
+//   finite.py:1974-1992) for every chain of a batch in one launch, one
+//   workgroup per chain running all K steps (SpMV + alpha, w update + beta,
+//   scale) with deterministic block reductions.  reorth = 1 adds two
+//   classical Gram-Schmidt passes against all stored Krylov vectors.
+// * eigen_kernel: replaces eigsh(which="SA") of eigensystem (finite.py:105):
+//   Lanczos with full reorthogonalization from a seeded random vector, the
+//   lowest Ritz pairs of the tridiagonal matrix by Sturm bisection and
+//   inverse iteration, checked every few steps against the residual
+//   |beta_m y_k[m-1]| <= tol * max(1, |lambda_k|).
+// * cf_kernel: the continued fraction of getGreen (spectra.py:677-680) on the
+//   omega grid for every chain, fused with the Boltzmann weights of
+//   thermal_average (average.py:11-45) and the sum over transition operators
+//   (spectra.py:233-238).
+#include <cmath>
+
+#include "krylov.h"
+
+namespace imped {
+
+namespace {
+
+constexpr int KT = 256;  // threads per Krylov workgroup
+
+__device__ __forceinline__ double block_sum(double x, double* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
+  __syncthreads();
+  double r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+// y = H x for the rows of one chain; returns the partial Re<x, y>.
+__device__ __forceinline__ double spmv_rows(const int64_t* rowptr, const int32_t* col, const double2* val,
+                                            int64_t base, int64_t n, const double2* x, double2* y) {
+  double pa = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += KT) {
+    const int64_t e0 = rowptr[base + i], e1 = rowptr[base + i + 1];
+    double ax = 0.0, ay = 0.0;
+    for (int64_t e = e0; e < e1; ++e) {
+      const double2 h = val[e];
+      const double2 v = x[col[e]];
+      ax += h.x * v.x - h.y * v.y;
+      ay += h.x * v.y + h.y * v.x;
+    }
+    y[i] = make_double2(ax, ay);
+    pa += ax * x[i].x + ay * x[i].y;
+  }
+  return pa;
+}
+
+// Two passes of classical Gram-Schmidt of w against V[0..m) (each n long).
+// coef must hold m doubles*2 in LDS-sized scratch (global here).
+__device__ void cgs2(const double2* V, int m, int64_t n, double2* w, double2* coef) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int l = wave; l < m; l += KT / 64) {
+      const double2* vl = V + (size_t)l * n;
+      double cx = 0.0, cy = 0.0;
+      for (int64_t i = lane; i < n; i += 64) {
+        const double2 a = vl[i], b = w[i];
+        cx += a.x * b.x + a.y * b.y;  // conj(a) * b
+        cy += a.x * b.y - a.y * b.x;
+      }
+      cx = wave_sum(cx);
+      cy = wave_sum(cy);
+      if (lane == 0) coef[l] = make_double2(cx, cy);
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < n; i += KT) {
+      double wx = w[i].x, wy = w[i].y;
+      for (int l = 0; l < m; ++l) {
+        const double2 c = coef[l];
+        const double2 a = V[(size_t)l * n + i];
+        wx -= a.x * c.x - a.y * c.y;
+        wy -= a.x * c.y + a.y * c.x;
+      }
+      w[i] = make_double2(wx, wy);
+    }
+    __syncthreads();
+  }
+}
+
+struct LanczosArgs {
+  const int64_t* row_base;
+  const int64_t* chain_n;
+  const int64_t* rowptr;
+  const int32_t* col;
+  const double2* val;
+  const double2* v0;
+  int K;
+  int reorth;
+  double2* ws;      // 3 vectors per chain at 3*row_base
+  double2* vstore;  // reorth: K vectors per chain at K*row_base
+  double2* coef;    // reorth: K per chain
+  double* alpha;
+  double* beta;
+  int32_t* kout;
+  int32_t* status;
+};
+
+__global__ __launch_bounds__(KT) void lanczos_kernel(LanczosArgs a) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  const int64_t n = a.chain_n[c];
+  const int64_t base = a.row_base[c];
+  double* alpha = a.alpha + (size_t)c * a.K;
+  double* beta = a.beta + (size_t)c * a.K;
+  if (n == 0) {
+    if (threadIdx.x == 0) {
+      a.kout[c] = 0;
+      a.status[c] = IMPED_OK;
+    }
+    return;
+  }
+  const int K = (int)(a.K < n ? a.K : n);
+  double2* v0 = a.ws + 3 * base;
+  double2* v1 = v0 + n;
+  double2* w = v1 + n;
+  double2* V = a.reorth ? a.vstore + (size_t)a.K * base : nullptr;
+  double2* coef = a.reorth ? a.coef + (size_t)a.K * c : nullptr;
+  for (int64_t i = threadIdx.x; i < n; i += KT) {
+    v0[i] = a.v0[base + i];
+    if (V) V[i] = v0[i];
+  }
+  __syncthreads();
+  // wp = H v0; alpha0 = Re(conj(wp) . v0); w = wp - alpha0 v0
+  double al = block_sum(spmv_rows(a.rowptr, a.col, a.val, base, n, v0, w), red);
+  if (threadIdx.x == 0) alpha[0] = al;
+  for (int64_t i = threadIdx.x; i < n; i += KT) w[i] = make_double2(w[i].x - al * v0[i].x, w[i].y - al * v0[i].y);
+  __syncthreads();
+  int status = IMPED_OK;
+  int kdone = K;
+  for (int j = 1; j < K; ++j) {
+    if (V) cgs2(V, j, n, w, coef);
+    double p = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += KT) p += w[i].x * w[i].x + w[i].y * w[i].y;
+    const double b = sqrt(block_sum(p, red));
+    if (threadIdx.x == 0) beta[j - 1] = b;
+    if (b == 0.0) {
+      status = IMPED_ERR_BREAKDOWN;
+      kdone = j;
+      break;
+    }
+    for (int64_t i = threadIdx.x; i < n; i += KT) {
+      v1[i] = make_double2(w[i].x / b, w[i].y / b);
+      if (V) V[(size_t)j * n + i] = v1[i];
+    }
+    __syncthreads();
+    al = block_sum(spmv_rows(a.rowptr, a.col, a.val, base, n, v1, w), red);
+    if (threadIdx.x == 0) alpha[j] = al;
+    for (int64_t i = threadIdx.x; i < n; i += KT) {
+      const double2 x = v1[i], y = v0[i];
+      w[i] = make_double2(w[i].x - al * x.x - b * y.x, w[i].y - al * x.y - b * y.y);
+      v0[i] = x;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.kout[c] = kdone;
+    a.status[c] = status;
+  }
+}
+
+// ------------------------------------------------------------------ eigen
+
+__device__ __forceinline__ uint64_t splitmix(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ double u01(uint64_t x) { return (double)(splitmix(x) >> 11) * (1.0 / 9007199254740992.0); }
+
+// Number of eigenvalues of the tridiagonal (a, b) below x (Sturm sequence).
+__device__ int sturm_count(const double* a, const double* b, int m, double x, double pivmin) {
+  int cnt = 0;
+  double q = a[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0) ++cnt;
+  for (int i = 1; i < m; ++i) {
+    q = a[i] - x - b[i - 1] * b[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+
+__device__ double kth_eigenvalue(const double* a, const double* b, int m, int k) {
+  double lo = a[0], hi = a[0], scale = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
+    lo = fmin(lo, a[i] - r);
+    hi = fmax(hi, a[i] + r);
+    scale = fmax(scale, fabs(a[i]) + r);
+  }
+  const double pivmin = 1e-300 + 2.2e-16 * 2.2e-16 * scale * scale;
+  lo -= 1e-14 * scale + 1e-300;
+  hi += 1e-14 * scale + 1e-300;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_count(a, b, m, mid, pivmin) > k) hi = mid; else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// Inverse iteration for the eigenvector of (a, b) at lambda; LU with partial
+// pivoting of the shifted tridiagonal.  y and work (5m) are thread-private.
+__device__ void tri_inverse_iteration(const double* a, const double* b, int m, double lambda, double* y, double* dd,
+                                      double* uu, double* ww, double* ll, int* piv, uint64_t seed) {
+  double scale = 0.0;
+  for (int i = 0; i < m; ++i) scale = fmax(scale, fabs(a[i]) + (i < m - 1 ? fabs(b[i]) : 0.0));
+  const double tiny = 2.2e-16 * (scale > 0 ? scale : 1.0);
+  for (int i = 0; i < m; ++i) {
+    dd[i] = a[i] - lambda;
+    uu[i] = i < m - 1 ? b[i] : 0.0;
+    ww[i] = 0.0;
+  }
+  for (int i = 0; i < m - 1; ++i) {
+    const double sub = b[i];
+    if (fabs(dd[i]) >= fabs(sub)) {
+      if (dd[i] == 0.0) dd[i] = tiny;
+      const double mult = sub / dd[i];
+      dd[i + 1] -= mult * uu[i];
+      ll[i] = mult;
+      piv[i] = 0;
+    } else {
+      const double mult = dd[i] / sub;
+      const double d1 = dd[i + 1];
+      const double u1 = uu[i + 1];
+      const double u0 = uu[i];
+      dd[i] = sub;
+      uu[i] = d1;
+      ww[i] = u1;
+      dd[i + 1] = u0 - mult * d1;
+      uu[i + 1] = -mult * u1;
+      ll[i] = mult;
+      piv[i] = 1;
+    }
+  }
+  if (dd[m - 1] == 0.0) dd[m - 1] = tiny;
+  for (int i = 0; i < m; ++i) y[i] = 0.5 + u01(seed + (uint64_t)i);
+  for (int it = 0; it < 3; ++it) {
+    for (int i = 0; i < m - 1; ++i) {
+      if (piv[i]) {
+        const double t = y[i];
+        y[i] = y[i + 1];
+        y[i + 1] = t;
+      }
+      y[i + 1] -= ll[i] * y[i];
+    }
+    for (int i = m - 1; i >= 0; --i) {
+      double r = y[i];
+      if (i + 1 < m) r -= uu[i] * y[i + 1];
+      if (i + 2 < m) r -= ww[i] * y[i + 2];
+      double d = dd[i];
+      if (fabs(d) < tiny) d = d < 0 ? -tiny : tiny;
+      y[i] = r / d;
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < m; ++i) nrm += y[i] * y[i];
+    nrm = sqrt(nrm);
+    for (int i = 0; i < m; ++i) y[i] /= nrm;
+  }
+}
+
+struct EigenArgs {
+  const int64_t* row_base;
+  const int64_t* chain_n;
+  const int64_t* rowptr;
+  const int32_t* col;
+  const double2* val;
+  int nev;
+  int max_steps;
+  double tol;
+  uint64_t seed;
+  double2* vstore;     // per chain: msteps(c) * n vectors, offset voff[c]
+  const int64_t* voff;
+  double2* wbuf;       // per chain: n at row_base
+  double2* coef;       // per chain: max_steps
+  double* ab;          // per chain: 2*max_steps (alpha then beta)
+  double* yk;          // per chain: nev*max_steps
+  double* tw;          // per chain: nev * 5 * max_steps
+  int* tpiv;           // per chain: nev * max_steps
+  double* evals;
+  double2* evecs;
+  int32_t* steps;
+  int32_t* conv;
+};
+
+__global__ __launch_bounds__(KT) void eigen_kernel(EigenArgs a) {
+  __shared__ double red[4];
+  __shared__ int s_done;
+  const int c = blockIdx.x;
+  const int64_t n = a.chain_n[c];
+  const int64_t base = a.row_base[c];
+  const int nev = a.nev;
+  const int ms = a.max_steps;
+  if (n == 0) {
+    if (threadIdx.x == 0) {
+      a.steps[c] = 0;
+      a.conv[c] = 1;
+    }
+    return;
+  }
+  const int mmax = (int)(ms < n ? ms : n);
+  double2* V = a.vstore + a.voff[c];
+  double2* w = a.wbuf + base;
+  double2* coef = a.coef + (size_t)c * ms;
+  double* alpha = a.ab + (size_t)c * 2 * ms;
+  double* beta = alpha + ms;
+  double* Y = a.yk + (size_t)c * nev * ms;
+  // random start vector
+  double p = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += KT) {
+    const uint64_t h = a.seed * 0x100000001B3ull + (uint64_t)c * 0x9E3779B97F4A7C15ull + 2 * (uint64_t)i;
+    const double2 x = make_double2(u01(h) - 0.5, u01(h + 1) - 0.5);
+    V[i] = x;
+    p += x.x * x.x + x.y * x.y;
+  }
+  double nrm = sqrt(block_sum(p, red));
+  for (int64_t i = threadIdx.x; i < n; i += KT) V[i] = make_double2(V[i].x / nrm, V[i].y / nrm);
+  __syncthreads();
+  int m = 0;
+  int converged = 0;
+  for (int j = 0; j < mmax; ++j) {
+    const double2* vj = V + (size_t)j * n;
+    const double al = block_sum(spmv_rows(a.rowptr, a.col, a.val, base, n, vj, w), red);
+    if (threadIdx.x == 0) alpha[j] = al;
+    const double bprev = j > 0 ? beta[j - 1] : 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += KT) {
+      double2 x = w[i];
+      x.x -= al * vj[i].x;
+      x.y -= al * vj[i].y;
+      if (j > 0) {
+        x.x -= bprev * vj[i - n].x;
+        x.y -= bprev * vj[i - n].y;
+      }
+      w[i] = x;
+    }
+    __syncthreads();
+    cgs2(V, j + 1, n, w, coef);
+    p = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += KT) p += w[i].x * w[i].x + w[i].y * w[i].y;
+    const double b = sqrt(block_sum(p, red));
+    m = j + 1;
+    const bool last = (m == mmax) || b <= 1e-13 * (fabs(al) + bprev + 1e-300);
+    if (threadIdx.x == 0) beta[j] = b;
+    __syncthreads();
+    if (last || (m >= nev && m % 8 == 0)) {
+      // Ritz pairs of T_m: thread k handles eigenvalue k
+      if (threadIdx.x == 0) s_done = 1;
+      __syncthreads();
+      if ((int)threadIdx.x < nev && (int)threadIdx.x < m) {
+        const int k = threadIdx.x;
+        const double lam = kth_eigenvalue(alpha, beta, m, k);
+        double* y = Y + (size_t)k * ms;
+        double* tw = a.tw + ((size_t)c * nev + k) * 5 * ms;
+        int* piv = a.tpiv + ((size_t)c * nev + k) * ms;
+        tri_inverse_iteration(alpha, beta, m, lam, y, tw, tw + ms, tw + 2 * ms, tw + 3 * ms, piv,
+                              a.seed + 7919 * (uint64_t)k);
+        a.evals[(size_t)c * nev + k] = lam;
+        const double res = fabs(b * y[m - 1]);
+        if (!(res <= a.tol * fmax(1.0, fabs(lam)))) atomicAnd(&s_done, 0);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0 && nev > 1) {
+        // orthogonalize inverse-iteration vectors of clustered eigenvalues
+        for (int k = 1; k < nev && k < m; ++k) {
+          double* yk = Y + (size_t)k * ms;
+          for (int q = 0; q < k; ++q) {
+            const double dl = fabs(a.evals[(size_t)c * nev + k] - a.evals[(size_t)c * nev + q]);
+            if (dl > 1e-3 * fmax(1.0, fabs(a.evals[(size_t)c * nev + k]))) continue;
+            const double* yq = Y + (size_t)q * ms;
+            double d = 0.0;
+            for (int i = 0; i < m; ++i) d += yq[i] * yk[i];
+            double nn = 0.0;
+            for (int i = 0; i < m; ++i) {
+              yk[i] -= d * yq[i];
+              nn += yk[i] * yk[i];
+            }
+            nn = sqrt(nn);
+            for (int i = 0; i < m; ++i) yk[i] /= nn;
+          }
+        }
+      }
+      __syncthreads();
+      if (s_done || last) {
+        converged = s_done;
+        break;
+      }
+    }
+    if (last) break;
+    for (int64_t i = threadIdx.x; i < n; i += KT) V[(size_t)(j + 1) * n + i] = make_double2(w[i].x / b, w[i].y / b);
+    __syncthreads();
+  }
+  // Ritz vectors x_k = sum_l y_k[l] V_l, normalized
+  double2* X = a.evecs + base * nev;
+  const int kk = nev < m ? nev : m;
+  for (int k = 0; k < kk; ++k) {
+    const double* y = Y + (size_t)k * ms;
+    double pn = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += KT) {
+      double xr = 0.0, xi = 0.0;
+      for (int l = 0; l < m; ++l) {
+        const double2 v = V[(size_t)l * n + i];
+        xr += y[l] * v.x;
+        xi += y[l] * v.y;
+      }
+      X[(size_t)k * n + i] = make_double2(xr, xi);
+      pn += xr * xr + xi * xi;
+    }
+    const double nn = sqrt(block_sum(pn, red));
+    for (int64_t i = threadIdx.x; i < n; i += KT)
+      X[(size_t)k * n + i] = make_double2(X[(size_t)k * n + i].x / nn, X[(size_t)k * n + i].y / nn);
+    __syncthreads();
+  }
+  for (int k = kk; k < nev; ++k) {
+    for (int64_t i = threadIdx.x; i < n; i += KT) X[(size_t)k * n + i] = make_double2(0.0, 0.0);
+    if (threadIdx.x == 0) a.evals[(size_t)c * nev + k] = NAN;
+  }
+  if (threadIdx.x == 0) {
+    a.steps[c] = m;
+    a.conv[c] = converged;
+  }
+}
+
+// ------------------------------------------------------------------ continued fraction
+
+struct CFArgs {
+  const double* alpha;
+  const double* beta;
+  const int32_t* kout;
+  int K;
+  const double* energy;
+  const double* norm2;
+  const int32_t* sign;
+  const int32_t* zflag;  // 1 on exactly one chain per thermal state of a spectrum
+  double kT;
+  const int32_t* spec_chain_off;  // chains of spectrum q: spec_chains[off[q] .. off[q+1])
+  const int32_t* spec_chains;
+  const double* omega;
+  int n_w;
+  double delta;
+  double2* g_out;  // optional [C][n_w]
+  double* spec_out;
+};
+
+// One thread per (spectrum, omega); the spectrum's chains are summed in order.
+__global__ __launch_bounds__(256) void cf_kernel(CFArgs a) {
+  const int q = blockIdx.y;
+  const int iw = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iw >= a.n_w) return;
+  const double om = a.omega[iw];
+  const int t0 = a.spec_chain_off[q], t1 = a.spec_chain_off[q + 1];
+  // Boltzmann weights exp(-(e - e_min)/kT) / Z over the spectrum's states
+  double emin = INFINITY;
+  for (int t = t0; t < t1; ++t) emin = fmin(emin, a.energy[a.spec_chains[t]]);
+  double Z = 0.0;
+  for (int t = t0; t < t1; ++t)
+    if (a.zflag[a.spec_chains[t]]) Z += exp(-(a.energy[a.spec_chains[t]] - emin) / a.kT);
+  double acc = 0.0;
+  for (int t = t0; t < t1; ++t) {
+    const int c = a.spec_chains[t];
+    const int K = a.kout[c];
+    const double sg = (double)a.sign[c];
+    // z = sg*omega + i*sg*delta + e  (photoemission uses -omega, -delta)
+    const double zr = sg * om + a.energy[c];
+    const double zi = sg * a.delta;
+    const double* al = a.alpha + (size_t)c * a.K;
+    const double* be = a.beta + (size_t)c * a.K;
+    double2 g = make_double2(0.0, 0.0);
+    for (int i = K - 1; i >= 0; --i) {
+      double2 d;
+      if (i == K - 1) {
+        d = make_double2(zr - al[i], zi);
+      } else {
+        const double b2 = be[i] * be[i];
+        d = make_double2(zr - al[i] - b2 * g.x, zi - b2 * g.y);
+      }
+      g = crecip(d);
+    }
+    // gs = sign * norm2 * g ; spectrum contribution weight * (-Im gs)
+    const double f = sg * a.norm2[c];
+    const double2 gs = make_double2(f * g.x, f * g.y);
+    if (a.g_out) a.g_out[(size_t)c * a.n_w + iw] = gs;
+    acc += exp(-(a.energy[c] - emin) / a.kT) * (-gs.y);
+  }
+  a.spec_out[(size_t)q * a.n_w + iw] = Z > 0.0 ? acc / Z : 0.0;
+}
+
+struct KScratch {
+  DevBuf ws, vstore, coef, ab, yk, tw, tpiv, voff, conv, wbuf;
+  DevBuf cf_alpha_tmp, e, nrm, sg, wt, soff, sch, om, gout, sout;
+};
+KScratch& kscratch() {
+  static KScratch k;
+  return k;
+}
+
+}  // namespace
+
+void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, double* alpha, double* beta,
+                 int32_t* kout, int32_t* status, hipStream_t s) {
+  KScratch& sc = kscratch();
+  int64_t R = 0;
+  for (auto x : kb.h_n) R += x;
+  sc.ws.reserve(std::max<int64_t>(3 * R, 1) * 16);
+  LanczosArgs a{};
+  a.row_base = kb.row_base;
+  a.chain_n = kb.chain_n;
+  a.rowptr = kb.rowptr;
+  a.col = kb.col;
+  a.val = kb.val;
+  a.v0 = v0;
+  a.K = K;
+  a.reorth = reorth;
+  a.ws = sc.ws.as<double2>();
+  if (reorth) {
+    sc.vstore.reserve(std::max<int64_t>(R * K, 1) * 16);
+    sc.coef.reserve((size_t)std::max(kb.C, 1) * K * 16);
+    a.vstore = sc.vstore.as<double2>();
+    a.coef = sc.coef.as<double2>();
+  }
+  a.alpha = alpha;
+  a.beta = beta;
+  a.kout = kout;
+  a.status = status;
+  if (kb.C == 0) return;
+  hipLaunchKernelGGL(lanczos_kernel, dim3(kb.C), dim3(KT), 0, s, a);
+  HIP_CHECK(hipGetLastError());
+}
+
+int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_t seed, double* evals,
+              double2* evecs, int32_t* steps, hipStream_t s) {
+  KScratch& sc = kscratch();
+  const int C = kb.C;
+  if (C == 0) return IMPED_OK;
+  std::vector<int64_t> voff(C + 1, 0);
+  int64_t R = 0;
+  for (int c = 0; c < C; ++c) {
+    const int64_t mm = std::min<int64_t>(max_steps, kb.h_n[c]);
+    voff[c + 1] = voff[c] + mm * kb.h_n[c];
+    R += kb.h_n[c];
+  }
+  h2d(sc.voff, voff.data(), C + 1, s);
+  sc.vstore.reserve(std::max<int64_t>(voff[C], 1) * 16);
+  sc.wbuf.reserve(std::max<int64_t>(R, 1) * 16);
+  sc.coef.reserve((size_t)C * max_steps * 16);
+  sc.ab.reserve((size_t)C * 2 * max_steps * 8);
+  sc.yk.reserve((size_t)C * nev * max_steps * 8);
+  sc.tw.reserve((size_t)C * nev * 5 * max_steps * 8);
+  sc.tpiv.reserve((size_t)C * nev * max_steps * 4);
+  sc.conv.reserve((size_t)C * 4);
+  EigenArgs a{};
+  a.row_base = kb.row_base;
+  a.chain_n = kb.chain_n;
+  a.rowptr = kb.rowptr;
+  a.col = kb.col;
+  a.val = kb.val;
+  a.nev = nev;
+  a.max_steps = max_steps;
+  a.tol = tol;
+  a.seed = seed;
+  a.vstore = sc.vstore.as<double2>();
+  a.voff = sc.voff.as<int64_t>();
+  a.wbuf = sc.wbuf.as<double2>();
+  a.coef = sc.coef.as<double2>();
+  a.ab = sc.ab.as<double>();
+  a.yk = sc.yk.as<double>();
+  a.tw = sc.tw.as<double>();
+  a.tpiv = sc.tpiv.as<int>();
+  a.evals = evals;
+  a.evecs = evecs;
+  a.steps = steps;
+  a.conv = sc.conv.as<int32_t>();
+  hipLaunchKernelGGL(eigen_kernel, dim3(C), dim3(KT), 0, s, a);
+  HIP_CHECK(hipGetLastError());
+  std::vector<int32_t> conv(C);
+  d2h(conv.data(), sc.conv.p, C, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+  for (int c = 0; c < C; ++c)
+    if (!conv[c]) return IMPED_ERR_NOCONV;
+  return IMPED_OK;
+}
+
+void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t* kout, const double* energy,
+            const double* norm2, const int32_t* sign, const int32_t* zflag, double kT, const int32_t* spec_id,
+            int n_spec, const double* omega, int n_w, double delta, double* g_out, double* spec_out, hipStream_t s) {
+  KScratch& sc = kscratch();
+  // chains grouped by spectrum, in chain order
+  std::vector<int32_t> off(n_spec + 1, 0), chains;
+  for (int c = 0; c < C; ++c)
+    if (spec_id[c] >= 0 && spec_id[c] < n_spec) off[spec_id[c] + 1]++;
+  for (int q = 0; q < n_spec; ++q) off[q + 1] += off[q];
+  chains.assign(std::max(off[n_spec], 1), 0);
+  std::vector<int32_t> fill(off.begin(), off.end() - 1);
+  for (int c = 0; c < C; ++c)
+    if (spec_id[c] >= 0 && spec_id[c] < n_spec) chains[fill[spec_id[c]]++] = c;
+  h2d(sc.e, energy, std::max(C, 1), s);
+  h2d(sc.nrm, norm2, std::max(C, 1), s);
+  h2d(sc.sg, sign, std::max(C, 1), s);
+  h2d(sc.wt, zflag, std::max(C, 1), s);
+  h2d(sc.soff, off.data(), n_spec + 1, s);
+  h2d(sc.sch, chains.data(), chains.size(), s);
+  h2d(sc.om, omega, n_w, s);
+  if (g_out) sc.gout.reserve((size_t)std::max(C, 1) * n_w * 16);
+  sc.sout.reserve((size_t)std::max(n_spec, 1) * n_w * 8);
+  CFArgs a{alpha, beta, kout, K, sc.e.as<double>(), sc.nrm.as<double>(), sc.sg.as<int32_t>(), sc.wt.as<int32_t>(),
+           kT, sc.soff.as<int32_t>(), sc.sch.as<int32_t>(), sc.om.as<double>(), n_w, delta,
+           g_out ? sc.gout.as<double2>() : nullptr, sc.sout.as<double>()};
+  if (n_spec > 0 && n_w > 0) {
+    hipLaunchKernelGGL(cf_kernel, dim3(blocks_for(n_w, 256), n_spec), dim3(256), 0, s, a);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (g_out) d2h(g_out, sc.gout.p, (size_t)C * n_w * 2, s);
+  d2h(spec_out, sc.sout.p, (size_t)n_spec * n_w, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace imped

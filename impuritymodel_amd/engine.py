@@ -1,0 +1,239 @@
+"""Python handles over libimped.so: operator tables, sparse states, batches.
+
+Thin objects that own a C handle each and convert between numpy arrays and
+the C ABI of include/imped.h.  All numerical work happens in the library's
+HIP kernels.
+"""
+
+import numpy as np
+
+from . import opcompile
+from ._lib import IMPED_ERR_BREAKDOWN, IMPED_ERR_NOCONV, C, check, lib, ptr
+
+
+def _c64(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a.view(np.float64).reshape(-1) if a.size else np.zeros(2, np.float64)
+
+
+class Ops:
+    """Compiled operators of several models (one restriction set per model)."""
+
+    def __init__(self, ops, restrictions, n_orb, W=None):
+        self.n_orb = int(n_orb)
+        self.W = W or opcompile.n_words(n_orb)
+        self.M = len(ops)
+        a = opcompile.compile_models(ops, restrictions, self.W)
+        self._arrays = a
+        h = C.c_void_p()
+        check(
+            lib.imped_ops_create(
+                self.W, self.n_orb, self.M, ptr(a["model_group_off"]), ptr(a["group_term_off"]),
+                ptr(_nz(a["group_flip"])), ptr(_nz(a["term_masks"])), ptr(_nz(a["term_meta"])),
+                ptr(_nz(a["term_coef"])), ptr(a["model_restr_off"]), ptr(_nz(a["restr_mask"])),
+                ptr(_nz(a["restr_lohi"])), C.byref(h),
+            )
+        )
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.imped_ops_destroy(self.h)
+            self.h = None
+
+
+def _nz(a):
+    """Non-empty contiguous array (the C side never reads past the counts)."""
+    a = np.ascontiguousarray(a)
+    if a.size == 0:
+        return np.zeros(1, dtype=a.dtype)
+    return a
+
+
+class Vec:
+    """A set of sparse states {key: amplitude} living on the device."""
+
+    def __init__(self, W, handle=None, keys=None, amps=None, offsets=None):
+        self.W = W
+        if handle is None:
+            keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, W)
+            offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+            h = C.c_void_p()
+            check(lib.imped_vec_create(W, len(offsets) - 1, ptr(offsets), ptr(_nz(keys.reshape(-1))),
+                                       ptr(_nz(_c64(amps))), C.byref(h)))
+            handle = h
+        self.h = handle
+        self._off = None
+
+    @property
+    def offsets(self):
+        if self._off is None:
+            n = C.c_int64()
+            # number of vectors is len(offsets)-1; query via a large-enough buffer
+            raise RuntimeError("offsets must be set by the producer")
+        return self._off
+
+    def set_count(self, V):
+        off = np.zeros(V + 1, np.int64)
+        check(lib.imped_vec_offsets(self.h, ptr(off)))
+        self._off = off
+        return self
+
+    def get(self):
+        """Return (keys [N, W] uint64, amps [N] complex, norm2 [V] or zeros)."""
+        off = self.offsets
+        N = int(off[-1])
+        keys = np.zeros((max(N, 1), self.W), np.uint64)
+        amps = np.zeros(max(N, 1), np.complex128)
+        norm2 = np.zeros(max(len(off) - 1, 1), np.float64)
+        check(lib.imped_vec_get(self.h, ptr(keys), ptr(amps.view(np.float64)), ptr(norm2)))
+        return keys[:N], amps[:N], norm2[: len(off) - 1]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.imped_vec_destroy(self.h)
+            self.h = None
+
+
+def vec_from_host(W, keys_list, amps_list):
+    """Vec from lists of per-vector key arrays [n_v, W] and amplitude arrays."""
+    off = np.zeros(len(keys_list) + 1, np.int64)
+    for v, k in enumerate(keys_list):
+        off[v + 1] = off[v] + len(k)
+    keys = np.concatenate([np.asarray(k, np.uint64).reshape(-1, W) for k in keys_list]) if keys_list else np.zeros((0, W), np.uint64)
+    amps = np.concatenate([np.asarray(a, np.complex128).reshape(-1) for a in amps_list]) if amps_list else np.zeros(0, complex)
+    v = Vec(W, keys=keys, amps=amps, offsets=off)
+    v._off = off
+    return v
+
+
+def vec_apply(ops, src, out_src, out_model, weight_min, normalize=False):
+    """Output vector o = ops model out_model[o] applied to src vector out_src[o]."""
+    out_src = np.ascontiguousarray(out_src, np.int32)
+    out_model = np.ascontiguousarray(out_model, np.int32)
+    h = C.c_void_p()
+    check(lib.imped_vec_apply(ops.h, src.h, len(out_src), ptr(_nz(out_src)), ptr(_nz(out_model)),
+                              float(weight_min), int(bool(normalize)), C.byref(h)))
+    return Vec(ops.W, handle=h).set_count(len(out_src))
+
+
+class Batch:
+    """Chains of a batch: bases, CSR Hamiltonians, Lanczos and eigen results."""
+
+    def __init__(self, ops, chain_model):
+        self.ops = ops
+        self.W = ops.W
+        self.chain_model = np.ascontiguousarray(chain_model, np.int32)
+        self.C = len(self.chain_model)
+        h = C.c_void_p()
+        check(lib.imped_batch_create(ops.h, self.C, ptr(_nz(self.chain_model)), C.byref(h)))
+        self.h = h
+        self.chain_n = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.imped_batch_destroy(self.h)
+            self.h = None
+
+    def set_seeds(self, keys_list, amps_list=None):
+        off = np.zeros(self.C + 1, np.int64)
+        for c, k in enumerate(keys_list):
+            off[c + 1] = off[c] + len(k)
+        keys = (np.concatenate([np.asarray(k, np.uint64).reshape(-1, self.W) for k in keys_list])
+                if self.C else np.zeros((0, self.W), np.uint64))
+        amps = None
+        if amps_list is not None:
+            amps = _c64(np.concatenate([np.asarray(a, np.complex128).reshape(-1) for a in amps_list]))
+        check(lib.imped_batch_set_seeds(self.h, ptr(off), ptr(_nz(keys.reshape(-1))), ptr(amps)))
+
+    def seed_from_vec(self, vec):
+        check(lib.imped_batch_seed_from_vec(self.h, vec.h))
+
+    def build(self, fixed_basis, weight_min=1e-12, max_layers=0):
+        check(lib.imped_batch_build(self.h, int(bool(fixed_basis)), float(weight_min), int(max_layers)))
+        self.chain_n = np.zeros(max(self.C, 1), np.int64)
+        self.chain_nnz = np.zeros(max(self.C, 1), np.int64)
+        tot = np.zeros(2, np.int64)
+        check(lib.imped_batch_sizes(self.h, ptr(self.chain_n), ptr(self.chain_nnz), ptr(tot)))
+        self.chain_n = self.chain_n[: self.C]
+        self.chain_nnz = self.chain_nnz[: self.C]
+        self.rows, self.nnz = int(tot[0]), int(tot[1])
+        self.row_base = np.concatenate([[0], np.cumsum(self.chain_n)]).astype(np.int64)
+        return self
+
+    def set_csr(self, chain_n, rowptr, col, val):
+        """Use a given CSR (rows of H, chain-local columns)."""
+        chain_n = np.ascontiguousarray(chain_n, np.int64)
+        rowptr = np.ascontiguousarray(rowptr, np.int64)
+        col = np.ascontiguousarray(col, np.int32)
+        check(lib.imped_batch_set_csr(self.h, ptr(_nz(chain_n)), ptr(rowptr), ptr(_nz(col)), ptr(_nz(_c64(val)))))
+        self.chain_n = chain_n[: self.C]
+        self.rows = int(chain_n[: self.C].sum())
+        self.nnz = int(rowptr[-1])
+        self.row_base = np.concatenate([[0], np.cumsum(self.chain_n)]).astype(np.int64)
+        self.chain_nnz = np.array([rowptr[self.row_base[c + 1]] - rowptr[self.row_base[c]] for c in range(self.C)],
+                                  np.int64)
+        return self
+
+    def basis_words(self):
+        keys = np.zeros((max(self.rows, 1), self.W), np.uint64)
+        check(lib.imped_batch_get_basis(self.h, ptr(keys)))
+        return keys[: self.rows]
+
+    def csr(self):
+        rowptr = np.zeros(self.rows + 1, np.int64)
+        col = np.zeros(max(self.nnz, 1), np.int32)
+        val = np.zeros(max(self.nnz, 1), np.complex128)
+        check(lib.imped_batch_get_csr(self.h, ptr(rowptr), ptr(col), ptr(val.view(np.float64))))
+        return rowptr, col[: self.nnz], val[: self.nnz]
+
+    def lanczos(self, K, reorth=0, v0=None):
+        alpha = np.zeros((max(self.C, 1), K), np.float64)
+        beta = np.zeros((max(self.C, 1), K), np.float64)
+        kout = np.zeros(max(self.C, 1), np.int32)
+        status = np.zeros(max(self.C, 1), np.int32)
+        v0a = None if v0 is None else _c64(v0)
+        check(lib.imped_batch_lanczos(self.h, ptr(v0a), int(K), int(reorth), ptr(alpha), ptr(beta), ptr(kout),
+                                      ptr(status)))
+        self.K = K
+        return alpha[: self.C], beta[: self.C], kout[: self.C], status[: self.C]
+
+    def eigen(self, nev, tol=1e-12, max_steps=600, seed=12345, want_vectors=True, allow_noconv=False):
+        evals = np.zeros((max(self.C, 1), nev), np.float64)
+        evecs = np.zeros(max(self.rows * nev, 1), np.complex128) if want_vectors else None
+        steps = np.zeros(max(self.C, 1), np.int32)
+        rc = check(
+            lib.imped_batch_eigen(self.h, int(nev), float(tol), int(max_steps), int(seed), ptr(evals),
+                                  None if evecs is None else ptr(evecs.view(np.float64)), ptr(steps)),
+            allow=(IMPED_ERR_NOCONV,) if allow_noconv else (),
+        )
+        self.nev = nev
+        return evals[: self.C], evecs, steps[: self.C], rc == 0
+
+    def eigvec(self, evecs, c, k):
+        """Eigenvector k of chain c from the flat evecs array."""
+        n = int(self.chain_n[c])
+        base = int(self.row_base[c]) * self.nev
+        return evecs[base + k * n: base + (k + 1) * n]
+
+    def vec_from_eigen(self, chains, which, weight_min):
+        chains = np.ascontiguousarray(chains, np.int32)
+        which = np.ascontiguousarray(which, np.int32)
+        h = C.c_void_p()
+        check(lib.imped_vec_from_eigen(self.h, len(chains), ptr(_nz(chains)), ptr(_nz(which)), float(weight_min),
+                                       C.byref(h)))
+        return Vec(self.W, handle=h).set_count(len(chains))
+
+    def cf(self, energy, norm2, sign, zflag, kT, spec_id, n_spec, omega, delta, want_g=False):
+        omega = np.ascontiguousarray(omega, np.float64)
+        g = np.zeros((max(self.C, 1), len(omega)), np.complex128) if want_g else None
+        spec = np.zeros((max(n_spec, 1), len(omega)), np.float64)
+        check(lib.imped_batch_cf(
+            self.h, ptr(_nz(np.asarray(energy, np.float64))), ptr(_nz(np.asarray(norm2, np.float64))),
+            ptr(_nz(np.asarray(sign, np.int32))), ptr(_nz(np.asarray(zflag, np.int32))), float(kT),
+            ptr(_nz(np.asarray(spec_id, np.int32))), int(n_spec), ptr(omega), len(omega), float(delta),
+            None if g is None else ptr(g.view(np.float64)), ptr(spec)))
+        return (g[: self.C] if want_g else None), spec[:n_spec]
+
+
+BREAKDOWN = IMPED_ERR_BREAKDOWN

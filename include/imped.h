@@ -1,0 +1,155 @@
+/* C ABI of libimped.so, the MI355X exact-diagonalization engine.
+ *
+ * impurityModel has no FFI: its hot path is a Python call surface.  Each
+ * entry point below replaces one reference function (cited as
+ * impurityModel/ed/<file>.py:<lines>); the Python package impuritymodel_amd
+ * binds them with ctypes and keeps the reference's names, arguments and
+ * exceptions (finite.applyOp, finite.expand_basis_and_hamiltonian, ...).
+ *
+ * Conventions
+ *  - Product states are W packed uint64 words (W = 1 or 2), spin-orbital i in
+ *    word i/64 at bit 63 - i%64; word-wise lexicographic order equals the
+ *    order of the reference's bytes keys (product_state_representation.py:131-161).
+ *  - Complex numbers are interleaved (re, im) doubles.
+ *  - All pointers are host pointers unless stated otherwise; handles own
+ *    their device memory.  Calls are synchronous on the library's stream.
+ *  - Every function returns IMPED_OK (0) or a negative status; the message
+ *    of the last error on the calling thread is imped_last_error().
+ */
+#ifndef IMPED_H
+#define IMPED_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMPED_ABI_VERSION 1
+
+/* status codes */
+#define IMPED_OK 0
+#define IMPED_ERR_INVALID -1   /* bad argument */
+#define IMPED_ERR_HIP -2       /* HIP runtime error */
+#define IMPED_ERR_CAPACITY -3  /* size limit exceeded */
+#define IMPED_ERR_BREAKDOWN -4 /* Lanczos beta == 0 (finite.py:1963, 1988) */
+#define IMPED_ERR_NOCONV -5    /* eigensolver did not converge */
+
+typedef struct imped_ops_s* imped_ops;
+typedef struct imped_vec_s* imped_vec;
+typedef struct imped_batch_s* imped_batch;
+
+const char* imped_last_error(void);
+int imped_abi_version(void);
+/* Stream for all later calls (a hipStream_t; NULL = the null stream). */
+int imped_set_stream(void* stream);
+int imped_device_synchronize(void);
+
+/* Compiled operators of n_models models (the process dicts of applyOp,
+ * finite.py:1301-1340, plus the occupation restrictions checked at
+ * finite.py:1382-1385).  Model m owns groups [model_group_off[m],
+ * model_group_off[m+1]); group g maps a state s to s ^ group_flip[g] and owns
+ * terms [group_term_off[g], group_term_off[g+1]).  Term t is non-zero on s iff
+ * (s & care) == val with care, val, sgn = term_masks[t*3W + {0, W, 2W}], has
+ * sign (-1)^(popcount(s & sgn) + (term_meta[t] & 1)), value term_coef[2t..2t+1]
+ * and dict position term_meta[t] >> 1.  Restriction r of model m:
+ * restr_lohi[2r] <= popcount(s & restr_mask[rW..]) <= restr_lohi[2r+1]. */
+int imped_ops_create(int W, int n_orb, int n_models, const int32_t* model_group_off, const int32_t* group_term_off,
+                     const uint64_t* group_flip, const uint64_t* term_masks, const uint32_t* term_meta,
+                     const double* term_coef, const int32_t* model_restr_off, const uint64_t* restr_mask,
+                     const int32_t* restr_lohi, imped_ops* out);
+int imped_ops_destroy(imped_ops ops);
+
+/* ---- sparse states (multi-configurational dicts {state: amplitude}) ---- */
+
+/* n_vec vectors, vector v = entries [off[v], off[v+1]) of keys/amps. */
+int imped_vec_create(int W, int n_vec, const int64_t* off, const uint64_t* keys, const double* amps, imped_vec* out);
+int imped_vec_destroy(imped_vec vec);
+int imped_vec_offsets(imped_vec vec, int64_t* off);
+/* keys, amps (and norm2 per vector when computed) copied to host. */
+int imped_vec_get(imped_vec vec, uint64_t* keys, double* amps, double* norm2);
+
+/* applyOp(n, op, psi, slaterWeightMin, restrictions) (finite.py:1301-1511,
+ * branches without opResult): output vector o is operator model out_model[o]
+ * applied to source vector out_src[o].  Targets failing the model's
+ * restrictions are dropped, |amp|^2 < weight_min is pruned after summation and
+ * the dict insertion order is kept.  normalize != 0 divides each output by its
+ * norm as getSpectra does (spectra.py:786-790); norm2 is stored either way. */
+int imped_vec_apply(imped_ops ops, imped_vec src, int n_out, const int32_t* out_src, const int32_t* out_model,
+                    double weight_min, int normalize, imped_vec* out);
+
+/* Eigenvector dicts of eigensystem (finite.py:114-117): output o holds the
+ * components of eigenvector which[o] of chain chain[o] with
+ * |v_i|^2 >= weight_min, in basis order, not renormalized. */
+int imped_vec_from_eigen(imped_batch batch, int n_out, const int32_t* chain, const int32_t* which, double weight_min,
+                         imped_vec* out);
+
+/* ---- batches of Hilbert-space sectors ---- */
+
+int imped_batch_create(imped_ops ops, int n_chains, const int32_t* chain_model, imped_batch* out);
+int imped_batch_destroy(imped_batch batch);
+/* Seed states of every chain (and optional start amplitudes). */
+int imped_batch_set_seeds(imped_batch batch, const int64_t* seed_off, const uint64_t* seed_keys,
+                          const double* seed_amps);
+/* Seeds and start amplitudes from a vector set (vector c -> chain c). */
+int imped_batch_seed_from_vec(imped_batch batch, imped_vec vec);
+
+/* fixed_basis = 1: get_hamiltonian_matrix (finite.py:1514-1610): the seeds
+ *   are the basis, the operator is applied without restrictions, targets
+ *   outside the basis are dropped.
+ * fixed_basis = 0: expand_basis_and_hamiltonian (finite.py:1696-1894): BFS
+ *   closure of the seeds under the operator with the model's restrictions,
+ *   seeds first, then each new layer in ascending key order.
+ * Values with |h|^2 < weight_min are pruned (1e-12 in the reference).
+ * max_layers > 0 stops the closure early (testing). */
+int imped_batch_build(imped_batch batch, int fixed_basis, double weight_min, int max_layers);
+/* Use a given CSR instead of building one (the matrix argument h of
+ * get_tridiagonal_krylov_vectors, finite.py:1897): rows of H with chain-local
+ * int32 columns, rowptr int64 over all chains (rows of chain c follow chain
+ * c-1), complex values.  Columns are checked against the chain sizes. */
+int imped_batch_set_csr(imped_batch batch, const int64_t* chain_n, const int64_t* rowptr, const int32_t* col,
+                        const double* val);
+/* chain_n[C], chain_nnz[C], totals[2] = {rows, nnz}; any may be NULL. */
+int imped_batch_sizes(imped_batch batch, int64_t* chain_n, int64_t* chain_nnz, int64_t* totals);
+/* Basis keys of all chains, rows*W words, chain after chain. */
+int imped_batch_get_basis(imped_batch batch, uint64_t* keys);
+/* CSR rows (H row r = conj of the operator applied to basis state r, i.e.
+ * rowptr[rows+1] int64, chain-local int32 columns, complex values);
+ * the reference's matrix h[i, j] = <i|H|j> is its conjugate transpose
+ * mapped back: h[col, row] = conj(val). */
+int imped_batch_get_csr(imped_batch batch, int64_t* rowptr, int32_t* col, double* val);
+
+/* get_tridiagonal_krylov_vectors (finite.py:1897-1995) for every chain from
+ * v0 (rows complex, or NULL = the seed amplitudes in the first rows of each
+ * chain).  reorth = 0 is the reference's plain recurrence; reorth = 1 adds
+ * full reorthogonalization (no reference counterpart).  alpha, beta are
+ * [C][K]; kout[c] = min(K, n_c), or the step of a beta == 0 breakdown with
+ * status[c] = IMPED_ERR_BREAKDOWN. */
+int imped_batch_lanczos(imped_batch batch, const double* v0, int K, int reorth, double* alpha, double* beta,
+                        int32_t* kout, int32_t* status);
+
+/* eigsh(h, k=nev, which="SA", tol) of eigensystem (finite.py:95-109) for every
+ * chain: evals[C][nev] ascending, evecs (may be NULL) rows*nev complex with
+ * chain c's block at row_base[c]*nev and eigenvector k at offset k*n_c.
+ * Returns IMPED_ERR_NOCONV if some chain missed tol within max_steps. */
+int imped_batch_eigen(imped_batch batch, int nev, double tol, int max_steps, uint64_t seed, double* evals,
+                      double* evecs, int32_t* steps);
+
+/* Continued fractions (spectra.py:677-680) of every chain after
+ * imped_batch_lanczos, on omega[n_w] with broadening delta:
+ *   g_c(w) = sign_c * norm2_c / (z - a0 - b0^2/(z - a1 - ...)),
+ *   z = sign_c*(w + i delta) + energy_c
+ * (sign -1 is the photoemission convention, spectra.py:136-137), written to
+ * g_out[C][n_w] (complex, may be NULL); and spectra
+ *   spec[q][w] = sum_{c in q} p_c * (-Im g_c(w)),
+ *   p_c = exp(-(e_c - min e)/kT) / Z, Z summed over chains with zflag = 1
+ * (thermal_average, average.py:11-45, summed over operators). */
+int imped_batch_cf(imped_batch batch, const double* energy, const double* norm2, const int32_t* sign,
+                   const int32_t* zflag, double kT, const int32_t* spec_id, int n_spec, const double* omega, int n_w,
+                   double delta, double* g_out, double* spec_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IMPED_H */
