@@ -31,6 +31,11 @@ SIGNATURES = {
     "imped_abi_version": [],
     "imped_set_stream": [_p],
     "imped_device_synchronize": [],
+    "imped_timing_enable": [_i32],
+    "imped_timing_reset": [],
+    "imped_timing_query": [C.c_char_p, _p, _p],
+    "imped_set_tuning": [C.c_char_p, _i64],
+    "imped_enumerate_basis": [_i32, _i32, _i32, _p, _i64, _p, _p],
     "imped_ops_create": [_i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "imped_ops_destroy": [_p],
     "imped_vec_create": [_i32, _i32, _p, _p, _p, _p],

@@ -27,7 +27,7 @@ import numpy as np
 
 from . import opcompile
 from .engine import Batch, Ops, vec_apply
-from .finite import get_basis
+from .finite import get_basis_words
 from .models import K_B
 from .operators import getDipoleOperators
 
@@ -47,11 +47,9 @@ class XASResult:
 
 
 def ground_bases(models, W):
-    out = []
-    for m in models:
-        b = get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
-        out.append(opcompile.bytes_to_words(list(b), m.n_spin_orbitals, W))
-    return out
+    """get_basis of every model as [N, W] words (native enumeration)."""
+    return [get_basis_words(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps, W)
+            for m in models]
 
 
 def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0, energy_cut=10.0, reorth=0,

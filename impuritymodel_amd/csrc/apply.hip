@@ -202,6 +202,7 @@ static void launch_w(int mode, bool fill, const ApplyArgs& a, hipStream_t s) {
   const int threads = 256;
   const unsigned blocks = blocks_for(a.n_tasks, threads / 64);
   if (a.n_tasks == 0) return;
+  TimedLaunch tl(fill ? "apply_fill" : "apply_count", s);
 #define IMPED_LAUNCH(M, F) hipLaunchKernelGGL((apply_kernel<W, M, F>), dim3(blocks), dim3(threads), 0, s, a)
   if (mode == MODE_EXPAND) {
     if (fill) IMPED_LAUNCH(MODE_EXPAND, true); else IMPED_LAUNCH(MODE_EXPAND, false);

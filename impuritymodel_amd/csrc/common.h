@@ -87,6 +87,24 @@ inline void d2h(T* dst, const void* src, size_t n, hipStream_t s) {
   if (n) HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
 }
 
+// Launch shapes chosen per chain-dimension bucket (set by the autotuner).
+struct Tuning {
+  int64_t lanczos_wave_max_n = 4096;    // chains up to this size: one wave each
+  int64_t lanczos_wg256_max_n = 65536;  // then 256 threads, above: 1024
+  int64_t eigen_small_n = 512;          // eigensolver: 256 threads up to this size
+  int64_t eigen_large_bt = 1024;        // threads per chain above it (512 or 1024)
+};
+extern Tuning g_tuning;
+
+// Optional HIP-event timing of kernel launches, accumulated per kernel name.
+struct TimedLaunch {
+  const char* name;
+  hipStream_t s;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  TimedLaunch(const char* n, hipStream_t st);
+  ~TimedLaunch();
+};
+
 inline unsigned blocks_for(int64_t n, int threads) {
   int64_t b = (n + threads - 1) / threads;
   return (unsigned)(b < 1 ? 1 : b);

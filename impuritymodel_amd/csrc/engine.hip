@@ -15,6 +15,8 @@
 // eigensystem (finite.py:58), the continued fraction of getGreen
 // (spectra.py:677-680) and thermal_average (average.py:11).
 #include <algorithm>
+#include <array>
+#include <stdexcept>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -27,6 +29,45 @@ namespace imped {
 
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+Tuning g_tuning;
+
+// ------------------------------------------------------------------ launch timing
+
+struct TimingRecord {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+};
+static bool g_timing = false;
+static std::vector<std::pair<std::string, TimingRecord>> g_timings;
+
+static TimingRecord& timing_record(const char* name) {
+  for (auto& kv : g_timings)
+    if (kv.first == name) return kv.second;
+  g_timings.emplace_back(name, TimingRecord{});
+  return g_timings.back().second;
+}
+
+TimedLaunch::TimedLaunch(const char* n, hipStream_t st) : name(n), s(st) {
+  if (!g_timing) return;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipEventRecord(e0, s));
+}
+
+TimedLaunch::~TimedLaunch() {
+  if (!e0) return;
+  (void)hipEventRecord(e1, s);
+  timing_record(name).events.emplace_back(e0, e1);
+}
+
+static void timing_clear() {
+  for (auto& kv : g_timings)
+    for (auto& ev : kv.second.events) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+  g_timings.clear();
+}
 
 static hipStream_t g_stream = nullptr;
 
@@ -115,15 +156,23 @@ __global__ void layer_heads_kernel(const int32_t* pool_chain, int64_t P0, int64_
   if (i == 0 || pool_chain[P0 + i - 1] != c) seg_start[c] = i;
 }
 
+// Local index of each new state: chain size before the layer + rank in its
+// chain's segment.  chain_size is only read here ...
 __global__ void layer_local_kernel(const int32_t* pool_chain, int64_t P0, int64_t n, const int64_t* seg_start,
-                                   int64_t* chain_size, int32_t* pool_local) {
+                                   const int64_t* chain_size, int32_t* pool_local) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int32_t c = pool_chain[P0 + i];
-  int64_t r = i - seg_start[c];
-  pool_local[P0 + i] = (int32_t)(chain_size[c] + r);
-  bool last = (i == n - 1) || pool_chain[P0 + i + 1] != c;
-  if (last) chain_size[c] += r + 1;
+  pool_local[P0 + i] = (int32_t)(chain_size[c] + (i - seg_start[c]));
+}
+
+// ... and grown afterwards by the last state of each segment.
+__global__ void layer_grow_kernel(const int32_t* pool_chain, int64_t P0, int64_t n, const int64_t* seg_start,
+                                  int64_t* chain_size) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t c = pool_chain[P0 + i];
+  if (i == n - 1 || pool_chain[P0 + i + 1] != c) chain_size[c] += i - seg_start[c] + 1;
 }
 
 __global__ void record_cols_kernel(const int64_t* offsets, const int32_t* counts, int64_t n, int64_t base,
@@ -408,6 +457,9 @@ static void build(Batch& b, int mode, double weight_min, int max_layers) {
     hipLaunchKernelGGL(layer_local_kernel, dim3(blocks_for(n_alive, 256)), dim3(256), 0, s,
                        b.pool_chain.as<int32_t>(), P0, n_alive, b.seg_start.as<int64_t>(),
                        b.chain_size.as<int64_t>(), b.pool_local.as<int32_t>());
+    hipLaunchKernelGGL(layer_grow_kernel, dim3(blocks_for(n_alive, 256)), dim3(256), 0, s,
+                       b.pool_chain.as<int32_t>(), P0, n_alive, b.seg_start.as<int64_t>(),
+                       b.chain_size.as<int64_t>());
     HIP_CHECK(hipGetLastError());
     b.P = P0 + n_alive;
     f0 = f1;
@@ -494,6 +546,148 @@ int imped_set_stream(void* stream) {
 int imped_device_synchronize(void) {
   API_BEGIN
   HIP_CHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
+int imped_timing_enable(int on) {
+  API_BEGIN
+  g_timing = on != 0;
+  if (!g_timing) timing_clear();
+  API_END
+}
+
+int imped_timing_reset(void) {
+  API_BEGIN
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  timing_clear();
+  API_END
+}
+
+int imped_timing_query(const char* name, double* total_ms, int64_t* launches) {
+  API_BEGIN
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  double t = 0.0;
+  int64_t k = 0;
+  for (auto& kv : g_timings) {
+    if (kv.first != name) continue;
+    for (auto& ev : kv.second.events) {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev.first, ev.second));
+      t += ms;
+      ++k;
+    }
+  }
+  *total_ms = t;
+  *launches = k;
+  API_END
+}
+
+// ------------------------------------------------------------------ restricted basis (host)
+
+namespace {
+
+using Mask = std::array<uint64_t, IMPED_MAXW>;
+
+void set_bit(Mask& m, int i) { m[i / 64] |= 1ull << (63 - i % 64); }
+
+// All k-subsets of orbitals [first, first+n) in itertools.combinations order.
+void combinations(int first, int n, int k, std::vector<Mask>& out) {
+  out.clear();
+  if (k < 0 || k > n) {
+    if (k < 0) throw std::invalid_argument("negative occupation in basis enumeration");
+    return;
+  }
+  std::vector<int> idx(k);
+  for (int i = 0; i < k; ++i) idx[i] = i;
+  for (;;) {
+    Mask m{};
+    for (int i = 0; i < k; ++i) set_bit(m, first + idx[i]);
+    out.push_back(m);
+    int i = k - 1;
+    while (i >= 0 && idx[i] == n - k + i) --i;
+    if (i < 0) break;
+    ++idx[i];
+    for (int j = i + 1; j < k; ++j) idx[j] = idx[j - 1] + 1;
+  }
+}
+
+}  // namespace
+
+int imped_enumerate_basis(int n_orb, int W, int n_shells, const int32_t* shell, int64_t cap, int64_t* n_out,
+                          uint64_t* keys) {
+  API_BEGIN
+  if (W < 1 || W > IMPED_MAXW || n_orb > 64 * W || n_shells < 1) {
+    set_error("imped_enumerate_basis: invalid W / n_orb / n_shells");
+    return IMPED_ERR_INVALID;
+  }
+  std::vector<std::vector<Mask>> per_shell(n_shells);
+  std::vector<Mask> ci, cv, cc;
+  for (int l = 0; l < n_shells; ++l) {
+    const int32_t* p = shell + 10 * l;
+    const int imp_first = p[0], n_imp = p[1], val_first = p[2], n_val = p[3], con_first = p[4], n_con = p[5];
+    const int n0 = p[6], dn_tol = p[7], dn_val_max = p[8], dn_con_max = p[9];
+    for (int dv = 0; dv <= dn_val_max; ++dv)
+      for (int dc = 0; dc <= dn_con_max; ++dc) {
+        const int dn = dv - dc;
+        if (std::abs(dn) > dn_tol) continue;
+        const int nimp = n0 + dn, nval = n_val - dv, ncon = dc;
+        if (nimp > n_imp || ncon > n_con) throw std::invalid_argument("over-occupied shell in basis enumeration");
+        combinations(imp_first, n_imp, nimp, ci);
+        if (n_val == 0) cv.assign(1, Mask{});
+        else combinations(val_first, n_val, nval, cv);
+        if (n_con == 0) cc.assign(1, Mask{});
+        else combinations(con_first, n_con, ncon, cc);
+        for (auto& a : ci)
+          for (auto& b : cv)
+            for (auto& c : cc) {
+              Mask m;
+              for (int w = 0; w < IMPED_MAXW; ++w) m[w] = a[w] | b[w] | c[w];
+              per_shell[l].push_back(m);
+            }
+      }
+  }
+  int64_t total = 1;
+  for (auto& v : per_shell) total *= (int64_t)v.size();
+  *n_out = total;
+  if (!keys) return IMPED_OK;
+  if (total > cap) {
+    set_error("imped_enumerate_basis: output capacity too small");
+    return IMPED_ERR_CAPACITY;
+  }
+  // itertools.product order: the first shell varies slowest
+  std::vector<size_t> idx(n_shells, 0);
+  for (int64_t k = 0; k < total; ++k) {
+    Mask m{};
+    for (int l = 0; l < n_shells; ++l)
+      for (int w = 0; w < IMPED_MAXW; ++w) m[w] |= per_shell[l][idx[l]][w];
+    for (int w = 0; w < W; ++w) keys[k * W + w] = m[w];
+    for (int l = n_shells - 1; l >= 0; --l) {
+      if (++idx[l] < per_shell[l].size()) break;
+      idx[l] = 0;
+    }
+  }
+  API_END
+}
+
+int imped_set_tuning(const char* name, int64_t value) {
+  API_BEGIN
+  const std::string n(name);
+  if (n == "lanczos_wave_max_n")
+    g_tuning.lanczos_wave_max_n = value;
+  else if (n == "lanczos_wg256_max_n")
+    g_tuning.lanczos_wg256_max_n = value;
+  else if (n == "eigen_small_n")
+    g_tuning.eigen_small_n = value;
+  else if (n == "eigen_large_bt") {
+    if (value != 512 && value != 1024) {
+      set_error("imped_set_tuning: eigen_large_bt must be 512 or 1024");
+      return IMPED_ERR_INVALID;
+    }
+    g_tuning.eigen_large_bt = value;
+  } else {
+    set_error("imped_set_tuning: unknown parameter " + n);
+    return IMPED_ERR_INVALID;
+  }
   API_END
 }
 

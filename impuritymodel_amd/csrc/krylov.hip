@@ -22,14 +22,18 @@ namespace imped {
 
 namespace {
 
-constexpr int KT = 256;  // threads per Krylov workgroup
-
+// Sum over the workgroup, identical in every thread; fixed order, so runs
+// are reproducible.  A one-wave workgroup needs no LDS round trip.
+template <int BT>
 __device__ __forceinline__ double block_sum(double x, double* lds) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  if (BT == 64) return x;
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
   __syncthreads();
-  double r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  double r = 0.0;
+#pragma unroll
+  for (int k = 0; k < BT / 64; ++k) r += lds[k];
   __syncthreads();
   return r;
 }
@@ -41,10 +45,11 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 
 // y = H x for the rows of one chain; returns the partial Re<x, y>.
+template <int BT>
 __device__ __forceinline__ double spmv_rows(const int64_t* rowptr, const int32_t* col, const double2* val,
                                             int64_t base, int64_t n, const double2* x, double2* y) {
   double pa = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += KT) {
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
     const int64_t e0 = rowptr[base + i], e1 = rowptr[base + i + 1];
     double ax = 0.0, ay = 0.0;
     for (int64_t e = e0; e < e1; ++e) {
@@ -61,10 +66,11 @@ __device__ __forceinline__ double spmv_rows(const int64_t* rowptr, const int32_t
 
 // Two passes of classical Gram-Schmidt of w against V[0..m) (each n long).
 // coef must hold m doubles*2 in LDS-sized scratch (global here).
+template <int BT>
 __device__ void cgs2(const double2* V, int m, int64_t n, double2* w, double2* coef) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int pass = 0; pass < 2; ++pass) {
-    for (int l = wave; l < m; l += KT / 64) {
+    for (int l = wave; l < m; l += BT / 64) {
       const double2* vl = V + (size_t)l * n;
       double cx = 0.0, cy = 0.0;
       for (int64_t i = lane; i < n; i += 64) {
@@ -77,7 +83,7 @@ __device__ void cgs2(const double2* V, int m, int64_t n, double2* w, double2* co
       if (lane == 0) coef[l] = make_double2(cx, cy);
     }
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < n; i += KT) {
+    for (int64_t i = threadIdx.x; i < n; i += BT) {
       double wx = w[i].x, wy = w[i].y;
       for (int l = 0; l < m; ++l) {
         const double2 c = coef[l];
@@ -107,11 +113,13 @@ struct LanczosArgs {
   double* beta;
   int32_t* kout;
   int32_t* status;
+  const int32_t* chain_list;  // chains handled by this launch, one per workgroup
 };
 
-__global__ __launch_bounds__(KT) void lanczos_kernel(LanczosArgs a) {
-  __shared__ double red[4];
-  const int c = blockIdx.x;
+template <int BT>
+__global__ __launch_bounds__(BT) void lanczos_kernel(LanczosArgs a) {
+  __shared__ double red[BT / 64];
+  const int c = a.chain_list[blockIdx.x];
   const int64_t n = a.chain_n[c];
   const int64_t base = a.row_base[c];
   double* alpha = a.alpha + (size_t)c * a.K;
@@ -129,37 +137,37 @@ __global__ __launch_bounds__(KT) void lanczos_kernel(LanczosArgs a) {
   double2* w = v1 + n;
   double2* V = a.reorth ? a.vstore + (size_t)a.K * base : nullptr;
   double2* coef = a.reorth ? a.coef + (size_t)a.K * c : nullptr;
-  for (int64_t i = threadIdx.x; i < n; i += KT) {
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
     v0[i] = a.v0[base + i];
     if (V) V[i] = v0[i];
   }
   __syncthreads();
   // wp = H v0; alpha0 = Re(conj(wp) . v0); w = wp - alpha0 v0
-  double al = block_sum(spmv_rows(a.rowptr, a.col, a.val, base, n, v0, w), red);
+  double al = block_sum<BT>(spmv_rows<BT>(a.rowptr, a.col, a.val, base, n, v0, w), red);
   if (threadIdx.x == 0) alpha[0] = al;
-  for (int64_t i = threadIdx.x; i < n; i += KT) w[i] = make_double2(w[i].x - al * v0[i].x, w[i].y - al * v0[i].y);
+  for (int64_t i = threadIdx.x; i < n; i += BT) w[i] = make_double2(w[i].x - al * v0[i].x, w[i].y - al * v0[i].y);
   __syncthreads();
   int status = IMPED_OK;
   int kdone = K;
   for (int j = 1; j < K; ++j) {
-    if (V) cgs2(V, j, n, w, coef);
+    if (V) cgs2<BT>(V, j, n, w, coef);
     double p = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += KT) p += w[i].x * w[i].x + w[i].y * w[i].y;
-    const double b = sqrt(block_sum(p, red));
+    for (int64_t i = threadIdx.x; i < n; i += BT) p += w[i].x * w[i].x + w[i].y * w[i].y;
+    const double b = sqrt(block_sum<BT>(p, red));
     if (threadIdx.x == 0) beta[j - 1] = b;
     if (b == 0.0) {
       status = IMPED_ERR_BREAKDOWN;
       kdone = j;
       break;
     }
-    for (int64_t i = threadIdx.x; i < n; i += KT) {
+    for (int64_t i = threadIdx.x; i < n; i += BT) {
       v1[i] = make_double2(w[i].x / b, w[i].y / b);
       if (V) V[(size_t)j * n + i] = v1[i];
     }
     __syncthreads();
-    al = block_sum(spmv_rows(a.rowptr, a.col, a.val, base, n, v1, w), red);
+    al = block_sum<BT>(spmv_rows<BT>(a.rowptr, a.col, a.val, base, n, v1, w), red);
     if (threadIdx.x == 0) alpha[j] = al;
-    for (int64_t i = threadIdx.x; i < n; i += KT) {
+    for (int64_t i = threadIdx.x; i < n; i += BT) {
       const double2 x = v1[i], y = v0[i];
       w[i] = make_double2(w[i].x - al * x.x - b * y.x, w[i].y - al * x.y - b * y.y);
       v0[i] = x;
@@ -298,12 +306,14 @@ struct EigenArgs {
   double2* evecs;
   int32_t* steps;
   int32_t* conv;
+  const int32_t* chain_list;
 };
 
-__global__ __launch_bounds__(KT) void eigen_kernel(EigenArgs a) {
-  __shared__ double red[4];
+template <int BT>
+__global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
+  __shared__ double red[BT / 64];
   __shared__ int s_done;
-  const int c = blockIdx.x;
+  const int c = a.chain_list[blockIdx.x];
   const int64_t n = a.chain_n[c];
   const int64_t base = a.row_base[c];
   const int nev = a.nev;
@@ -324,23 +334,23 @@ __global__ __launch_bounds__(KT) void eigen_kernel(EigenArgs a) {
   double* Y = a.yk + (size_t)c * nev * ms;
   // random start vector
   double p = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += KT) {
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
     const uint64_t h = a.seed * 0x100000001B3ull + (uint64_t)c * 0x9E3779B97F4A7C15ull + 2 * (uint64_t)i;
     const double2 x = make_double2(u01(h) - 0.5, u01(h + 1) - 0.5);
     V[i] = x;
     p += x.x * x.x + x.y * x.y;
   }
-  double nrm = sqrt(block_sum(p, red));
-  for (int64_t i = threadIdx.x; i < n; i += KT) V[i] = make_double2(V[i].x / nrm, V[i].y / nrm);
+  double nrm = sqrt(block_sum<BT>(p, red));
+  for (int64_t i = threadIdx.x; i < n; i += BT) V[i] = make_double2(V[i].x / nrm, V[i].y / nrm);
   __syncthreads();
   int m = 0;
   int converged = 0;
   for (int j = 0; j < mmax; ++j) {
     const double2* vj = V + (size_t)j * n;
-    const double al = block_sum(spmv_rows(a.rowptr, a.col, a.val, base, n, vj, w), red);
+    const double al = block_sum<BT>(spmv_rows<BT>(a.rowptr, a.col, a.val, base, n, vj, w), red);
     if (threadIdx.x == 0) alpha[j] = al;
     const double bprev = j > 0 ? beta[j - 1] : 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += KT) {
+    for (int64_t i = threadIdx.x; i < n; i += BT) {
       double2 x = w[i];
       x.x -= al * vj[i].x;
       x.y -= al * vj[i].y;
@@ -351,10 +361,10 @@ __global__ __launch_bounds__(KT) void eigen_kernel(EigenArgs a) {
       w[i] = x;
     }
     __syncthreads();
-    cgs2(V, j + 1, n, w, coef);
+    cgs2<BT>(V, j + 1, n, w, coef);
     p = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += KT) p += w[i].x * w[i].x + w[i].y * w[i].y;
-    const double b = sqrt(block_sum(p, red));
+    for (int64_t i = threadIdx.x; i < n; i += BT) p += w[i].x * w[i].x + w[i].y * w[i].y;
+    const double b = sqrt(block_sum<BT>(p, red));
     m = j + 1;
     const bool last = (m == mmax) || b <= 1e-13 * (fabs(al) + bprev + 1e-300);
     if (threadIdx.x == 0) beta[j] = b;
@@ -403,7 +413,7 @@ __global__ __launch_bounds__(KT) void eigen_kernel(EigenArgs a) {
       }
     }
     if (last) break;
-    for (int64_t i = threadIdx.x; i < n; i += KT) V[(size_t)(j + 1) * n + i] = make_double2(w[i].x / b, w[i].y / b);
+    for (int64_t i = threadIdx.x; i < n; i += BT) V[(size_t)(j + 1) * n + i] = make_double2(w[i].x / b, w[i].y / b);
     __syncthreads();
   }
   // Ritz vectors x_k = sum_l y_k[l] V_l, normalized
@@ -412,7 +422,7 @@ __global__ __launch_bounds__(KT) void eigen_kernel(EigenArgs a) {
   for (int k = 0; k < kk; ++k) {
     const double* y = Y + (size_t)k * ms;
     double pn = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += KT) {
+    for (int64_t i = threadIdx.x; i < n; i += BT) {
       double xr = 0.0, xi = 0.0;
       for (int l = 0; l < m; ++l) {
         const double2 v = V[(size_t)l * n + i];
@@ -422,13 +432,13 @@ __global__ __launch_bounds__(KT) void eigen_kernel(EigenArgs a) {
       X[(size_t)k * n + i] = make_double2(xr, xi);
       pn += xr * xr + xi * xi;
     }
-    const double nn = sqrt(block_sum(pn, red));
-    for (int64_t i = threadIdx.x; i < n; i += KT)
+    const double nn = sqrt(block_sum<BT>(pn, red));
+    for (int64_t i = threadIdx.x; i < n; i += BT)
       X[(size_t)k * n + i] = make_double2(X[(size_t)k * n + i].x / nn, X[(size_t)k * n + i].y / nn);
     __syncthreads();
   }
   for (int k = kk; k < nev; ++k) {
-    for (int64_t i = threadIdx.x; i < n; i += KT) X[(size_t)k * n + i] = make_double2(0.0, 0.0);
+    for (int64_t i = threadIdx.x; i < n; i += BT) X[(size_t)k * n + i] = make_double2(0.0, 0.0);
     if (threadIdx.x == 0) a.evals[(size_t)c * nev + k] = NAN;
   }
   if (threadIdx.x == 0) {
@@ -502,7 +512,7 @@ __global__ __launch_bounds__(256) void cf_kernel(CFArgs a) {
 }
 
 struct KScratch {
-  DevBuf ws, vstore, coef, ab, yk, tw, tpiv, voff, conv, wbuf;
+  DevBuf ws, vstore, coef, ab, yk, tw, tpiv, voff, conv, wbuf, clist;
   DevBuf cf_alpha_tmp, e, nrm, sg, wt, soff, sch, om, gout, sout;
 };
 KScratch& kscratch() {
@@ -539,8 +549,33 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
   a.kout = kout;
   a.status = status;
   if (kb.C == 0) return;
-  hipLaunchKernelGGL(lanczos_kernel, dim3(kb.C), dim3(KT), 0, s, a);
-  HIP_CHECK(hipGetLastError());
+  // chains bucketed by dimension: one wave, 256 or 1024 threads per chain
+  std::vector<int32_t> lists[3];
+  for (int c = 0; c < kb.C; ++c) {
+    const int64_t n = kb.h_n[c];
+    const int bkt = n <= g_tuning.lanczos_wave_max_n ? 0 : (n <= g_tuning.lanczos_wg256_max_n ? 1 : 2);
+    lists[bkt].push_back(c);
+  }
+  std::vector<int32_t> all;
+  for (auto& l : lists) all.insert(all.end(), l.begin(), l.end());
+  h2d(sc.clist, all.data(), all.size(), s);
+  size_t off = 0;
+  for (int bkt = 0; bkt < 3; ++bkt) {
+    const unsigned nb = (unsigned)lists[bkt].size();
+    if (!nb) continue;
+    a.chain_list = sc.clist.as<int32_t>() + off;
+    off += nb;
+    TimedLaunch tl("lanczos", s);
+    if (bkt == 0)
+      hipLaunchKernelGGL(lanczos_kernel<64>, dim3(nb), dim3(64), 0, s, a);
+    else if (bkt == 1)
+      hipLaunchKernelGGL(lanczos_kernel<256>, dim3(nb), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL(lanczos_kernel<1024>, dim3(nb), dim3(1024), 0, s, a);
+    HIP_CHECK(hipGetLastError());
+  }
+  // the chain list buffer must outlive the launches
+  HIP_CHECK(hipStreamSynchronize(s));
 }
 
 int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_t seed, double* evals,
@@ -586,8 +621,26 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   a.evecs = evecs;
   a.steps = steps;
   a.conv = sc.conv.as<int32_t>();
-  hipLaunchKernelGGL(eigen_kernel, dim3(C), dim3(KT), 0, s, a);
-  HIP_CHECK(hipGetLastError());
+  std::vector<int32_t> small, large;
+  for (int c = 0; c < C; ++c) (kb.h_n[c] <= g_tuning.eigen_small_n ? small : large).push_back(c);
+  std::vector<int32_t> all(small);
+  all.insert(all.end(), large.begin(), large.end());
+  h2d(sc.clist, all.data(), all.size(), s);
+  if (!small.empty()) {
+    a.chain_list = sc.clist.as<int32_t>();
+    TimedLaunch tl("eigen", s);
+    hipLaunchKernelGGL(eigen_kernel<256>, dim3((unsigned)small.size()), dim3(256), 0, s, a);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (!large.empty()) {
+    a.chain_list = sc.clist.as<int32_t>() + small.size();
+    TimedLaunch tl("eigen", s);
+    if (g_tuning.eigen_large_bt == 512)
+      hipLaunchKernelGGL(eigen_kernel<512>, dim3((unsigned)large.size()), dim3(512), 0, s, a);
+    else
+      hipLaunchKernelGGL(eigen_kernel<1024>, dim3((unsigned)large.size()), dim3(1024), 0, s, a);
+    HIP_CHECK(hipGetLastError());
+  }
   std::vector<int32_t> conv(C);
   d2h(conv.data(), sc.conv.p, C, s);
   HIP_CHECK(hipStreamSynchronize(s));
@@ -622,6 +675,7 @@ void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t
            kT, sc.soff.as<int32_t>(), sc.sch.as<int32_t>(), sc.om.as<double>(), n_w, delta,
            g_out ? sc.gout.as<double2>() : nullptr, sc.sout.as<double>()};
   if (n_spec > 0 && n_w > 0) {
+    TimedLaunch tl("cf", s);
     hipLaunchKernelGGL(cf_kernel, dim3(blocks_for(n_w, 256), n_spec), dim3(256), 0, s, a);
     HIP_CHECK(hipGetLastError());
   }

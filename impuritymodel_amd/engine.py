@@ -237,3 +237,41 @@ class Batch:
 
 
 BREAKDOWN = IMPED_ERR_BREAKDOWN
+
+
+# ------------------------------------------------------------------ timing, tuning, enumeration
+
+
+def timing_enable(on=True):
+    check(lib.imped_timing_enable(int(bool(on))))
+
+
+def timing_reset():
+    check(lib.imped_timing_reset())
+
+
+def timing_query(name):
+    """(total milliseconds, launches) recorded for one kernel name."""
+    ms = np.zeros(1, np.float64)
+    k = np.zeros(1, np.int64)
+    check(lib.imped_timing_query(name.encode(), ptr(ms), ptr(k)))
+    return float(ms[0]), int(k[0])
+
+
+def set_tuning(name, value):
+    check(lib.imped_set_tuning(name.encode(), int(value)))
+
+
+def enumerate_basis(n_orb, W, shells):
+    """Restricted basis words [N, W] in get_basis order (host C++).
+
+    ``shells``: list of 10-tuples per angular-momentum shell (see
+    imped_enumerate_basis).
+    """
+    sh = np.ascontiguousarray(np.array(shells, np.int32).reshape(-1))
+    n = np.zeros(1, np.int64)
+    check(lib.imped_enumerate_basis(int(n_orb), int(W), len(shells), ptr(sh), 0, ptr(n), None))
+    N = int(n[0])
+    keys = np.zeros((max(N, 1), W), np.uint64)
+    check(lib.imped_enumerate_basis(int(n_orb), int(W), len(shells), ptr(sh), N, ptr(n), ptr(keys)))
+    return keys[:N]

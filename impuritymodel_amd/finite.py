@@ -156,47 +156,44 @@ def norm2(psi):
 # ------------------------------------------------------------------ bases and Hamiltonians
 
 
+def basis_shells(nBaths, valBaths, dnValBaths, dnConBaths, dnTol, n0imp):
+    """Per-shell enumeration parameters of get_basis for imped_enumerate_basis.
+
+    Impurity spin-orbitals (l, s, m) of a shell are consecutive in c2i order,
+    and so are its valence and conduction bath states.
+    """
+    shells = []
+    for l in nBaths.keys():
+        assert valBaths[l] <= nBaths[l]
+        imp_first = c2i(nBaths, (l, 0, -l))
+        nv = valBaths[l]
+        nc = nBaths[l] - valBaths[l]
+        v_first = c2i(nBaths, (l, 0)) if nv > 0 else 0
+        c_first = c2i(nBaths, (l, nv)) if nc > 0 else 0
+        shells.append((imp_first, 2 * (2 * l + 1), v_first, nv, c_first, nc, n0imp[l], dnTol[l], dnValBaths[l],
+                       dnConBaths[l]))
+    return shells
+
+
+def get_basis_words(nBaths, valBaths, dnValBaths, dnConBaths, dnTol, n0imp, W=None):
+    """get_basis as packed words [N, W] (native enumeration)."""
+    from .engine import enumerate_basis
+
+    n = sum(2 * (2 * ang + 1) + nb for ang, nb in nBaths.items())
+    W = W or opcompile.n_words(n)
+    return enumerate_basis(n, W, basis_shells(nBaths, valBaths, dnValBaths, dnConBaths, dnTol, n0imp))
+
+
 def get_basis(nBaths, valBaths, dnValBaths, dnConBaths, dnTol, n0imp):
     """Restricted initial basis as bytes keys in itertools order (finite.py:294-374).
 
-    Host enumeration of occupation partitions (negligible cost); the order
-    is the reference's: nested combinations per shell, shells combined by
-    itertools.product, not sorted.
+    Enumerated by libimped's host enumerator: for each shell the nested
+    combinations of impurity, valence and conduction indices per occupation
+    partition, shells combined as itertools.product, not sorted.
     """
-    import itertools
-
-    for l in nBaths.keys():
-        assert valBaths[l] <= nBaths[l]
-    basisL = {}
-    for l in nBaths.keys():
-        basisL[l] = []
-        for dnVal in range(dnValBaths[l] + 1):
-            for dnCon in range(dnConBaths[l] + 1):
-                dn = dnVal - dnCon
-                if abs(dn) <= dnTol[l]:
-                    nImp, nVal, nCon = n0imp[l] + dn, valBaths[l] - dnVal, dnCon
-                    assert nVal <= valBaths[l]
-                    assert nCon <= nBaths[l] - valBaths[l]
-                    assert nImp <= 2 * (2 * l + 1)
-                    imp = [c2i(nBaths, (l, s, m)) for s in range(2) for m in range(-l, l + 1)]
-                    bImp = tuple(itertools.combinations(imp, nImp))
-                    bVal = ((),) if valBaths[l] == 0 else tuple(
-                        itertools.combinations([c2i(nBaths, (l, b)) for b in range(valBaths[l])], nVal))
-                    bCon = ((),) if nBaths[l] - valBaths[l] == 0 else tuple(
-                        itertools.combinations([c2i(nBaths, (l, b)) for b in range(valBaths[l], nBaths[l])], nCon))
-                    for x in bImp:
-                        for y in bVal:
-                            for z in bCon:
-                                basisL[l].append(x + y + z)
     n = sum(2 * (2 * ang + 1) + nb for ang, nb in nBaths.items())
-    nb = (n + 7) // 8
-    out = []
-    for conf in itertools.product(*basisL.values()):
-        v = 0
-        for i in itertools.chain.from_iterable(conf):
-            v |= 1 << (8 * nb - 1 - i)
-        out.append(v.to_bytes(nb, "big"))
-    return tuple(out)
+    words = get_basis_words(nBaths, valBaths, dnValBaths, dnConBaths, dnTol, n0imp)
+    return tuple(opcompile.words_to_bytes(words, n))
 
 
 def _reference_csr(b, chain=0):

@@ -117,6 +117,14 @@ def i2c(nBaths, i):
     raise Exception("Can not find spin-orbital state corresponding to index.")
 
 
+@lru_cache(maxsize=64)
+def _index_map(nbaths_items):
+    """{spin-orbital tuple: c2i index} for every spin-orbital of a layout."""
+    nBaths = dict(nbaths_items)
+    n = n_spin_orbitals(nBaths)
+    return {i2c(nBaths, i): i for i in range(n)}
+
+
 def n_spin_orbitals(nBaths):
     """Total spin-orbital count (get_spectra.py:175)."""
     return sum(2 * (2 * ang + 1) + nb for ang, nb in nBaths.items())
@@ -288,9 +296,10 @@ def get_hamiltonian_operator_from_h0(nBaths, slaterCondon, SOCs, DCinfo, hField,
                 eDC[(((l, s, m), "c"), ((l, s, m), "a"))] = -dc[il]
     hH = gethHfieldop(hx, hy, hz, l=2)
     hsum = addOps([uOp, hH, soc2p, soc3d, eDC, h0_operator])
+    index = _index_map(tuple(nBaths.items()))
     hOp = {}
     for proc, value in hsum.items():
-        hOp[tuple((c2i(nBaths, so), action) for so, action in proc)] = value
+        hOp[tuple((index[so], action) for so, action in proc)] = value
     assert_hermitian(hOp)
     return hOp
 
@@ -312,6 +321,8 @@ def get_bath_h0(e_imp, tenDq, eps_bath, v_bath, l=2):
     np.fill_diagonal(h, (e_eg, e_eg, e_t2g, e_t2g, e_t2g))
     u = get_spherical_2_cubic_matrix(spinpol=False, l=l)
     h = np.dot(u, np.dot(h, np.conj(u.T)))
+    # the rotation can leave h[i, j] and conj(h[j, i]) one ulp apart
+    h = 0.5 * (h + np.conj(h.T))
     op = {}
     for i, mi in enumerate(range(-l, l + 1)):
         for j, mj in enumerate(range(-l, l + 1)):

@@ -45,6 +45,28 @@ int imped_abi_version(void);
 int imped_set_stream(void* stream);
 int imped_device_synchronize(void);
 
+/* Launch timing: when enabled, every kernel launch of the engine is bracketed
+ * by HIP events on the library's stream; imped_timing_query sums the elapsed
+ * time of the launches recorded under a kernel name ("apply_count",
+ * "apply_fill", "eigen", "lanczos", "cf").  No reference counterpart. */
+int imped_timing_enable(int on);
+int imped_timing_reset(void);
+int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
+
+/* Launch-shape parameters picked per chain-dimension bucket
+ * ("lanczos_wave_max_n", "lanczos_wg256_max_n", "eigen_small_n",
+ * "eigen_large_bt"); set by the autotuner.  No reference counterpart. */
+int imped_set_tuning(const char* name, int64_t value);
+
+/* get_basis (finite.py:294-374) on the host: the restricted initial basis in
+ * the reference's itertools order.  shell[10*l ..] per angular-momentum shell:
+ * {first impurity index, impurity count, first valence-bath index, valence
+ * count, first conduction-bath index, conduction count, n0imp, dnTol,
+ * dnValBaths, dnConBaths}; shells combined like itertools.product (first
+ * shell slowest).  keys == NULL only counts (*n_out). */
+int imped_enumerate_basis(int n_orb, int W, int n_shells, const int32_t* shell, int64_t cap, int64_t* n_out,
+                          uint64_t* keys);
+
 /* Compiled operators of n_models models (the process dicts of applyOp,
  * finite.py:1301-1340, plus the occupation restrictions checked at
  * finite.py:1382-1385).  Model m owns groups [model_group_off[m],

@@ -1,0 +1,272 @@
+"""GPU parity: the HIP engine against the CPU oracle and the golden vectors.
+
+Bars: product states, insertion order, bases and sparsity patterns
+bit-exact; matrix elements 1e-12; eigenvalues 1e-10; spectra 1e-8 relative
+(to the largest value); Lanczos alpha/beta 1e-10 while the recurrence is
+orthogonal (K = 40 on a 66-spin-orbital chain).
+"""
+
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse
+
+from oracle import applyop, basis as obasis, hamiltonian as oham, krylov as okry, spectra as ospec
+from oracle import states as psr
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def words_to_ints(w, n):
+    from impuritymodel_amd import opcompile
+
+    return opcompile.words_to_ints(np.asarray(w, np.uint64), n)
+
+
+def ints_to_bytes(states, n):
+    return [psr.int2bytes(s, n) for s in states]
+
+
+def bytes_to_ints(keys, n):
+    return [psr.bytes2int(k, n) for k in keys]
+
+
+# ------------------------------------------------------------------ applyOp
+
+
+def test_applyop_reference_kats(gpu):
+    from impuritymodel_amd import finite
+
+    vac = "000000"
+    psi = {psr.str2bytes(vac): 7}
+    for i in range(6):
+        assert finite.applyOp(6, {((i, "c"),): 9}, psi) == {psr.str2bytes(vac[:i] + "1" + vac[i + 1:]): 63}
+        assert finite.applyOp(6, {((i, "a"),): 9}, psi) == {}
+    for i in range(6):
+        for j in range(6):
+            r = finite.applyOp(6, {((i, "c"), (j, "c")): 9}, psi)
+            if i == j:
+                assert r == {}
+            else:
+                a, b = min(i, j), max(i, j)
+                s = vac[:a] + "1" + vac[a + 1:b] + "1" + vac[b + 1:]
+                assert r == {psr.str2bytes(s): 7 * 9 * (2 * (i < j) - 1)}
+            rr = finite.applyOp(6, {((i, "c"), (j, "c")): 9}, psi, restrictions={frozenset([1, 4, 5]): (0, 1)})
+            if i in (1, 4, 5) and j in (1, 4, 5):
+                assert rr == {}
+    ps = "011000"
+    r = finite.applyOp(6, {((1, "c"), (1, "a")): 9, ((2, "c"), (2, "a")): 11}, {psr.str2bytes(ps): 7})
+    assert r == {psr.str2bytes(ps): 140}
+
+
+def test_applyop_opresult_memo(gpu):
+    from impuritymodel_amd import finite
+
+    vac = psr.str2bytes("000000")
+    op = {((0, "c"),): 9, ((3, "c"),): 11}
+    memo = {}
+    r = finite.applyOp(6, op, {vac: 7}, opResult=memo)
+    assert r == {psr.str2bytes("100000"): 63, psr.str2bytes("000100"): 77}
+    assert memo == {vac: {psr.str2bytes("100000"): 9, psr.str2bytes("000100"): 11}}
+    wrong = {vac: {psr.str2bytes("111111"): 2}}
+    assert finite.applyOp(6, op, {vac: 7}, opResult=wrong) == {psr.str2bytes("111111"): 14}
+
+
+@pytest.mark.parametrize("n", [10, 26, 66, 100])
+def test_applyop_random_vs_oracle(gpu, n):
+    from impuritymodel_amd import finite
+
+    rng = np.random.default_rng(100 + n)
+    for trial in range(12):
+        op = {}
+        for _ in range(int(rng.integers(1, 30))):
+            k = int(rng.choice([1, 2, 4]))
+            proc = tuple((int(rng.integers(n)), str(rng.choice(["c", "a"]))) for _ in range(k))
+            op[proc] = complex(rng.normal(), rng.normal())
+        states = {int.from_bytes(rng.bytes(16), "big") >> (128 - n) for _ in range(int(rng.integers(1, 40)))}
+        psi = {s: complex(rng.normal(), rng.normal()) for s in states}
+        restr = None
+        if trial % 2:
+            idx = frozenset(int(x) for x in rng.choice(n, size=min(n, 8), replace=False))
+            restr = {idx: (2, 5)}
+        ref = applyop.apply_op(n, op, psi, 1e-12, restr)
+        got = finite.applyOp(n, op, dict(zip(ints_to_bytes(list(psi), n), psi.values())), 1e-12, restr)
+        assert bytes_to_ints(list(got), n) == list(ref), "insertion order / membership differs"
+        np.testing.assert_allclose(list(got.values()), list(ref.values()), rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------------------------ Hamiltonians
+
+
+def test_fixed_basis_hamiltonian_golden(gpu):
+    from impuritymodel_amd import finite
+    from impuritymodel_amd.models import make_model
+
+    g = np.load(os.path.join(GOLD, "ground_10bath_s7.npz"))
+    m = make_model(n_bath=10, seed=int(g["seed"]))
+    n = m.n_spin_orbitals
+    b = finite.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    assert bytes_to_ints(b, n) == words_to_ints(g["basis"], n)
+    h = finite.get_hamiltonian_matrix(n, m.hOp, b).tocoo()
+    got = {(r, c): v for r, c, v in zip(h.row, h.col, h.data)}
+    ref = {(r, c): v for r, c, v in zip(g["h_row"], g["h_col"], g["h_val"])}
+    assert set(got) == set(ref)
+    assert max(abs(got[k] - ref[k]) for k in ref) <= 1e-12
+
+
+def test_expand_chain_golden_w2(gpu):
+    from impuritymodel_amd import finite
+    from impuritymodel_amd.models import make_model
+
+    g = np.load(os.path.join(GOLD, "chain_50bath_s2.npz"))
+    m = make_model(n_bath=50, seed=int(g["seed"]))
+    n = m.n_spin_orbitals
+    seeds = ints_to_bytes(words_to_ints(g["seed_keys"], n), n)
+    h, index = finite.expand_basis_and_hamiltonian(n, {}, m.hOp, seeds, m.restrictions)
+    assert bytes_to_ints(list(index), n) == words_to_ints(g["basis"], n)
+    h = h.tocoo()
+    got = {(r, c): v for r, c, v in zip(h.row, h.col, h.data)}
+    ref = {(r, c): v for r, c, v in zip(g["h_row"], g["h_col"], g["h_val"])}
+    assert set(got) == set(ref)
+    assert max(abs(got[k] - ref[k]) for k in ref) <= 1e-12
+    psi0 = np.zeros(len(index), complex)
+    vals = g["seed_vals"] / np.sqrt(np.sum(np.abs(g["seed_vals"]) ** 2))
+    for s, v in zip(seeds, vals):
+        psi0[index[s]] = v
+    a, b = finite.get_tridiagonal_krylov_vectors(scipy.sparse.csr_matrix(h), psi0, 40)
+    np.testing.assert_allclose(a, g["alpha"], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(b, g["beta"], rtol=0, atol=1e-10)
+
+
+def test_expand_many_seeds_vs_oracle(gpu):
+    from impuritymodel_amd import finite
+    from impuritymodel_amd.models import make_model
+
+    m = make_model(n_bath=20, seed=11, n_con=4, dnConBath=1)
+    n = m.n_spin_orbitals
+    b = obasis.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    rs = applyop.restriction_masks(n, m.restrictions)
+    seeds = [s for s in b[::37] if applyop._passes(s, rs)][:12]
+    ho, io = oham.expand_basis_and_hamiltonian(n, {}, m.hOp, seeds, m.restrictions)
+    hg, ig = finite.expand_basis_and_hamiltonian(n, {}, m.hOp, ints_to_bytes(seeds, n), m.restrictions)
+    assert bytes_to_ints(list(ig), n) == list(io)
+    d = (hg - ho)
+    assert (abs(d).max() if d.nnz else 0.0) <= 1e-12
+    assert hg.nnz == ho.nnz
+
+
+# ------------------------------------------------------------------ Krylov
+
+
+def test_eigensystem_vs_dense(gpu):
+    from impuritymodel_amd import finite
+    from impuritymodel_amd.models import make_model
+
+    g = np.load(os.path.join(GOLD, "ground_10bath_s7.npz"))
+    m = make_model(n_bath=10, seed=int(g["seed"]))
+    n = m.n_spin_orbitals
+    b = finite.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    es, psis = finite.eigensystem(n, m.hOp, b, 3, eigenValueTol=1e-12)
+    np.testing.assert_allclose(es, g["evals"], rtol=0, atol=1e-10)
+    bo = obasis.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    _, po = okry.eigensystem(n, m.hOp, bo, 3, groundDiagMode="full")
+    for pg, p in zip(psis, po):
+        keys = bytes_to_ints(list(pg), n)
+        assert keys == [k for k in bo if k in p]  # basis order, same truncation
+        ov = abs(sum(np.conj(pg[psr.int2bytes(k, n)]) * p[k] for k in keys))
+        assert abs(ov - 1.0) < 1e-8
+
+
+def test_lanczos_random_hermitian_and_breakdown(gpu):
+    from impuritymodel_amd import finite
+
+    rng = np.random.default_rng(5)
+    n = 60
+    a = scipy.sparse.random(n, n, density=0.1, random_state=1) + 1j * scipy.sparse.random(n, n, density=0.1,
+                                                                                         random_state=2)
+    h = scipy.sparse.csr_matrix((a + a.conj().T) * 0.5 + scipy.sparse.diags(rng.uniform(-2, 2, n)))
+    psi = rng.normal(size=n) + 1j * rng.normal(size=n)
+    psi /= np.linalg.norm(psi)
+    for K in (1, 2, 15):
+        ag, bg = finite.get_tridiagonal_krylov_vectors(h, psi, K)
+        ao, bo = okry.lanczos(h, psi, K)
+        np.testing.assert_allclose(ag, ao, atol=1e-12)
+        np.testing.assert_allclose(bg, bo, atol=1e-12)
+    ag, bg = finite.get_tridiagonal_krylov_vectors(h, psi, n, reorth="full")
+    ao, bo = okry.lanczos(h, psi, n, reorth="full")
+    np.testing.assert_allclose(ag, ao, atol=1e-9)
+    np.testing.assert_allclose(bg, bo, atol=1e-9)
+    d = scipy.sparse.csr_matrix(np.diag([1.0, 2.0, 3.0]).astype(complex))
+    with pytest.raises(ValueError, match="beta==0"):
+        finite.get_tridiagonal_krylov_vectors(d, np.array([1, 0, 0], complex), 3)
+
+
+def test_green_function_vs_oracle(gpu):
+    from impuritymodel_amd import spectra
+    from impuritymodel_amd.models import make_model
+
+    m = make_model(n_bath=10, seed=2)
+    n = m.n_spin_orbitals
+    b = obasis.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    rs = applyop.restriction_masks(n, m.restrictions)
+    psi = {s: 1.0 / np.sqrt(3) for s in [x for x in b if applyop._passes(x, rs)][:3]}
+    w = np.linspace(-10, 10, 301)
+    go = ospec.get_green(n, -700.0, psi, m.hOp, w, 0.3, 500, 1e-7, m.restrictions, reorth="full")
+    gg = spectra.getGreen(n, -700.0, dict(zip(ints_to_bytes(list(psi), n), psi.values())), m.hOp, w, 0.3, 500, 1e-7,
+                          m.restrictions, reorth="full")
+    np.testing.assert_allclose(gg, go, rtol=0, atol=1e-8 * np.abs(go).max())
+
+
+# ------------------------------------------------------------------ spectra
+
+
+def test_xas_batched_vs_golden_and_plain(gpu):
+    from impuritymodel_amd.batched import plain_xas, xas_batch
+    from impuritymodel_amd.models import make_model
+
+    g = np.load(os.path.join(GOLD, "ground_10bath_s7.npz"))
+    m = make_model(n_bath=10, seed=int(g["seed"]))
+    w = g["omega"]
+    res = xas_batch([m], omega=w, krylovSize=200, nPsiMax=3, reorth=1)
+    scale = np.abs(g["xas_sum"]).max()
+    np.testing.assert_allclose(res.spectra[0], g["xas_sum"], rtol=0, atol=1e-8 * scale)
+    plain = plain_xas(m, omega=w, krylovSize=200, nPsiMax=3, reorth=1)
+    np.testing.assert_allclose(plain, res.spectra[0], rtol=0, atol=1e-10 * scale)
+
+
+def test_photoemission_spectra_vs_oracle(gpu):
+    from impuritymodel_amd import finite, spectra
+    from impuritymodel_amd.models import make_model
+    from impuritymodel_amd.operators import getInversePhotoEmissionOperators, getPhotoEmissionOperators
+
+    m = make_model(n_bath=10, seed=4)
+    n = m.n_spin_orbitals
+    bo = obasis.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    es, po = okry.eigensystem(n, m.hOp, bo, 2, groundDiagMode="full")
+    w = np.linspace(-15, 15, 301)
+    tps = getPhotoEmissionOperators(m.nBaths, l=2)[:3]
+    tips = getInversePhotoEmissionOperators(m.nBaths, l=2)[:3]
+    gs_o, _, _ = ospec.photoemission(n, m.hOp, tps, tips, po, es, w, 0.2, m.restrictions, reorth="full")
+    pg = [dict(zip(ints_to_bytes(list(p), n), p.values())) for p in po]
+    gs_g = -spectra.getSpectra(n, m.hOp, tps, pg, es, -w, -0.2, m.restrictions, reorth="full")
+    gs_g = gs_g + spectra.getSpectra(n, m.hOp, tips, pg, es, w, 0.2, m.restrictions, reorth="full")
+    scale = np.abs(gs_o).max()
+    np.testing.assert_allclose(gs_g, gs_o, rtol=0, atol=1e-8 * scale)
+
+
+def test_batched_equals_single_chain_runs(gpu):
+    """Many mixed-size models (W = 1 and 2) in one batch reproduce the per-model
+    runs: same chain bases, CSR and spectra."""
+    from impuritymodel_amd.batched import xas_batch
+    from impuritymodel_amd.models import batched_params, model_from_params
+
+    P = batched_params(24, seed=3)
+    models = [model_from_params(p) for p in P]
+    w = np.linspace(-25, 25, 300)
+    big = xas_batch(models, omega=w, krylovSize=200, reorth=1)
+    for k in (0, 5, 11, 17, 23):
+        one = xas_batch([models[k]], omega=w, krylovSize=200, reorth=1)
+        scale = np.abs(one.spectra[0]).max()
+        np.testing.assert_allclose(big.spectra[k], one.spectra[0], rtol=0, atol=1e-10 * scale)

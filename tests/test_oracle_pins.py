@@ -1,0 +1,281 @@
+"""Pins of the oracle beyond the reference's unit tests.
+
+Each check is chosen so that a plausible mistake (a dropped term, a wrong
+sign or index, a transposed operand) fails it: Jordan-Wigner matrices for
+operator application, closed-form atomic multiplets and spin-orbit spectra
+for the operator builders (Gaunt coefficients, the u/2 convention, the Pauli
+filter, fermion signs), sympy's Gaunt (the reference's dependency), the NIXS
+radial integrals printed in the reference's golden log, exact resolvents for
+the Lanczos continued fraction, and closed-form basis counts.
+"""
+
+import itertools
+import os
+from collections import OrderedDict
+
+import numpy as np
+import pytest
+import scipy.sparse
+
+from impuritymodel_amd import operators as ops
+from oracle import applyop, basis, hamiltonian, krylov, spectra
+from oracle import states as psr
+
+RADIAL = os.path.join(os.path.dirname(__file__), "golden", "Ni3d.dat")
+
+
+# ------------------------------------------------------------ Jordan-Wigner
+
+
+def jw_annihilator(n, i):
+    """Dense c_i on 2^n states indexed by the reference's integer form."""
+    dim = 1 << n
+    m = np.zeros((dim, dim))
+    bit = 1 << (n - 1 - i)
+    for s in range(dim):
+        if s & bit:
+            sign = -1 if bin(s >> (n - i)).count("1") % 2 else 1
+            m[s ^ bit, s] = sign
+    return m
+
+
+def op_matrix(n, op):
+    dim = 1 << n
+    a = [jw_annihilator(n, i) for i in range(n)]
+    total = np.zeros((dim, dim), complex)
+    for proc, h in op.items():
+        m = np.eye(dim)
+        for i, act in proc:  # leftmost factor first: product in written order
+            m = m @ (a[i].T if act == "c" else a[i])
+        total += h * m
+    return total
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_apply_op_matches_jordan_wigner(seed):
+    rng = np.random.default_rng(seed)
+    n = 7
+    op = {}
+    for _ in range(6):
+        k = rng.choice([1, 2, 4])
+        proc = tuple((int(rng.integers(n)), str(rng.choice(["c", "a"]))) for _ in range(k))
+        op[proc] = complex(rng.normal(), rng.normal())
+    states = rng.choice(1 << n, size=4, replace=False)
+    psi = {int(s): complex(rng.normal(), rng.normal()) for s in states}
+    vec = np.zeros(1 << n, complex)
+    for s, a in psi.items():
+        vec[s] = a
+    ref = op_matrix(n, op) @ vec
+    got = applyop.apply_op(n, op, psi, slaterWeightMin=0.0)
+    dense = np.zeros(1 << n, complex)
+    for s, a in got.items():
+        dense[s] = a
+    np.testing.assert_allclose(dense, ref, atol=1e-12)
+
+
+def test_anticommutators():
+    n = 5
+    for i in range(n):
+        for j in range(n):
+            ci, cj = jw_annihilator(n, i), jw_annihilator(n, j)
+            ac = ci @ cj.T + cj.T @ ci
+            np.testing.assert_allclose(ac, np.eye(1 << n) * (i == j), atol=0)
+            # the oracle's create/remove agree with the matrices
+    for s in range(1 << n):
+        for i in range(n):
+            t, amp = psr.create(n, i, s)
+            col = jw_annihilator(n, i).T[:, s]
+            if amp == 0:
+                assert not col.any()
+            else:
+                assert col[t] == amp
+
+
+# ------------------------------------------------------------ angular algebra
+
+
+def test_gaunt_matches_sympy():
+    from sympy.physics.wigner import gaunt
+
+    for k in range(5):
+        for l in (1, 2):
+            for lp in (1, 2):
+                for m in range(-l, l + 1):
+                    for mp in range(-lp, lp + 1):
+                        ref = float(np.sqrt(4 * np.pi / (2 * k + 1)) * (-1) ** m
+                                    * gaunt(l, k, lp, -m, m - mp, mp, prec=16))
+                        assert abs(ops.gauntC(k, l, m, lp, mp) - ref) < 1e-15
+
+
+def _dense_h(n, op, states):
+    idx = {s: i for i, s in enumerate(states)}
+    h = np.zeros((len(states), len(states)), complex)
+    for j, s in enumerate(states):
+        for t, v in applyop.apply_op(n, op, {s: 1}).items():
+            h[idx[t], j] = v
+    return h
+
+
+def test_d2_multiplets_slater_condon():
+    """d^2 terms: 3F, 1D, 3P, 1G, 1S in Condon-Shortley F_2 = F^2/49, F_4 = F^4/441."""
+    F0, F2, F4 = 3.0, 7.0, 5.0
+    nb = OrderedDict([(2, 0)])
+    uop = ops.getUop(2, 2, 2, 2, (F0, 0, F2, 0, F4))
+    op = {tuple((ops.c2i(nb, so), a) for so, a in p): v for p, v in uop.items()}
+    n = 10
+    states = [psr.tuple2int(t, n) for t in itertools.combinations(range(n), 2)]
+    e = np.linalg.eigvalsh(_dense_h(n, op, states))
+    f2, f4 = F2 / 49, F4 / 441
+    terms = {"3F": (F0 - 8 * f2 - 9 * f4, 21), "1D": (F0 - 3 * f2 + 36 * f4, 5), "3P": (F0 + 7 * f2 - 84 * f4, 9),
+             "1G": (F0 + 4 * f2 + f4, 9), "1S": (F0 + 14 * f2 + 126 * f4, 1)}
+    expect = np.sort(np.concatenate([[v] * d for v, d in terms.values()]))
+    np.testing.assert_allclose(e, expect, atol=1e-12)
+
+
+@pytest.mark.parametrize("l,xi", [(2, 0.08), (1, 11.5)])
+def test_spin_orbit_spectrum(l, xi):
+    """One electron: j = l + 1/2 at xi*l/2, j = l - 1/2 at -xi*(l+1)/2."""
+    nb = OrderedDict([(l, 0)])
+    op = {tuple((ops.c2i(nb, so), a) for so, a in p): v for p, v in ops.getSOCop(xi, l).items()}
+    n = 2 * (2 * l + 1)
+    states = [psr.tuple2int((i,), n) for i in range(n)]
+    e = np.linalg.eigvalsh(_dense_h(n, op, states))
+    expect = np.sort([xi * l / 2] * (2 * l + 2) + [-xi * (l + 1) / 2] * (2 * l))
+    np.testing.assert_allclose(e, expect, atol=1e-13)
+
+
+def test_dipole_operator_is_p_to_d_and_normalized():
+    """sum over polarizations and spins of |T_ij|^2 per 2p orbital = 2/5 * ... (Gaunt sum rule).
+
+    sum_{m} c^1(2 m, 1 m')^2 = (2l'+1)... : for each 2p m', sum_m c1^2 = 2/3 (closed form)."""
+    nb = OrderedDict([(1, 0), (2, 0)])
+    for mp in (-1, 0, 1):
+        s = sum(ops.gauntC(1, 2, m, 1, mp) ** 2 for m in range(-2, 3))
+        assert abs(s - 2 / 3) < 1e-14
+    t = ops.getDipoleOperator(nb, [0, 0, 1])
+    assert all(p[0][1] == "c" and p[1][1] == "a" and 6 <= p[0][0] < 16 and p[1][0] < 6 for p in t)
+
+
+def test_nixs_radial_integrals_match_reference_log():
+    """refout/output.txt:579-583 prints these Rintegral values."""
+    r, R = np.loadtxt(RADIAL).T
+    got = [ops.radial_integral(k, q, R, R, r) for q in (2.0, 7.0) for k in (2, 4)]
+    ref = [0.06931194874611417, 0.0024194888780995167, 0.1783277037191155, 0.08530206004884185]
+    np.testing.assert_allclose(got, ref, rtol=1e-13)
+
+
+# ------------------------------------------------------------ bases
+
+
+def test_basis_count_and_order_50_bath_layout():
+    """45 + 500 + 1225 = 1770 states (refout/output.txt:19) in itertools order."""
+    nb = OrderedDict([(1, 0), (2, 50)])
+    b = basis.get_basis(nb, nb, OrderedDict([(1, 0), (2, 2)]), OrderedDict([(1, 0), (2, 0)]),
+                        OrderedDict([(1, 0), (2, 2)]), OrderedDict([(1, 6), (2, 8)]))
+    assert len(b) == 1770
+    n = 66
+    first = psr.int2tuple(b[0], n)
+    assert first == tuple(range(6)) + (6, 7, 8, 9, 10, 11, 12, 13) + tuple(range(16, 66))
+    assert len(set(b)) == 1770
+
+
+def test_restrictions_layout():
+    nb = OrderedDict([(1, 0), (2, 10)])
+    r = basis.get_restrictions(nb, nb, {1: 6, 2: 8}, {1: 0, 2: 2}, {1: 0, 2: 2}, {1: 0, 2: 0})
+    assert r[frozenset(range(6, 16))] == (7, 11)
+    assert r[frozenset(range(16, 26))] == (8, 10)
+    assert r[frozenset()] == (0, 0)
+
+
+def test_expand_basis_invariants():
+    from impuritymodel_amd.models import make_model
+
+    m = make_model(n_bath=10, seed=5)
+    n = m.n_spin_orbitals
+    b0 = basis.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    seeds = list(b0[:3])
+    rs = applyop.restriction_masks(n, m.restrictions)
+    seeds = [s for s in seeds if applyop._passes(s, rs)]
+    h_dict = {}
+    full = hamiltonian.expand_basis(n, h_dict, m.hOp, seeds, m.restrictions)
+    assert list(full[: len(seeds)]) == seeds
+    sset = set(full)
+    for s in full:
+        assert set(h_dict[s]).issubset(sset)  # closed under H
+        assert applyop._passes(s, rs)
+    # BFS layers by distance from the seeds, each appended in ascending order
+    seen = set(seeds)
+    frontier = list(seeds)
+    pos = len(seeds)
+    while frontier:
+        layer = sorted({t for s in frontier for t in h_dict[s]} - seen)
+        assert list(full[pos:pos + len(layer)]) == layer
+        pos += len(layer)
+        seen.update(layer)
+        frontier = layer
+    assert pos == len(full)
+
+
+# ------------------------------------------------------------ Krylov
+
+
+def _random_hermitian(n, density, seed):
+    rng = np.random.default_rng(seed)
+    a = scipy.sparse.random(n, n, density=density, random_state=seed) + 1j * scipy.sparse.random(
+        n, n, density=density, random_state=seed + 1)
+    h = (a + a.conj().T) * 0.5 + scipy.sparse.diags(rng.uniform(-3, 3, n))
+    return scipy.sparse.csr_matrix(h)
+
+
+def test_lanczos_cf_equals_exact_resolvent():
+    n = 40
+    h = _random_hermitian(n, 0.15, 3)
+    rng = np.random.default_rng(4)
+    psi = rng.normal(size=n) + 1j * rng.normal(size=n)
+    psi /= np.linalg.norm(psi)
+    a, b = krylov.lanczos(h, psi, n, reorth="full")
+    w = np.linspace(-6, 6, 301)
+    g = krylov.continued_fraction(a, b, w, 0.3, 0.7)
+    e, v = np.linalg.eigh(h.toarray())
+    wts = np.abs(v.conj().T @ psi) ** 2
+    exact = np.array([np.sum(wts / (x + 0.3j + 0.7 - e)) for x in w])
+    np.testing.assert_allclose(g, exact, rtol=1e-10, atol=1e-12)
+
+
+def test_lanczos_moments():
+    n = 30
+    h = _random_hermitian(n, 0.2, 8).toarray()
+    rng = np.random.default_rng(9)
+    psi = rng.normal(size=n) + 1j * rng.normal(size=n)
+    psi /= np.linalg.norm(psi)
+    a, b = krylov.lanczos(scipy.sparse.csr_matrix(h), psi, 5)
+    m1 = np.vdot(psi, h @ psi).real
+    m2 = np.vdot(h @ psi, h @ psi).real
+    assert abs(a[0] - m1) < 1e-12
+    assert abs(b[0] ** 2 - (m2 - m1**2)) < 1e-11
+
+
+def test_lanczos_beta_zero_raises():
+    h = scipy.sparse.csr_matrix(np.diag([1.0, 2.0, 3.0]).astype(complex))
+    with pytest.raises(ValueError, match="beta==0"):
+        krylov.lanczos(h, np.array([1, 0, 0], complex), 3)
+
+
+def test_sum_rule_of_spectrum():
+    """int -Im G dw -> pi |psi|^2 for a wide window (Lorentzian tails cut)."""
+    n = 25
+    h = _random_hermitian(n, 0.2, 11)
+    psi = np.ones(n, complex) / np.sqrt(n)
+    a, b = krylov.lanczos(h, psi, n, reorth="full")
+    w = np.linspace(-400, 400, 400001)
+    g = krylov.continued_fraction(a, b, w, 0.05, 0.0)
+    integral = np.trapezoid(-g.imag, w)
+    assert abs(integral - np.pi) < 1e-3
+
+
+def test_thermal_average_closed_form():
+    e = np.array([0.0, 0.01, 0.05])
+    o = np.array([1.0, 2.0, 3.0])
+    w = np.exp(-(e - e.min()) / (spectra.k_B * 300))
+    assert abs(spectra.thermal_average(e, o, 300) - np.sum(w * o) / np.sum(w)) < 1e-15
+    assert abs(spectra.k_B - 8.617333262145179e-05) < 1e-20
