@@ -35,6 +35,7 @@ SIGNATURES = {
     "imped_timing_reset": [],
     "imped_timing_query": [C.c_char_p, _p, _p],
     "imped_set_tuning": [C.c_char_p, _i64],
+    "imped_sort_u64": [_p, _i64, _i32, _i32, _p],
     "imped_enumerate_basis": [_i32, _i32, _i32, _p, _i64, _p, _p],
     "imped_ops_create": [_i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "imped_ops_destroy": [_p],

@@ -669,6 +669,21 @@ int imped_enumerate_basis(int n_orb, int W, int n_shells, const int32_t* shell, 
   API_END
 }
 
+int imped_sort_u64(const uint64_t* keys, int64_t n, int begin_bit, int end_bit, int32_t* perm) {
+  API_BEGIN
+  if (n < 0 || begin_bit < 0 || end_bit > 64 || begin_bit > end_bit || n > INT32_MAX) {
+    set_error("imped_sort_u64: invalid arguments");
+    return IMPED_ERR_INVALID;
+  }
+  DevBuf dk, dp;
+  h2d(dk, keys, std::max<int64_t>(n, 1), stream());
+  h2d(dp, perm, std::max<int64_t>(n, 1), stream());
+  sort_u64_perm(dk.as<uint64_t>(), dp.as<int32_t>(), n, begin_bit, end_bit, stream());
+  d2h(perm, dp.p, n, stream());
+  HIP_CHECK(hipStreamSynchronize(stream()));
+  API_END
+}
+
 int imped_set_tuning(const char* name, int64_t value) {
   API_BEGIN
   const std::string n(name);

@@ -105,6 +105,8 @@ void sort_chain_keys(int W, int n_orb, int n_chain_bits, const uint64_t* keys, c
 // Stable sort of records (vec, key) by LSD passes; same interface.
 void sort_vec_keys_stable(int W, int n_orb, int n_vec_bits, const uint64_t* keys, const int32_t* vec,
                           int32_t* perm, int64_t n, hipStream_t s);
+// Stable sort of perm by keys[perm[i]] on bits [begin_bit, end_bit).
+void sort_u64_perm(const uint64_t* keys, int32_t* perm, int64_t n, int begin_bit, int end_bit, hipStream_t s);
 // Stable sort of perm by 64-bit secondary key then 32-bit primary key.
 void sort_pairs_u64_u32(const uint64_t* sec, const int32_t* prim, int n_prim_bits, int32_t* perm, int64_t n,
                         hipStream_t s);

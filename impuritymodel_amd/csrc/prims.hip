@@ -7,7 +7,6 @@
 // sorted by stable LSD radix passes over the key words (least significant
 // first) and then the chain id, using rocPRIM's device radix sort.
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "kernels.h"
@@ -17,7 +16,7 @@ namespace imped {
 namespace {
 
 struct Scratch {
-  DevBuf temp, k0, k1, v1, k64a, k64b;
+  DevBuf temp, wide, k0, k1, v1, hist, hoff, gather;
 };
 Scratch& scratch() {
   static Scratch s;
@@ -29,15 +28,24 @@ __global__ void widen_kernel(const int32_t* in, int64_t* out, int64_t n) {
   if (i < n) out[i] = in[i];
 }
 
+// Exclusive scan of int32 counts into int64 offsets, stream-ordered.
+void scan_counts_async(const int32_t* counts, int64_t n, int64_t* offsets, hipStream_t s) {
+  if (n == 0) return;
+  Scratch& sc = scratch();
+  sc.wide.reserve(n * sizeof(int64_t));
+  int64_t* wide = sc.wide.as<int64_t>();
+  hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, counts, wide, n);
+  size_t tb = 0;
+  HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, wide, offsets, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s));
+  sc.temp.reserve(tb);
+  HIP_CHECK(
+      rocprim::exclusive_scan(sc.temp.p, tb, wide, offsets, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s));
+}
+
 __global__ void gather_word_kernel(const uint64_t* keys, int W, int w, const int32_t* perm, uint64_t* out,
                                    int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = keys[(size_t)perm[i] * W + w];
-}
-
-__global__ void gather_i32_kernel(const int32_t* src, const int32_t* perm, uint32_t* out, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (uint32_t)src[perm[i]];
 }
 
 __global__ void gather_u64_kernel(const uint64_t* src, const int32_t* perm, uint64_t* out, int64_t n) {
@@ -45,18 +53,109 @@ __global__ void gather_u64_kernel(const uint64_t* src, const int32_t* perm, uint
   if (i < n) out[i] = src[perm[i]];
 }
 
-template <class K>
-void sort_pass(const K* keys_in, int32_t* perm, int64_t n, int begin_bit, int end_bit, hipStream_t s) {
+// ---- stable LSD radix sort of (uint64 key, int32 payload) pairs, 8-bit digits
+//
+// Pass over one digit: (1) per-tile digit histograms, (2) one exclusive scan
+// over the digit-major histogram matrix (digit d of tile b lands after all
+// smaller digits and after digit d of tiles < b), (3) a stable scatter: each
+// tile walks its elements in index order, 256 at a time; inside a wave the
+// lanes holding the same digit are found with 8 ballots and ranked by lane,
+// waves are ordered through LDS counters.
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+
+__global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const uint64_t* keys, int64_t n, int shift,
+                                                             int n_tiles, int32_t* hist) {
+  __shared__ int32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    const int64_t i = base + (int64_t)k * RS_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t* keys, const int32_t* vals, int64_t n,
+                                                                int shift, int n_tiles, const int64_t* offs,
+                                                                uint64_t* keys_out, int32_t* vals_out) {
+  __shared__ int64_t run[256];
+  __shared__ int32_t wcnt[RS_THREADS / 64][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  run[threadIdx.x] = offs[(int64_t)threadIdx.x * n_tiles + blockIdx.x];
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    for (int j = threadIdx.x; j < (RS_THREADS / 64) * 256; j += RS_THREADS) (&wcnt[0][0])[j] = 0;
+    __syncthreads();
+    const int64_t i = base + (int64_t)k * RS_THREADS + threadIdx.x;
+    const bool valid = i < n;
+    uint64_t key = 0;
+    int32_t val = 0;
+    uint32_t d = 0;
+    if (valid) {
+      key = keys[i];
+      val = vals[i];
+      d = (uint32_t)(key >> shift) & 255u;
+    }
+    uint64_t eq = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      eq &= bit ? m : ~m;
+    }
+    const int rank = __popcll(eq & lt);
+    if (valid && rank == 0) wcnt[wave][d] = __popcll(eq);
+    __syncthreads();
+    if (valid) {
+      int64_t pos = run[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+    int32_t add = 0;
+    for (int w = 0; w < RS_THREADS / 64; ++w) add += wcnt[w][threadIdx.x];
+    run[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// Stable sort of perm by keys_in[i] (already gathered in perm order) on bits
+// [begin_bit, end_bit).  perm is updated in place.
+void sort_pass(const uint64_t* keys_in, int32_t* perm, int64_t n, int begin_bit, int end_bit, hipStream_t s) {
+  if (n <= 1 || begin_bit >= end_bit) return;
+  if (n > INT32_MAX) throw HipError{"sort: too many records"};
   Scratch& sc = scratch();
-  sc.k1.reserve(n * sizeof(K));
-  sc.v1.reserve(n * sizeof(int32_t));
-  size_t tb = 0;
-  HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, keys_in, sc.k1.as<K>(), perm, sc.v1.as<int32_t>(), (size_t)n,
-                                      begin_bit, end_bit, s));
-  sc.temp.reserve(tb);
-  HIP_CHECK(rocprim::radix_sort_pairs(sc.temp.p, tb, keys_in, sc.k1.as<K>(), perm, sc.v1.as<int32_t>(), (size_t)n,
-                                      begin_bit, end_bit, s));
-  HIP_CHECK(hipMemcpyAsync(perm, sc.v1.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  const int n_tiles = (int)((n + RS_TILE - 1) / RS_TILE);
+  sc.k0.reserve(n * 8);
+  sc.k1.reserve(n * 8);
+  sc.v1.reserve(n * 4);
+  sc.hist.reserve((size_t)256 * n_tiles * 4);
+  sc.hoff.reserve(((size_t)256 * n_tiles + 1) * 8);
+  if (keys_in != sc.k0.as<uint64_t>())
+    HIP_CHECK(hipMemcpyAsync(sc.k0.p, keys_in, n * 8, hipMemcpyDeviceToDevice, s));
+  uint64_t* ka = sc.k0.as<uint64_t>();
+  uint64_t* kb = sc.k1.as<uint64_t>();
+  int32_t* va = perm;
+  int32_t* vb = sc.v1.as<int32_t>();
+  for (int shift = begin_bit; shift < end_bit; shift += 8) {
+    hipLaunchKernelGGL(rs_hist_kernel, dim3(n_tiles), dim3(RS_THREADS), 0, s, ka, n, shift, n_tiles,
+                       sc.hist.as<int32_t>());
+    HIP_CHECK(hipGetLastError());
+    scan_counts_async(sc.hist.as<int32_t>(), (int64_t)256 * n_tiles, sc.hoff.as<int64_t>(), s);
+    hipLaunchKernelGGL(rs_scatter_kernel, dim3(n_tiles), dim3(RS_THREADS), 0, s, ka, va, n, shift, n_tiles,
+                       sc.hoff.as<int64_t>(), kb, vb);
+    HIP_CHECK(hipGetLastError());
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (va != perm) HIP_CHECK(hipMemcpyAsync(perm, va, n * 4, hipMemcpyDeviceToDevice, s));
 }
 
 int bits_for(int64_t n) {
@@ -65,39 +164,44 @@ int bits_for(int64_t n) {
   return b;
 }
 
+// Digit passes of a word whose used bits are [64 - used, 64), rounded to
+// whole 8-bit digits from the bottom of the used range.
+void sort_word(const uint64_t* gathered_in_k0, int32_t* perm, int64_t n, int used_bits, hipStream_t s) {
+  int begin = 64 - used_bits;
+  begin -= begin % 8;
+  sort_pass(gathered_in_k0, perm, n, begin, 64, s);
+}
+
+__global__ void gather_i32_as_u64_kernel(const int32_t* src, const int32_t* perm, uint64_t* out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint64_t)(uint32_t)src[perm[i]];
+}
+
 void sort_words_then(int W, int n_orb, int n_prim_bits, const uint64_t* keys, const int32_t* prim, int32_t* perm,
                      int64_t n, hipStream_t s) {
   if (n <= 1) return;
   Scratch& sc = scratch();
-  sc.k64a.reserve(n * sizeof(uint64_t));
+  sc.k0.reserve(n * sizeof(uint64_t));
   for (int w = W - 1; w >= 0; --w) {
     // Orbital bits of word w occupy its top bits; lower bits are zero.
     int used = n_orb - 64 * w;
     if (used <= 0) continue;
     if (used > 64) used = 64;
     hipLaunchKernelGGL(gather_word_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, W, w, perm,
-                       sc.k64a.as<uint64_t>(), n);
-    sort_pass<uint64_t>(sc.k64a.as<uint64_t>(), perm, n, 64 - used, 64, s);
+                       sc.k0.as<uint64_t>(), n);
+    sort_word(sc.k0.as<uint64_t>(), perm, n, used, s);
   }
-  sc.k0.reserve(n * sizeof(uint32_t));
-  hipLaunchKernelGGL(gather_i32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, prim, perm, sc.k0.as<uint32_t>(),
-                     n);
-  sort_pass<uint32_t>(sc.k0.as<uint32_t>(), perm, n, 0, n_prim_bits, s);
+  sc.k0.reserve(n * sizeof(uint64_t));
+  hipLaunchKernelGGL(gather_i32_as_u64_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, prim, perm,
+                     sc.k0.as<uint64_t>(), n);
+  sort_pass(sc.k0.as<uint64_t>(), perm, n, 0, n_prim_bits, s);
 }
 
 }  // namespace
 
 int64_t scan_counts(const int32_t* counts, int64_t n, int64_t* offsets, hipStream_t s) {
   if (n == 0) return 0;
-  Scratch& sc = scratch();
-  sc.k64b.reserve((n + 1) * sizeof(int64_t));
-  int64_t* wide = sc.k64b.as<int64_t>();
-  hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, counts, wide, n);
-  size_t tb = 0;
-  HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, wide, offsets, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s));
-  sc.temp.reserve(tb);
-  HIP_CHECK(
-      rocprim::exclusive_scan(sc.temp.p, tb, wide, offsets, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s));
+  scan_counts_async(counts, n, offsets, s);
   int64_t last_off = 0;
   int32_t last_cnt = 0;
   HIP_CHECK(hipMemcpyAsync(&last_off, offsets + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -120,14 +224,23 @@ void sort_pairs_u64_u32(const uint64_t* sec, const int32_t* prim, int n_prim_bit
                         hipStream_t s) {
   if (n <= 1) return;
   Scratch& sc = scratch();
-  sc.k64a.reserve(n * sizeof(uint64_t));
-  hipLaunchKernelGGL(gather_u64_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, sec, perm, sc.k64a.as<uint64_t>(),
+  sc.k0.reserve(n * sizeof(uint64_t));
+  hipLaunchKernelGGL(gather_u64_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, sec, perm, sc.k0.as<uint64_t>(),
                      n);
-  sort_pass<uint64_t>(sc.k64a.as<uint64_t>(), perm, n, 0, 64, s);
-  sc.k0.reserve(n * sizeof(uint32_t));
-  hipLaunchKernelGGL(gather_i32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, prim, perm, sc.k0.as<uint32_t>(),
+  sort_pass(sc.k0.as<uint64_t>(), perm, n, 0, 64, s);
+  hipLaunchKernelGGL(gather_i32_as_u64_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, prim, perm,
+                     sc.k0.as<uint64_t>(), n);
+  sort_pass(sc.k0.as<uint64_t>(), perm, n, 0, n_prim_bits, s);
+}
+
+// Test hook: stable sort of perm by 64-bit keys (bits [begin, end)).
+void sort_u64_perm(const uint64_t* keys, int32_t* perm, int64_t n, int begin_bit, int end_bit, hipStream_t s) {
+  if (n <= 1) return;
+  Scratch& sc = scratch();
+  sc.k0.reserve(n * sizeof(uint64_t));
+  hipLaunchKernelGGL(gather_u64_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, perm, sc.k0.as<uint64_t>(),
                      n);
-  sort_pass<uint32_t>(sc.k0.as<uint32_t>(), perm, n, 0, n_prim_bits, s);
+  sort_pass(sc.k0.as<uint64_t>(), perm, n, begin_bit, end_bit, s);
 }
 
 // ------------------------------------------------------------------ wave tasks

@@ -275,3 +275,12 @@ def enumerate_basis(n_orb, W, shells):
     keys = np.zeros((max(N, 1), W), np.uint64)
     check(lib.imped_enumerate_basis(int(n_orb), int(W), len(shells), ptr(sh), N, ptr(n), ptr(keys)))
     return keys[:N]
+
+
+def sort_u64(keys, begin_bit=0, end_bit=64, perm=None):
+    """Stable device radix sort: permutation sorting ``keys`` on bits [begin, end)."""
+    keys = np.ascontiguousarray(keys, np.uint64)
+    n = len(keys)
+    perm = np.arange(n, dtype=np.int32) if perm is None else np.ascontiguousarray(perm, np.int32).copy()
+    check(lib.imped_sort_u64(ptr(_nz(keys)), n, int(begin_bit), int(end_bit), ptr(_nz(perm))))
+    return perm

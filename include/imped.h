@@ -58,6 +58,12 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * "eigen_large_bt"); set by the autotuner.  No reference counterpart. */
 int imped_set_tuning(const char* name, int64_t value);
 
+/* The engine's device radix sort (the one that orders BFS layers and
+ * applyOp outputs), exposed for testing: stable sort of the index array
+ * perm[n] by keys[perm[i]] on bits [begin_bit, end_bit).  No reference
+ * counterpart (the reference calls Python's sorted(), finite.py:1749). */
+int imped_sort_u64(const uint64_t* keys, int64_t n, int begin_bit, int end_bit, int32_t* perm);
+
 /* get_basis (finite.py:294-374) on the host: the restricted initial basis in
  * the reference's itertools order.  shell[10*l ..] per angular-momentum shell:
  * {first impurity index, impurity count, first valence-bath index, valence
