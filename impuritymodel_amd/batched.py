@@ -46,6 +46,18 @@ class XASResult:
     kout: np.ndarray = None
 
 
+def _dipole_tables(models, W):
+    """Dipole term tables (x, y, z) of every model with its restrictions."""
+    out = []
+    for m in models:
+        cache = m.__dict__.setdefault("_dip_tables", {})
+        if W not in cache:
+            dip = getDipoleOperators(m.nBaths, POLARIZATIONS)
+            cache[W] = [opcompile.TermTable(d, m.restrictions, W) for d in dip]
+        out.extend(cache[W])
+    return out
+
+
 def ground_bases(models, W):
     """get_basis of every model as [N, W] words (native enumeration)."""
     return [get_basis_words(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps, W)
@@ -63,9 +75,8 @@ def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0,
     M = len(models)
     n_orb = max(m.n_spin_orbitals for m in models)
     W = opcompile.n_words(n_orb)
-    ops_h = Ops([m.hOp for m in models], [m.restrictions for m in models], n_orb, W)
-    dip = getDipoleOperators(models[0].nBaths, POLARIZATIONS)
-    ops_t = Ops([d for m in models for d in dip], [m.restrictions for m in models for _ in dip], n_orb, W)
+    ops_h = Ops([m.term_table(W) for m in models], None, n_orb, W)
+    ops_t = Ops(_dipole_tables(models, W), None, n_orb, W)
     if bases is None:
         bases = ground_bases(models, W)
     tim["host_setup"] = time.perf_counter() - t0
@@ -92,7 +103,7 @@ def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0,
     for k, (m, i) in enumerate(zip(psi_chain, psi_which)):
         psi_index[(int(m), int(i))] = k
     psis = gb.vec_from_eigen(psi_chain, psi_which, 1e-7)
-    nt = len(dip)
+    nt = len(POLARIZATIONS)
     out_src, out_model, chain_model, energy, spec_id, zflag = [], [], [], [], [], []
     for m in range(M):
         for t in range(nt):

@@ -91,8 +91,10 @@ inline void d2h(T* dst, const void* src, size_t n, hipStream_t s) {
 struct Tuning {
   int64_t lanczos_wave_max_n = 4096;    // chains up to this size: one wave each
   int64_t lanczos_wg256_max_n = 65536;  // then 256 threads, above: 1024
-  int64_t eigen_small_n = 512;          // eigensolver: 256 threads up to this size
-  int64_t eigen_large_bt = 1024;        // threads per chain above it (512 or 1024)
+  int64_t eigen_bt = 512;               // eigensolver threads per chain (256, 512, 1024)
+  int64_t eigen_real = 1;               // real Krylov vectors when H is real
+  int64_t eigen_dgks = 1;               // second Gram-Schmidt pass only when needed
+  int64_t eigen_check_every = 4;        // Ritz convergence check interval
 };
 extern Tuning g_tuning;
 

@@ -73,6 +73,7 @@ static hipStream_t g_stream = nullptr;
 
 struct Ops {
   int W = 1, M = 0, n_orb = 0;
+  bool real = false;  // every term value is real, so H is real in the product basis
   DevBuf model_group_off, group_term_off, group_flip, term_masks, term_meta, term_coef, model_restr_off, restr_mask,
       restr_lohi;
   OpsView view() const {
@@ -121,6 +122,7 @@ struct Batch {
   int nev = 0;
   DevBuf evals, evecs;
   bool built = false;
+  bool real = false;  // all CSR values real
 };
 
 // ------------------------------------------------------------------ small kernels
@@ -382,16 +384,32 @@ static void finalize_sizes(Batch& b) {
   h2d(b.chain_n, b.h_n.data(), b.C, stream());
 }
 
+__global__ void gather_rowptr_kernel(const int64_t* rowptr, const int64_t* row_base, int C, int64_t* out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= C) out[c] = rowptr[row_base[c]];
+}
+
 static void compute_chain_nnz(Batch& b) {
   std::vector<int64_t> rp(b.C + 1);
   b.h_nnz.assign(b.C, 0);
-  for (int c = 0; c <= b.C; ++c) {
-    HIP_CHECK(hipMemcpyAsync(&rp[c], b.rowptr.as<int64_t>() + b.h_row_base[c], sizeof(int64_t),
-                             hipMemcpyDeviceToHost, stream()));
-  }
+  DevBuf tmp;
+  tmp.reserve((b.C + 1) * sizeof(int64_t));
+  hipLaunchKernelGGL(gather_rowptr_kernel, dim3(blocks_for(b.C + 1, 256)), dim3(256), 0, stream(),
+                     b.rowptr.as<int64_t>(), b.row_base.as<int64_t>(), b.C, tmp.as<int64_t>());
+  HIP_CHECK(hipGetLastError());
+  d2h(rp.data(), tmp.p, b.C + 1, stream());
   HIP_CHECK(hipStreamSynchronize(stream()));
   for (int c = 0; c < b.C; ++c) b.h_nnz[c] = rp[c + 1] - rp[c];
   b.NNZ = rp[b.C];
+}
+
+// Start vectors: the seed amplitudes of chain c fill its first local rows.
+__global__ void place_seed_amps_kernel(const int64_t* seed_off, const int64_t* row_base, int C, const double2* amps,
+                                       double2* v0) {
+  int c = blockIdx.x;
+  if (c >= C) return;
+  const int64_t a = seed_off[c], b = seed_off[c + 1], r = row_base[c];
+  for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) v0[r + (i - a)] = amps[i];
 }
 
 static void build(Batch& b, int mode, double weight_min, int max_layers) {
@@ -427,6 +445,7 @@ static void build(Batch& b, int mode, double weight_min, int max_layers) {
                          b.chain_n.as<int64_t>(), b.C, b.row_chain.as<int32_t>());
     compute_chain_nnz(b);
     b.built = true;
+    b.real = b.ops->real;
     return;
   }
 
@@ -513,6 +532,7 @@ static void build(Batch& b, int mode, double weight_min, int max_layers) {
   if (err) throw HipError{"internal error: CSR target missing from basis"};
   compute_chain_nnz(b);
   b.built = true;
+  b.real = b.ops->real;
 }
 
 }  // namespace imped
@@ -691,14 +711,22 @@ int imped_set_tuning(const char* name, int64_t value) {
     g_tuning.lanczos_wave_max_n = value;
   else if (n == "lanczos_wg256_max_n")
     g_tuning.lanczos_wg256_max_n = value;
-  else if (n == "eigen_small_n")
-    g_tuning.eigen_small_n = value;
-  else if (n == "eigen_large_bt") {
-    if (value != 512 && value != 1024) {
-      set_error("imped_set_tuning: eigen_large_bt must be 512 or 1024");
+  else if (n == "eigen_bt") {
+    if (value != 256 && value != 512 && value != 1024) {
+      set_error("imped_set_tuning: eigen_bt must be 256, 512 or 1024");
       return IMPED_ERR_INVALID;
     }
-    g_tuning.eigen_large_bt = value;
+    g_tuning.eigen_bt = value;
+  } else if (n == "eigen_real")
+    g_tuning.eigen_real = value != 0;
+  else if (n == "eigen_dgks")
+    g_tuning.eigen_dgks = value != 0;
+  else if (n == "eigen_check_every") {
+    if (value < 1) {
+      set_error("imped_set_tuning: eigen_check_every must be >= 1");
+      return IMPED_ERR_INVALID;
+    }
+    g_tuning.eigen_check_every = value;
   } else {
     set_error("imped_set_tuning: unknown parameter " + n);
     return IMPED_ERR_INVALID;
@@ -722,6 +750,9 @@ int imped_ops_create(int W, int n_orb, int n_models, const int32_t* model_group_
   const int G = model_group_off[n_models];
   const int T = group_term_off[G];
   const int R = model_restr_off[n_models];
+  o->real = true;
+  for (int t = 0; t < T; ++t)
+    if (term_coef[2 * t + 1] != 0.0) o->real = false;
   hipStream_t s = stream();
   h2d(o->model_group_off, model_group_off, n_models + 1, s);
   h2d(o->group_term_off, group_term_off, G + 1, s);
@@ -856,6 +887,9 @@ int imped_batch_set_csr(imped_batch batch, const int64_t* chain_n, const int64_t
   b->h_nnz.assign(b->C, 0);
   for (int c = 0; c < b->C; ++c) b->h_nnz[c] = rowptr[b->h_row_base[c + 1]] - rowptr[b->h_row_base[c]];
   HIP_CHECK(hipStreamSynchronize(s));
+  b->real = true;
+  for (int64_t e = 0; e < b->NNZ; ++e)
+    if (val[2 * e + 1] != 0.0) b->real = false;
   b->built = true;
   API_END
 }
@@ -915,11 +949,10 @@ int imped_batch_lanczos(imped_batch batch, const double* v0_host, int K, int reo
     // seed amplitudes occupy the first local rows of every chain
     v0.reserve(std::max<int64_t>(b->R, 1) * 16);
     HIP_CHECK(hipMemsetAsync(v0.p, 0, std::max<int64_t>(b->R, 1) * 16, stream()));
-    for (int c = 0; c < b->C; ++c) {
-      int64_t ns = b->h_seed_off[c + 1] - b->h_seed_off[c];
-      if (ns)
-        HIP_CHECK(hipMemcpyAsync(v0.as<double2>() + b->h_row_base[c], b->seed_amps.as<double2>() + b->h_seed_off[c],
-                                 ns * 16, hipMemcpyDeviceToDevice, stream()));
+    if (b->C) {
+      hipLaunchKernelGGL(place_seed_amps_kernel, dim3(b->C), dim3(64), 0, stream(), b->seed_off_dev.as<int64_t>(),
+                         b->row_base.as<int64_t>(), b->C, b->seed_amps.as<double2>(), v0.as<double2>());
+      HIP_CHECK(hipGetLastError());
     }
     v0d = v0.as<double2>();
   }
@@ -968,7 +1001,8 @@ int imped_batch_eigen(imped_batch batch, int nev, double tol, int max_steps, uin
   DevBuf steps;
   steps.reserve(std::max(b->C, 1) * 4);
   int rc = run_eigen(kb, nev, tol, max_steps, seed, b->evals.as<double>(), b->evecs.as<double2>(),
-                     steps.as<int32_t>(), stream());
+                     steps.as<int32_t>(), b->real, stream());
+  if (rc == IMPED_ERR_INVALID) return rc;
   if (evals) d2h(evals, b->evals.p, (size_t)b->C * nev, stream());
   if (evecs_host) d2h(evecs_host, b->evecs.p, (size_t)b->R * nev * 2, stream());
   if (steps_out) d2h(steps_out, steps.p, b->C, stream());

@@ -26,8 +26,10 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
 
 // Lowest nev eigenpairs of every chain.  evals [C][nev]; evecs: chain c's
 // block starts at row_base[c]*nev, eigenvector k at offset k*n_c.
+// real_values: every CSR value has zero imaginary part, so the solver may
+// run on real vectors (when the eigen_real tuning flag allows it).
 int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_t seed, double* evals,
-              double2* evecs, int32_t* steps, hipStream_t s);
+              double2* evecs, int32_t* steps, bool real_values, hipStream_t s);
 
 // Continued fractions of all chains on the omega grid plus the Boltzmann
 // average over each spectrum's chains (host arrays are uploaded).

@@ -294,9 +294,9 @@ struct EigenArgs {
   int max_steps;
   double tol;
   uint64_t seed;
-  double2* vstore;     // per chain: msteps(c) * n vectors, offset voff[c]
+  void* vstore;        // per chain: msteps(c) * n vectors (double or double2), offset voff[c]
   const int64_t* voff;
-  double2* wbuf;       // per chain: n at row_base
+  void* wbuf;          // per chain: n at row_base
   double2* coef;       // per chain: max_steps
   double* ab;          // per chain: 2*max_steps (alpha then beta)
   double* yk;          // per chain: nev*max_steps
@@ -307,12 +307,217 @@ struct EigenArgs {
   int32_t* steps;
   int32_t* conv;
   const int32_t* chain_list;
+  int dgks;          // 1: second Gram-Schmidt pass only when needed
+  int check_every;   // Ritz check interval (steps)
 };
 
+// Element helpers: the eigensolver runs on complex (double2) vectors, or on
+// real (double) vectors when H is real -- then a real start vector keeps
+// every Krylov vector real and halves the memory traffic.
+__device__ __forceinline__ double re_(double x) { return x; }
+__device__ __forceinline__ double re_(double2 x) { return x.x; }
+__device__ __forceinline__ double im_(double) { return 0.0; }
+__device__ __forceinline__ double im_(double2 x) { return x.y; }
+template <class T>
+__device__ __forceinline__ T mk(double r, double i);
+template <>
+__device__ __forceinline__ double mk<double>(double r, double) {
+  return r;
+}
+template <>
+__device__ __forceinline__ double2 mk<double2>(double r, double i) {
+  return make_double2(r, i);
+}
+template <class T>
+constexpr bool is_real_v = sizeof(T) == sizeof(double);
+
+// y = H x on one chain (T-typed vectors); returns the partial Re<x, y>.
+template <int BT, class T>
+__device__ __forceinline__ double spmv_t(const int64_t* rowptr, const int32_t* col, const double2* val, int64_t base,
+                                         int64_t n, const T* x, T* y) {
+  double pa = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
+    const int64_t e0 = rowptr[base + i], e1 = rowptr[base + i + 1];
+    double ax = 0.0, ay = 0.0;
+    for (int64_t e = e0; e < e1; ++e) {
+      const double2 h = val[e];
+      const T v = x[col[e]];
+      if constexpr (is_real_v<T>) {
+        ax += h.x * re_(v);
+      } else {
+        ax += h.x * re_(v) - h.y * im_(v);
+        ay += h.x * im_(v) + h.y * re_(v);
+      }
+    }
+    y[i] = mk<T>(ax, ay);
+    pa += ax * re_(x[i]) + ay * im_(x[i]);
+  }
+  return pa;
+}
+
+// One classical Gram-Schmidt pass of w against V[0..m): coefficients by
+// wave-level dot products, then one update sweep.
+template <int BT, class T>
+__device__ void cgs_pass(const T* V, int m, int64_t n, T* w, double2* coef) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int l = wave; l < m; l += BT / 64) {
+    const T* vl = V + (size_t)l * n;
+    double cx = 0.0, cy = 0.0;
+    for (int64_t i = lane; i < n; i += 64) {
+      const T a = vl[i], b = w[i];
+      cx += re_(a) * re_(b) + im_(a) * im_(b);  // conj(a) * b
+      if constexpr (!is_real_v<T>) cy += re_(a) * im_(b) - im_(a) * re_(b);
+    }
+    cx = wave_sum(cx);
+    if constexpr (!is_real_v<T>) cy = wave_sum(cy);
+    if (lane == 0) coef[l] = make_double2(cx, cy);
+  }
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
+    double wx = re_(w[i]), wy = im_(w[i]);
+    for (int l = 0; l < m; ++l) {
+      const double2 c = coef[l];
+      const T a = V[(size_t)l * n + i];
+      if constexpr (is_real_v<T>) {
+        wx -= re_(a) * c.x;
+      } else {
+        wx -= re_(a) * c.x - im_(a) * c.y;
+        wy -= re_(a) * c.y + im_(a) * c.x;
+      }
+    }
+    w[i] = mk<T>(wx, wy);
+  }
+  __syncthreads();
+}
+
+template <int BT, class T>
+__device__ __forceinline__ double norm2_t(const T* w, int64_t n, double* red) {
+  double p = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += BT) p += re_(w[i]) * re_(w[i]) + im_(w[i]) * im_(w[i]);
+  return block_sum<BT>(p, red);
+}
+
+constexpr int MAX_NEV = 16;
+
+// cl[l] = <V_l, w> for l < m (and cl[m] = <w, w> when self), four vectors per
+// wave at a time so w is read once per four dot products; results in LDS.
+template <int BT, class T>
+__device__ void dots_lds(const T* V, int m, int64_t n, const T* w, double2* cl, bool self) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int NW = BT / 64;
+  const int total = m + (self ? 1 : 0);
+  for (int l0 = wave * 4; l0 < total; l0 += NW * 4) {
+    double cx[4] = {0.0, 0.0, 0.0, 0.0}, cy[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = lane; i < n; i += 64) {
+      const T b = w[i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int l = l0 + u;
+        if (l < total) {
+          const T av = l < m ? V[(size_t)l * n + i] : b;
+          cx[u] += re_(av) * re_(b) + im_(av) * im_(b);
+          if constexpr (!is_real_v<T>) cy[u] += re_(av) * im_(b) - im_(av) * re_(b);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double x = wave_sum(cx[u]);
+      const double y = is_real_v<T> ? 0.0 : wave_sum(cy[u]);
+      if (lane == 0 && l0 + u < total) cl[l0 + u] = make_double2(x, y);
+    }
+  }
+  __syncthreads();
+}
+
+// w -= sum_l cl[l] V_l over l < m; returns |w|^2 (same in every thread).
+template <int BT, class T>
+__device__ double update_norm(const T* V, int m, int64_t n, T* w, const double2* cl, double* red) {
+  double p = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
+    double wx = re_(w[i]), wy = im_(w[i]);
+    for (int l = 0; l < m; ++l) {
+      const double2 c = cl[l];
+      const T av = V[(size_t)l * n + i];
+      if constexpr (is_real_v<T>) {
+        wx -= re_(av) * c.x;
+      } else {
+        wx -= re_(av) * c.x - im_(av) * c.y;
+        wy -= re_(av) * c.y + im_(av) * c.x;
+      }
+    }
+    w[i] = mk<T>(wx, wy);
+    p += wx * wx + wy * wy;
+  }
+  return block_sum<BT>(p, red);
+}
+
+// Lowest nev eigenvalues of the symmetric tridiagonal (a, b) of size m by
+// multisection: the workgroup splits into nev groups; each evaluates Sturm
+// counts at G interior points of its interval per round and keeps the
+// sub-interval where the count crosses k, shrinking it G+1 times per round.
 template <int BT>
+__device__ void ritz_values(const double* a, const double* b, int m, int nev, double* s_lam, double* s_lo,
+                            double* s_hi, int* s_first, int* s_flag) {
+  __shared__ double s_bounds[3];
+  if (threadIdx.x == 0) {
+    double lo = a[0], hi = a[0], scale = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
+      lo = fmin(lo, a[i] - r);
+      hi = fmax(hi, a[i] + r);
+      scale = fmax(scale, fabs(a[i]) + r);
+    }
+    s_bounds[0] = lo - 1e-14 * scale - 1e-300;
+    s_bounds[1] = hi + 1e-14 * scale + 1e-300;
+    s_bounds[2] = 1e-300 + 2.2e-16 * 2.2e-16 * scale * scale;
+  }
+  __syncthreads();
+  const int G = BT / nev;
+  const int g = threadIdx.x / G, r = threadIdx.x % G;
+  if ((int)threadIdx.x < nev) {
+    s_lo[threadIdx.x] = s_bounds[0];
+    s_hi[threadIdx.x] = s_bounds[1];
+  }
+  const double pivmin = s_bounds[2];
+  __syncthreads();
+  for (int round = 0; round < 64; ++round) {
+    if ((int)threadIdx.x < nev) s_first[threadIdx.x] = G;
+    if (threadIdx.x == 0) *s_flag = 0;
+    __syncthreads();
+    if (g < nev) {
+      const double lo = s_lo[g], hi = s_hi[g];
+      const double x = lo + (hi - lo) * (double)(r + 1) / (double)(G + 1);
+      if (sturm_count(a, b, m, x, pivmin) > g) atomicMin(&s_first[g], r);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nev) {
+      const int k = threadIdx.x;
+      const double lo = s_lo[k], hi = s_hi[k];
+      const int f = s_first[k];
+      const double nlo = f == 0 ? lo : lo + (hi - lo) * (double)f / (double)(G + 1);
+      const double nhi = f == G ? hi : lo + (hi - lo) * (double)(f + 1) / (double)(G + 1);
+      s_lo[k] = nlo;
+      s_hi[k] = nhi;
+      const double mid = 0.5 * (nlo + nhi);
+      if (nhi - nlo > 2.2e-16 * fmax(fabs(nlo), fabs(nhi)) + 1e-300 && mid > nlo && mid < nhi) atomicOr(s_flag, 1);
+    }
+    __syncthreads();
+    if (*s_flag == 0) break;
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < nev) s_lam[threadIdx.x] = 0.5 * (s_lo[threadIdx.x] + s_hi[threadIdx.x]);
+  __syncthreads();
+}
+
+template <int BT, class T>
 __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   __shared__ double red[BT / 64];
   __shared__ int s_done;
+  __shared__ double s_lam[MAX_NEV], s_lo[MAX_NEV], s_hi[MAX_NEV];
+  __shared__ int s_first[MAX_NEV];
+  __shared__ int s_flag;
+  extern __shared__ double2 cl[];  // max_steps + 2 coefficients
   const int c = a.chain_list[blockIdx.x];
   const int64_t n = a.chain_n[c];
   const int64_t base = a.row_base[c];
@@ -326,56 +531,69 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
     return;
   }
   const int mmax = (int)(ms < n ? ms : n);
-  double2* V = a.vstore + a.voff[c];
-  double2* w = a.wbuf + base;
+  T* V = reinterpret_cast<T*>(a.vstore) + a.voff[c];
+  T* w = reinterpret_cast<T*>(a.wbuf) + base;
   double2* coef = a.coef + (size_t)c * ms;
   double* alpha = a.ab + (size_t)c * 2 * ms;
   double* beta = alpha + ms;
   double* Y = a.yk + (size_t)c * nev * ms;
-  // random start vector
+  // seeded random start vector (real when T is real)
   double p = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += BT) {
     const uint64_t h = a.seed * 0x100000001B3ull + (uint64_t)c * 0x9E3779B97F4A7C15ull + 2 * (uint64_t)i;
-    const double2 x = make_double2(u01(h) - 0.5, u01(h + 1) - 0.5);
-    V[i] = x;
-    p += x.x * x.x + x.y * x.y;
+    const double xr = u01(h) - 0.5, xi = is_real_v<T> ? 0.0 : u01(h + 1) - 0.5;
+    V[i] = mk<T>(xr, xi);
+    p += xr * xr + xi * xi;
   }
-  double nrm = sqrt(block_sum<BT>(p, red));
-  for (int64_t i = threadIdx.x; i < n; i += BT) V[i] = make_double2(V[i].x / nrm, V[i].y / nrm);
+  const double nrm = sqrt(block_sum<BT>(p, red));
+  for (int64_t i = threadIdx.x; i < n; i += BT) V[i] = mk<T>(re_(V[i]) / nrm, im_(V[i]) / nrm);
   __syncthreads();
   int m = 0;
   int converged = 0;
   for (int j = 0; j < mmax; ++j) {
-    const double2* vj = V + (size_t)j * n;
-    const double al = block_sum<BT>(spmv_rows<BT>(a.rowptr, a.col, a.val, base, n, vj, w), red);
-    if (threadIdx.x == 0) alpha[j] = al;
+    const T* vj = V + (size_t)j * n;
+    // three-term step: w = H v_j - alpha_j v_j - beta_{j-1} v_{j-1}, with the
+    // big cancellation done before any Gram-Schmidt coefficient is formed
+    // (coefficients taken from H v_j itself would carry errors of order
+    // eps*|H v_j| and lose orthogonality geometrically)
+    double al = block_sum<BT>(spmv_t<BT, T>(a.rowptr, a.col, a.val, base, n, vj, w), red);
     const double bprev = j > 0 ? beta[j - 1] : 0.0;
+    double p0 = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += BT) {
-      double2 x = w[i];
-      x.x -= al * vj[i].x;
-      x.y -= al * vj[i].y;
+      double xr = re_(w[i]) - al * re_(vj[i]);
+      double xi = im_(w[i]) - al * im_(vj[i]);
       if (j > 0) {
-        x.x -= bprev * vj[i - n].x;
-        x.y -= bprev * vj[i - n].y;
+        xr -= bprev * re_(vj[i - n]);
+        xi -= bprev * im_(vj[i - n]);
       }
-      w[i] = x;
+      w[i] = mk<T>(xr, xi);
+      p0 += xr * xr + xi * xi;
     }
-    __syncthreads();
-    cgs2<BT>(V, j + 1, n, w, coef);
-    p = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += BT) p += w[i].x * w[i].x + w[i].y * w[i].y;
-    const double b = sqrt(block_sum<BT>(p, red));
+    p0 = block_sum<BT>(p0, red);
+    // full reorthogonalization against V_0..V_j; a second pass when the
+    // first removed more than half of |w|^2 (Daniel-Gragg-Kaufman-Stewart),
+    // always when the criterion is switched off
+    dots_lds<BT, T>(V, j + 1, n, w, cl, false);
+    double p1 = update_norm<BT, T>(V, j + 1, n, w, cl, red);
+    if (!a.dgks || p1 < 0.5 * p0) {
+      dots_lds<BT, T>(V, j + 1, n, w, cl, false);
+      p1 = update_norm<BT, T>(V, j + 1, n, w, cl, red);
+    }
+    if (threadIdx.x == 0) alpha[j] = al;
+    const double b = sqrt(p1);
     m = j + 1;
     const bool last = (m == mmax) || b <= 1e-13 * (fabs(al) + bprev + 1e-300);
     if (threadIdx.x == 0) beta[j] = b;
     __syncthreads();
-    if (last || (m >= nev && m % 8 == 0)) {
-      // Ritz pairs of T_m: thread k handles eigenvalue k
+    if (last || (m >= nev && m % a.check_every == 0)) {
+      // Ritz pairs of T_m: eigenvalues by workgroup multisection, then
+      // thread k refines eigenvector k by inverse iteration
+      ritz_values<BT>(alpha, beta, m, nev < m ? nev : m, s_lam, s_lo, s_hi, s_first, &s_flag);
       if (threadIdx.x == 0) s_done = 1;
       __syncthreads();
       if ((int)threadIdx.x < nev && (int)threadIdx.x < m) {
         const int k = threadIdx.x;
-        const double lam = kth_eigenvalue(alpha, beta, m, k);
+        const double lam = s_lam[k];
         double* y = Y + (size_t)k * ms;
         double* tw = a.tw + ((size_t)c * nev + k) * 5 * ms;
         int* piv = a.tpiv + ((size_t)c * nev + k) * ms;
@@ -413,10 +631,10 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
       }
     }
     if (last) break;
-    for (int64_t i = threadIdx.x; i < n; i += BT) V[(size_t)(j + 1) * n + i] = make_double2(w[i].x / b, w[i].y / b);
+    for (int64_t i = threadIdx.x; i < n; i += BT) V[(size_t)(j + 1) * n + i] = mk<T>(re_(w[i]) / b, im_(w[i]) / b);
     __syncthreads();
   }
-  // Ritz vectors x_k = sum_l y_k[l] V_l, normalized
+  // Ritz vectors x_k = sum_l y_k[l] V_l, normalized, stored complex
   double2* X = a.evecs + base * nev;
   const int kk = nev < m ? nev : m;
   for (int k = 0; k < kk; ++k) {
@@ -425,9 +643,9 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
     for (int64_t i = threadIdx.x; i < n; i += BT) {
       double xr = 0.0, xi = 0.0;
       for (int l = 0; l < m; ++l) {
-        const double2 v = V[(size_t)l * n + i];
-        xr += y[l] * v.x;
-        xi += y[l] * v.y;
+        const T v = V[(size_t)l * n + i];
+        xr += y[l] * re_(v);
+        xi += y[l] * im_(v);
       }
       X[(size_t)k * n + i] = make_double2(xr, xi);
       pn += xr * xr + xi * xi;
@@ -579,10 +797,14 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
 }
 
 int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_t seed, double* evals,
-              double2* evecs, int32_t* steps, hipStream_t s) {
+              double2* evecs, int32_t* steps, bool real_values, hipStream_t s) {
   KScratch& sc = kscratch();
   const int C = kb.C;
   if (C == 0) return IMPED_OK;
+  if (nev > MAX_NEV || max_steps < 1 || max_steps > 4000) {
+    set_error("eigensolver: nev must be <= 16 and max_steps in [1, 4000]");
+    return IMPED_ERR_INVALID;
+  }
   std::vector<int64_t> voff(C + 1, 0);
   int64_t R = 0;
   for (int c = 0; c < C; ++c) {
@@ -609,9 +831,9 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   a.max_steps = max_steps;
   a.tol = tol;
   a.seed = seed;
-  a.vstore = sc.vstore.as<double2>();
+  a.vstore = sc.vstore.p;
   a.voff = sc.voff.as<int64_t>();
-  a.wbuf = sc.wbuf.as<double2>();
+  a.wbuf = sc.wbuf.p;
   a.coef = sc.coef.as<double2>();
   a.ab = sc.ab.as<double>();
   a.yk = sc.yk.as<double>();
@@ -621,24 +843,29 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   a.evecs = evecs;
   a.steps = steps;
   a.conv = sc.conv.as<int32_t>();
-  std::vector<int32_t> small, large;
-  for (int c = 0; c < C; ++c) (kb.h_n[c] <= g_tuning.eigen_small_n ? small : large).push_back(c);
-  std::vector<int32_t> all(small);
-  all.insert(all.end(), large.begin(), large.end());
-  h2d(sc.clist, all.data(), all.size(), s);
-  if (!small.empty()) {
-    a.chain_list = sc.clist.as<int32_t>();
+  a.dgks = (int)g_tuning.eigen_dgks;
+  a.check_every = (int)std::max<int64_t>(1, g_tuning.eigen_check_every);
+  // one launch, largest chains first so the longest workgroups start first
+  std::vector<int32_t> order(C);
+  for (int c = 0; c < C; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kb.h_n[x] > kb.h_n[y]; });
+  h2d(sc.clist, order.data(), order.size(), s);
+  a.chain_list = sc.clist.as<int32_t>();
+  const bool real = real_values && g_tuning.eigen_real;
+  const size_t shm = (size_t)(max_steps + 2) * sizeof(double2);
+  {
     TimedLaunch tl("eigen", s);
-    hipLaunchKernelGGL(eigen_kernel<256>, dim3((unsigned)small.size()), dim3(256), 0, s, a);
-    HIP_CHECK(hipGetLastError());
-  }
-  if (!large.empty()) {
-    a.chain_list = sc.clist.as<int32_t>() + small.size();
-    TimedLaunch tl("eigen", s);
-    if (g_tuning.eigen_large_bt == 512)
-      hipLaunchKernelGGL(eigen_kernel<512>, dim3((unsigned)large.size()), dim3(512), 0, s, a);
-    else
-      hipLaunchKernelGGL(eigen_kernel<1024>, dim3((unsigned)large.size()), dim3(1024), 0, s, a);
+    const dim3 g((unsigned)C);
+    if (g_tuning.eigen_bt == 256) {
+      if (real) hipLaunchKernelGGL((eigen_kernel<256, double>), g, dim3(256), shm, s, a);
+      else hipLaunchKernelGGL((eigen_kernel<256, double2>), g, dim3(256), shm, s, a);
+    } else if (g_tuning.eigen_bt == 512) {
+      if (real) hipLaunchKernelGGL((eigen_kernel<512, double>), g, dim3(512), shm, s, a);
+      else hipLaunchKernelGGL((eigen_kernel<512, double2>), g, dim3(512), shm, s, a);
+    } else {
+      if (real) hipLaunchKernelGGL((eigen_kernel<1024, double>), g, dim3(1024), shm, s, a);
+      else hipLaunchKernelGGL((eigen_kernel<1024, double2>), g, dim3(1024), shm, s, a);
+    }
     HIP_CHECK(hipGetLastError());
   }
   std::vector<int32_t> conv(C);

@@ -37,6 +37,20 @@ class ImpurityModel:
     def n_spin_orbitals(self):
         return ops.n_spin_orbitals(self.nBaths)
 
+    def term_table(self, W):
+        """Compiled H term table (with restrictions) for W-word states.
+
+        Built once per model: the model is an immutable input, its operator
+        dict is converted to the engine's table format the first time a batch
+        needs it.
+        """
+        from .opcompile import TermTable
+
+        cache = self.__dict__.setdefault("_tables", {})
+        if W not in cache:
+            cache[W] = TermTable(self.hOp, self.restrictions, W)
+        return cache[W]
+
 
 def make_model(
     n_bath=10,

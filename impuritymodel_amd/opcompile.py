@@ -127,32 +127,49 @@ def restriction_arrays(restrictions, W):
     return np.array(masks, np.uint64).reshape(-1, W), np.array(lohi, np.int32).reshape(-1, 2)
 
 
+class TermTable:
+    """One operator's compiled term table (with its restrictions) for W words."""
+
+    __slots__ = ("W", "group_term_off", "group_flip", "masks", "meta", "coef", "rmask", "rlohi")
+
+    def __init__(self, op, restrictions, W):
+        g_off, g_flip, masks, met, src = _structure(tuple(op.keys()), W)
+        vals = np.fromiter(op.values(), dtype=np.complex128, count=len(op)) if len(op) else np.zeros(0, complex)
+        self.W = W
+        self.group_term_off = g_off
+        self.group_flip = g_flip
+        self.masks = masks
+        self.meta = met
+        self.coef = vals[src]
+        self.rmask, self.rlohi = restriction_arrays(restrictions, W)
+
+
 def compile_models(ops, restrictions, W):
     """Concatenate the term tables of several operators.
 
-    ``ops`` is a list of operator dicts, ``restrictions`` a list (same length)
-    of restriction dicts or None.  Returns a dict of arrays matching
-    ``imped_ops_create``.
+    ``ops`` is a list of operator dicts (or ready ``TermTable`` objects),
+    ``restrictions`` a list (same length) of restriction dicts or None.
+    Returns a dict of arrays matching ``imped_ops_create``.
     """
-    M = len(ops)
     mg, gt, gf, tm, meta, coef, mr, rm, rl = [0], [], [], [], [], [], [0], [], []
     t_base = 0
     for m, op in enumerate(ops):
-        procs = tuple(op.keys())
-        g_off, g_flip, masks, met, src = _structure(procs, W)
-        vals = np.fromiter(op.values(), dtype=np.complex128, count=len(op)) if len(op) else np.zeros(0, complex)
+        if isinstance(op, TermTable):
+            assert op.W == W, "term table compiled for another word count"
+            tt = op
+        else:
+            tt = TermTable(op, restrictions[m] if restrictions is not None else None, W)
+        g_off = tt.group_term_off
         gt.append(g_off[:-1] + t_base if len(g_off) > 1 else np.zeros(0, np.int32))
-        gf.append(g_flip)
-        tm.append(masks)
-        meta.append(met)
-        coef.append(vals[src])
-        t_base += len(src)
+        gf.append(tt.group_flip)
+        tm.append(tt.masks)
+        meta.append(tt.meta)
+        coef.append(tt.coef)
+        t_base += len(tt.coef)
         mg.append(mg[-1] + len(g_off) - 1)
-        r = restrictions[m] if restrictions is not None else None
-        rmask, rlohi = restriction_arrays(r, W)
-        rm.append(rmask)
-        rl.append(rlohi)
-        mr.append(mr[-1] + len(rmask))
+        rm.append(tt.rmask)
+        rl.append(tt.rlohi)
+        mr.append(mr[-1] + len(tt.rmask))
     group_term_off = np.concatenate(gt + [np.array([t_base], np.int32)]).astype(np.int32)
     coef = np.concatenate(coef) if coef else np.zeros(0, complex)
     return dict(

@@ -188,6 +188,46 @@ def test_eigensystem_vs_dense(gpu):
         np.testing.assert_allclose(vg * (ov / abs(ov)), vo, rtol=0, atol=1e-8)
 
 
+@pytest.mark.parametrize("hy", [0.0, 2e-3])
+def test_eigensolver_variants_vs_dense(gpu, hy):
+    """Real (hy = 0) and complex (hy != 0) Hamiltonians; every launch shape,
+    one or two Gram-Schmidt passes, real or complex Krylov vectors."""
+    from impuritymodel_amd import engine
+    from impuritymodel_amd.batched import ground_bases
+    from impuritymodel_amd.models import make_model
+
+    models = [make_model(n_bath=nb, seed=20 + nb, hField=(0.0, hy, 1e-4)) for nb in (10, 20, 30)]
+    ops = engine.Ops([m.hOp for m in models], [m.restrictions for m in models], max(m.n_spin_orbitals for m in models))
+    b = engine.Batch(ops, np.arange(3, dtype=np.int32))
+    b.set_seeds(ground_bases(models, ops.W))
+    b.build(fixed_basis=True)
+    rowptr, col, val = b.csr()
+    ref = []
+    for c in range(3):
+        r0, r1 = b.row_base[c], b.row_base[c + 1]
+        n = r1 - r0
+        dense = np.zeros((n, n), complex)
+        for r in range(n):
+            for e in range(rowptr[r0 + r], rowptr[r0 + r + 1]):
+                dense[r, col[e]] = val[e]
+        ref.append(np.linalg.eigvalsh(dense)[:3])
+    try:
+        for bt in (256, 1024):
+            for real in (0, 1):
+                for dgks in (0, 1):
+                    engine.set_tuning("eigen_bt", bt)
+                    engine.set_tuning("eigen_real", real)
+                    engine.set_tuning("eigen_dgks", dgks)
+                    ev, _, steps, ok = b.eigen(3, tol=1e-12)
+                    assert ok
+                    for c in range(3):
+                        np.testing.assert_allclose(ev[c], ref[c], rtol=0, atol=1e-10)
+    finally:
+        engine.set_tuning("eigen_bt", 512)
+        engine.set_tuning("eigen_real", 1)
+        engine.set_tuning("eigen_dgks", 1)
+
+
 def test_lanczos_random_hermitian_and_breakdown(gpu):
     from impuritymodel_amd import finite
 
