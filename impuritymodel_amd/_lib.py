@@ -25,6 +25,7 @@ _i32 = C.c_int32
 _i64 = C.c_int64
 _u64 = C.c_uint64
 _d = C.c_double
+_f = C.c_float
 
 # name -> argtypes (all return int)
 SIGNATURES = {
@@ -45,6 +46,8 @@ SIGNATURES = {
     "imped_vec_get": [_p, _p, _p, _p],
     "imped_vec_apply": [_p, _p, _i32, _p, _p, _d, _i32, _p],
     "imped_vec_from_eigen": [_p, _i32, _p, _p, _d, _p],
+    "imped_vec_prune": [_p, _d, _p],
+    "imped_batch_solve": [_p, _i32, _p, _d, _i32, _p, _p, _p],
     "imped_batch_create": [_p, _i32, _p, _p],
     "imped_batch_destroy": [_p],
     "imped_batch_set_seeds": [_p, _p, _p, _p],
@@ -56,7 +59,16 @@ SIGNATURES = {
     "imped_batch_get_csr": [_p, _p, _p, _p],
     "imped_batch_lanczos": [_p, _p, _i32, _i32, _p, _p, _p, _p],
     "imped_batch_eigen": [_p, _i32, _d, _i32, _u64, _p, _p, _p],
-    "imped_batch_cf": [_p, _p, _p, _p, _p, _d, _p, _i32, _p, _i32, _d, _p, _p],
+    "imped_batch_cf": [_p, _p, _p, _p, _p, _p, _p, _i32, _p, _i32, _d, _p, _p],
+    "imped_mlp_create": [_i32, _i32, _p, _i32, _p],
+    "imped_mlp_destroy": [_p],
+    "imped_mlp_param_count": [_p, _p],
+    "imped_mlp_set_params": [_p, _p],
+    "imped_mlp_get_params": [_p, _p],
+    "imped_mlp_forward": [_p, _p, _i32, _p],
+    "imped_mlp_gradients": [_p, _p, _p, _i32, _p, _p],
+    "imped_mlp_set_data": [_p, _p, _p, _i64],
+    "imped_mlp_train": [_p, _p, _i32, _i32, _f, _f, _f, _f, _p],
 }
 
 

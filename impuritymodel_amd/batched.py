@@ -92,10 +92,11 @@ def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0,
     tim["ground_eigen"] = time.perf_counter() - t1
 
     t1 = time.perf_counter()
-    lim = energy_cut * K_B * T
+    T_m = np.broadcast_to(np.asarray(T, np.float64), (M,))
     kept = []
     for m in range(M):
         e = evals[m]
+        lim = energy_cut * K_B * T_m[m]
         kept.append([i for i in range(nev) if e[i] - e[0] < lim])
     psi_chain = np.array([m for m in range(M) for i in kept[m]], np.int32)
     psi_which = np.array([i for m in range(M) for i in kept[m]], np.int32)
@@ -130,7 +131,7 @@ def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0,
         raise ValueError("Warning: beta==0, implementation absent!")
     tim["lanczos"] = time.perf_counter() - t1
     t1 = time.perf_counter()
-    _, spec = xb.cf(np.array(energy), norm2, np.ones(len(energy), np.int32), np.array(zflag, np.int32), K_B * T,
+    _, spec = xb.cf(np.array(energy), norm2, np.ones(len(energy), np.int32), np.array(zflag, np.int32), K_B * T_m,
                     np.array(spec_id, np.int32), M, omega, delta)
     tim["cf"] = time.perf_counter() - t1
     tim["total"] = time.perf_counter() - t0

@@ -24,6 +24,7 @@
 #include "../../include/imped.h"
 #include "kernels.h"
 #include "krylov.h"
+#include "mlp.h"
 
 namespace imped {
 
@@ -122,8 +123,18 @@ struct Batch {
   int nev = 0;
   DevBuf evals, evecs;
   bool built = false;
-  bool real = false;  // all CSR values real
+  bool row_keys = false;  // pool_keys holds the basis in row order (built, not set_csr)
+  bool real = false;      // all CSR values real
 };
+
+// Seed amplitudes are placed on the first rows of their chain: every chain
+// must hold at least as many rows as it has seeds.
+static bool seeds_fit(const Batch& b) {
+  if ((int)b.h_seed_off.size() != b.C + 1) return false;
+  for (int c = 0; c < b.C; ++c)
+    if (b.h_seed_off[c + 1] - b.h_seed_off[c] > b.h_n[c]) return false;
+  return true;
+}
 
 // ------------------------------------------------------------------ small kernels
 
@@ -445,6 +456,7 @@ static void build(Batch& b, int mode, double weight_min, int max_layers) {
                          b.chain_n.as<int64_t>(), b.C, b.row_chain.as<int32_t>());
     compute_chain_nnz(b);
     b.built = true;
+    b.row_keys = true;
     b.real = b.ops->real;
     return;
   }
@@ -532,6 +544,7 @@ static void build(Batch& b, int mode, double weight_min, int max_layers) {
   if (err) throw HipError{"internal error: CSR target missing from basis"};
   compute_chain_nnz(b);
   b.built = true;
+  b.row_keys = true;
   b.real = b.ops->real;
 }
 
@@ -721,7 +734,15 @@ int imped_set_tuning(const char* name, int64_t value) {
     g_tuning.eigen_real = value != 0;
   else if (n == "eigen_dgks")
     g_tuning.eigen_dgks = value != 0;
-  else if (n == "eigen_check_every") {
+  else if (n == "flat_min_n")
+    g_tuning.flat_min_n = value;
+  else if (n == "spmv_lpr") {
+    if (value != 4 && value != 8 && value != 16 && value != 32 && value != 64) {
+      set_error("imped_set_tuning: spmv_lpr must be 4, 8, 16, 32 or 64");
+      return IMPED_ERR_INVALID;
+    }
+    g_tuning.spmv_lpr = value;
+  } else if (n == "eigen_check_every") {
     if (value < 1) {
       set_error("imped_set_tuning: eigen_check_every must be >= 1");
       return IMPED_ERR_INVALID;
@@ -891,6 +912,7 @@ int imped_batch_set_csr(imped_batch batch, const int64_t* chain_n, const int64_t
   for (int64_t e = 0; e < b->NNZ; ++e)
     if (val[2 * e + 1] != 0.0) b->real = false;
   b->built = true;
+  b->row_keys = false;
   API_END
 }
 
@@ -913,6 +935,10 @@ int imped_batch_sizes(imped_batch batch, int64_t* chain_n, int64_t* chain_nnz, i
 int imped_batch_get_basis(imped_batch batch, uint64_t* keys) {
   API_BEGIN
   Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b->row_keys) {
+    set_error("imped_batch_get_basis: the batch has no basis keys (not built by imped_batch_build)");
+    return IMPED_ERR_INVALID;
+  }
   d2h(keys, b->pool_keys.p, (size_t)b->R * b->W, stream());
   HIP_CHECK(hipStreamSynchronize(stream()));
   API_END
@@ -942,8 +968,8 @@ int imped_batch_lanczos(imped_batch batch, const double* v0_host, int K, int reo
     h2d(v0, v0_host, std::max<int64_t>(b->R, 1) * 2, stream());
     v0d = v0.as<double2>();
   } else {
-    if (!b->have_seed_amps) {
-      set_error("imped_batch_lanczos: no start vector");
+    if (!b->have_seed_amps || !seeds_fit(*b)) {
+      set_error("imped_batch_lanczos: no start vector (or more seeds than rows in a chain)");
       return IMPED_ERR_INVALID;
     }
     // seed amplitudes occupy the first local rows of every chain
@@ -1014,8 +1040,53 @@ int imped_batch_eigen(imped_batch batch, int nev, double tol, int max_steps, uin
   API_END
 }
 
+int imped_batch_solve(imped_batch batch, int n_shift, const double* shifts, double tol, int max_steps,
+                      int32_t* steps_out, double* resid_out, double* x_out) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b->built || n_shift < 1) {
+    set_error("imped_batch_solve: batch not built or no shifts");
+    return IMPED_ERR_INVALID;
+  }
+  if (!b->have_seed_amps || !seeds_fit(*b)) {
+    set_error("imped_batch_solve: no right-hand side (or more seeds than rows in a chain)");
+    return IMPED_ERR_INVALID;
+  }
+  DevBuf y;
+  y.reserve(std::max<int64_t>(b->R, 1) * 16);
+  HIP_CHECK(hipMemsetAsync(y.p, 0, std::max<int64_t>(b->R, 1) * 16, stream()));
+  if (b->C) {
+    hipLaunchKernelGGL(place_seed_amps_kernel, dim3(b->C), dim3(64), 0, stream(), b->seed_off_dev.as<int64_t>(),
+                       b->row_base.as<int64_t>(), b->C, b->seed_amps.as<double2>(), y.as<double2>());
+    HIP_CHECK(hipGetLastError());
+  }
+  b->nev = 0;
+  b->evecs.reserve(std::max<int64_t>(b->R, 1) * n_shift * 16);
+  KrylovBatch kb{b->C,
+                 b->row_base.as<int64_t>(),
+                 b->chain_n.as<int64_t>(),
+                 b->rowptr.as<int64_t>(),
+                 b->col.as<int32_t>(),
+                 b->val.as<double2>(),
+                 b->h_n,
+                 b->h_nnz};
+  const int rc = run_solve(kb, y.as<double2>(), n_shift, shifts, tol, max_steps, b->evecs.as<double2>(), steps_out,
+                           resid_out, stream());
+  if (rc == IMPED_ERR_INVALID) return rc;
+  b->nev = n_shift;
+  if (x_out) {
+    d2h(x_out, b->evecs.p, (size_t)b->R * n_shift * 2, stream());
+    HIP_CHECK(hipStreamSynchronize(stream()));
+  }
+  if (rc != IMPED_OK) {
+    set_error("imped_batch_solve: residual above tolerance after max_steps");
+    return rc;
+  }
+  API_END
+}
+
 int imped_batch_cf(imped_batch batch, const double* energy, const double* norm2, const int32_t* sign,
-                   const int32_t* zflag, double kT, const int32_t* spec_id, int n_spec, const double* omega, int n_w,
+                   const int32_t* zflag, const double* kT, const int32_t* spec_id, int n_spec, const double* omega, int n_w,
                    double delta, double* g_out, double* spec_out) {
   API_BEGIN
   Batch* b = reinterpret_cast<Batch*>(batch);
@@ -1023,10 +1094,11 @@ int imped_batch_cf(imped_batch batch, const double* energy, const double* norm2,
     set_error("imped_batch_cf: run imped_batch_lanczos first");
     return IMPED_ERR_INVALID;
   }
-  if (!(kT > 0.0)) {
-    set_error("imped_batch_cf: kT must be positive");
-    return IMPED_ERR_INVALID;
-  }
+  for (int q = 0; q < n_spec; ++q)
+    if (!(kT[q] > 0.0)) {
+      set_error("imped_batch_cf: kT must be positive");
+      return IMPED_ERR_INVALID;
+    }
   run_cf(b->C, b->K, b->alpha.as<double>(), b->beta.as<double>(), b->kout.as<int32_t>(), energy, norm2, sign, zflag,
          kT, spec_id, n_spec, omega, n_w, delta, g_out, spec_out, stream());
   API_END
@@ -1093,12 +1165,33 @@ int imped_vec_apply(imped_ops ops, imped_vec src, int n_out, const int32_t* out_
   API_END
 }
 
+int imped_vec_prune(imped_vec src, double weight_min, imped_vec* out) {
+  API_BEGIN
+  Vec* v = reinterpret_cast<Vec*>(src);
+  auto r = std::make_unique<Vec>();
+  r->W = v->W;
+  r->V = v->V;
+  prune_vec(v->W, v->h_off, v->keys.as<uint64_t>(), v->amps.as<double2>(), weight_min, r->h_off, r->keys, r->amps,
+            stream());
+  if (v->norm2.p && v->V) {
+    r->norm2.reserve((size_t)v->V * 8);
+    HIP_CHECK(hipMemcpyAsync(r->norm2.p, v->norm2.p, (size_t)v->V * 8, hipMemcpyDeviceToDevice, stream()));
+    HIP_CHECK(hipStreamSynchronize(stream()));
+  }
+  *out = reinterpret_cast<imped_vec>(r.release());
+  API_END
+}
+
 int imped_vec_from_eigen(imped_batch batch, int n_out, const int32_t* chain, const int32_t* which, double weight_min,
                          imped_vec* out) {
   API_BEGIN
   Batch* b = reinterpret_cast<Batch*>(batch);
   if (b->nev < 1) {
-    set_error("imped_vec_from_eigen: run imped_batch_eigen first");
+    set_error("imped_vec_from_eigen: run imped_batch_eigen or imped_batch_solve first");
+    return IMPED_ERR_INVALID;
+  }
+  if (!b->row_keys) {
+    set_error("imped_vec_from_eigen: the batch has no basis keys (not built by imped_batch_build)");
     return IMPED_ERR_INVALID;
   }
   for (int i = 0; i < n_out; ++i)
@@ -1112,6 +1205,93 @@ int imped_vec_from_eigen(imped_batch batch, int n_out, const int32_t* chain, con
   truncate_eigvecs(b->W, b->pool_keys.as<uint64_t>(), b->evecs.as<double2>(), b->nev, b->h_row_base, b->h_n, n_out,
                    chain, which, weight_min, r->h_off, r->keys, r->amps, stream());
   *out = reinterpret_cast<imped_vec>(r.release());
+  API_END
+}
+
+// ------------------------------------------------------------------ surrogate network
+
+int imped_mlp_create(int n_members, int n_layers, const int32_t* dims, int max_batch, imped_mlp* out) {
+  API_BEGIN
+  if (n_members < 1 || n_members > 64 || n_layers < 1 || n_layers > 16 || max_batch < 1) {
+    set_error("imped_mlp_create: need 1..64 members, 1..16 layers and max_batch >= 1");
+    return IMPED_ERR_INVALID;
+  }
+  for (int l = 0; l <= n_layers; ++l)
+    if (dims[l] < 1 || dims[l] > 65536) {
+      set_error("imped_mlp_create: layer widths must be in [1, 65536]");
+      return IMPED_ERR_INVALID;
+    }
+  *out = reinterpret_cast<imped_mlp>(mlp_create(n_members, n_layers, dims, max_batch));
+  API_END
+}
+
+int imped_mlp_destroy(imped_mlp net) {
+  mlp_destroy(reinterpret_cast<Mlp*>(net));
+  return IMPED_OK;
+}
+
+int imped_mlp_param_count(imped_mlp net, int64_t* n) {
+  *n = mlp_param_count(*reinterpret_cast<Mlp*>(net));
+  return IMPED_OK;
+}
+
+int imped_mlp_set_params(imped_mlp net, const float* params) {
+  API_BEGIN
+  mlp_set_params(*reinterpret_cast<Mlp*>(net), params, stream());
+  API_END
+}
+
+int imped_mlp_get_params(imped_mlp net, float* params) {
+  API_BEGIN
+  mlp_get_params(*reinterpret_cast<Mlp*>(net), params, stream());
+  API_END
+}
+
+int imped_mlp_forward(imped_mlp net, const float* x, int batch, float* y) {
+  API_BEGIN
+  if (batch < 1) {
+    set_error("imped_mlp_forward: batch must be >= 1");
+    return IMPED_ERR_INVALID;
+  }
+  mlp_forward(*reinterpret_cast<Mlp*>(net), x, batch, y, stream());
+  API_END
+}
+
+int imped_mlp_gradients(imped_mlp net, const float* x, const float* y, int batch, float* grads, double* loss) {
+  API_BEGIN
+  if (batch < 1) {
+    set_error("imped_mlp_gradients: batch must be >= 1");
+    return IMPED_ERR_INVALID;
+  }
+  mlp_gradients(*reinterpret_cast<Mlp*>(net), x, y, batch, grads, loss, stream());
+  API_END
+}
+
+int imped_mlp_set_data(imped_mlp net, const float* x, const float* y, int64_t n) {
+  API_BEGIN
+  if (n < 1) {
+    set_error("imped_mlp_set_data: empty data set");
+    return IMPED_ERR_INVALID;
+  }
+  mlp_set_data(*reinterpret_cast<Mlp*>(net), x, y, n, stream());
+  API_END
+}
+
+int imped_mlp_train(imped_mlp net, const int32_t* batches, int steps, int batch, float lr, float beta1, float beta2,
+                    float eps, double* losses) {
+  API_BEGIN
+  Mlp* m = reinterpret_cast<Mlp*>(net);
+  const int64_t n = mlp_data_count(*m);
+  if (steps < 0 || batch < 1 || n < 1) {
+    set_error("imped_mlp_train: set the data first; steps >= 0, batch >= 1");
+    return IMPED_ERR_INVALID;
+  }
+  for (int64_t i = 0; i < (int64_t)steps * batch; ++i)
+    if (batches[i] < 0 || batches[i] >= n) {
+      set_error("imped_mlp_train: minibatch row index out of range");
+      return IMPED_ERR_INVALID;
+    }
+  mlp_train(*m, batches, steps, batch, lr, beta1, beta2, eps, losses, stream());
   API_END
 }
 

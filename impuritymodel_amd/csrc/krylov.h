@@ -31,10 +31,18 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
 int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_t seed, double* evals,
               double2* evecs, int32_t* steps, bool real_values, hipStream_t s);
 
+// (z_{c,s} - H_c) x = y_c for S shifts per chain (z_host: [C][S] complex,
+// every Im z != 0), one Krylov space per chain.  y: one entry per batch row.
+// X: chain c's block starts at row_base[c]*S, shift s at offset s*n_c (the
+// eigenvector layout with nev = S).  resid_host [C][S]: relative residuals.
+// Returns IMPED_ERR_NOCONV when some shift stays above tol after max_steps.
+int run_solve(const KrylovBatch& kb, const double2* y, int S, const double* z_host, double tol, int max_steps,
+              double2* X, int32_t* steps_host, double* resid_host, hipStream_t s);
+
 // Continued fractions of all chains on the omega grid plus the Boltzmann
 // average over each spectrum's chains (host arrays are uploaded).
 void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t* kout, const double* energy,
-            const double* norm2, const int32_t* sign, const int32_t* zflag, double kT, const int32_t* spec_id,
+            const double* norm2, const int32_t* sign, const int32_t* zflag, const double* kT, const int32_t* spec_id,
             int n_spec, const double* omega, int n_w, double delta, double* g_out, double* spec_out, hipStream_t s);
 
 // out vector o = op[out_model[o]] applied to source vector out_src[o].
@@ -42,6 +50,10 @@ void vector_apply(OpsView ops, int W, const std::vector<int64_t>& src_off, const
                   const double2* src_amps, int n_out, const int32_t* out_src, const int32_t* out_model,
                   double weight_min, bool normalize, std::vector<int64_t>& out_off, DevBuf& out_keys,
                   DevBuf& out_amps, DevBuf& out_norm2, hipStream_t s, int n_orb);
+
+// Entries of every vector of a set with |amp|^2 >= weight_min, in stored order.
+void prune_vec(int W, const std::vector<int64_t>& off, const uint64_t* keys, const double2* amps, double weight_min,
+               std::vector<int64_t>& out_off, DevBuf& out_keys, DevBuf& out_amps, hipStream_t s);
 
 // Eigenvector components with |v_i|^2 >= weight_min, in basis order.
 void truncate_eigvecs(int W, const uint64_t* basis_keys, const double2* evecs, int nev,

@@ -10,6 +10,9 @@
 //   lowest Ritz pairs of the tridiagonal matrix by Sturm bisection and
 //   inverse iteration, checked every few steps against the residual
 //   |beta_m y_k[m-1]| <= tol * max(1, |lambda_k|).
+// * solve_kernel: the resolvent solves of getRIXSmap (spectra.py:1066-1080)
+//   for all incoming energies of a chain from one Krylov space (shifted
+//   Lanczos/FOM with full reorthogonalization), residual-checked per shift.
 // * cf_kernel: the continued fraction of getGreen (spectra.py:677-680) on the
 //   omega grid for every chain, fused with the Boltzmann weights of
 //   thermal_average (average.py:11-45) and the sum over transition operators
@@ -665,6 +668,205 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ shifted solves
+//
+// (z_s - H) x_s = y for S complex shifts z_s of one chain (the incoming
+// energies of getRIXSmap, spectra.py:1066-1080) share one Krylov space:
+// Lanczos with full reorthogonalization from y/|y| gives H V = V T + b v e^T,
+// and the Galerkin solution x_s = |y| V (z_s - T)^-1 e_1 has the residual
+// |y| b |e_m^T (z_s - T)^-1 e_1|, so one SpMV per step serves every shift
+// and the residual of each is known without forming x_s.
+
+struct SolveArgs {
+  const int64_t* row_base;
+  const int64_t* chain_n;
+  const int64_t* rowptr;
+  const int32_t* col;
+  const double2* val;
+  const double2* y;    // right-hand side of every chain, one entry per row
+  const double2* z;    // [C][S] shifts
+  int S;
+  int max_steps;
+  double tol;          // relative residual |y - (z - H) x| / |y|
+  double2* vstore;     // per chain: min(max_steps, n) * n Krylov vectors at voff[c]
+  const int64_t* voff;
+  double2* wbuf;       // per chain: n at row_base
+  double* ab;          // [C][2 * max_steps]: alpha then beta
+  double2* cz;         // [C][S][2 * max_steps]: pivots, then |y| (z - T)^-1 e_1
+  int32_t* steps;
+  double* resid;       // [C][S]
+  int32_t* conv;
+  const int32_t* chain_list;
+  int dgks;
+  int check_every;
+};
+
+// Last entry of (z - T_m)^-1 e_1 by downward elimination; the pivots
+// d_i = z - a_i - b_{i-1}^2 / d_{i-1} keep |Im d_i| >= |Im z| > 0.
+__device__ double2 tri_shift_last(const double* a, const double* b, int m, double2 z) {
+  double2 d = make_double2(z.x - a[0], z.y);
+  double2 f = make_double2(1.0, 0.0);
+  for (int i = 1; i < m; ++i) {
+    const double2 r = crecip(d);
+    const double2 fr = cmul(f, r);
+    f = make_double2(fr.x * b[i - 1], fr.y * b[i - 1]);
+    const double bb = b[i - 1] * b[i - 1];
+    d = make_double2(z.x - a[i] - bb * r.x, z.y - bb * r.y);
+  }
+  return cmul(f, crecip(d));
+}
+
+__device__ __forceinline__ unsigned long long res_bits(double r) {
+  // non-negative doubles order like their bit patterns; NaN sorts above inf
+  return (unsigned long long)__double_as_longlong(r != r ? __longlong_as_double(0x7ff8000000000000ll) : fabs(r));
+}
+
+template <int BT>
+__global__ __launch_bounds__(BT) void solve_kernel(SolveArgs a) {
+  __shared__ double red[BT / 64];
+  __shared__ unsigned long long s_max;
+  extern __shared__ double2 cl[];  // max_steps + 2 coefficients
+  const int c = a.chain_list[blockIdx.x];
+  const int64_t n = a.chain_n[c];
+  const int64_t base = a.row_base[c];
+  const int ms = a.max_steps;
+  const int S = a.S;
+  double* alpha = a.ab + (size_t)c * 2 * ms;
+  double* beta = alpha + ms;
+  double2* V = a.vstore + a.voff[c];
+  double2* w = a.wbuf + base;
+  double p = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
+    const double2 v = a.y[base + i];
+    p += v.x * v.x + v.y * v.y;
+  }
+  const double ynorm = sqrt(block_sum<BT>(p, red));
+  if (n == 0 || !(ynorm > 0.0)) {
+    for (int s = threadIdx.x; s < S; s += BT) a.resid[(size_t)c * S + s] = 0.0;
+    if (threadIdx.x == 0) {
+      a.steps[c] = 0;
+      a.conv[c] = 1;
+    }
+    return;
+  }
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
+    const double2 v = a.y[base + i];
+    V[i] = make_double2(v.x / ynorm, v.y / ynorm);
+  }
+  __syncthreads();
+  const int mmax = (int)(ms < n ? ms : n);
+  int m = 0, converged = 0;
+  for (int j = 0; j < mmax; ++j) {
+    const double2* vj = V + (size_t)j * n;
+    const double al = block_sum<BT>(spmv_t<BT, double2>(a.rowptr, a.col, a.val, base, n, vj, w), red);
+    const double bprev = j > 0 ? beta[j - 1] : 0.0;
+    double p0 = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += BT) {
+      double xr = w[i].x - al * vj[i].x, xi = w[i].y - al * vj[i].y;
+      if (j > 0) {
+        xr -= bprev * vj[i - n].x;
+        xi -= bprev * vj[i - n].y;
+      }
+      w[i] = make_double2(xr, xi);
+      p0 += xr * xr + xi * xi;
+    }
+    p0 = block_sum<BT>(p0, red);
+    dots_lds<BT, double2>(V, j + 1, n, w, cl, false);
+    double p1 = update_norm<BT, double2>(V, j + 1, n, w, cl, red);
+    if (!a.dgks || p1 < 0.5 * p0) {
+      dots_lds<BT, double2>(V, j + 1, n, w, cl, false);
+      p1 = update_norm<BT, double2>(V, j + 1, n, w, cl, red);
+    }
+    const double b = sqrt(p1);
+    if (threadIdx.x == 0) {
+      alpha[j] = al;
+      beta[j] = b;
+    }
+    m = j + 1;
+    // an invariant subspace (b ~ 0) holds the exact solutions
+    const bool last = (m == mmax) || b <= 1e-13 * (fabs(al) + bprev + 1e-300);
+    __syncthreads();
+    if (last || m % a.check_every == 0) {
+      if (threadIdx.x == 0) s_max = 0ull;
+      __syncthreads();
+      for (int s = threadIdx.x; s < S; s += BT) {
+        const double2 l = tri_shift_last(alpha, beta, m, a.z[(size_t)c * S + s]);
+        const double r = last && b <= 1e-13 * (fabs(al) + bprev + 1e-300) ? 0.0 : b * hypot(l.x, l.y);
+        a.resid[(size_t)c * S + s] = r;
+        atomicMax(&s_max, res_bits(r));
+      }
+      __syncthreads();
+      const double worst = __longlong_as_double((long long)s_max);
+      if (worst <= a.tol) {
+        converged = 1;
+        break;
+      }
+      if (last) break;
+    }
+    for (int64_t i = threadIdx.x; i < n; i += BT) V[(size_t)(j + 1) * n + i] = make_double2(w[i].x / b, w[i].y / b);
+    __syncthreads();
+  }
+  // coefficients |y| (z_s - T_m)^-1 e_1: elimination pivots D, right-hand
+  // side F, then back substitution into F
+  for (int s = threadIdx.x; s < S; s += BT) {
+    const double2 z = a.z[(size_t)c * S + s];
+    double2* D = a.cz + ((size_t)c * S + s) * 2 * ms;
+    double2* F = D + ms;
+    D[0] = make_double2(z.x - alpha[0], z.y);
+    F[0] = make_double2(ynorm, 0.0);
+    for (int i = 1; i < m; ++i) {
+      const double2 r = crecip(D[i - 1]);
+      const double2 fr = cmul(F[i - 1], r);
+      F[i] = make_double2(fr.x * beta[i - 1], fr.y * beta[i - 1]);
+      const double bb = beta[i - 1] * beta[i - 1];
+      D[i] = make_double2(z.x - alpha[i] - bb * r.x, z.y - bb * r.y);
+    }
+    F[m - 1] = cmul(F[m - 1], crecip(D[m - 1]));
+    for (int i = m - 2; i >= 0; --i) {
+      const double2 t = make_double2(F[i].x + beta[i] * F[i + 1].x, F[i].y + beta[i] * F[i + 1].y);
+      F[i] = cmul(t, crecip(D[i]));
+    }
+  }
+  if (threadIdx.x == 0) {
+    a.steps[c] = m;
+    a.conv[c] = converged;
+  }
+}
+
+// x_s[i] = sum_l coef_s[l] V_l[i] for every shift; eight shifts per pass so
+// each Krylov entry is read S/8 times (from cache after the first).
+__global__ __launch_bounds__(256) void solve_combine_kernel(const int64_t* row_base, const int64_t* chain_n,
+                                                            const int32_t* chain_list, const double2* vstore,
+                                                            const int64_t* voff, const double2* cz,
+                                                            const int32_t* steps, int S, int ms, double2* X) {
+  const int c = chain_list[blockIdx.y];
+  const int64_t n = chain_n[c];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int m = steps[c];
+  const double2* V = vstore + voff[c];
+  double2* Xc = X + row_base[c] * S;
+  for (int s0 = 0; s0 < S; s0 += 8) {
+    double ax[8], ay[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) ax[u] = ay[u] = 0.0;
+    for (int l = 0; l < m; ++l) {
+      const double2 v = V[(size_t)l * n + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (s0 + u < S) {
+          const double2 k = cz[((size_t)c * S + s0 + u) * 2 * ms + ms + l];
+          ax[u] += k.x * v.x - k.y * v.y;
+          ay[u] += k.x * v.y + k.y * v.x;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (s0 + u < S) Xc[(size_t)(s0 + u) * n + i] = make_double2(ax[u], ay[u]);
+  }
+}
+
 // ------------------------------------------------------------------ continued fraction
 
 struct CFArgs {
@@ -676,7 +878,7 @@ struct CFArgs {
   const double* norm2;
   const int32_t* sign;
   const int32_t* zflag;  // 1 on exactly one chain per thermal state of a spectrum
-  double kT;
+  const double* kT;               // per spectrum
   const int32_t* spec_chain_off;  // chains of spectrum q: spec_chains[off[q] .. off[q+1])
   const int32_t* spec_chains;
   const double* omega;
@@ -694,11 +896,12 @@ __global__ __launch_bounds__(256) void cf_kernel(CFArgs a) {
   const double om = a.omega[iw];
   const int t0 = a.spec_chain_off[q], t1 = a.spec_chain_off[q + 1];
   // Boltzmann weights exp(-(e - e_min)/kT) / Z over the spectrum's states
+  const double kT = a.kT[q];
   double emin = INFINITY;
   for (int t = t0; t < t1; ++t) emin = fmin(emin, a.energy[a.spec_chains[t]]);
   double Z = 0.0;
   for (int t = t0; t < t1; ++t)
-    if (a.zflag[a.spec_chains[t]]) Z += exp(-(a.energy[a.spec_chains[t]] - emin) / a.kT);
+    if (a.zflag[a.spec_chains[t]]) Z += exp(-(a.energy[a.spec_chains[t]] - emin) / kT);
   double acc = 0.0;
   for (int t = t0; t < t1; ++t) {
     const int c = a.spec_chains[t];
@@ -724,18 +927,452 @@ __global__ __launch_bounds__(256) void cf_kernel(CFArgs a) {
     const double f = sg * a.norm2[c];
     const double2 gs = make_double2(f * g.x, f * g.y);
     if (a.g_out) a.g_out[(size_t)c * a.n_w + iw] = gs;
-    acc += exp(-(a.energy[c] - emin) / a.kT) * (-gs.y);
+    acc += exp(-(a.energy[c] - emin) / kT) * (-gs.y);
   }
   a.spec_out[(size_t)q * a.n_w + iw] = Z > 0.0 ? acc / Z : 0.0;
+}
+
+// ------------------------------------------------------------------ flat (multi-workgroup) Krylov
+//
+// For sectors too large for one workgroup (10^5..10^7 states): every
+// Lanczos step is a short sequence of grid-wide launches on one chain, with
+// alpha/beta kept on the device (no host round trip per step):
+//   spmv (+ per-block Re<v, Hv>) -> reduce -> three-term update (+ |w|^2)
+//   -> reduce (beta = sqrt) -> [Gram-Schmidt: dots -> reduce -> update]
+//   -> scale into the next Krylov vector.
+// The SpMV gives LPR lanes to a row (the autotuned launch shape).
+
+constexpr int FT = 256;  // threads per flat block
+
+template <int LPR, class T>
+__global__ __launch_bounds__(FT) void flat_spmv_kernel(const int64_t* rowptr, const int32_t* col, const double2* val,
+                                                       int64_t base, int64_t n, const T* x, T* y, double* partial) {
+  __shared__ double red[FT / 64];
+  const int g = threadIdx.x / LPR, r = threadIdx.x % LPR;
+  const int64_t row = (int64_t)blockIdx.x * (FT / LPR) + g;
+  double ax = 0.0, ay = 0.0;
+  if (row < n) {
+    const int64_t e0 = rowptr[base + row], e1 = rowptr[base + row + 1];
+    for (int64_t e = e0 + r; e < e1; e += LPR) {
+      const double2 h = val[e];
+      const T v = x[col[e]];
+      if constexpr (is_real_v<T>) {
+        ax += h.x * re_(v);
+      } else {
+        ax += h.x * re_(v) - h.y * im_(v);
+        ay += h.x * im_(v) + h.y * re_(v);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) {
+    ax += __shfl_xor(ax, off);
+    if constexpr (!is_real_v<T>) ay += __shfl_xor(ay, off);
+  }
+  double pa = 0.0;
+  if (row < n && r == 0) {
+    y[row] = mk<T>(ax, ay);
+    pa = ax * re_(x[row]) + ay * im_(x[row]);
+  }
+  pa = block_sum<FT>(pa, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = pa;
+}
+
+// out[k] = sum of partial[k*stride .. k*stride + count) in a fixed order;
+// mode 1 takes the square root (beta), mode 2 stores into alpha and adds
+// the value to an accumulator slot.
+__global__ __launch_bounds__(FT) void flat_reduce_kernel(const double* partial, int64_t count, int64_t stride,
+                                                         double* out, int sqrt_mode) {
+  __shared__ double red[FT / 64];
+  const int k = blockIdx.x;
+  double p = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += FT) p += partial[(int64_t)k * stride + i];
+  p = block_sum<FT>(p, red);
+  if (threadIdx.x == 0) out[k] = sqrt_mode ? sqrt(p) : p;
+}
+
+// w = w - alpha v - beta_prev v_prev (beta_prev may be null), partial |w|^2.
+template <class T>
+__global__ __launch_bounds__(FT) void flat_update_kernel(T* w, const T* v, const T* vprev, const double* alpha,
+                                                         const double* beta_prev, int64_t n, double* partial) {
+  __shared__ double red[FT / 64];
+  const double al = *alpha;
+  const double bp = beta_prev ? *beta_prev : 0.0;
+  double p = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * FT + threadIdx.x; i < n; i += (int64_t)gridDim.x * FT) {
+    double xr = re_(w[i]) - al * re_(v[i]);
+    double xi = im_(w[i]) - al * im_(v[i]);
+    if (vprev) {
+      xr -= bp * re_(vprev[i]);
+      xi -= bp * im_(vprev[i]);
+    }
+    w[i] = mk<T>(xr, xi);
+    p += xr * xr + xi * xi;
+  }
+  p = block_sum<FT>(p, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = p;
+}
+
+// dst = src / beta; a zero beta flags a breakdown at step `step`.
+template <class T>
+__global__ __launch_bounds__(FT) void flat_scale_kernel(const T* src, T* dst, const double* beta, int64_t n,
+                                                        int32_t* status, int32_t* kout, int step) {
+  const double b = *beta;
+  if (b == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *status == IMPED_OK) {
+      *status = IMPED_ERR_BREAKDOWN;
+      *kout = step;
+    }
+    return;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * FT + threadIdx.x; i < n; i += (int64_t)gridDim.x * FT)
+    dst[i] = mk<T>(re_(src[i]) / b, im_(src[i]) / b);
+}
+
+// Gram-Schmidt coefficients: partial[l][blk] = sum over this block's rows of
+// conj(V_l) w, for l in [0, m).  Rows are split into grid.x chunks; each
+// block keeps 8 coefficients in flight.
+template <class T>
+__global__ __launch_bounds__(FT) void flat_dots_kernel(const T* V, int m, int64_t n, const T* w, double2* partial,
+                                                       int64_t chunk) {
+  __shared__ double red[FT / 64];
+  const int64_t i0 = (int64_t)blockIdx.x * chunk;
+  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+  for (int l0 = 0; l0 < m; l0 += 8) {
+    double cx[8], cy[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cx[u] = cy[u] = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += FT) {
+      const T b = w[i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (l0 + u < m) {
+          const T a = V[(size_t)(l0 + u) * n + i];
+          cx[u] += re_(a) * re_(b) + im_(a) * im_(b);
+          if constexpr (!is_real_v<T>) cy[u] += re_(a) * im_(b) - im_(a) * re_(b);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (l0 + u >= m) break;
+      const double x = block_sum<FT>(cx[u], red);
+      const double y = is_real_v<T> ? 0.0 : block_sum<FT>(cy[u], red);
+      if (threadIdx.x == 0) partial[(size_t)(l0 + u) * gridDim.x + blockIdx.x] = make_double2(x, y);
+    }
+  }
+}
+
+__global__ __launch_bounds__(FT) void flat_reduce_c_kernel(const double2* partial, int nb, double2* coef) {
+  __shared__ double red[FT / 64];
+  const int l = blockIdx.x;
+  double x = 0.0, y = 0.0;
+  for (int i = threadIdx.x; i < nb; i += FT) {
+    x += partial[(size_t)l * nb + i].x;
+    y += partial[(size_t)l * nb + i].y;
+  }
+  x = block_sum<FT>(x, red);
+  y = block_sum<FT>(y, red);
+  if (threadIdx.x == 0) coef[l] = make_double2(x, y);
+}
+
+// w -= sum_l coef[l] V_l, partial |w|^2 per block.
+template <class T>
+__global__ __launch_bounds__(FT) void flat_gs_update_kernel(const T* V, int m, int64_t n, T* w, const double2* coef,
+                                                            double* partial) {
+  __shared__ double red[FT / 64];
+  double p = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * FT + threadIdx.x; i < n; i += (int64_t)gridDim.x * FT) {
+    double wx = re_(w[i]), wy = im_(w[i]);
+    for (int l = 0; l < m; ++l) {
+      const double2 c = coef[l];
+      const T a = V[(size_t)l * n + i];
+      if constexpr (is_real_v<T>) {
+        wx -= re_(a) * c.x;
+      } else {
+        wx -= re_(a) * c.x - im_(a) * c.y;
+        wy -= re_(a) * c.y + im_(a) * c.x;
+      }
+    }
+    w[i] = mk<T>(wx, wy);
+    p += wx * wx + wy * wy;
+  }
+  p = block_sum<FT>(p, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = p;
+}
+
+// Ritz check of T_m (one workgroup): eigenvalues, inverse-iteration vectors,
+// residuals |beta_m y_k[m-1]| against tol; flags convergence.
+__global__ __launch_bounds__(FT) void flat_ritz_kernel(const double* alpha, const double* beta, int m, int nev,
+                                                       double tol, int ms, double* evals, double* Y, double* tw,
+                                                       int* tpiv, int32_t* converged, uint64_t seed) {
+  __shared__ double s_lam[MAX_NEV], s_lo[MAX_NEV], s_hi[MAX_NEV];
+  __shared__ int s_first[MAX_NEV];
+  __shared__ int s_flag, s_done;
+  const int ne = nev < m ? nev : m;
+  ritz_values<FT>(alpha, beta, m, ne, s_lam, s_lo, s_hi, s_first, &s_flag);
+  if (threadIdx.x == 0) s_done = 1;
+  __syncthreads();
+  if ((int)threadIdx.x < ne) {
+    const int k = threadIdx.x;
+    double* y = Y + (size_t)k * ms;
+    double* t = tw + (size_t)k * 5 * ms;
+    tri_inverse_iteration(alpha, beta, m, s_lam[k], y, t, t + ms, t + 2 * ms, t + 3 * ms, tpiv + (size_t)k * ms,
+                          seed + 7919 * (uint64_t)k);
+    evals[k] = s_lam[k];
+    if (!(fabs(beta[m - 1] * y[m - 1]) <= tol * fmax(1.0, fabs(s_lam[k])))) atomicAnd(&s_done, 0);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // orthogonalize inverse-iteration vectors of clustered eigenvalues
+    for (int k = 1; k < ne; ++k) {
+      double* yk = Y + (size_t)k * ms;
+      for (int q = 0; q < k; ++q) {
+        if (fabs(evals[k] - evals[q]) > 1e-3 * fmax(1.0, fabs(evals[k]))) continue;
+        const double* yq = Y + (size_t)q * ms;
+        double d = 0.0, nn = 0.0;
+        for (int i = 0; i < m; ++i) d += yq[i] * yk[i];
+        for (int i = 0; i < m; ++i) {
+          yk[i] -= d * yq[i];
+          nn += yk[i] * yk[i];
+        }
+        nn = sqrt(nn);
+        for (int i = 0; i < m; ++i) yk[i] /= nn;
+      }
+    }
+    *converged = s_done;
+  }
+}
+
+// X_k = sum_l y_k[l] V_l for k < ne (complex output), partial |X_k|^2.
+template <class T>
+__global__ __launch_bounds__(FT) void flat_ritz_vectors_kernel(const T* V, int m, int64_t n, const double* Y, int ms,
+                                                               int ne, double2* X, double* partial) {
+  __shared__ double red[FT / 64];
+  for (int k = 0; k < ne; ++k) {
+    const double* y = Y + (size_t)k * ms;
+    double p = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * FT + threadIdx.x; i < n; i += (int64_t)gridDim.x * FT) {
+      double xr = 0.0, xi = 0.0;
+      for (int l = 0; l < m; ++l) {
+        const T v = V[(size_t)l * n + i];
+        xr += y[l] * re_(v);
+        xi += y[l] * im_(v);
+      }
+      X[(size_t)k * n + i] = make_double2(xr, xi);
+      p += xr * xr + xi * xi;
+    }
+    p = block_sum<FT>(p, red);
+    if (threadIdx.x == 0) partial[(size_t)k * gridDim.x + blockIdx.x] = p;
+  }
+}
+
+__global__ __launch_bounds__(FT) void flat_normalize_kernel(double2* X, int64_t n, const double* norms2, int ne) {
+  for (int k = 0; k < ne; ++k) {
+    const double nn = sqrt(norms2[k]);
+    for (int64_t i = (int64_t)blockIdx.x * FT + threadIdx.x; i < n; i += (int64_t)gridDim.x * FT)
+      X[(size_t)k * n + i] = make_double2(X[(size_t)k * n + i].x / nn, X[(size_t)k * n + i].y / nn);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(FT) void flat_random_kernel(T* v, int64_t n, uint64_t seed, int chain, double* partial) {
+  __shared__ double red[FT / 64];
+  double p = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * FT + threadIdx.x; i < n; i += (int64_t)gridDim.x * FT) {
+    const uint64_t h = seed * 0x100000001B3ull + (uint64_t)chain * 0x9E3779B97F4A7C15ull + 2 * (uint64_t)i;
+    const double xr = u01(h) - 0.5, xi = is_real_v<T> ? 0.0 : u01(h + 1) - 0.5;
+    v[i] = mk<T>(xr, xi);
+    p += xr * xr + xi * xi;
+  }
+  p = block_sum<FT>(p, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = p;
+}
+
+template <class T>
+__global__ __launch_bounds__(FT) void flat_copy_kernel(const double2* src, T* dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * FT + threadIdx.x; i < n; i += (int64_t)gridDim.x * FT)
+    dst[i] = mk<T>(src[i].x, src[i].y);
 }
 
 struct KScratch {
   DevBuf ws, vstore, coef, ab, yk, tw, tpiv, voff, conv, wbuf, clist;
   DevBuf cf_alpha_tmp, e, nrm, sg, wt, soff, sch, om, gout, sout;
+  DevBuf fpart, fcpart, fcoef, fscal;
+  DevBuf sz, scz, sres, ssteps, skt;
 };
 KScratch& kscratch() {
   static KScratch k;
   return k;
+}
+
+int64_t chain_base(const KrylovBatch& kb, int c) {
+  int64_t b = 0;
+  for (int i = 0; i < c; ++i) b += kb.h_n[i];
+  return b;
+}
+
+unsigned vec_blocks(int64_t n) { return (unsigned)std::min<int64_t>(std::max<int64_t>((n + FT - 1) / FT, 1), 2048); }
+
+template <class T>
+void launch_spmv(int lpr, const KrylovBatch& kb, int64_t base, int64_t n, const T* x, T* y, double* partial,
+                 unsigned* nb_out, hipStream_t s) {
+  const unsigned nb = (unsigned)((n + FT / lpr - 1) / (FT / lpr));
+  *nb_out = nb;
+  TimedLaunch tl("spmv", s);
+#define IMPED_SPMV(L) \
+  hipLaunchKernelGGL((flat_spmv_kernel<L, T>), dim3(nb), dim3(FT), 0, s, kb.rowptr, kb.col, kb.val, base, n, x, y, partial)
+  switch (lpr) {
+    case 4: IMPED_SPMV(4); break;
+    case 8: IMPED_SPMV(8); break;
+    case 16: IMPED_SPMV(16); break;
+    case 32: IMPED_SPMV(32); break;
+    default: IMPED_SPMV(64); break;
+  }
+#undef IMPED_SPMV
+  HIP_CHECK(hipGetLastError());
+}
+
+// Plain Lanczos of one large chain, everything on the device (no per-step
+// host synchronization).  alpha/beta/kout/status point at this chain's slots.
+void lanczos_flat_chain(const KrylovBatch& kb, int c, const double2* v0, int K, double* alpha, double* beta,
+                        int32_t* kout, int32_t* status, hipStream_t s) {
+  KScratch& sc = kscratch();
+  const int64_t n = kb.h_n[c], base = chain_base(kb, c);
+  const int Kc = (int)std::min<int64_t>(K, n);
+  const int lpr = (int)g_tuning.spmv_lpr;
+  sc.wbuf.reserve(3 * n * sizeof(double2));
+  double2* P[3] = {sc.wbuf.as<double2>(), sc.wbuf.as<double2>() + n, sc.wbuf.as<double2>() + 2 * n};
+  const unsigned nbv = vec_blocks(n);
+  sc.fpart.reserve(std::max<int64_t>((n + 3) / 4 + 1, 4096) * sizeof(double));
+  double* part = sc.fpart.as<double>();
+  const int32_t init[2] = {Kc, IMPED_OK};
+  HIP_CHECK(hipMemcpy(kout, &init[0], 4, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(status, &init[1], 4, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpyAsync(P[1], v0 + base, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+  int iv = 1, ip = 0, iw = 2;  // v_j, v_{j-1}, w
+  for (int j = 0; j < Kc; ++j) {
+    if (j > 0) {
+      // beta_{j-1} = |w|, v_j = w / beta_{j-1} into the v_{j-2} buffer
+      hipLaunchKernelGGL(flat_reduce_kernel, dim3(1), dim3(FT), 0, s, part, (int64_t)nbv, (int64_t)0, beta + j - 1, 1);
+      hipLaunchKernelGGL((flat_scale_kernel<double2>), dim3(nbv), dim3(FT), 0, s, P[iw], P[ip], beta + j - 1, n,
+                         status, kout, j);
+      const int nv = ip;
+      ip = iv;
+      iv = nv;
+    }
+    unsigned nbs = 0;
+    launch_spmv<double2>(lpr, kb, base, n, P[iv], P[iw], part, &nbs, s);
+    hipLaunchKernelGGL(flat_reduce_kernel, dim3(1), dim3(FT), 0, s, part, (int64_t)nbs, (int64_t)0, alpha + j, 0);
+    hipLaunchKernelGGL((flat_update_kernel<double2>), dim3(nbv), dim3(FT), 0, s, P[iw], P[iv],
+                       j > 0 ? P[ip] : (const double2*)nullptr, alpha + j, j > 0 ? beta + j - 1 : nullptr, n, part);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+// Lowest eigenpairs of one large chain: Lanczos with full reorthogonalization
+// by grid-wide Gram-Schmidt, host-controlled (one small read-back per step).
+template <class T>
+int eigen_flat_chain(const KrylovBatch& kb, int c, int nev, double tol, int max_steps, uint64_t seed, double* evals,
+                     double2* X, int32_t* steps_out, hipStream_t s) {
+  KScratch& sc = kscratch();
+  const int64_t n = kb.h_n[c], base = chain_base(kb, c);
+  const int mmax = (int)std::min<int64_t>(max_steps, n);
+  const int lpr = (int)g_tuning.spmv_lpr;
+  const unsigned nbv = vec_blocks(n);
+  sc.vstore.reserve((size_t)mmax * n * sizeof(T));
+  sc.wbuf.reserve(n * sizeof(T));
+  T* V = reinterpret_cast<T*>(sc.vstore.p);
+  T* w = reinterpret_cast<T*>(sc.wbuf.p);
+  const int64_t chunk = std::max<int64_t>(FT * 16, (n + 255) / 256);
+  const unsigned nbd = (unsigned)((n + chunk - 1) / chunk);
+  sc.fpart.reserve(std::max<int64_t>((n + 3) / 4 + 1, 4096) * sizeof(double));
+  sc.fcpart.reserve((size_t)(mmax + 1) * nbd * sizeof(double2));
+  sc.fcoef.reserve((size_t)(mmax + 1) * sizeof(double2));
+  sc.fscal.reserve(32 * sizeof(double));
+  sc.ab.reserve((size_t)2 * mmax * sizeof(double));
+  sc.yk.reserve((size_t)nev * mmax * sizeof(double));
+  sc.tw.reserve((size_t)nev * 5 * mmax * sizeof(double));
+  sc.tpiv.reserve((size_t)nev * mmax * sizeof(int));
+  sc.conv.reserve(4);
+  double* part = sc.fpart.as<double>();
+  double* scal = sc.fscal.as<double>();
+  double* alpha = sc.ab.as<double>();
+  double* beta = alpha + mmax;
+  hipLaunchKernelGGL((flat_random_kernel<T>), dim3(nbv), dim3(FT), 0, s, V, n, seed, c, part);
+  hipLaunchKernelGGL(flat_reduce_kernel, dim3(1), dim3(FT), 0, s, part, (int64_t)nbv, (int64_t)0, scal, 1);
+  hipLaunchKernelGGL((flat_scale_kernel<T>), dim3(nbv), dim3(FT), 0, s, V, V, scal, n, sc.conv.as<int32_t>(),
+                     sc.conv.as<int32_t>(), 0);
+  HIP_CHECK(hipGetLastError());
+  int m = 0, converged = 0;
+  double hb[4];
+  auto gs_pass = [&](int mm) {
+    hipLaunchKernelGGL((flat_dots_kernel<T>), dim3(nbd), dim3(FT), 0, s, V, mm, n, w, sc.fcpart.as<double2>(), chunk);
+    hipLaunchKernelGGL(flat_reduce_c_kernel, dim3(mm), dim3(FT), 0, s, sc.fcpart.as<double2>(), (int)nbd,
+                       sc.fcoef.as<double2>());
+    hipLaunchKernelGGL((flat_gs_update_kernel<T>), dim3(nbv), dim3(FT), 0, s, V, mm, n, w, sc.fcoef.as<double2>(),
+                       part);
+    hipLaunchKernelGGL(flat_reduce_kernel, dim3(1), dim3(FT), 0, s, part, (int64_t)nbv, (int64_t)0, scal + 2, 0);
+    HIP_CHECK(hipGetLastError());
+  };
+  for (int j = 0; j < mmax; ++j) {
+    const T* vj = V + (size_t)j * n;
+    unsigned nbs = 0;
+    launch_spmv<T>(lpr, kb, base, n, vj, w, part, &nbs, s);
+    hipLaunchKernelGGL(flat_reduce_kernel, dim3(1), dim3(FT), 0, s, part, (int64_t)nbs, (int64_t)0, alpha + j, 0);
+    hipLaunchKernelGGL((flat_update_kernel<T>), dim3(nbv), dim3(FT), 0, s, w, vj, j > 0 ? vj - n : (const T*)nullptr,
+                       alpha + j, j > 0 ? beta + j - 1 : nullptr, n, part);
+    hipLaunchKernelGGL(flat_reduce_kernel, dim3(1), dim3(FT), 0, s, part, (int64_t)nbv, (int64_t)0, scal + 1, 0);
+    gs_pass(j + 1);
+    HIP_CHECK(hipMemcpyAsync(hb, scal, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    double p1 = hb[2];
+    if (!g_tuning.eigen_dgks || p1 < 0.5 * hb[1]) {
+      gs_pass(j + 1);
+      HIP_CHECK(hipMemcpyAsync(&p1, scal + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+    }
+    double al = 0.0;
+    HIP_CHECK(hipMemcpy(&al, alpha + j, sizeof(double), hipMemcpyDeviceToHost));
+    const double b = std::sqrt(p1);
+    const double bprev = j > 0 ? hb[3] : 0.0;
+    hb[3] = b;
+    HIP_CHECK(hipMemcpy(beta + j, &b, sizeof(double), hipMemcpyHostToDevice));
+    m = j + 1;
+    const bool last = (m == mmax) || b <= 1e-13 * (std::fabs(al) + bprev + 1e-300);
+    if (last || (m >= nev && m % g_tuning.eigen_check_every == 0)) {
+      hipLaunchKernelGGL(flat_ritz_kernel, dim3(1), dim3(FT), 0, s, alpha, beta, m, nev, tol, mmax,
+                         evals, sc.yk.as<double>(), sc.tw.as<double>(), sc.tpiv.as<int>(), sc.conv.as<int32_t>(),
+                         seed);
+      HIP_CHECK(hipGetLastError());
+      int32_t conv = 0;
+      HIP_CHECK(hipMemcpyAsync(&conv, sc.conv.p, 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      if (conv || last) {
+        converged = conv;
+        break;
+      }
+    }
+    HIP_CHECK(hipMemcpy(scal + 3, &b, sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL((flat_scale_kernel<T>), dim3(nbv), dim3(FT), 0, s, w, V + (size_t)(j + 1) * n, scal + 3, n,
+                       sc.conv.as<int32_t>(), sc.conv.as<int32_t>(), 0);
+    HIP_CHECK(hipGetLastError());
+  }
+  const int ne = std::min(nev, m);
+  const unsigned nbr = vec_blocks(n);
+  hipLaunchKernelGGL((flat_ritz_vectors_kernel<T>), dim3(nbr), dim3(FT), 0, s, V, m, n, sc.yk.as<double>(), mmax, ne,
+                     X, part);
+  hipLaunchKernelGGL(flat_reduce_kernel, dim3(ne), dim3(FT), 0, s, part, (int64_t)nbr, (int64_t)nbr, scal + 4, 0);
+  hipLaunchKernelGGL(flat_normalize_kernel, dim3(nbr), dim3(FT), 0, s, X, n, scal + 4, ne);
+  HIP_CHECK(hipGetLastError());
+  if (ne < nev) {
+    HIP_CHECK(hipMemsetAsync(X + (size_t)ne * n, 0, (size_t)(nev - ne) * n * sizeof(double2), s));
+    std::vector<double> nanv(nev - ne, NAN);
+    HIP_CHECK(hipMemcpyAsync(evals + ne, nanv.data(), nanv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  HIP_CHECK(hipMemcpyAsync(steps_out, &m, 4, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  return converged ? IMPED_OK : IMPED_ERR_NOCONV;
 }
 
 }  // namespace
@@ -767,13 +1404,20 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
   a.kout = kout;
   a.status = status;
   if (kb.C == 0) return;
-  // chains bucketed by dimension: one wave, 256 or 1024 threads per chain
-  std::vector<int32_t> lists[3];
+  // chains bucketed by dimension: one wave, 256 or 1024 threads per chain;
+  // the largest (plain recurrence) run grid-wide, one chain at a time
+  std::vector<int32_t> lists[3], flat;
   for (int c = 0; c < kb.C; ++c) {
     const int64_t n = kb.h_n[c];
+    if (!reorth && n > g_tuning.flat_min_n) {
+      flat.push_back(c);
+      continue;
+    }
     const int bkt = n <= g_tuning.lanczos_wave_max_n ? 0 : (n <= g_tuning.lanczos_wg256_max_n ? 1 : 2);
     lists[bkt].push_back(c);
   }
+  for (int c : flat)
+    lanczos_flat_chain(kb, c, v0, K, alpha + (size_t)c * K, beta + (size_t)c * K, kout + c, status + c, s);
   std::vector<int32_t> all;
   for (auto& l : lists) all.insert(all.end(), l.begin(), l.end());
   h2d(sc.clist, all.data(), all.size(), s);
@@ -805,12 +1449,18 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
     set_error("eigensolver: nev must be <= 16 and max_steps in [1, 4000]");
     return IMPED_ERR_INVALID;
   }
+  const bool real = real_values && g_tuning.eigen_real;
+  // chains above flat_min_n are solved grid-wide, one at a time, after the
+  // one-workgroup-per-chain launch of the others
+  std::vector<int32_t> order, large;
+  for (int c = 0; c < C; ++c) (kb.h_n[c] > g_tuning.flat_min_n ? large : order).push_back(c);
   std::vector<int64_t> voff(C + 1, 0);
   int64_t R = 0;
   for (int c = 0; c < C; ++c) {
-    const int64_t mm = std::min<int64_t>(max_steps, kb.h_n[c]);
+    const bool small = kb.h_n[c] <= g_tuning.flat_min_n;
+    const int64_t mm = small ? std::min<int64_t>(max_steps, kb.h_n[c]) : 0;
     voff[c + 1] = voff[c] + mm * kb.h_n[c];
-    R += kb.h_n[c];
+    R += small ? kb.h_n[c] : 0;
   }
   h2d(sc.voff, voff.data(), C + 1, s);
   sc.vstore.reserve(std::max<int64_t>(voff[C], 1) * 16);
@@ -846,16 +1496,13 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   a.dgks = (int)g_tuning.eigen_dgks;
   a.check_every = (int)std::max<int64_t>(1, g_tuning.eigen_check_every);
   // one launch, largest chains first so the longest workgroups start first
-  std::vector<int32_t> order(C);
-  for (int c = 0; c < C; ++c) order[c] = c;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kb.h_n[x] > kb.h_n[y]; });
-  h2d(sc.clist, order.data(), order.size(), s);
+  if (!order.empty()) h2d(sc.clist, order.data(), order.size(), s);
   a.chain_list = sc.clist.as<int32_t>();
-  const bool real = real_values && g_tuning.eigen_real;
   const size_t shm = (size_t)(max_steps + 2) * sizeof(double2);
-  {
+  if (!order.empty()) {
     TimedLaunch tl("eigen", s);
-    const dim3 g((unsigned)C);
+    const dim3 g((unsigned)order.size());
     if (g_tuning.eigen_bt == 256) {
       if (real) hipLaunchKernelGGL((eigen_kernel<256, double>), g, dim3(256), shm, s, a);
       else hipLaunchKernelGGL((eigen_kernel<256, double2>), g, dim3(256), shm, s, a);
@@ -868,8 +1515,103 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
     }
     HIP_CHECK(hipGetLastError());
   }
+  std::vector<int32_t> conv(C, 1);
+  d2h(conv.data(), sc.conv.p, C, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+  int rc = IMPED_OK;
+  for (int c : order)
+    if (!conv[c]) rc = IMPED_ERR_NOCONV;
+  for (int c : large) {
+    const int64_t base = chain_base(kb, c);
+    double2* X = evecs + base * nev;
+    TimedLaunch tl("eigen_flat", s);
+    const int r = real ? eigen_flat_chain<double>(kb, c, nev, tol, max_steps, seed, evals + (size_t)c * nev, X,
+                                                  steps + c, s)
+                       : eigen_flat_chain<double2>(kb, c, nev, tol, max_steps, seed, evals + (size_t)c * nev, X,
+                                                   steps + c, s);
+    if (r != IMPED_OK) rc = r;
+  }
+  return rc;
+}
+
+int run_solve(const KrylovBatch& kb, const double2* y, int S, const double* z_host, double tol, int max_steps,
+              double2* X, int32_t* steps_host, double* resid_host, hipStream_t s) {
+  KScratch& sc = kscratch();
+  const int C = kb.C;
+  if (S < 1 || max_steps < 1 || max_steps > 4000) {
+    set_error("shifted solve: need at least one shift and max_steps in [1, 4000]");
+    return IMPED_ERR_INVALID;
+  }
+  for (size_t q = 0; q < (size_t)C * S; ++q)
+    if (!(z_host[2 * q + 1] != 0.0) || !std::isfinite(z_host[2 * q]) || !std::isfinite(z_host[2 * q + 1])) {
+      set_error("shifted solve: every shift needs a finite, nonzero imaginary part");
+      return IMPED_ERR_INVALID;
+    }
+  if (C == 0) return IMPED_OK;
+  std::vector<int64_t> voff(C + 1, 0);
+  int64_t R = 0, nmax = 1;
+  for (int c = 0; c < C; ++c) {
+    voff[c + 1] = voff[c] + std::min<int64_t>(max_steps, kb.h_n[c]) * kb.h_n[c];
+    R += kb.h_n[c];
+    nmax = std::max(nmax, kb.h_n[c]);
+  }
+  h2d(sc.voff, voff.data(), C + 1, s);
+  h2d(sc.sz, z_host, (size_t)C * S * 2, s);
+  sc.vstore.reserve(std::max<int64_t>(voff[C], 1) * 16);
+  sc.wbuf.reserve(std::max<int64_t>(R, 1) * 16);
+  sc.ab.reserve((size_t)C * 2 * max_steps * 8);
+  sc.scz.reserve((size_t)C * S * 2 * max_steps * 16);
+  sc.sres.reserve((size_t)C * S * 8);
+  sc.ssteps.reserve((size_t)C * 4);
+  sc.conv.reserve((size_t)C * 4);
+  std::vector<int32_t> order(C);
+  for (int c = 0; c < C; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return kb.h_n[p] > kb.h_n[q]; });
+  h2d(sc.clist, order.data(), C, s);
+  SolveArgs a{};
+  a.row_base = kb.row_base;
+  a.chain_n = kb.chain_n;
+  a.rowptr = kb.rowptr;
+  a.col = kb.col;
+  a.val = kb.val;
+  a.y = y;
+  a.z = sc.sz.as<double2>();
+  a.S = S;
+  a.max_steps = max_steps;
+  a.tol = tol;
+  a.vstore = sc.vstore.as<double2>();
+  a.voff = sc.voff.as<int64_t>();
+  a.wbuf = sc.wbuf.as<double2>();
+  a.ab = sc.ab.as<double>();
+  a.cz = sc.scz.as<double2>();
+  a.steps = sc.ssteps.as<int32_t>();
+  a.resid = sc.sres.as<double>();
+  a.conv = sc.conv.as<int32_t>();
+  a.chain_list = sc.clist.as<int32_t>();
+  a.dgks = (int)g_tuning.eigen_dgks;
+  a.check_every = (int)std::max<int64_t>(1, g_tuning.eigen_check_every);
+  const size_t shm = (size_t)(max_steps + 2) * sizeof(double2);
+  {
+    TimedLaunch tl("solve", s);
+    if (g_tuning.eigen_bt == 256)
+      hipLaunchKernelGGL(solve_kernel<256>, dim3(C), dim3(256), shm, s, a);
+    else if (g_tuning.eigen_bt == 512)
+      hipLaunchKernelGGL(solve_kernel<512>, dim3(C), dim3(512), shm, s, a);
+    else
+      hipLaunchKernelGGL(solve_kernel<1024>, dim3(C), dim3(1024), shm, s, a);
+    HIP_CHECK(hipGetLastError());
+  }
+  {
+    TimedLaunch tl("solve_combine", s);
+    hipLaunchKernelGGL(solve_combine_kernel, dim3(blocks_for(nmax, 256), C), dim3(256), 0, s, kb.row_base,
+                       kb.chain_n, sc.clist.as<int32_t>(), sc.vstore.as<double2>(), sc.voff.as<int64_t>(),
+                       sc.scz.as<double2>(), sc.ssteps.as<int32_t>(), S, max_steps, X);
+    HIP_CHECK(hipGetLastError());
+  }
   std::vector<int32_t> conv(C);
   d2h(conv.data(), sc.conv.p, C, s);
+  if (steps_host) d2h(steps_host, sc.ssteps.p, C, s);
+  if (resid_host) d2h(resid_host, sc.sres.p, (size_t)C * S, s);
   HIP_CHECK(hipStreamSynchronize(s));
   for (int c = 0; c < C; ++c)
     if (!conv[c]) return IMPED_ERR_NOCONV;
@@ -877,7 +1619,7 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
 }
 
 void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t* kout, const double* energy,
-            const double* norm2, const int32_t* sign, const int32_t* zflag, double kT, const int32_t* spec_id,
+            const double* norm2, const int32_t* sign, const int32_t* zflag, const double* kT, const int32_t* spec_id,
             int n_spec, const double* omega, int n_w, double delta, double* g_out, double* spec_out, hipStream_t s) {
   KScratch& sc = kscratch();
   // chains grouped by spectrum, in chain order
@@ -896,10 +1638,11 @@ void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t
   h2d(sc.soff, off.data(), n_spec + 1, s);
   h2d(sc.sch, chains.data(), chains.size(), s);
   h2d(sc.om, omega, n_w, s);
+  h2d(sc.skt, kT, std::max(n_spec, 1), s);
   if (g_out) sc.gout.reserve((size_t)std::max(C, 1) * n_w * 16);
   sc.sout.reserve((size_t)std::max(n_spec, 1) * n_w * 8);
   CFArgs a{alpha, beta, kout, K, sc.e.as<double>(), sc.nrm.as<double>(), sc.sg.as<int32_t>(), sc.wt.as<int32_t>(),
-           kT, sc.soff.as<int32_t>(), sc.sch.as<int32_t>(), sc.om.as<double>(), n_w, delta,
+           sc.skt.as<double>(), sc.soff.as<int32_t>(), sc.sch.as<int32_t>(), sc.om.as<double>(), n_w, delta,
            g_out ? sc.gout.as<double2>() : nullptr, sc.sout.as<double>()};
   if (n_spec > 0 && n_w > 0) {
     TimedLaunch tl("cf", s);

@@ -1,0 +1,460 @@
+// Surrogate network kernels: an ensemble of E fully connected GELU networks
+// of identical shape, trained and evaluated together (blockIdx.z = member).
+//
+// * gemm_bf16_kernel: C[m][n] = sum_k A(m,k) B(n,k) on the matrix cores
+//   (v_mfma_f32_32x32x8 bf16, FP32 accumulation), 64x64 output tiles of four
+//   waves, 32-deep K slices staged through LDS; either operand may be stored
+//   k-contiguous or row-contiguous, so forward (X W^T), input gradient
+//   (dZ W) and weight gradient (dZ^T X) are one kernel.  Epilogues fuse the
+//   bias, GELU (pre-activation kept in FP32 for the backward pass) and the
+//   GELU derivative.
+// * mse_grad_kernel, bias_grad_kernel, adam_kernel, gather_kernel: loss
+//   gradient, bias gradients, the Adam update (FP32 master weights, BF16
+//   copies for the GEMMs) and minibatch assembly.
+#include <cmath>
+#include <memory>
+
+#include "common.h"
+
+namespace imped {
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// round-to-nearest-even float -> bfloat16 (NaN stays NaN)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// GELU(x) = x Phi(x) and its derivative Phi(x) + x phi(x)
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
+enum { EPI_BIAS_GELU = 0, EPI_BIAS = 1, EPI_DGELU = 2, EPI_F32 = 3 };
+
+struct GemmArgs {
+  const uint16_t* A;  // AK: A(m,k) = A[m*lda + k]; else A[k*lda + m]
+  int64_t lda, a_z;
+  const uint16_t* B;  // BK: B(n,k) = B[n*ldb + k]; else B[k*ldb + n]
+  int64_t ldb, b_z;
+  int M, N, K;
+  float* Cf;  // row-major [M][ldc]
+  uint16_t* Cb;
+  int64_t ldc, c_z;
+  const float* bias;  // [N] per member
+  int64_t bias_z;
+  const float* aux;  // EPI_DGELU: pre-activation, layout of C
+};
+
+constexpr int TM = 64, TN = 64, TK = 32, LK = TK + 8;
+
+// Stage a 64 x 32 slice (rows r0.., k0..) of an operand into tile[row][k],
+// zero outside [0, R) x [0, K).
+template <bool KC>
+__device__ __forceinline__ void stage(const uint16_t* src, int64_t ld, int R, int K, int r0, int k0,
+                                      uint16_t (*tile)[LK], int t) {
+  if constexpr (KC) {
+    const int r = t >> 2, kc = (t & 3) * 8;
+    const int gr = r0 + r, gk = k0 + kc;
+    uint16_t v[8];
+    const uint16_t* p = src + (int64_t)gr * ld + gk;
+    if (gr < R && gk + 8 <= K && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+      *reinterpret_cast<uint4*>(v) = *reinterpret_cast<const uint4*>(p);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = (gr < R && gk + i < K) ? src[(int64_t)gr * ld + gk + i] : (uint16_t)0;
+    }
+    *reinterpret_cast<uint4*>(&tile[r][kc]) = *reinterpret_cast<const uint4*>(v);
+  } else {
+    const int k = t >> 3, rc = (t & 7) * 8;
+    const int gk = k0 + k, gr = r0 + rc;
+    uint16_t v[8];
+    const uint16_t* p = src + (int64_t)gk * ld + gr;
+    if (gk < K && gr + 8 <= R && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+      *reinterpret_cast<uint4*>(v) = *reinterpret_cast<const uint4*>(p);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = (gk < K && gr + i < R) ? src[(int64_t)gk * ld + gr + i] : (uint16_t)0;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) tile[rc + i][k] = v[i];
+  }
+}
+
+template <bool AK, bool BK, int EPI>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
+  __shared__ __align__(16) uint16_t As[TM][LK];
+  __shared__ __align__(16) uint16_t Bs[TN][LK];
+  const int z = blockIdx.z;
+  const uint16_t* A = g.A + (int64_t)z * g.a_z;
+  const uint16_t* B = g.B + (int64_t)z * g.b_z;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int k0 = 0; k0 < g.K; k0 += TK) {
+    stage<AK>(A, g.lda, g.M, g.K, m0, k0, As, t);
+    stage<BK>(B, g.ldb, g.N, g.K, n0, k0, Bs, t);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 8) {
+      // lane l supplies row l%32 and k = kk + 4*(l/32) .. +3 of each operand
+      const int kq = kk + 4 * (lane >> 5);
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(&As[wm + (lane & 31)][kq]);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(&Bs[wn + (lane & 31)][kq]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // accumulator r of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32
+  const int64_t cz = (int64_t)z * g.c_z;
+  const int n = n0 + wn + (lane & 31);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int m = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (m >= g.M || n >= g.N) continue;
+    const int64_t o = cz + (int64_t)m * g.ldc + n;
+    const float v = acc[r];
+    if constexpr (EPI == EPI_BIAS_GELU) {
+      const float zv = v + g.bias[(int64_t)z * g.bias_z + n];
+      g.Cf[o] = zv;
+      g.Cb[o] = f2bf(gelu(zv));
+    } else if constexpr (EPI == EPI_BIAS) {
+      g.Cf[o] = v + g.bias[(int64_t)z * g.bias_z + n];
+    } else if constexpr (EPI == EPI_DGELU) {
+      g.Cb[o] = f2bf(v * gelu_grad(g.aux[o]));
+    } else {
+      g.Cf[o] = v;
+    }
+  }
+}
+
+__global__ void to_bf16_kernel(const float* x, int64_t n, uint16_t* y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+// minibatch rows idx[b] of the data set: inputs to BF16, targets as FP32
+__global__ void gather_kernel(const float* X, int dx, const float* Y, int dy, const int32_t* idx, int B,
+                              uint16_t* xb, float* yb) {
+  const int b = blockIdx.x;
+  if (b >= B) return;
+  const int64_t r = idx[b];
+  for (int j = threadIdx.x; j < dx; j += blockDim.x) xb[(int64_t)b * dx + j] = f2bf(X[r * dx + j]);
+  for (int j = threadIdx.x; j < dy; j += blockDim.x) yb[(int64_t)b * dy + j] = Y[r * dy + j];
+}
+
+// dY = 2 (Y - T) / (B N) in BF16 per member; loss[z] += sum (Y - T)^2 / (B N)
+__global__ __launch_bounds__(256) void mse_grad_kernel(const float* Y, const float* T, int64_t BN, uint16_t* dY,
+                                                       double* loss) {
+  __shared__ double red[4];
+  const int z = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double s = 0.0;
+  if (i < BN) {
+    const float d = Y[(int64_t)z * BN + i] - T[i];
+    dY[(int64_t)z * BN + i] = f2bf(2.0f * d / (float)BN);
+    s = (double)d * (double)d;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&loss[z], (red[0] + red[1] + red[2] + red[3]) / (double)BN);
+}
+
+// db[z][n] = sum_b dZ[z][b][n]
+__global__ void bias_grad_kernel(const uint16_t* dZ, int B, int N, int64_t dz_z, float* db, int64_t db_z) {
+  const int z = blockIdx.y;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.0f;
+  for (int b = 0; b < B; ++b) s += bf2f(dZ[(int64_t)z * dz_z + (int64_t)b * N + n]);
+  db[(int64_t)z * db_z + n] = s;
+}
+
+__global__ void adam_kernel(float* w, const float* g, float* m, float* v, uint16_t* wb, int64_t n, float lr, float b1,
+                            float b2, float eps, float bc1, float bc2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  w[i] = wi;
+  wb[i] = f2bf(wi);
+}
+
+template <bool AK, bool BK, int EPI>
+void gemm(const GemmArgs& g, int E, hipStream_t s) {
+  const dim3 grid(blocks_for(g.N, TN), blocks_for(g.M, TM), E);
+  hipLaunchKernelGGL((gemm_bf16_kernel<AK, BK, EPI>), grid, dim3(256), 0, s, g);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ network state
+
+struct Mlp {
+  int E = 0, L = 0, max_batch = 0;
+  std::vector<int> dims;            // L + 1
+  std::vector<int64_t> w_off, b_off;  // per layer offsets in a member's flat parameter vector
+  int64_t P = 0;                    // parameters per member
+  int64_t step = 0;                 // Adam steps taken
+  DevBuf w, wb, g, m, v;            // [E][P] fp32 master, bf16 copy, gradient, Adam moments
+  static constexpr int MAXL = 16;
+  DevBuf zf[MAXL], hb[MAXL], dz[MAXL];  // per layer: pre-activation fp32, activation bf16, dZ bf16
+  bool acts = false;
+  DevBuf xb, yb, out, loss, idx;    // minibatch input bf16, targets fp32, output fp32 [E][B][dL]
+  DevBuf dX, dY;                    // data set (fp32)
+  int64_t n_data = 0;
+};
+
+namespace {
+
+void alloc_acts(Mlp& net, int B) {
+  if (B <= net.max_batch && net.acts) return;
+  net.max_batch = std::max(B, net.max_batch);
+  net.acts = true;
+  const int64_t Bm = net.max_batch;
+  for (int l = 0; l < net.L; ++l) {
+    const int64_t out = net.dims[l + 1];
+    net.zf[l].reserve(net.E * Bm * out * 4);
+    net.hb[l].reserve(net.E * Bm * out * 2);
+    net.dz[l].reserve(net.E * Bm * out * 2);
+  }
+  net.xb.reserve(Bm * net.dims[0] * 2);
+  net.yb.reserve(Bm * net.dims[net.L] * 4);
+  net.out.reserve(net.E * Bm * net.dims[net.L] * 4);
+  net.idx.reserve(Bm * 4);
+  net.loss.reserve(net.E * 8);
+}
+
+// forward pass of every member on the BF16 minibatch in xb; output in out
+void forward(Mlp& net, int B, hipStream_t s) {
+  const uint16_t* x = net.xb.as<uint16_t>();
+  int64_t x_z = 0;  // the input is shared by all members
+  for (int l = 0; l < net.L; ++l) {
+    const int in = net.dims[l], outd = net.dims[l + 1];
+    GemmArgs g{};
+    g.A = x;
+    g.lda = in;
+    g.a_z = x_z;
+    g.B = net.wb.as<uint16_t>() + net.w_off[l];
+    g.ldb = in;
+    g.b_z = net.P;
+    g.M = B;
+    g.N = outd;
+    g.K = in;
+    g.ldc = outd;
+    g.c_z = (int64_t)B * outd;
+    g.bias = net.w.as<float>() + net.b_off[l];
+    g.bias_z = net.P;
+    TimedLaunch tl("mlp_gemm", s);
+    if (l + 1 < net.L) {
+      g.Cf = net.zf[l].as<float>();
+      g.Cb = net.hb[l].as<uint16_t>();
+      gemm<true, true, EPI_BIAS_GELU>(g, net.E, s);
+      x = net.hb[l].as<uint16_t>();
+      x_z = (int64_t)B * outd;
+    } else {
+      g.Cf = net.out.as<float>();
+      gemm<true, true, EPI_BIAS>(g, net.E, s);
+    }
+  }
+}
+
+// gradients of the mean-squared error against yb into g; loss per member
+void backward(Mlp& net, int B, hipStream_t s) {
+  const int L = net.L;
+  const int64_t BN = (int64_t)B * net.dims[L];
+  HIP_CHECK(hipMemsetAsync(net.loss.p, 0, net.E * 8, s));
+  hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks_for(BN, 256), net.E), dim3(256), 0, s, net.out.as<float>(),
+                     net.yb.as<float>(), BN, net.dz[L - 1].as<uint16_t>(), net.loss.as<double>());
+  HIP_CHECK(hipGetLastError());
+  for (int l = L - 1; l >= 0; --l) {
+    const int in = net.dims[l], outd = net.dims[l + 1];
+    const uint16_t* dZ = net.dz[l].as<uint16_t>();
+    const uint16_t* X = l > 0 ? net.hb[l - 1].as<uint16_t>() : net.xb.as<uint16_t>();
+    const int64_t x_z = l > 0 ? (int64_t)B * in : 0;
+    {
+      // dW[o][i] = sum_b dZ[b][o] X[b][i]
+      GemmArgs g{};
+      g.A = dZ;
+      g.lda = outd;
+      g.a_z = (int64_t)B * outd;
+      g.B = X;
+      g.ldb = in;
+      g.b_z = x_z;
+      g.M = outd;
+      g.N = in;
+      g.K = B;
+      g.Cf = net.g.as<float>() + net.w_off[l];
+      g.ldc = in;
+      g.c_z = net.P;
+      TimedLaunch tl("mlp_gemm", s);
+      gemm<false, false, EPI_F32>(g, net.E, s);
+    }
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(blocks_for(outd, 256), net.E), dim3(256), 0, s, dZ, B, outd,
+                       (int64_t)B * outd, net.g.as<float>() + net.b_off[l], net.P);
+    HIP_CHECK(hipGetLastError());
+    if (l > 0) {
+      // dZ_{l-1}[b][i] = (sum_o dZ[b][o] W[o][i]) * gelu'(Z_{l-1}[b][i])
+      GemmArgs g{};
+      g.A = dZ;
+      g.lda = outd;
+      g.a_z = (int64_t)B * outd;
+      g.B = net.wb.as<uint16_t>() + net.w_off[l];
+      g.ldb = in;
+      g.b_z = net.P;
+      g.M = B;
+      g.N = in;
+      g.K = outd;
+      g.Cb = net.dz[l - 1].as<uint16_t>();
+      g.ldc = in;
+      g.c_z = (int64_t)B * in;
+      g.aux = net.zf[l - 1].as<float>();
+      TimedLaunch tl("mlp_gemm", s);
+      gemm<true, false, EPI_DGELU>(g, net.E, s);
+    }
+  }
+}
+
+void upload_batch_input(Mlp& net, const float* x_host, int B, hipStream_t s) {
+  DevBuf tmp;
+  h2d(tmp, x_host, (size_t)B * net.dims[0], s);
+  const int64_t n = (int64_t)B * net.dims[0];
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, tmp.as<float>(), n,
+                     net.xb.as<uint16_t>());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+Mlp* mlp_create(int E, int L, const int32_t* dims, int max_batch) {
+  auto net = std::make_unique<Mlp>();
+  net->E = E;
+  net->L = L;
+  net->dims.assign(dims, dims + L + 1);
+  int64_t off = 0;
+  for (int l = 0; l < L; ++l) {
+    net->w_off.push_back(off);
+    off += (int64_t)dims[l + 1] * dims[l];
+    net->b_off.push_back(off);
+    off += dims[l + 1];
+  }
+  // members start 16-element aligned so every BF16 row load stays aligned
+  net->P = (off + 15) / 16 * 16;
+  const int64_t tot = (int64_t)E * net->P;
+  net->w.reserve(tot * 4);
+  net->wb.reserve(tot * 2);
+  net->g.reserve(tot * 4);
+  net->m.reserve(tot * 4);
+  net->v.reserve(tot * 4);
+  HIP_CHECK(hipMemset(net->w.p, 0, tot * 4));
+  HIP_CHECK(hipMemset(net->wb.p, 0, tot * 2));
+  HIP_CHECK(hipMemset(net->g.p, 0, tot * 4));
+  HIP_CHECK(hipMemset(net->m.p, 0, tot * 4));
+  HIP_CHECK(hipMemset(net->v.p, 0, tot * 4));
+  alloc_acts(*net, std::max(max_batch, 1));
+  return net.release();
+}
+
+void mlp_destroy(Mlp* net) { delete net; }
+int64_t mlp_param_count(const Mlp& net) { return net.P; }
+int64_t mlp_data_count(const Mlp& net) { return net.n_data; }
+int mlp_layers(const Mlp& net, int32_t* dims) {
+  if (dims)
+    for (int l = 0; l <= net.L; ++l) dims[l] = net.dims[l];
+  return net.L;
+}
+
+void mlp_get_params(Mlp& net, float* params, hipStream_t s) {
+  d2h(params, net.w.p, (size_t)net.E * net.P, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void mlp_set_params(Mlp& net, const float* params, hipStream_t s) {
+  const int64_t tot = (int64_t)net.E * net.P;
+  HIP_CHECK(hipMemcpyAsync(net.w.p, params, tot * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, s, net.w.as<float>(), tot,
+                     net.wb.as<uint16_t>());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemsetAsync(net.m.p, 0, tot * 4, s));
+  HIP_CHECK(hipMemsetAsync(net.v.p, 0, tot * 4, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  net.step = 0;
+}
+
+void mlp_forward(Mlp& net, const float* x_host, int B, float* y_host, hipStream_t s) {
+  alloc_acts(net, B);
+  upload_batch_input(net, x_host, B, s);
+  forward(net, B, s);
+  d2h(y_host, net.out.p, (size_t)net.E * B * net.dims[net.L], s);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void mlp_gradients(Mlp& net, const float* x_host, const float* y_host, int B, float* g_host, double* loss,
+                   hipStream_t s) {
+  alloc_acts(net, B);
+  upload_batch_input(net, x_host, B, s);
+  h2d(net.yb, y_host, (size_t)B * net.dims[net.L], s);
+  HIP_CHECK(hipMemsetAsync(net.g.p, 0, (size_t)net.E * net.P * 4, s));
+  forward(net, B, s);
+  backward(net, B, s);
+  if (g_host) d2h(g_host, net.g.p, (size_t)net.E * net.P, s);
+  if (loss) d2h(loss, net.loss.p, net.E, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void mlp_set_data(Mlp& net, const float* X, const float* Y, int64_t n, hipStream_t s) {
+  h2d(net.dX, X, (size_t)n * net.dims[0], s);
+  h2d(net.dY, Y, (size_t)n * net.dims[net.L], s);
+  HIP_CHECK(hipStreamSynchronize(s));
+  net.n_data = n;
+}
+
+void mlp_train(Mlp& net, const int32_t* batches, int steps, int B, float lr, float b1, float b2, float eps,
+               double* losses, hipStream_t s) {
+  alloc_acts(net, B);
+  DevBuf didx;
+  h2d(didx, batches, (size_t)steps * B, s);
+  const int64_t tot = (int64_t)net.E * net.P;
+  DevBuf lbuf;
+  lbuf.reserve((size_t)std::max(steps, 1) * net.E * 8);
+  for (int it = 0; it < steps; ++it) {
+    hipLaunchKernelGGL(gather_kernel, dim3(B), dim3(256), 0, s, net.dX.as<float>(), net.dims[0], net.dY.as<float>(),
+                       net.dims[net.L], didx.as<int32_t>() + (size_t)it * B, B, net.xb.as<uint16_t>(),
+                       net.yb.as<float>());
+    HIP_CHECK(hipGetLastError());
+    forward(net, B, s);
+    backward(net, B, s);
+    HIP_CHECK(hipMemcpyAsync(lbuf.as<double>() + (size_t)it * net.E, net.loss.p, net.E * 8, hipMemcpyDeviceToDevice,
+                             s));
+    ++net.step;
+    const float bc1 = 1.0f - powf(b1, (float)net.step), bc2 = 1.0f - powf(b2, (float)net.step);
+    TimedLaunch tl("mlp_adam", s);
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, s, net.w.as<float>(), net.g.as<float>(),
+                       net.m.as<float>(), net.v.as<float>(), net.wb.as<uint16_t>(), tot, lr, b1, b2, eps, bc1, bc2);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (losses) d2h(losses, lbuf.p, (size_t)steps * net.E, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace imped

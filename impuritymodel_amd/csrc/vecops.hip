@@ -139,6 +139,29 @@ __global__ void trunc_scatter_kernel(int W, const double2* evecs, const uint64_t
   oa[d] = evecs[vbase[o] + i];
 }
 
+// Entries of a vector set with |amp|^2 >= weight_min (the pruning of
+// spectra.py:1108-1111), kept in stored order.
+__global__ void prune_flags_kernel(const double2* amps, int64_t n, double weight_min, int32_t* flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = abs2_ref(amps[i]) >= weight_min ? 1 : 0;
+}
+
+__global__ void prune_scatter_kernel(int W, const uint64_t* keys, const double2* amps, int64_t n,
+                                     const int32_t* flags, const int64_t* pos, uint64_t* ok, double2* oa) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flags[i]) return;
+  const int64_t d = pos[i];
+  for (int w = 0; w < W; ++w) ok[d * W + w] = keys[i * W + w];
+  oa[d] = amps[i];
+}
+
+// out[o] = exclusive-scan position of entry at[o] (total when at[o] == n).
+__global__ void gather_pos_kernel(const int64_t* pos, int64_t n, int64_t total, const int64_t* at, int m,
+                                  int64_t* out) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < m) out[o] = at[o] < n ? pos[at[o]] : total;
+}
+
 struct VScratch {
   DevBuf so_off, sbeg, skeys, samps, spos, tasks, counts, offs, ekeys, eval, evec, esrc, efirst, perm, keep, sums,
       okey, pos, ck, cv, ca, co, perm2, cnt, off, flags;
@@ -156,6 +179,22 @@ int bits_for(int64_t n) {
 
 // Exclusive scan of int32 flags into int64 positions (total returned).
 int64_t scan_flags(const int32_t* flags, int64_t n, int64_t* pos, hipStream_t s) { return scan_counts(flags, n, pos, s); }
+
+// pos_at[o] = scan position (in vscratch().pos over n flags, total kept) of
+// segment start seg[o], o <= V, with one gather and one copy.
+void segment_positions(const std::vector<int64_t>& seg, int64_t n, int64_t kept, std::vector<int64_t>& pos_at,
+                       hipStream_t s) {
+  VScratch& sc = vscratch();
+  const int m = (int)seg.size();
+  h2d(sc.off, seg.data(), m, s);
+  sc.cnt.reserve((size_t)m * 8);
+  hipLaunchKernelGGL(gather_pos_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, s, sc.pos.as<int64_t>(), n, kept,
+                     sc.off.as<int64_t>(), m, sc.cnt.as<int64_t>());
+  HIP_CHECK(hipGetLastError());
+  pos_at.resize(m);
+  d2h(pos_at.data(), sc.cnt.p, m, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
 
 }  // namespace
 
@@ -330,10 +369,35 @@ void truncate_eigvecs(int W, const uint64_t* basis_keys, const double2* evecs, i
   HIP_CHECK(hipGetLastError());
   // per-output counts from the scan positions at segment starts
   std::vector<int64_t> pos_at(n_out + 1);
-  for (int o = 0; o < n_out; ++o) d2h(&pos_at[o], sc.pos.as<int64_t>() + fl_off[o], 1, s);
-  HIP_CHECK(hipStreamSynchronize(s));
-  pos_at[n_out] = NU;
+  segment_positions(fl_off, NF, NU, pos_at, s);
   for (int o = 0; o < n_out; ++o) out_off[o + 1] = out_off[o] + (pos_at[o + 1] - pos_at[o]);
+}
+
+void prune_vec(int W, const std::vector<int64_t>& off, const uint64_t* keys, const double2* amps, double weight_min,
+               std::vector<int64_t>& out_off, DevBuf& out_keys, DevBuf& out_amps, hipStream_t s) {
+  VScratch& sc = vscratch();
+  const int V = (int)off.size() - 1;
+  const int64_t N = off.empty() ? 0 : off.back();
+  out_off.assign(V + 1, 0);
+  if (N == 0) {
+    out_keys.reserve(8);
+    out_amps.reserve(16);
+    return;
+  }
+  sc.flags.reserve(N * 4);
+  hipLaunchKernelGGL(prune_flags_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, s, amps, N, weight_min,
+                     sc.flags.as<int32_t>());
+  HIP_CHECK(hipGetLastError());
+  sc.pos.reserve((N + 1) * 8);
+  const int64_t NU = scan_flags(sc.flags.as<int32_t>(), N, sc.pos.as<int64_t>(), s);
+  out_keys.reserve(std::max<int64_t>(NU, 1) * W * 8);
+  out_amps.reserve(std::max<int64_t>(NU, 1) * 16);
+  hipLaunchKernelGGL(prune_scatter_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, s, W, keys, amps, N,
+                     sc.flags.as<int32_t>(), sc.pos.as<int64_t>(), out_keys.as<uint64_t>(), out_amps.as<double2>());
+  HIP_CHECK(hipGetLastError());
+  std::vector<int64_t> pos_at(V + 1);
+  segment_positions(off, N, NU, pos_at, s);
+  for (int o = 0; o < V; ++o) out_off[o + 1] = out_off[o] + (pos_at[o + 1] - pos_at[o]);
 }
 
 }  // namespace imped

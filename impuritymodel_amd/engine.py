@@ -68,8 +68,6 @@ class Vec:
     @property
     def offsets(self):
         if self._off is None:
-            n = C.c_int64()
-            # number of vectors is len(offsets)-1; query via a large-enough buffer
             raise RuntimeError("offsets must be set by the producer")
         return self._off
 
@@ -78,6 +76,19 @@ class Vec:
         check(lib.imped_vec_offsets(self.h, ptr(off)))
         self._off = off
         return self
+
+    def norm2(self):
+        """Per-vector |psi|^2 recorded by the producer (before normalization)."""
+        off = self.offsets
+        out = np.zeros(max(len(off) - 1, 1), np.float64)
+        check(lib.imped_vec_get(self.h, None, None, ptr(out)))
+        return out[: len(off) - 1]
+
+    def prune(self, weight_min):
+        """New vector set without the entries of |amp|^2 < weight_min."""
+        h = C.c_void_p()
+        check(lib.imped_vec_prune(self.h, float(weight_min), C.byref(h)))
+        return Vec(self.W, handle=h).set_count(len(self.offsets) - 1)
 
     def get(self):
         """Return (keys [N, W] uint64, amps [N] complex, norm2 [V] or zeros)."""
@@ -210,6 +221,23 @@ class Batch:
         self.nev = nev
         return evals[: self.C], evecs, steps[: self.C], rc == 0
 
+    def solve(self, shifts, tol=1e-12, max_steps=1000, allow_noconv=False, want_x=False):
+        """(z I - H_c) x = seed vector of chain c for every shift z in
+        ``shifts[c]`` ([C, S] complex); the solutions feed ``vec_from_eigen``.
+        With ``want_x`` the solutions are also returned (chain c's block at
+        row_base[c]*S, shift s at offset s*n_c)."""
+        z = np.asarray(shifts, np.complex128).reshape(self.C, -1)
+        S = z.shape[1]
+        steps = np.zeros(max(self.C, 1), np.int32)
+        resid = np.zeros((max(self.C, 1), S), np.float64)
+        x = np.zeros(max(self.rows * S, 1), np.complex128) if want_x else None
+        rc = check(lib.imped_batch_solve(self.h, int(S), ptr(_c64(z)), float(tol), int(max_steps), ptr(steps),
+                                         ptr(resid), None if x is None else ptr(x.view(np.float64))),
+                   allow=(IMPED_ERR_NOCONV,) if allow_noconv else ())
+        self.nev = S
+        out = (steps[: self.C], resid[: self.C], rc == 0)
+        return out + (x,) if want_x else out
+
     def eigvec(self, evecs, c, k):
         """Eigenvector k of chain c from the flat evecs array."""
         n = int(self.chain_n[c])
@@ -230,7 +258,8 @@ class Batch:
         spec = np.zeros((max(n_spec, 1), len(omega)), np.float64)
         check(lib.imped_batch_cf(
             self.h, ptr(_nz(np.asarray(energy, np.float64))), ptr(_nz(np.asarray(norm2, np.float64))),
-            ptr(_nz(np.asarray(sign, np.int32))), ptr(_nz(np.asarray(zflag, np.int32))), float(kT),
+            ptr(_nz(np.asarray(sign, np.int32))), ptr(_nz(np.asarray(zflag, np.int32))),
+            ptr(np.ascontiguousarray(np.broadcast_to(np.asarray(kT, np.float64), (max(n_spec, 1),)))),
             ptr(_nz(np.asarray(spec_id, np.int32))), int(n_spec), ptr(omega), len(omega), float(delta),
             None if g is None else ptr(g.view(np.float64)), ptr(spec)))
         return (g[: self.C] if want_g else None), spec[:n_spec]

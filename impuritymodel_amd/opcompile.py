@@ -130,7 +130,7 @@ def restriction_arrays(restrictions, W):
 class TermTable:
     """One operator's compiled term table (with its restrictions) for W words."""
 
-    __slots__ = ("W", "group_term_off", "group_flip", "masks", "meta", "coef", "rmask", "rlohi")
+    __slots__ = ("W", "group_term_off", "group_flip", "masks", "meta", "coef", "rmask", "rlohi", "src")
 
     def __init__(self, op, restrictions, W):
         g_off, g_flip, masks, met, src = _structure(tuple(op.keys()), W)
@@ -140,8 +140,17 @@ class TermTable:
         self.group_flip = g_flip
         self.masks = masks
         self.meta = met
+        self.src = src
         self.coef = vals[src]
         self.rmask, self.rlohi = restriction_arrays(restrictions, W)
+
+    def with_coef(self, values):
+        """The same term structure with new values (given in dict-key order)."""
+        t = TermTable.__new__(TermTable)
+        for name in TermTable.__slots__:
+            setattr(t, name, getattr(self, name))
+        t.coef = np.asarray(values, np.complex128)[self.src]
+        return t
 
 
 def compile_models(ops, restrictions, W):

@@ -1,0 +1,379 @@
+"""Learned XAS surrogate with exact-ED fallback (no reference counterpart).
+
+A parameter vector of a NiO-like 10-bath model,
+
+    (10Dq, F0, F2/F4 scale, Delta, V_eg, V_t2g, xi_3d, T),
+
+maps to its thermal, polarization-summed L2,3 XAS spectrum on a fixed
+1000-point grid, normalized to unit area.  The exact spectra come from the
+batched GPU ED (:func:`impuritymodel_amd.batched.xas_batch`); the network is
+an ensemble of MLPs trained and evaluated by the library's own BF16-MFMA
+kernels (``imped_mlp_*``), and the fitting loop sends a parameter set to the
+surrogate when the ensemble agrees and to exact ED otherwise.
+
+The model family is affine in its first seven parameters (every Hamiltonian
+term is a Slater integral, a crystal-field, hopping, spin-orbit or
+double-counting coefficient times a fixed operator), so the operator of any
+parameter set is ``O_0 + sum_i theta_i O_i`` over one shared term structure:
+models are built by one matrix product instead of operator algebra.
+"""
+
+from collections import OrderedDict
+
+import numpy as np
+
+from . import operators as ops
+from .models import get_restrictions
+from .opcompile import TermTable
+
+PARAM_NAMES = ("tenDq", "F0", "Fscale", "delta", "V_eg", "V_t2g", "xi_3d", "T")
+PARAM_RANGES = np.array([
+    [0.3, 1.5],    # 10Dq (eV)
+    [5.0, 9.0],    # F0_dd (eV)
+    [0.6, 1.0],    # scale of F2_dd = 10 eV and F4_dd = 6.2 eV
+    [1.0, 8.0],    # charge-transfer correction of the double counting (eV)
+    [1.0, 3.0],    # eg bath hopping (eV)
+    [0.5, 2.0],    # t2g bath hopping (eV)
+    [0.04, 0.12],  # 3d spin-orbit coupling (eV)
+    [10.0, 600.0],  # temperature (K)
+])
+
+OMEGA = np.linspace(-20.0, 30.0, 1000)  # XAS grid (eV)
+DELTA = 0.3                             # Lorentzian half width (eV)
+
+
+def cubic_bath_h0(e_imp, tenDq, eps_eg, eps_t2g, v_eg, v_t2g, l=2):
+    """3d crystal field plus one bath state per cubic 3d spin-orbital.
+
+    Bath state b = 5 s + k couples to cubic orbital k (0, 1 = eg; 2, 3, 4 =
+    t2g of the spherical->cubic matrix) with spin s; c_k = sum_m conj(u[m, k]) c_m.
+    """
+    e_eg = e_imp + 3 / 5 * tenDq
+    e_t2g = e_imp - 2 / 5 * tenDq
+    u = ops.get_spherical_2_cubic_matrix(spinpol=False, l=l)
+    h = np.dot(u, np.dot(np.diag([e_eg, e_eg, e_t2g, e_t2g, e_t2g]).astype(complex), np.conj(u.T)))
+    h = 0.5 * (h + np.conj(h.T))
+    op = {}
+    for i, mi in enumerate(range(-l, l + 1)):
+        for j, mj in enumerate(range(-l, l + 1)):
+            if h[i, j] != 0:
+                for s in range(2):
+                    op[(((l, s, mi), "c"), ((l, s, mj), "a"))] = h[i, j]
+    for s in range(2):
+        for k in range(5):
+            b = 5 * s + k
+            eg = k < 2
+            eps, v = (eps_eg, v_eg) if eg else (eps_t2g, v_t2g)
+            if eps != 0:
+                op[(((l, b), "c"), ((l, b), "a"))] = complex(eps)
+            for i, m in enumerate(range(-l, l + 1)):
+                c = v * np.conj(u[i, k])
+                if c != 0 and v != 0:
+                    op[(((l, b), "c"), ((l, s, m), "a"))] = complex(c)
+                    op[(((l, s, m), "c"), ((l, b), "a"))] = complex(np.conj(c))
+    return op
+
+
+class FamilyModel:
+    """One member of :class:`XASFamily` (duck-types ImpurityModel for xas_batch)."""
+
+    def __init__(self, family, theta, coef):
+        self.family = family
+        self.theta = theta
+        self._coef = coef
+        self.n_bath = 10
+        self.nBaths = family.nBaths
+        self.nValBaths = family.nValBaths
+        self.n0imps = family.n0imps
+        self.dnTols = family.dnTols
+        self.dnValBaths = family.dnValBaths
+        self.dnConBaths = family.dnConBaths
+        self.restrictions = family.restrictions
+        self.params = dict(zip(PARAM_NAMES, theta))
+        # dipole tables are identical for the whole family
+        self._dip_tables = family._dip_tables
+
+    @property
+    def n_spin_orbitals(self):
+        return ops.n_spin_orbitals(self.nBaths)
+
+    @property
+    def hOp(self):
+        return {k: complex(v) for k, v in zip(self.family.keys, self._coef) if v != 0}
+
+    def term_table(self, W):
+        return self.family.table(W).with_coef(self._coef)
+
+
+class XASFamily:
+    """Affine family hOp(theta) = O_0 + sum_i theta_i O_i of 10-bath models."""
+
+    def __init__(self, e_imp=-1.31796, eps_eg=-2.0, eps_t2g=-1.5, Fpd=(8.5, 0.0, 6.0), Gpd=(0.0, 4.5, 0.0, 2.5),
+                 xi_2p=11.5, hField=(0.0, 0.0, 1e-4)):
+        self.fixed = dict(e_imp=e_imp, eps_eg=eps_eg, eps_t2g=eps_t2g, Fpd=Fpd, Gpd=Gpd, xi_2p=xi_2p, hField=hField)
+        self.nBaths = OrderedDict([(1, 0), (2, 10)])
+        self.nValBaths = OrderedDict([(1, 0), (2, 10)])
+        self.n0imps = OrderedDict([(1, 6), (2, 8)])
+        self.dnTols = OrderedDict([(1, 0), (2, 2)])
+        self.dnValBaths = OrderedDict([(1, 0), (2, 2)])
+        self.dnConBaths = OrderedDict([(1, 0), (2, 0)])
+        self.restrictions = get_restrictions(self.nBaths, self.nValBaths, self.n0imps, self.dnTols,
+                                             self.dnValBaths, self.dnConBaths)
+        base = self.hop(np.zeros(7))
+        units = [self.hop(np.eye(7)[i]) for i in range(7)]
+        keys = list(base.keys())
+        seen = set(keys)
+        for u in units:
+            for k in u:
+                if k not in seen:
+                    seen.add(k)
+                    keys.append(k)
+        self.keys = keys
+        self.c0 = np.array([base.get(k, 0.0) for k in keys], np.complex128)
+        self.cmat = np.array([[u.get(k, 0.0) for k in keys] for u in units], np.complex128) - self.c0[None, :]
+        self._tables = {}
+        self._dip_tables = {}
+
+    def hop(self, theta):
+        """The operator dict of one parameter set, assembled term by term."""
+        tenDq, F0, fs, delta, v_eg, v_t2g, xi_3d = (float(x) for x in theta[:7])
+        f = self.fixed
+        h0 = cubic_bath_h0(f["e_imp"], tenDq, f["eps_eg"], f["eps_t2g"], v_eg, v_t2g)
+        Fdd = (F0, 0.0, 10.0 * fs, 0.0, 6.2 * fs)
+        return ops.get_hamiltonian_operator_from_h0(
+            self.nBaths, [Fdd, (0.0, 0.0, 0.0), f["Fpd"], f["Gpd"]], [f["xi_2p"], xi_3d], [self.n0imps, delta],
+            f["hField"], h0)
+
+    def coef(self, thetas):
+        """Term coefficients [M, n_terms] of parameter sets [M, >=7]."""
+        th = np.asarray(thetas, np.float64).reshape(-1, np.asarray(thetas).shape[-1])[:, :7]
+        return self.c0[None, :] + th @ self.cmat
+
+    def table(self, W):
+        if W not in self._tables:
+            self._tables[W] = TermTable(dict(zip(self.keys, [1.0] * len(self.keys))), self.restrictions, W)
+        return self._tables[W]
+
+    def models(self, thetas):
+        thetas = np.asarray(thetas, np.float64)
+        cf = self.coef(thetas)
+        return [FamilyModel(self, thetas[i], cf[i]) for i in range(len(thetas))]
+
+
+class Ensemble:
+    """Handle of an ensemble of MLPs in libimped (``imped_mlp_*``)."""
+
+    def __init__(self, members=5, dims=(8, 512, 512, 512, 1000), max_batch=1024, seed=0):
+        from ._lib import C, check, lib, ptr
+
+        self._lib, self._check, self._ptr = lib, check, ptr
+        self.E = int(members)
+        self.dims = tuple(int(d) for d in dims)
+        d = np.array(self.dims, np.int32)
+        h = C.c_void_p()
+        check(lib.imped_mlp_create(self.E, len(d) - 1, ptr(d), int(max_batch), C.byref(h)))
+        self.h = h
+        n = np.zeros(1, np.int64)
+        check(lib.imped_mlp_param_count(self.h, ptr(n)))
+        self.P = int(n[0])
+        self.set_params(self.init_params(seed))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.imped_mlp_destroy(self.h)
+            self.h = None
+
+    def init_params(self, seed):
+        """He-normal weights (sqrt(2/fan_in); sqrt(1/fan_in) for the output layer), zero biases."""
+        rng = np.random.default_rng(seed)
+        p = np.zeros((self.E, self.P), np.float32)
+        L = len(self.dims) - 1
+        for e in range(self.E):
+            off = 0
+            for l in range(L):
+                i, o = self.dims[l], self.dims[l + 1]
+                std = np.sqrt((2.0 if l + 1 < L else 1.0) / i)
+                p[e, off: off + o * i] = rng.normal(0.0, std, o * i)
+                off += o * i + o
+        return p
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32).reshape(self.E, self.P)
+        self._check(self._lib.imped_mlp_set_params(self.h, self._ptr(p)))
+
+    def get_params(self):
+        p = np.zeros((self.E, self.P), np.float32)
+        self._check(self._lib.imped_mlp_get_params(self.h, self._ptr(p)))
+        return p
+
+    def forward(self, x):
+        """Outputs of every member [E, B, dims[-1]] for inputs x [B, dims[0]]."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
+        y = np.zeros((self.E, len(x), self.dims[-1]), np.float32)
+        self._check(self._lib.imped_mlp_forward(self.h, self._ptr(x), len(x), self._ptr(y)))
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
+        t = np.ascontiguousarray(t, np.float32).reshape(len(x), self.dims[-1])
+        g = np.zeros((self.E, self.P), np.float32)
+        loss = np.zeros(self.E, np.float64)
+        self._check(self._lib.imped_mlp_gradients(self.h, self._ptr(x), self._ptr(t), len(x), self._ptr(g),
+                                                  self._ptr(loss)))
+        return loss, g
+
+    def train(self, x, t, epochs, batch=1024, lr=1e-3, lr_final=None, seed=0, chunk=200):
+        """Adam on minibatches of shuffled rows; cosine decay from lr to lr_final.
+
+        Returns the loss of every step [steps, E]."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
+        t = np.ascontiguousarray(t, np.float32).reshape(len(x), self.dims[-1])
+        n = len(x)
+        self._check(self._lib.imped_mlp_set_data(self.h, self._ptr(x), self._ptr(t), n))
+        rng = np.random.default_rng(seed)
+        batch = min(batch, n)
+        per_epoch = max(n // batch, 1)
+        idx = np.concatenate([rng.permutation(n)[: per_epoch * batch] for _ in range(epochs)]).astype(np.int32)
+        steps = per_epoch * epochs
+        idx = idx.reshape(steps, batch)
+        lr_final = lr if lr_final is None else lr_final
+        losses = np.zeros((steps, self.E))
+        for s0 in range(0, steps, chunk):
+            s1 = min(steps, s0 + chunk)
+            frac = (s0 + s1) / 2 / max(steps, 1)
+            rate = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * frac))
+            part = np.zeros((s1 - s0, self.E))
+            self._check(self._lib.imped_mlp_train(self.h, self._ptr(np.ascontiguousarray(idx[s0:s1])), s1 - s0,
+                                                  batch, float(rate), 0.9, 0.999, 1e-8, self._ptr(part)))
+            losses[s0:s1] = part
+        return losses
+
+
+def sample_params(n, seed=0):
+    """Parameter sets drawn uniformly from PARAM_RANGES."""
+    rng = np.random.default_rng(seed)
+    lo, hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
+    return lo + (hi - lo) * rng.uniform(size=(n, len(lo)))
+
+
+def normalize_params(p):
+    """Map parameters to [-1, 1] per coordinate (network input)."""
+    lo, hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
+    return (2.0 * (np.asarray(p) - lo) / (hi - lo) - 1.0).astype(np.float32)
+
+
+def area_normalize(spectra, omega=OMEGA):
+    """Spectra scaled to unit trapezoid area on omega."""
+    s = np.asarray(spectra, np.float64)
+    area = np.trapezoid(s, omega, axis=-1)
+    return s / area[..., None]
+
+
+def l1_error(a, b, omega=OMEGA):
+    """L1 distance of area-normalized spectra: sum |a - b| d omega."""
+    return np.trapezoid(np.abs(area_normalize(a, omega) - area_normalize(b, omega)), omega, axis=-1)
+
+
+SCALE = float(OMEGA[-1] - OMEGA[0])  # unit-area spectra times the window: mean value 1
+
+
+class Surrogate:
+    """Ensemble network over an :class:`XASFamily` with a calibrated trust threshold.
+
+    ``predict`` returns the area-normalized ensemble mean and the ensemble
+    disagreement: the mean L1 distance of the members' spectra from the
+    mean.  Predictions with disagreement below ``threshold`` are trusted.
+    """
+
+    def __init__(self, family, ensemble):
+        self.family = family
+        self.net = ensemble
+        self.threshold = 0.0
+
+    def fit(self, params, spectra, epochs=200, batch=1024, lr=2e-3, lr_final=2e-5, seed=0):
+        return self.net.train(normalize_params(params), np.asarray(spectra) * SCALE, epochs, batch, lr, lr_final,
+                              seed)
+
+    def predict(self, params):
+        y = self.net.forward(normalize_params(params)).astype(np.float64) / SCALE
+        y = np.maximum(y, 0.0)
+        members = area_normalize(y)
+        mean = area_normalize(members.mean(axis=0))
+        dis = np.trapezoid(np.abs(members - mean[None]), OMEGA, axis=-1).mean(axis=0)
+        return mean, dis
+
+    def calibrate(self, params, exact, bar=1e-2):
+        """Largest threshold under which every validation prediction meets the L1 bar."""
+        pred, dis = self.predict(params)
+        err = l1_error(pred, exact)
+        order = np.argsort(dis)
+        ok = np.cumprod(err[order] <= bar).astype(bool)
+        k = int(ok.sum())
+        self.threshold = float(dis[order[k - 1]]) * (1 + 1e-9) if k else 0.0
+        return err, dis
+
+
+def fitting_loop(family, target, surrogate=None, n_walkers=256, n_iter=8, step=0.05, temperature=0.02, seed=0,
+                 **xas_kw):
+    """Metropolis walkers in parameter space fitting a target spectrum.
+
+    Every proposal's spectrum comes from the surrogate when it is trusted
+    and from batched exact ED otherwise.  Returns a dict with the counts, the
+    wall time and the surrogate-produced (params, spectra) for scoring.
+    """
+    import time
+
+    rng = np.random.default_rng(seed)
+    lo, hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
+    stats = dict(n_spectra=0, n_exact=0, n_surrogate=0, sur_params=[], sur_spectra=[])
+
+    def evaluate(P):
+        out = np.zeros((len(P), len(OMEGA)))
+        use = np.zeros(len(P), bool)
+        if surrogate is not None:
+            pred, dis = surrogate.predict(P)
+            use = dis < surrogate.threshold
+            out[use] = pred[use]
+            stats["sur_params"].append(P[use])
+            stats["sur_spectra"].append(pred[use])
+        if (~use).any():
+            ex, _ = exact_spectra(family, P[~use], **xas_kw)
+            out[~use] = ex
+        stats["n_spectra"] += len(P)
+        stats["n_exact"] += int((~use).sum())
+        stats["n_surrogate"] += int(use.sum())
+        return out
+
+    t0 = time.perf_counter()
+    theta = sample_params(n_walkers, seed=seed + 1)
+    cur = l1_error(evaluate(theta), target[None])
+    best = cur.min()
+    for _ in range(n_iter):
+        prop = theta + step * (hi - lo) * rng.normal(size=theta.shape)
+        prop = np.where(prop < lo, 2 * lo - prop, prop)
+        prop = np.where(prop > hi, 2 * hi - prop, prop)
+        prop = np.clip(prop, lo, hi)
+        new = l1_error(evaluate(prop), target[None])
+        acc = rng.uniform(size=n_walkers) < np.exp(-(new - cur) / temperature)
+        theta[acc] = prop[acc]
+        cur[acc] = new[acc]
+        best = min(best, cur.min())
+    stats["seconds"] = time.perf_counter() - t0
+    stats["best_l1"] = float(best)
+    for k in ("sur_params", "sur_spectra"):
+        stats[k] = np.concatenate(stats[k]) if stats[k] else np.zeros((0, 8 if k == "sur_params" else len(OMEGA)))
+    return stats
+
+
+def exact_spectra(family, thetas, krylovSize=200, reorth=0, **kw):
+    """Exact XAS of parameter sets by the batched GPU ED, area-normalized."""
+    from .batched import ground_bases, xas_batch
+
+    models = family.models(thetas)
+    from . import opcompile
+
+    W = opcompile.n_words(models[0].n_spin_orbitals)
+    basis = ground_bases(models[:1], W)[0]
+    res = xas_batch(models, omega=OMEGA, delta=DELTA, krylovSize=krylovSize, T=np.asarray(thetas)[:, 7],
+                    reorth=reorth, bases=[basis] * len(models), **kw)
+    return area_normalize(res.spectra), res

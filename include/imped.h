@@ -106,9 +106,16 @@ int imped_vec_get(imped_vec vec, uint64_t* keys, double* amps, double* norm2);
 int imped_vec_apply(imped_ops ops, imped_vec src, int n_out, const int32_t* out_src, const int32_t* out_model,
                     double weight_min, int normalize, imped_vec* out);
 
+/* The pruning of getRIXSmap after normalizing psi3 (spectra.py:1108-1111):
+ * a new vector set with the entries of src whose |amp|^2 >= weight_min, in
+ * stored order; src's per-vector norm2 is carried over. */
+int imped_vec_prune(imped_vec src, double weight_min, imped_vec* out);
+
 /* Eigenvector dicts of eigensystem (finite.py:114-117): output o holds the
  * components of eigenvector which[o] of chain chain[o] with
- * |v_i|^2 >= weight_min, in basis order, not renormalized. */
+ * |v_i|^2 >= weight_min, in basis order, not renormalized.  After
+ * imped_batch_solve, which[o] names a shift and the vector is that solution
+ * (psi2 of getRIXSmap, spectra.py:1087-1090, with weight_min = 0). */
 int imped_vec_from_eigen(imped_batch batch, int n_out, const int32_t* chain, const int32_t* which, double weight_min,
                          imped_vec* out);
 
@@ -163,6 +170,21 @@ int imped_batch_lanczos(imped_batch batch, const double* v0, int K, int reorth, 
 int imped_batch_eigen(imped_batch batch, int nev, double tol, int max_steps, uint64_t seed, double* evals,
                       double* evecs, int32_t* steps);
 
+/* The resolvent solves of getRIXSmap (spectra.py:1066-1080, which calls
+ * scipy.sparse.linalg.bicgstab on (wIn + i delta1 + e) I - H):
+ *   (z_{c,s} I - H_c) x_{c,s} = y_c,  s < n_shift,
+ * y_c = the chain's seed amplitudes (imped_batch_seed_from_vec), shifts
+ * [C][n_shift] complex with nonzero imaginary parts.  All shifts of a chain
+ * share one Krylov space (Lanczos with full reorthogonalization, Galerkin
+ * solution); tol bounds the relative residual |y - (z - H) x| / |y| of every
+ * shift.  Solutions replace the eigenvectors of the batch (layout of
+ * imped_batch_eigen with nev = n_shift) for imped_vec_from_eigen.
+ * steps[C], resid[C][n_shift] and x (a host copy of the solutions, complex,
+ * rows * n_shift) may be NULL.  Returns IMPED_ERR_NOCONV (with the solutions
+ * written) when some residual stays above tol. */
+int imped_batch_solve(imped_batch batch, int n_shift, const double* shifts, double tol, int max_steps,
+                      int32_t* steps, double* resid, double* x);
+
 /* Continued fractions (spectra.py:677-680) of every chain after
  * imped_batch_lanczos, on omega[n_w] with broadening delta:
  *   g_c(w) = sign_c * norm2_c / (z - a0 - b0^2/(z - a1 - ...)),
@@ -170,11 +192,39 @@ int imped_batch_eigen(imped_batch batch, int nev, double tol, int max_steps, uin
  * (sign -1 is the photoemission convention, spectra.py:136-137), written to
  * g_out[C][n_w] (complex, may be NULL); and spectra
  *   spec[q][w] = sum_{c in q} p_c * (-Im g_c(w)),
- *   p_c = exp(-(e_c - min e)/kT) / Z, Z summed over chains with zflag = 1
+ *   p_c = exp(-(e_c - min e)/kT[q]) / Z, Z summed over chains with zflag = 1
  * (thermal_average, average.py:11-45, summed over operators). */
 int imped_batch_cf(imped_batch batch, const double* energy, const double* norm2, const int32_t* sign,
-                   const int32_t* zflag, double kT, const int32_t* spec_id, int n_spec, const double* omega, int n_w,
+                   const int32_t* zflag, const double* kT, const int32_t* spec_id, int n_spec, const double* omega, int n_w,
                    double delta, double* g_out, double* spec_out);
+
+/* ---- surrogate network (no reference counterpart) ----
+ *
+ * An ensemble of n_members fully connected networks of layer widths
+ * dims[0..n_layers] with GELU between layers, for the parameters -> XAS
+ * spectrum surrogate of the fitting loop.  GEMMs run on the matrix cores in
+ * BF16 with FP32 accumulation; master weights, gradients and Adam moments are
+ * FP32.  Member e's parameters are params[e*P .. e*P + P) (P from
+ * imped_mlp_param_count): W_0 [dims[1]][dims[0]] row-major, b_0 [dims[1]],
+ * W_1, b_1, ..., then zero padding. */
+typedef struct imped_mlp_s* imped_mlp;
+int imped_mlp_create(int n_members, int n_layers, const int32_t* dims, int max_batch, imped_mlp* out);
+int imped_mlp_destroy(imped_mlp net);
+int imped_mlp_param_count(imped_mlp net, int64_t* per_member);
+/* Loads all members' parameters [n_members][P] and resets the Adam state. */
+int imped_mlp_set_params(imped_mlp net, const float* params);
+int imped_mlp_get_params(imped_mlp net, float* params);
+/* y[e][b][dims[L]] = network e applied to x[b][dims[0]] (inputs rounded to BF16). */
+int imped_mlp_forward(imped_mlp net, const float* x, int batch, float* y);
+/* Gradients [n_members][P] of the mean squared error mean_{b,j} (y_e - t)^2
+ * over one batch (x, t), and the loss of each member (either may be NULL). */
+int imped_mlp_gradients(imped_mlp net, const float* x, const float* t, int batch, float* grads, double* loss);
+/* Training set kept on the device: x [n][dims[0]], t [n][dims[L]]. */
+int imped_mlp_set_data(imped_mlp net, const float* x, const float* t, int64_t n);
+/* steps Adam steps on minibatches batches[step][batch] (row indices of the
+ * training set; every member sees the same rows); losses [steps][n_members]. */
+int imped_mlp_train(imped_mlp net, const int32_t* batches, int steps, int batch, float lr, float beta1, float beta2,
+                    float eps, double* losses);
 
 #ifdef __cplusplus
 }

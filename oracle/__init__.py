@@ -30,6 +30,11 @@ How each part is pinned (the tests named here live in ``tests/``):
   ``int A dw = pi |T psi|^2``.
 * ``eigen``: dense ``eigh`` brute force.
 * ``average``: closed form of the Boltzmann average.
+* ``spectra.rixs_map``: dense Kramers-Heisenberg amplitudes in the full Fock
+  space of a small system (``test_oracle_pins.py``).
+* ``mlp`` (the surrogate network, no reference counterpart): float64
+  gradients against central finite differences; the BF16 emulation mode
+  against the float64 mode within BF16 rounding.
 
 The reference's own pickled data (``h0/*.pickle``, the object arrays of
 ``referenceOutput/.../data.npz``) is not loaded: the safe loader

@@ -2,7 +2,8 @@
 
 Restates ``getGreen`` (impurityModel/ed/spectra.py:551-681, sparse branch),
 ``getSpectra`` (spectra.py:684-806, one-rank ``H_build`` branch 778-803),
-the XAS/PS/XPS composition of ``simulate_spectra`` (spectra.py:124-238) and
+``getRIXSmap`` (spectra.py:809-1130, ``wIn`` branch 1012-1129), the
+XAS/PS/XPS/RIXS composition of ``simulate_spectra`` (spectra.py:124-293) and
 ``thermal_average`` (impurityModel/ed/average.py:11-45).
 """
 
@@ -72,6 +73,91 @@ def get_spectra(n, hOp, tOps, psis, es, w, delta, restrictions=None, krylovSize=
                 n, es[i], psiR, hOp, w, delta, krylovSize, slaterWeightMin, restrictions, h, reorth=reorth
             )
     return gs
+
+
+def rixs_map(n, hOp, tOpsIn, tOpsOut, psis, es, wIns, wLoss, delta1, delta2, restrictions=None, krylovSize=150,
+             slaterWeightMin=1e-7, alias_memo=False, reorth="none"):
+    """RIXS Green's functions gs[iE, tIn, tOut, iwIn, :] (spectra.py:809-1130).
+
+    Per (tIn, eigenstate): psi1 = T_in psi (memo per tIn, 1012-1021), the
+    core-hole sector H by BFS expansion from psi1's keys (one h_dict shared
+    by all, 1024-1045), y = psi1 as a vector (1051-1054); per incoming
+    energy x = (wIn + i delta1 + e - H)^-1 y (1066-1080) and psi2 = x as a
+    dict over the nonzero entries in basis order (1087-1090); per outgoing
+    operator psi3 = T_out psi2, normalized, pruned at slaterWeightMin
+    (1094-1111), and norm^2 * getGreen(e, psi3, wLoss, delta2) (1113-1125).
+
+    The reference solves with scipy's ``bicgstab`` at its default tolerance
+    (relative residual 1e-5), an approximation of the exact solution; this
+    restatement solves exactly (sparse LU), which is what a tight-tolerance
+    solver converges to.  ``alias_memo`` reproduces the reference's single
+    memo shared by all out-going operators (``[{}] * len(tOpsOut)``,
+    spectra.py:910): the first operator fills it for every state it meets,
+    so later operators reuse the first one's T_out|state>.
+    """
+    import scipy.sparse
+    import scipy.sparse.linalg
+
+    ne = len(es)
+    gs = np.zeros((ne, len(tOpsIn), len(tOpsOut), len(wIns), len(wLoss)), dtype=complex)
+    h_exc = {}
+    h_ground = {}
+    shared = {}
+    t_out_memo = [shared] * len(tOpsOut) if alias_memo else [{} for _ in tOpsOut]
+    for tIn, tOpIn in enumerate(tOpsIn):
+        t_in_memo = {}
+        for iE in range(ne):
+            e = es[iE]
+            psi1 = apply_op(n, tOpIn, psis[iE], slaterWeightMin, restrictions, t_in_memo)
+            h, index = expand_basis_and_hamiltonian(n, h_exc, hOp, psi1.keys(), restrictions)
+            nb = len(index)
+            y = np.zeros(nb, dtype=complex)
+            for ps, amp in psi1.items():
+                y[index[ps]] = amp
+            states = [None] * nb
+            for ps, i in index.items():
+                states[i] = ps
+            for iw, wIn in enumerate(wIns):
+                a = scipy.sparse.identity(nb, dtype=complex, format="csc") * (wIn + 1j * delta1 + e) - h.tocsc()
+                x = scipy.sparse.linalg.spsolve(a, y) if nb else np.zeros(0, complex)
+                x = np.atleast_1d(x)
+                psi2 = {states[i]: amp for i, amp in enumerate(x) if amp != 0}
+                for tOut, tOpOut in enumerate(tOpsOut):
+                    psi3 = apply_op(n, tOpOut, psi2, slaterWeightMin, restrictions, t_out_memo[tOut])
+                    normalization = sqrt(norm2(psi3))
+                    for s in psi3.keys():
+                        psi3[s] /= normalization
+                    for s, amp in list(psi3.items()):
+                        if abs(amp) ** 2 < slaterWeightMin:
+                            psi3.pop(s)
+                    gs[iE, tIn, tOut, iw, :] = normalization**2 * get_green(
+                        n, e, psi3, hOp, wLoss, delta2, krylovSize, slaterWeightMin, restrictions, h_ground,
+                        reorth=reorth)
+    return gs
+
+
+def rixs(n, hOp, tOpsIn, tOpsOut, psis, es, wIns, wLoss, delta1, delta2, restrictions, T=300, krylovSize=150,
+         alias_memo=False, reorth="none"):
+    """RIXS branch of simulate_spectra (spectra.py:251-293).
+
+    Returns ``(gs, a, aSum)``: the Green's functions, the thermal average of
+    -Im gs over eigenstates and its sum over in/out operators, [nwIn, nwLoss].
+    """
+    gs = rixs_map(n, hOp, tOpsIn, tOpsOut, psis, es, wIns, wLoss, delta1, delta2, restrictions, krylovSize,
+                  alias_memo=alias_memo, reorth=reorth)
+    a = thermal_average(es[: np.shape(gs)[0]], -gs.imag, T=T)
+    return gs, a, np.sum(a, axis=(0, 1))
+
+
+def rixs_bin(wIns, wLoss, aSum):
+    """The float32 RIXS.bin table (spectra.py:287-293): [0,0] = len(wIn),
+    first row wIn, first column wLoss, body aSum transposed to [wLoss, wIn]."""
+    tmp = np.zeros((len(wLoss) + 1, len(wIns) + 1), dtype=np.float32)
+    tmp[0, 0] = len(wIns)
+    tmp[0, 1:] = wIns
+    tmp[1:, 0] = wLoss
+    tmp[1:, 1:] = np.asarray(aSum).T
+    return tmp
 
 
 def energy_cut(es, T=300, cut=10.0):

@@ -228,6 +228,45 @@ def test_eigensolver_variants_vs_dense(gpu, hy):
         engine.set_tuning("eigen_dgks", 1)
 
 
+def test_flat_kernels_match_workgroup_kernels(gpu):
+    """The grid-wide (flat) Lanczos and eigensolver used for large sectors give
+    the workgroup kernels' results; every SpMV lanes-per-row shape."""
+    from impuritymodel_amd import engine
+    from impuritymodel_amd.batched import ground_bases
+    from impuritymodel_amd.models import make_model
+
+    models = [make_model(n_bath=nb, seed=40 + nb, hField=(0.0, hy, 1e-4)) for nb, hy in ((20, 0.0), (30, 1e-3))]
+    ops = engine.Ops([m.hOp for m in models], [m.restrictions for m in models], max(m.n_spin_orbitals for m in models))
+    b = engine.Batch(ops, np.arange(2, dtype=np.int32))
+    b.set_seeds(ground_bases(models, ops.W))
+    b.build(fixed_basis=True)
+    ref_ev, _, _, ok = b.eigen(3, tol=1e-12)
+    assert ok
+    rng = np.random.default_rng(1)
+    v0 = rng.normal(size=b.rows) + 1j * rng.normal(size=b.rows)
+    for c in range(2):
+        r0, r1 = b.row_base[c], b.row_base[c + 1]
+        v0[r0:r1] /= np.linalg.norm(v0[r0:r1])
+    ra, rb, rk, rs = b.lanczos(12, v0=v0)
+    try:
+        engine.set_tuning("flat_min_n", 0)
+        for lpr in (4, 8, 16, 32, 64):
+            engine.set_tuning("spmv_lpr", lpr)
+            fa, fb, fk, fs = b.lanczos(12, v0=v0)
+            assert np.array_equal(fk, rk) and np.array_equal(fs, rs)
+            np.testing.assert_allclose(fa, ra, rtol=1e-11, atol=0)
+            np.testing.assert_allclose(fb[:, :11], rb[:, :11], rtol=1e-10, atol=0)
+        for real in (0, 1):
+            engine.set_tuning("eigen_real", real)
+            ev, evecs, steps, ok = b.eigen(3, tol=1e-12)
+            assert ok
+            np.testing.assert_allclose(ev, ref_ev, rtol=0, atol=1e-10)
+    finally:
+        engine.set_tuning("flat_min_n", 32768)
+        engine.set_tuning("spmv_lpr", 8)
+        engine.set_tuning("eigen_real", 1)
+
+
 def test_lanczos_random_hermitian_and_breakdown(gpu):
     from impuritymodel_amd import finite
 
@@ -303,6 +342,73 @@ def test_photoemission_spectra_vs_oracle(gpu):
     gs_g = gs_g + spectra.getSpectra(n, m.hOp, tips, pg, es, w, 0.2, m.restrictions, reorth="full")
     scale = np.abs(gs_o).max()
     np.testing.assert_allclose(gs_g, gs_o, rtol=0, atol=1e-8 * scale)
+
+
+def test_shifted_solve_vs_dense(gpu):
+    """imped_batch_solve: (z - H) x = y for many shifts per chain against a
+    dense solve; chains of different sizes in one launch."""
+    from impuritymodel_amd import engine
+
+    rng = np.random.default_rng(5)
+    sizes = [1, 7, 60, 300]
+    mats = []
+    for n in sizes:
+        a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        mats.append(0.5 * (a + a.conj().T) + np.diag(rng.uniform(-5, 5, n)))
+    rowptr, col, val = [0], [], []
+    for h in mats:
+        for r in range(h.shape[0]):
+            nz = np.nonzero(h[r])[0]
+            col.extend(nz.tolist())
+            # the library stores row r as the conjugate of H's column r
+            val.extend(np.conj(h[nz, r]).tolist())
+            rowptr.append(len(col))
+    ops = engine.Ops([{((0, "c"), (0, "a")): 1.0}], [None], 8)
+    b = engine.Batch(ops, np.zeros(len(sizes), np.int32))
+    # right-hand sides are the seed amplitudes, placed on each chain's first rows
+    ys = [rng.normal(size=n) + 1j * rng.normal(size=n) for n in sizes]
+    keys = [np.arange(n, dtype=np.uint64).reshape(-1, 1) for n in sizes]
+    b.set_seeds(keys, ys)
+    b.set_csr(np.array(sizes), np.array(rowptr), np.array(col), np.array(val))
+    S = 5
+    shifts = rng.uniform(-4, 4, (len(sizes), S)) + 1j * rng.choice([0.3, -0.5, 1.0], (len(sizes), S))
+    steps, resid, ok, X = b.solve(shifts, tol=1e-12, max_steps=400, want_x=True)
+    assert ok and np.all(resid <= 1e-12)
+    assert np.all(steps >= 1) and np.all(steps <= np.array(sizes))
+    for c, (h, y) in enumerate(zip(mats, ys)):
+        n = len(y)
+        for s in range(S):
+            x = np.linalg.solve(shifts[c, s] * np.eye(n) - h, y)
+            base = int(b.row_base[c]) * S + s * n
+            np.testing.assert_allclose(X[base:base + n], x, rtol=0, atol=1e-10 * np.abs(x).max())
+    # a batch given only a CSR has no basis keys to attach to the solutions
+    with pytest.raises(Exception, match="no basis keys"):
+        b.vec_from_eigen(np.zeros(1, np.int32), np.zeros(1, np.int32), 0.0)
+
+
+@pytest.mark.parametrize("alias", [False, True])
+def test_rixs_map_vs_oracle(gpu, alias):
+    from impuritymodel_amd import spectra
+    from impuritymodel_amd.models import make_model
+    from impuritymodel_amd.operators import getDaggeredDipoleOperators, getDipoleOperators
+
+    m = make_model(n_bath=10, seed=6)
+    n = m.n_spin_orbitals
+    bo = obasis.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    es, po = okry.eigensystem(n, m.hOp, bo, 2, groundDiagMode="full")
+    eps = [[1, 0, 0], [0, 1, 0], [0, 0, 1]]
+    t_in = getDipoleOperators(m.nBaths, eps)
+    t_out = getDaggeredDipoleOperators(m.nBaths, eps)
+    w_in = np.array([-2.0, 0.5, 3.0])
+    w_loss = np.linspace(-2, 10, 121)
+    gs_o = ospec.rixs_map(n, m.hOp, t_in, t_out, po, es, w_in, w_loss, 0.5, 0.1, m.restrictions, 150,
+                          alias_memo=alias, reorth="full")
+    pg = [dict(zip(ints_to_bytes(list(p), n), p.values())) for p in po]
+    gs_g = spectra.getRIXSmap(n, m.hOp, t_in, t_out, pg, es, w_in, w_loss, 0.5, 0.1, m.restrictions, 150,
+                              alias_memo=alias, reorth="full")
+    scale = np.abs(gs_o).max()
+    assert scale > 0
+    np.testing.assert_allclose(gs_g, gs_o, rtol=0, atol=1e-7 * scale)
 
 
 def test_batched_equals_single_chain_runs(gpu):

@@ -279,3 +279,121 @@ def test_thermal_average_closed_form():
     w = np.exp(-(e - e.min()) / (spectra.k_B * 300))
     assert abs(spectra.thermal_average(e, o, 300) - np.sum(w * o) / np.sum(w)) < 1e-15
     assert abs(spectra.k_B - 8.617333262145179e-05) < 1e-20
+
+
+# ------------------------------------------------------------ RIXS
+
+
+def _rixs_toy(seed):
+    """8 spin-orbitals: 2 'core' (0, 1) and 6 'valence' (2..7), 4 electrons.
+
+    H = random Hermitian one-body part + density-density terms; T_in moves
+    an electron core -> valence, T_out valence -> core."""
+    rng = np.random.default_rng(seed)
+    n = 8
+    hop = {}
+    m = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    m = 0.5 * (m + m.conj().T)
+    for i in range(n):
+        for j in range(n):
+            hop[((i, "c"), (j, "a"))] = complex(m[i, j])
+    for i, j in [(2, 3), (4, 5), (0, 6), (1, 7)]:
+        hop[((i, "c"), (i, "a"), (j, "c"), (j, "a"))] = complex(rng.uniform(0.5, 2.0))
+    t_in = [{((v, "c"), (c, "a")): complex(rng.normal(), rng.normal()) for c in (0, 1) for v in (2, 5, 7)}
+            for _ in range(2)]
+    t_out = [{((c, "c"), (v, "a")): complex(rng.normal(), rng.normal()) for c in (0, 1) for v in (3, 4, 6)}
+             for _ in range(3)]
+    return n, hop, t_in, t_out
+
+
+def _fock_eigenstates(n, hop, n_el, k):
+    hd = op_matrix(n, hop)
+    idx = [s for s in range(1 << n) if bin(s).count("1") == n_el]
+    e, v = np.linalg.eigh(hd[np.ix_(idx, idx)])
+    psis = [{idx[i]: complex(v[i, q]) for i in range(len(idx)) if abs(v[i, q]) > 0} for q in range(k)]
+    return hd, list(e[:k]), psis
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_rixs_map_equals_kramers_heisenberg(seed):
+    """g = <y3|(wl + i d2 + e - H)^-1|y3>, y3 = T_out (win + i d1 + e - H)^-1 T_in psi,
+    evaluated with dense Fock-space matrices (no pruning at weight 0)."""
+    n, hop, t_in, t_out = _rixs_toy(seed)
+    hd, es, psis = _fock_eigenstates(n, hop, 4, 2)
+    w_in = np.array([-1.0, 0.3, 2.0])
+    w_loss = np.linspace(-2, 6, 41)
+    d1, d2 = 0.5, 0.2
+    gs = spectra.rixs_map(n, hop, t_in, t_out, psis, es, w_in, w_loss, d1, d2, None, krylovSize=400,
+                          slaterWeightMin=0.0, reorth="full")
+    eye = np.eye(1 << n)
+    for iE, (psi, e) in enumerate(zip(psis, es)):
+        v = np.zeros(1 << n, complex)
+        for s, a in psi.items():
+            v[s] = a
+        for a_, ti in enumerate(t_in):
+            y = op_matrix(n, ti) @ v
+            for b_, w in enumerate(w_in):
+                x = np.linalg.solve((w + 1j * d1 + e) * eye - hd, y)
+                for c_, to in enumerate(t_out):
+                    y3 = op_matrix(n, to) @ x
+                    ref = np.array([y3.conj() @ np.linalg.solve((wl + 1j * d2 + e) * eye - hd, y3) for wl in w_loss])
+                    np.testing.assert_allclose(gs[iE, a_, c_, b_], ref, rtol=1e-9, atol=1e-12)
+
+
+def test_rixs_memo_alias_reuses_first_out_operator():
+    """With one memo shared by the out-going operators every product state is
+    first met by operator 0, so all out-going channels equal channel 0."""
+    n, hop, t_in, t_out = _rixs_toy(3)
+    _, es, psis = _fock_eigenstates(n, hop, 4, 2)
+    w_in, w_loss = np.array([0.0, 1.0]), np.linspace(-2, 6, 11)
+    alias = spectra.rixs_map(n, hop, t_in, t_out, psis, es, w_in, w_loss, 0.5, 0.2, None, 100, alias_memo=True)
+    first = spectra.rixs_map(n, hop, t_in, [t_out[0]] * 3, psis, es, w_in, w_loss, 0.5, 0.2, None, 100)
+    np.testing.assert_array_equal(alias, first)
+    proper = spectra.rixs_map(n, hop, t_in, t_out, psis, es, w_in, w_loss, 0.5, 0.2, None, 100)
+    assert np.abs(proper[:, :, 1] - alias[:, :, 1]).max() > 1e-3
+
+
+def test_rixs_bin_layout():
+    w_in, w_loss = np.array([1.0, 2.0]), np.array([-1.0, 0.0, 1.0])
+    a = np.arange(6.0).reshape(2, 3)
+    t = spectra.rixs_bin(w_in, w_loss, a)
+    assert t.dtype == np.float32 and t.shape == (4, 3)
+    assert t[0, 0] == 2 and list(t[0, 1:]) == [1.0, 2.0] and list(t[1:, 0]) == [-1.0, 0.0, 1.0]
+    np.testing.assert_array_equal(t[1:, 1:], a.T)
+
+
+# ------------------------------------------------------------ surrogate network
+
+
+def _mlp_case(seed, dims=(8, 24, 20, 12)):
+    from oracle import mlp
+
+    rng = np.random.default_rng(seed)
+    P = sum(dims[i] * dims[i + 1] + dims[i + 1] for i in range(len(dims) - 1))
+    params = rng.normal(0, 0.5, P)
+    x = rng.uniform(-1, 1, (6, dims[0]))
+    t = rng.normal(size=(6, dims[-1]))
+    return mlp, dims, params, x, t
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    mlp, dims, params, x, t = _mlp_case(0)
+    loss, g = mlp.loss_and_grad(params, dims, x, t)
+    rng = np.random.default_rng(1)
+    for i in rng.choice(len(params), 40, replace=False):
+        h = 1e-6
+        p1, p2 = params.copy(), params.copy()
+        p1[i] += h
+        p2[i] -= h
+        fd = (mlp.loss_and_grad(p1, dims, x, t)[0] - mlp.loss_and_grad(p2, dims, x, t)[0]) / (2 * h)
+        assert abs(fd - g[i]) < 1e-7 * max(1.0, abs(g[i]))
+
+
+def test_mlp_oracle_bf16_mode_within_rounding():
+    mlp, dims, params, x, t = _mlp_case(2)
+    assert mlp.to_bf16(np.float32(1.0 + 2.0**-9)) == 1.0          # ties to even
+    assert mlp.to_bf16(np.float32(1.0 + 3 * 2.0**-9)) == 1.0 + 2.0**-7
+    l64, g64 = mlp.loss_and_grad(params, dims, x, t)
+    l16, g16 = mlp.loss_and_grad(params, dims, x, t, bf16=True)
+    assert abs(l16 - l64) < 2e-2 * l64
+    assert np.linalg.norm(g16 - g64) < 3e-2 * np.linalg.norm(g64)
