@@ -833,6 +833,8 @@ int imped_batch_set_seeds(imped_batch batch, const int64_t* seed_off, const uint
       return IMPED_ERR_INVALID;
     }
   b->h_seed_off.assign(seed_off, seed_off + b->C + 1);
+  // device offsets are needed by every seed consumer, built or not (set_csr)
+  h2d(b->seed_off_dev, b->h_seed_off.data(), b->C + 1, stream());
   const int64_t S = b->h_seed_off.back();
   h2d(b->seed_keys, seed_keys, std::max<int64_t>(S, 1) * b->W, stream());
   b->have_seed_amps = seed_amps != nullptr;
@@ -850,6 +852,7 @@ int imped_batch_seed_from_vec(imped_batch batch, imped_vec vec) {
     return IMPED_ERR_INVALID;
   }
   b->h_seed_off = v->h_off;
+  h2d(b->seed_off_dev, b->h_seed_off.data(), b->C + 1, stream());
   const int64_t S = v->total();
   b->seed_keys.reserve(std::max<int64_t>(S, 1) * b->W * 8);
   b->seed_amps.reserve(std::max<int64_t>(S, 1) * 16);
