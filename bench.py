@@ -1,19 +1,31 @@
-"""Benchmark: XAS spectra/s of batched exact diagonalization on one MI355X.
+"""Benchmark: XAS spectra/s of a parameter-fitting loop on one MI355X --
+batched exact diagonalization plus the learned surrogate, against plain
+(unbatched) GPU ED, at L1 error <= 1e-2.
 
-Workload (one "step"): the batched-ED configuration -- 128 NiO-like Anderson
-impurity models with parameters drawn from a seed (10Dq U(0.3, 1.5), F0 U(5, 9),
-charge transfer U(1, 8) eV, V U(0.5, 3), bath size from {10, 20, 30, 40, 50}),
-each giving its thermal, polarization-summed XAS: ground basis, H build,
-lowest 3 eigenstates (tol 1e-12), energy cut, 3 dipole polarizations x kept
-eigenstates Lanczos chains of 200 steps, continued fractions on 2000 omega
-points, Boltzmann average.  complex128 throughout.  The operator dicts are
-built once (synthetic input); everything from operator compilation to the
-spectra on the host is timed.
+Default workload ("fit"), one step = one iteration of a 256-walker
+Metropolis fitting loop over the 8 parameters (10Dq, F0, F2/F4 scale, charge
+transfer, V_eg, V_t2g, xi_3d, T) of a NiO-like 10-bath impurity model: every
+walker proposes a move and needs the thermal, polarization-summed L2,3 XAS of
+its proposal (3 lowest eigenstates, energy cut, 3 dipole polarizations,
+200-step Lanczos, continued fraction on 1000 points, complex128).  Proposals
+the ensemble agrees on are answered by the surrogate network at once; the
+others wait and are evaluated together by one batched exact-ED call once
+half the walkers wait (and at the end of the timed steps), so the timed
+region contains every proposal's spectrum.  Set-up (untimed, reported): a 100k-spectrum
+training set from the batched exact ED, 5-member ensemble training with the
+library's BF16 matrix-core kernels, trust-threshold calibration on 2048
+validation spectra.  After the timed steps every surrogate-served spectrum
+is recomputed by exact ED and its L1 error reported.  The same loop without
+the surrogate gives the batched-ED rate, and per-model unbatched runs of
+the same proposals give the plain GPU ED rate.
 
-Prints one JSON line (see the contract in the repository docs) with a
-roofline figure for the dominant kernel (HIP-event timed inside the timed
-steps) and a CPU baseline: the oracle (pure Python/numpy restatement of the
-reference) on a bounded sample of models, one model per worker process.
+``--workload batched128``: 128 seeded models with bath sizes 10-50 (26-66
+spin-orbitals) per step, exact path only (the batched-ED configuration).
+
+Prints one JSON line with a roofline figure for the dominant kernel
+(HIP-event timed inside the timed steps) and a CPU baseline: the oracle (a
+pure Python/numpy restatement of the reference algorithm) on a bounded
+sample of the same spectra, one model per worker process.
 """
 
 import argparse
@@ -52,8 +64,11 @@ def algorithmic_bytes_eigen(n, nnz, steps, nev):
     return tot
 
 
-def _oracle_xas(args):
-    """One model's XAS with the CPU oracle (reference algorithm, ARPACK ground state)."""
+# ------------------------------------------------------------------ CPU baseline (oracle)
+
+
+def _oracle_batched(args):
+    """One batched-workload model's XAS with the CPU oracle (reference algorithm)."""
     p, omega, K = args
     sys.path.insert(0, ROOT)
     from impuritymodel_amd.models import model_from_params
@@ -70,98 +85,210 @@ def _oracle_xas(args):
     return s
 
 
-def cpu_baseline(params, omega, K, workers, budget_s=25.0):
-    """Oracle spectra/s over a bounded sample: one model per worker."""
-    sample = params[:workers]
+def _oracle_family(theta):
+    """One fitting-loop spectrum with the CPU oracle."""
+    sys.path.insert(0, ROOT)
+    from impuritymodel_amd import surrogate as S
+    from impuritymodel_amd.operators import getDipoleOperators
+    from oracle import basis, krylov, spectra
+
+    fam = S.XASFamily()
+    m = fam.models(np.asarray([theta]))[0]
+    n = m.n_spin_orbitals
+    b = basis.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    es, psis = krylov.eigensystem(n, m.hOp, b, 3, eigenValueTol=1e-12)
+    es = spectra.energy_cut(tuple(es), T=theta[7])
+    tops = getDipoleOperators(m.nBaths, ([1, 0, 0], [0, 1, 0], [0, 0, 1]))
+    _, _, s = spectra.xas(n, m.hOp, tops, psis[: len(es)], es, S.OMEGA, S.DELTA, m.restrictions, T=theta[7],
+                          krylovSize=200)
+    return s
+
+
+def cpu_baseline(fn, items, workers, what):
+    """Oracle spectra/s over a bounded sample: one model per worker process."""
+    sample = items[:workers]
     t0 = time.perf_counter()
     ctx = mp.get_context("spawn")
     with ctx.Pool(workers) as pool:
-        pool.map(_oracle_xas, [(p, omega, K) for p in sample], chunksize=1)
+        pool.map(fn, sample, chunksize=1)
     dt = time.perf_counter() - t0
-    return dict(value=len(sample) / dt, unit="spectra/s", cores=workers, kind="port",
-                sample="%d models of the same parameter distribution (bath sizes %s), one per worker process, "
-                       "%.1f s wall" % (len(sample), sorted(p["n_bath"] for p in sample), dt))
+    return dict(value=round(len(sample) / dt, 4), unit="spectra/s", cores=workers, kind="port",
+                sample="%d %s, one per worker process, %.1f s wall (process start-up included)"
+                       % (len(sample), what, dt))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--models", type=int, default=128)
-    ap.add_argument("--omega", type=int, default=2000)
-    ap.add_argument("--krylov", type=int, default=200)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-workers", type=int, default=0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: the engine runs on a single MI355X; "
-              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
-        sys.exit(2)
+def _roofline(kernels, bytes_by_kernel):
+    """Roofline object of the kernel with the largest timed total."""
+    cand = {k: v for k, v in kernels.items() if k in bytes_by_kernel and v[1] > 0}
+    if not cand:
+        return None
+    dom = max(cand, key=lambda k: cand[k][0])
+    ms_total, launches = cand[dom]
+    per_launch = bytes_by_kernel[dom] / launches
+    avg_ms = ms_total / launches
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return dict(bound="hbm", kernel=dom + "_kernel", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, avg_launch_ms=round(avg_ms, 4),
+                launches=launches, algorithmic_bytes_per_launch=per_launch)
 
-    import torch
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    torch.cuda.set_device(0)
+KERNELS = ("eigen", "lanczos", "apply_count", "apply_fill", "cf", "mlp_gemm", "mlp_adam")
+
+
+# ------------------------------------------------------------------ workloads
+
+
+def run_fit(args, torch):
     from impuritymodel_amd import engine
-    from impuritymodel_amd.batched import xas_batch
+    from impuritymodel_amd import surrogate as S
+    from impuritymodel_amd.batched import plain_xas
+    from impuritymodel_amd.engine import Batch, Ops
+
+    fam = S.XASFamily()
+    sur, info = S.train_surrogate(fam, n_train=args.n_train, epochs=args.epochs, seed=0)
+    print("set-up: %s" % json.dumps({k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()}),
+          file=sys.stderr, flush=True)
+    target_theta = S.sample_params(1, seed=99)[0]
+    target = S.exact_spectra(fam, target_theta[None])[0][0]
+
+    def timed_loop(surrogate):
+        loop = S.FittingLoop(fam, target, surrogate, n_walkers=args.walkers, seed=1)
+        for _ in range(args.warmup):
+            loop.step()
+        loop.finish()
+        torch.cuda.synchronize()
+        engine.timing_reset()
+        loop.reset_counters()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            loop.step()
+        loop.finish()  # proposals still waiting for exact ED are evaluated inside the timed region
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return loop, dt, {k: engine.timing_query(k) for k in KERNELS}
+
+    engine.timing_enable(True)
+    loop_s, dt_s, kern_s = timed_loop(sur)
+    loop_e, dt_e, _ = timed_loop(None)
+    engine.timing_enable(False)
+    value = loop_s.n_spectra / dt_s
+    batched_rate = loop_e.n_spectra / dt_e
+    print("timed: surrogate loop %.3f s (%d spectra, %d by surrogate), exact loop %.3f s"
+          % (dt_s, loop_s.n_spectra, loop_s.n_surrogate, dt_e), file=sys.stderr, flush=True)
+
+    # plain GPU ED: one model and one chain at a time, same proposals
+    plainP = S.sample_params(args.plain_sample, seed=123)
+    t0 = time.perf_counter()
+    plain = [S.area_normalize(plain_xas(m, omega=S.OMEGA, delta=S.DELTA, krylovSize=200, T=p[7]))
+             for m, p in zip(fam.models(plainP), plainP)]
+    plain_rate = len(plainP) / (time.perf_counter() - t0)
+    batched_same = S.exact_spectra(fam, plainP)[0]
+    plain_vs_batched = float(np.abs(np.array(plain) - batched_same).max())
+
+    # score every surrogate-served spectrum against exact ED
+    sp, ss = loop_s.served()
+    if len(sp):
+        exact = np.concatenate([S.exact_spectra(fam, sp[i:i + 4096])[0] for i in range(0, len(sp), 4096)])
+        l1 = S.l1_error(ss, exact)
+        score = dict(n=int(len(l1)), l1_max=float(l1.max()), l1_mean=float(l1.mean()),
+                     l1_p99=float(np.percentile(l1, 99)), frac_le_1e_2=float((l1 <= 1e-2).mean()))
+    else:
+        score = dict(n=0)
+
+    # algorithmic bytes of the exact-ED kernels inside the timed surrogate loop
+    ops = Ops([fam.models(target_theta[None])[0].term_table(1)], None, 26, 1)
+    gb = Batch(ops, [0])
+    gb.set_seeds([fam._bases[1]])
+    gb.build(fixed_basis=True)
+    n_g, nnz_g = int(gb.chain_n[0]), int(gb.chain_nnz[0])
+    eig_bytes = sum(algorithmic_bytes_eigen([n_g] * len(r.eig_steps), [nnz_g] * len(r.eig_steps), r.eig_steps, 3)
+                    for r in loop_s.exact_results)
+    lan_bytes = sum(algorithmic_bytes_lanczos(r.chain_dims, r.chain_nnz, r.kout) for r in loop_s.exact_results)
+    roofline = _roofline(kern_s, {"eigen": eig_bytes, "lanczos": lan_bytes})
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        workers = args.cpu_workers or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(_oracle_family, [tuple(p) for p in S.sample_params(workers, seed=321)], workers,
+                           "fitting-loop spectra (10-bath family, uniform parameters)")
+    return {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "spectra/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt_s / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded parameters; exact spectra from the engine itself)",
+        "config": {
+            "workload": "fitting loop: %d Metropolis walkers over 8 parameters of a NiO-like 10-bath model "
+                        "(26 spin-orbitals), one XAS per proposal (3 eigenstates, 3 polarizations, Lanczos 200, "
+                        "%d omega points, delta %.2f eV, complex128); surrogate = 5 x MLP 8-512-512-512-%d "
+                        "(BF16 MFMA, FP32 accumulate), exact fallback by batched ED"
+                        % (args.walkers, len(S.OMEGA), S.DELTA, len(S.OMEGA)),
+            "walkers": args.walkers,
+            "spectra_timed": int(loop_s.n_spectra),
+            "exact_flushes_timed": int(loop_s.n_flushes),
+            "exact_only_spectra_timed": int(loop_e.n_spectra),
+            "surrogate_fraction": round(loop_s.n_surrogate / max(loop_s.n_spectra, 1), 4),
+            "surrogate_l1_vs_exact": score,
+            "batched_ed_only_spectra_per_s": round(batched_rate, 3),
+            "plain_gpu_ed_spectra_per_s": round(plain_rate, 4),
+            "plain_sample": int(len(plainP)),
+            "plain_vs_batched_max_abs_diff": plain_vs_batched,
+            "speedup_vs_plain_gpu_ed": round(value / plain_rate, 2),
+            "speedup_batched_ed_vs_plain": round(batched_rate / plain_rate, 2),
+            "setup": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()},
+            "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kern_s.items() if v[1]},
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+
+
+def run_batched128(args, torch):
+    from impuritymodel_amd import engine
+    from impuritymodel_amd.batched import ground_bases, xas_batch
+    from impuritymodel_amd.engine import Batch, Ops
     from impuritymodel_amd.models import batched_params, model_from_params
 
     params = batched_params(args.models, seed=0)
     models = [model_from_params(p) for p in params]
-    omega = np.linspace(-25.0, 25.0, args.omega)
-
+    omega = np.linspace(-25.0, 25.0, 2000)
     for _ in range(args.warmup):
-        res = xas_batch(models, omega=omega, krylovSize=args.krylov)
+        res = xas_batch(models, omega=omega, krylovSize=200)
     torch.cuda.synchronize()
     engine.timing_enable(True)
     engine.timing_reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = xas_batch(models, omega=omega, krylovSize=args.krylov)
+        res = xas_batch(models, omega=omega, krylovSize=200)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    kernels = {k: engine.timing_query(k) for k in ("eigen", "lanczos", "apply_count", "apply_fill", "cf")}
+    kernels = {k: engine.timing_query(k) for k in KERNELS}
     engine.timing_enable(False)
-    value = args.models * args.steps / dt
-
-    # dominant kernel and its algorithmic bytes per launch
-    dom = max(kernels, key=lambda k: kernels[k][0])
-    ms_total, launches = kernels[dom]
-    if dom == "eigen":
-        from impuritymodel_amd.batched import ground_bases
-        from impuritymodel_amd.engine import Batch, Ops
-
-        # sizes of the ground-state sectors (identical every step)
-        n_orb = max(m.n_spin_orbitals for m in models)
-        gb_ops = Ops([m.hOp for m in models], [m.restrictions for m in models], n_orb)
-        gb = Batch(gb_ops, np.arange(len(models), dtype=np.int32))
-        gb.set_seeds(ground_bases(models, gb_ops.W))
-        gb.build(fixed_basis=True)
-        per_step = algorithmic_bytes_eigen(gb.chain_n, gb.chain_nnz, res.eig_steps, 3)
-    elif dom == "lanczos":
-        per_step = algorithmic_bytes_lanczos(res.chain_dims, res.chain_nnz, res.kout)
-    else:
-        per_step = None
-    roofline = None
-    if per_step is not None:
-        launches_per_step = launches / args.steps
-        per_launch = per_step / launches_per_step
-        avg_ms = ms_total / launches
-        achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        roofline = dict(bound="hbm", kernel=dom + "_kernel", achieved=round(achieved, 2), peak=HBM_PEAK_GBS,
-                        unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                        avg_launch_ms=round(avg_ms, 4), algorithmic_bytes_per_launch=per_launch)
-
+    n_orb = max(m.n_spin_orbitals for m in models)
+    gb_ops = Ops([m.hOp for m in models], [m.restrictions for m in models], n_orb)
+    gb = Batch(gb_ops, np.arange(len(models), dtype=np.int32))
+    gb.set_seeds(ground_bases(models, gb_ops.W))
+    gb.build(fixed_basis=True)
+    roofline = _roofline(kernels, {
+        "eigen": args.steps * algorithmic_bytes_eigen(gb.chain_n, gb.chain_nnz, res.eig_steps, 3),
+        "lanczos": args.steps * algorithmic_bytes_lanczos(res.chain_dims, res.chain_nnz, res.kout)})
     cpu = None
     if not args.no_cpu_baseline:
         workers = args.cpu_workers or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(params, omega, args.krylov, workers)
-
-    out = {
+        cpu = cpu_baseline(_oracle_batched, [(p, omega, 200) for p in params], workers,
+                           "models of the same parameter distribution")
+    return {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(args.models * args.steps / dt, 3),
         "unit": "spectra/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -175,16 +302,41 @@ def main():
         "config": {
             "workload": "batched exact-diagonalization XAS, %d NiO-like impurity models per step (bath sizes "
                         "10-50 spin-orbitals, 26-66 spin-orbitals), 3 polarizations x <=3 thermal states, "
-                        "Lanczos %d steps, %d omega points, complex128; exact path (no surrogate)"
-                        % (args.models, args.krylov, args.omega),
+                        "Lanczos 200 steps, 2000 omega points, complex128; exact path (no surrogate)" % args.models,
             "models_per_step": args.models,
             "chains_per_step": int(len(res.chain_dims)),
             "chain_dim_range": [int(res.chain_dims.min()), int(res.chain_dims.max())],
-            "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kernels.items()},
+            "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kernels.items() if v[1]},
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("fit", "batched128"), default="fit")
+    ap.add_argument("--models", type=int, default=128)
+    ap.add_argument("--walkers", type=int, default=256)
+    ap.add_argument("--n-train", type=int, default=100000)
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the engine runs on a single MI355X; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    out = run_fit(args, torch) if args.workload == "fit" else run_batched128(args, torch)
     print(json.dumps(out))
 
 

@@ -116,7 +116,7 @@ def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0,
                 spec_id.append(m)
                 zflag.append(1 if t == 0 else 0)
     seeds = vec_apply(ops_t, psis, out_src, out_model, 1e-7, normalize=True)
-    _, _, norm2 = seeds.get()
+    norm2 = seeds.norm2()
     tim["transition"] = time.perf_counter() - t1
 
     t1 = time.perf_counter()

@@ -92,9 +92,10 @@ struct Tuning {
   int64_t lanczos_wave_max_n = 4096;    // chains up to this size: one wave each
   int64_t lanczos_wg256_max_n = 65536;  // then 256 threads, above: 1024
   int64_t eigen_bt = 512;               // eigensolver threads per chain (256, 512, 1024)
+  int64_t eigen_wave_max_n = 0;         // eigensolver chains up to this size: one wave each (off)
   int64_t eigen_real = 1;               // real Krylov vectors when H is real
   int64_t eigen_dgks = 1;               // second Gram-Schmidt pass only when needed
-  int64_t eigen_check_every = 4;        // Ritz convergence check interval
+  int64_t eigen_check_every = 8;        // Ritz convergence check interval
   int64_t flat_min_n = 32768;           // chains above this size use the grid-wide (flat) kernels
   int64_t spmv_lpr = 8;                 // lanes per CSR row in the flat SpMV (4..64)
 };

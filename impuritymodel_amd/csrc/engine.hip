@@ -730,7 +730,9 @@ int imped_set_tuning(const char* name, int64_t value) {
       return IMPED_ERR_INVALID;
     }
     g_tuning.eigen_bt = value;
-  } else if (n == "eigen_real")
+  } else if (n == "eigen_wave_max_n")
+    g_tuning.eigen_wave_max_n = value;
+  else if (n == "eigen_real")
     g_tuning.eigen_real = value != 0;
   else if (n == "eigen_dgks")
     g_tuning.eigen_dgks = value != 0;

@@ -228,60 +228,76 @@ __device__ double kth_eigenvalue(const double* a, const double* b, int m, int k)
 }
 
 // Inverse iteration for the eigenvector of (a, b) at lambda; LU with partial
-// pivoting of the shifted tridiagonal.  y and work (5m) are thread-private.
+// pivoting of the shifted tridiagonal.  y and work (4m + m ints) are
+// thread-private (LDS or global).  Every recurrence carries its running
+// values in registers, so no iteration waits on a store it just made.
 __device__ void tri_inverse_iteration(const double* a, const double* b, int m, double lambda, double* y, double* dd,
                                       double* uu, double* ww, double* ll, int* piv, uint64_t seed) {
   double scale = 0.0;
   for (int i = 0; i < m; ++i) scale = fmax(scale, fabs(a[i]) + (i < m - 1 ? fabs(b[i]) : 0.0));
   const double tiny = 2.2e-16 * (scale > 0 ? scale : 1.0);
-  for (int i = 0; i < m; ++i) {
-    dd[i] = a[i] - lambda;
-    uu[i] = i < m - 1 ? b[i] : 0.0;
-    ww[i] = 0.0;
-  }
+  // (d, u): diagonal and superdiagonal of the current row after elimination
+  double d = a[0] - lambda, u = m > 1 ? b[0] : 0.0;
   for (int i = 0; i < m - 1; ++i) {
     const double sub = b[i];
-    if (fabs(dd[i]) >= fabs(sub)) {
-      if (dd[i] == 0.0) dd[i] = tiny;
-      const double mult = sub / dd[i];
-      dd[i + 1] -= mult * uu[i];
+    const double dn = a[i + 1] - lambda;
+    const double un = i + 1 < m - 1 ? b[i + 1] : 0.0;
+    if (fabs(d) >= fabs(sub)) {
+      if (d == 0.0) d = tiny;
+      const double mult = sub / d;
+      dd[i] = d;
+      uu[i] = u;
+      ww[i] = 0.0;
       ll[i] = mult;
       piv[i] = 0;
+      d = dn - mult * u;
+      u = un;
     } else {
-      const double mult = dd[i] / sub;
-      const double d1 = dd[i + 1];
-      const double u1 = uu[i + 1];
-      const double u0 = uu[i];
+      // rows i and i+1 interchanged
+      const double mult = d / sub;
       dd[i] = sub;
-      uu[i] = d1;
-      ww[i] = u1;
-      dd[i + 1] = u0 - mult * d1;
-      uu[i + 1] = -mult * u1;
+      uu[i] = dn;
+      ww[i] = un;
       ll[i] = mult;
       piv[i] = 1;
+      d = u - mult * dn;
+      u = -mult * un;
     }
   }
-  if (dd[m - 1] == 0.0) dd[m - 1] = tiny;
+  if (d == 0.0) d = tiny;
+  dd[m - 1] = d;
+  uu[m - 1] = 0.0;
+  ww[m - 1] = 0.0;
   for (int i = 0; i < m; ++i) y[i] = 0.5 + u01(seed + (uint64_t)i);
   for (int it = 0; it < 3; ++it) {
+    // forward: L^-1 with the interchanges; yi is the running y[i]
+    double yi = y[0];
     for (int i = 0; i < m - 1; ++i) {
+      double yn = y[i + 1];
       if (piv[i]) {
-        const double t = y[i];
-        y[i] = y[i + 1];
-        y[i + 1] = t;
+        const double t = yi;
+        yi = yn;
+        yn = t;
       }
-      y[i + 1] -= ll[i] * y[i];
+      yn -= ll[i] * yi;
+      y[i] = yi;
+      yi = yn;
     }
+    y[m - 1] = yi;
+    // back substitution with U (bandwidth 3); y1, y2 = y[i+1], y[i+2]
+    double y1 = 0.0, y2 = 0.0, nrm = 0.0;
     for (int i = m - 1; i >= 0; --i) {
       double r = y[i];
-      if (i + 1 < m) r -= uu[i] * y[i + 1];
-      if (i + 2 < m) r -= ww[i] * y[i + 2];
-      double d = dd[i];
-      if (fabs(d) < tiny) d = d < 0 ? -tiny : tiny;
-      y[i] = r / d;
+      if (i + 1 < m) r -= uu[i] * y1;
+      if (i + 2 < m) r -= ww[i] * y2;
+      double dv = dd[i];
+      if (fabs(dv) < tiny) dv = dv < 0 ? -tiny : tiny;
+      const double v = r / dv;
+      y[i] = v;
+      nrm += v * v;
+      y2 = y1;
+      y1 = v;
     }
-    double nrm = 0.0;
-    for (int i = 0; i < m; ++i) nrm += y[i] * y[i];
     nrm = sqrt(nrm);
     for (int i = 0; i < m; ++i) y[i] /= nrm;
   }
@@ -312,6 +328,9 @@ struct EigenArgs {
   const int32_t* chain_list;
   int dgks;          // 1: second Gram-Schmidt pass only when needed
   int check_every;   // Ritz check interval (steps)
+  int lstride;       // LDS arrays: Gram-Schmidt coefficients (lstride + 2), then optionally
+  int lds_ab;        //   alpha/beta (2 lstride) and, per eigenvalue, y (lstride),
+  int lds_work;      //   LU work (4 lstride) and pivots (lstride ints); lstride >= every chain's steps
 };
 
 // Element helpers: the eigensolver runs on complex (double2) vectors, or on
@@ -439,6 +458,8 @@ __device__ double update_norm(const T* V, int m, int64_t n, T* w, const double2*
   double p = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += BT) {
     double wx = re_(w[i]), wy = im_(w[i]);
+    // unrolled so the loads of eight Krylov entries are in flight together
+#pragma unroll 8
     for (int l = 0; l < m; ++l) {
       const double2 c = cl[l];
       const T av = V[(size_t)l * n + i];
@@ -520,7 +541,7 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   __shared__ double s_lam[MAX_NEV], s_lo[MAX_NEV], s_hi[MAX_NEV];
   __shared__ int s_first[MAX_NEV];
   __shared__ int s_flag;
-  extern __shared__ double2 cl[];  // max_steps + 2 coefficients
+  extern __shared__ double2 cl[];  // lstride + 2 coefficients, then the optional arrays
   const int c = a.chain_list[blockIdx.x];
   const int64_t n = a.chain_n[c];
   const int64_t base = a.row_base[c];
@@ -536,10 +557,22 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   const int mmax = (int)(ms < n ? ms : n);
   T* V = reinterpret_cast<T*>(a.vstore) + a.voff[c];
   T* w = reinterpret_cast<T*>(a.wbuf) + base;
-  double2* coef = a.coef + (size_t)c * ms;
   double* alpha = a.ab + (size_t)c * 2 * ms;
   double* beta = alpha + ms;
-  double* Y = a.yk + (size_t)c * nev * ms;
+  // tridiagonal and Ritz work arrays: LDS when they fit (lstride >= mmax)
+  const int L = a.lstride;
+  double* tail = reinterpret_cast<double*>(cl + L + 2);
+  double* sa = alpha;
+  double* sb = beta;
+  if (a.lds_ab) {
+    sa = tail;
+    sb = tail + L;
+    tail += 2 * L;
+  }
+  const int ys = a.lds_work ? L : ms;  // stride of eigenvalue k's arrays
+  double* Y = a.lds_work ? tail : a.yk + (size_t)c * nev * ms;
+  double* twb = a.lds_work ? tail + (size_t)nev * L : a.tw + (size_t)c * nev * 5 * ms;
+  int* pivb = a.lds_work ? reinterpret_cast<int*>(twb + (size_t)nev * 4 * L) : a.tpiv + (size_t)c * nev * ms;
   // seeded random start vector (real when T is real)
   double p = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += BT) {
@@ -560,7 +593,7 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
     // (coefficients taken from H v_j itself would carry errors of order
     // eps*|H v_j| and lose orthogonality geometrically)
     double al = block_sum<BT>(spmv_t<BT, T>(a.rowptr, a.col, a.val, base, n, vj, w), red);
-    const double bprev = j > 0 ? beta[j - 1] : 0.0;
+    const double bprev = j > 0 ? sb[j - 1] : 0.0;
     double p0 = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += BT) {
       double xr = re_(w[i]) - al * re_(vj[i]);
@@ -582,25 +615,31 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
       dots_lds<BT, T>(V, j + 1, n, w, cl, false);
       p1 = update_norm<BT, T>(V, j + 1, n, w, cl, red);
     }
-    if (threadIdx.x == 0) alpha[j] = al;
+    if (threadIdx.x == 0) {
+      alpha[j] = al;
+      sa[j] = al;
+    }
     const double b = sqrt(p1);
     m = j + 1;
     const bool last = (m == mmax) || b <= 1e-13 * (fabs(al) + bprev + 1e-300);
-    if (threadIdx.x == 0) beta[j] = b;
+    if (threadIdx.x == 0) {
+      beta[j] = b;
+      sb[j] = b;
+    }
     __syncthreads();
     if (last || (m >= nev && m % a.check_every == 0)) {
       // Ritz pairs of T_m: eigenvalues by workgroup multisection, then
       // thread k refines eigenvector k by inverse iteration
-      ritz_values<BT>(alpha, beta, m, nev < m ? nev : m, s_lam, s_lo, s_hi, s_first, &s_flag);
+      ritz_values<BT>(sa, sb, m, nev < m ? nev : m, s_lam, s_lo, s_hi, s_first, &s_flag);
       if (threadIdx.x == 0) s_done = 1;
       __syncthreads();
       if ((int)threadIdx.x < nev && (int)threadIdx.x < m) {
         const int k = threadIdx.x;
         const double lam = s_lam[k];
-        double* y = Y + (size_t)k * ms;
-        double* tw = a.tw + ((size_t)c * nev + k) * 5 * ms;
-        int* piv = a.tpiv + ((size_t)c * nev + k) * ms;
-        tri_inverse_iteration(alpha, beta, m, lam, y, tw, tw + ms, tw + 2 * ms, tw + 3 * ms, piv,
+        double* y = Y + (size_t)k * ys;
+        double* tw = twb + (size_t)k * (a.lds_work ? 4 * ys : 5 * ys);
+        int* piv = pivb + (size_t)k * ys;
+        tri_inverse_iteration(sa, sb, m, lam, y, tw, tw + ys, tw + 2 * ys, tw + 3 * ys, piv,
                               a.seed + 7919 * (uint64_t)k);
         a.evals[(size_t)c * nev + k] = lam;
         const double res = fabs(b * y[m - 1]);
@@ -610,11 +649,11 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
       if (threadIdx.x == 0 && nev > 1) {
         // orthogonalize inverse-iteration vectors of clustered eigenvalues
         for (int k = 1; k < nev && k < m; ++k) {
-          double* yk = Y + (size_t)k * ms;
+          double* yk = Y + (size_t)k * ys;
           for (int q = 0; q < k; ++q) {
             const double dl = fabs(a.evals[(size_t)c * nev + k] - a.evals[(size_t)c * nev + q]);
             if (dl > 1e-3 * fmax(1.0, fabs(a.evals[(size_t)c * nev + k]))) continue;
-            const double* yq = Y + (size_t)q * ms;
+            const double* yq = Y + (size_t)q * ys;
             double d = 0.0;
             for (int i = 0; i < m; ++i) d += yq[i] * yk[i];
             double nn = 0.0;
@@ -641,10 +680,11 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   double2* X = a.evecs + base * nev;
   const int kk = nev < m ? nev : m;
   for (int k = 0; k < kk; ++k) {
-    const double* y = Y + (size_t)k * ms;
+    const double* y = Y + (size_t)k * ys;
     double pn = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += BT) {
       double xr = 0.0, xi = 0.0;
+#pragma unroll 8
       for (int l = 0; l < m; ++l) {
         const T v = V[(size_t)l * n + i];
         xr += y[l] * re_(v);
@@ -1445,8 +1485,10 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   KScratch& sc = kscratch();
   const int C = kb.C;
   if (C == 0) return IMPED_OK;
-  if (nev > MAX_NEV || max_steps < 1 || max_steps > 4000) {
-    set_error("eigensolver: nev must be <= 16 and max_steps in [1, 4000]");
+  // max_steps bound: the Gram-Schmidt coefficients (max_steps + 2 complex)
+  // live in LDS next to the kernel's static arrays
+  if (nev > MAX_NEV || max_steps < 1 || max_steps > 3500) {
+    set_error("eigensolver: nev must be <= 16 and max_steps in [1, 3500]");
     return IMPED_ERR_INVALID;
   }
   const bool real = real_values && g_tuning.eigen_real;
@@ -1495,14 +1537,41 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   a.conv = sc.conv.as<int32_t>();
   a.dgks = (int)g_tuning.eigen_dgks;
   a.check_every = (int)std::max<int64_t>(1, g_tuning.eigen_check_every);
-  // one launch, largest chains first so the longest workgroups start first
+  // largest chains first so the longest workgroups start first; chains up to
+  // eigen_wave_max_n get one wave each (reductions by shuffles, no barriers),
+  // the others eigen_bt threads
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kb.h_n[x] > kb.h_n[y]; });
   if (!order.empty()) h2d(sc.clist, order.data(), order.size(), s);
-  a.chain_list = sc.clist.as<int32_t>();
-  const size_t shm = (size_t)(max_steps + 2) * sizeof(double2);
-  if (!order.empty()) {
+  size_t n_wide = 0;
+  while (n_wide < order.size() && kb.h_n[order[n_wide]] > g_tuning.eigen_wave_max_n) ++n_wide;
+  // LDS layout of a launch whose largest chain is order[first]: coefficients
+  // always, alpha/beta and the Ritz work arrays when they fit in 60 KiB (the
+  // 64 KiB per-workgroup limit less the kernel's static arrays)
+  size_t shm = 0;
+  auto layout = [&](size_t first) {
+    const int L = (int)std::min<int64_t>(max_steps, kb.h_n[order[first]]);
+    const size_t coef = (size_t)(L + 2) * sizeof(double2), ab = 2 * (size_t)L * 8;
+    const size_t work = (size_t)nev * ((size_t)L * 8 + 4 * (size_t)L * 8 + (size_t)L * 4);
+    constexpr size_t budget = 60 * 1024;
+    a.lstride = L;
+    a.lds_ab = coef + ab <= budget;
+    a.lds_work = a.lds_ab && coef + ab + work <= budget;
+    shm = coef + (a.lds_ab ? ab : 0) + (a.lds_work ? work : 0);
+  };
+  if (n_wide < order.size()) {
     TimedLaunch tl("eigen", s);
-    const dim3 g((unsigned)order.size());
+    layout(n_wide);
+    a.chain_list = sc.clist.as<int32_t>() + n_wide;
+    const dim3 g((unsigned)(order.size() - n_wide));
+    if (real) hipLaunchKernelGGL((eigen_kernel<64, double>), g, dim3(64), shm, s, a);
+    else hipLaunchKernelGGL((eigen_kernel<64, double2>), g, dim3(64), shm, s, a);
+    HIP_CHECK(hipGetLastError());
+  }
+  a.chain_list = sc.clist.as<int32_t>();
+  if (n_wide > 0) {
+    TimedLaunch tl("eigen", s);
+    layout(0);
+    const dim3 g((unsigned)n_wide);
     if (g_tuning.eigen_bt == 256) {
       if (real) hipLaunchKernelGGL((eigen_kernel<256, double>), g, dim3(256), shm, s, a);
       else hipLaunchKernelGGL((eigen_kernel<256, double2>), g, dim3(256), shm, s, a);
@@ -1538,8 +1607,8 @@ int run_solve(const KrylovBatch& kb, const double2* y, int S, const double* z_ho
               double2* X, int32_t* steps_host, double* resid_host, hipStream_t s) {
   KScratch& sc = kscratch();
   const int C = kb.C;
-  if (S < 1 || max_steps < 1 || max_steps > 4000) {
-    set_error("shifted solve: need at least one shift and max_steps in [1, 4000]");
+  if (S < 1 || max_steps < 1 || max_steps > 3500) {
+    set_error("shifted solve: need at least one shift and max_steps in [1, 3500]");
     return IMPED_ERR_INVALID;
   }
   for (size_t q = 0; q < (size_t)C * S; ++q)

@@ -153,6 +153,50 @@ class TermTable:
         return t
 
 
+def _compile_periodic(tts, W):
+    """compile_models for a list whose term structures repeat with a period p
+    (tables sharing their arrays, as a model family or the per-model dipole
+    operators give): the structure arrays are tiled, only the values are
+    gathered per model.  None when the list has no such period."""
+    M = len(tts)
+    if M < 2:
+        return None
+    p = 1
+    while p <= M // 2 and tts[p].masks is not tts[0].masks:
+        p += 1
+    if p > M // 2 or M % p:
+        return None
+    for i in range(p, M):
+        a, b = tts[i], tts[i % p]
+        if a.masks is not b.masks or a.rmask is not b.rmask or a.rlohi is not b.rlohi:
+            return None
+    reps = M // p
+    # structure of one period
+    g_cnt = np.array([len(t.group_term_off) - 1 for t in tts[:p]], np.int64)
+    t_cnt = np.array([len(t.coef) for t in tts[:p]], np.int64)
+    r_cnt = np.array([len(t.rmask) for t in tts[:p]], np.int64)
+    T = int(t_cnt.sum())
+    g_off = np.concatenate([t.group_term_off[:-1] + o for t, o in zip(tts[:p], np.concatenate([[0], np.cumsum(t_cnt)[:-1]]))])
+    group_term_off = (g_off[None, :].astype(np.int64) + T * np.arange(reps)[:, None]).reshape(-1)
+    group_term_off = np.concatenate([group_term_off, [T * reps]]).astype(np.int32)
+    coef = np.concatenate([t.coef for t in tts]) if T else np.zeros(0, complex)
+    return dict(
+        model_group_off=np.concatenate([[0], np.cumsum(np.tile(g_cnt, reps))]).astype(np.int32),
+        group_term_off=group_term_off,
+        group_flip=np.ascontiguousarray(np.tile(np.concatenate([t.group_flip for t in tts[:p]]).reshape(-1, W),
+                                                (reps, 1)), np.uint64),
+        term_masks=np.ascontiguousarray(np.tile(np.concatenate([t.masks for t in tts[:p]]).reshape(-1, 3 * W),
+                                                (reps, 1)), np.uint64),
+        term_meta=np.ascontiguousarray(np.tile(np.concatenate([t.meta for t in tts[:p]]), reps), np.uint32),
+        term_coef=np.ascontiguousarray(np.stack([coef.real, coef.imag], axis=-1).reshape(-1), np.float64),
+        model_restr_off=np.concatenate([[0], np.cumsum(np.tile(r_cnt, reps))]).astype(np.int32),
+        restr_mask=np.ascontiguousarray(np.tile(np.concatenate([t.rmask for t in tts[:p]]).reshape(-1, W),
+                                                (reps, 1)), np.uint64),
+        restr_lohi=np.ascontiguousarray(np.tile(np.concatenate([t.rlohi for t in tts[:p]]).reshape(-1), reps),
+                                        np.int32),
+    )
+
+
 def compile_models(ops, restrictions, W):
     """Concatenate the term tables of several operators.
 
@@ -160,14 +204,19 @@ def compile_models(ops, restrictions, W):
     ``restrictions`` a list (same length) of restriction dicts or None.
     Returns a dict of arrays matching ``imped_ops_create``.
     """
-    mg, gt, gf, tm, meta, coef, mr, rm, rl = [0], [], [], [], [], [], [0], [], []
-    t_base = 0
+    tts = []
     for m, op in enumerate(ops):
         if isinstance(op, TermTable):
             assert op.W == W, "term table compiled for another word count"
-            tt = op
+            tts.append(op)
         else:
-            tt = TermTable(op, restrictions[m] if restrictions is not None else None, W)
+            tts.append(TermTable(op, restrictions[m] if restrictions is not None else None, W))
+    tiled = _compile_periodic(tts, W)
+    if tiled is not None:
+        return tiled
+    mg, gt, gf, tm, meta, coef, mr, rm, rl = [0], [], [], [], [], [], [0], [], []
+    t_base = 0
+    for tt in tts:
         g_off = tt.group_term_off
         gt.append(g_off[:-1] + t_base if len(g_off) > 1 else np.zeros(0, np.int32))
         gf.append(tt.group_flip)

@@ -83,7 +83,7 @@ def getSpectra(n_spin_orbitals, hOp, tOps, psis, es, w, delta, restrictions=None
     out_src = np.array([i for t in range(nt) for i in range(ne)], np.int32)
     out_model = np.array([t for t in range(nt) for i in range(ne)], np.int32)
     seeds = vec_apply(tops, src, out_src, out_model, slaterWeightMin, normalize=True)
-    _, _, norm2 = seeds.get()
+    norm2 = seeds.norm2()
     hops = _ops_for(n, hOp, restrictions)
     b = Batch(hops, np.zeros(len(out_src), np.int32))
     b.seed_from_vec(seeds)

@@ -133,6 +133,7 @@ class XASFamily:
         self.cmat = np.array([[u.get(k, 0.0) for k in keys] for u in units], np.complex128) - self.c0[None, :]
         self._tables = {}
         self._dip_tables = {}
+        self._bases = {}
 
     def hop(self, theta):
         """The operator dict of one parameter set, assembled term by term."""
@@ -313,67 +314,149 @@ class Surrogate:
         return err, dis
 
 
-def fitting_loop(family, target, surrogate=None, n_walkers=256, n_iter=8, step=0.05, temperature=0.02, seed=0,
-                 **xas_kw):
-    """Metropolis walkers in parameter space fitting a target spectrum.
+def train_surrogate(family, n_train=100000, epochs=200, n_val=2048, members=5, seed=0, bar=1e-2, margin=0.8,
+                    chunk=4096, **fit_kw):
+    """Training set by batched exact ED, ensemble training, threshold calibration.
 
-    Every proposal's spectrum comes from the surrogate when it is trusted
-    and from batched exact ED otherwise.  Returns a dict with the counts, the
-    wall time and the surrogate-produced (params, spectra) for scoring.
+    The trust threshold is calibrated to ``margin * bar`` on the validation
+    set, leaving room for proposals that sit where validation is sparse.
+    Returns ``(surrogate, info)``; info holds the wall times of each stage
+    and the validation errors.
     """
     import time
 
-    rng = np.random.default_rng(seed)
-    lo, hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
-    stats = dict(n_spectra=0, n_exact=0, n_surrogate=0, sur_params=[], sur_spectra=[])
-
-    def evaluate(P):
-        out = np.zeros((len(P), len(OMEGA)))
-        use = np.zeros(len(P), bool)
-        if surrogate is not None:
-            pred, dis = surrogate.predict(P)
-            use = dis < surrogate.threshold
-            out[use] = pred[use]
-            stats["sur_params"].append(P[use])
-            stats["sur_spectra"].append(pred[use])
-        if (~use).any():
-            ex, _ = exact_spectra(family, P[~use], **xas_kw)
-            out[~use] = ex
-        stats["n_spectra"] += len(P)
-        stats["n_exact"] += int((~use).sum())
-        stats["n_surrogate"] += int(use.sum())
-        return out
-
+    info = {}
     t0 = time.perf_counter()
-    theta = sample_params(n_walkers, seed=seed + 1)
-    cur = l1_error(evaluate(theta), target[None])
-    best = cur.min()
-    for _ in range(n_iter):
-        prop = theta + step * (hi - lo) * rng.normal(size=theta.shape)
+    P = sample_params(n_train, seed=seed + 10)
+    Y = np.concatenate([exact_spectra(family, P[i:i + chunk])[0] for i in range(0, n_train, chunk)])
+    Pv = sample_params(n_val, seed=seed + 11)
+    Yv = np.concatenate([exact_spectra(family, Pv[i:i + chunk])[0] for i in range(0, n_val, chunk)])
+    info["train_set_seconds"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sur = Surrogate(family, Ensemble(members, (8, 512, 512, 512, len(OMEGA)), max_batch=1024, seed=seed))
+    losses = sur.fit(P, Y, epochs=epochs, seed=seed, **fit_kw)
+    info["train_seconds"] = time.perf_counter() - t0
+    info["train_steps"] = int(len(losses))
+    info["final_loss"] = float(losses[-50:].mean())
+    err, dis = sur.calibrate(Pv, Yv, bar=bar * margin)
+    info.update(n_train=n_train, epochs=epochs, n_val=n_val, threshold=sur.threshold,
+                val_l1_median=float(np.median(err)), val_trusted=float((dis < sur.threshold).mean()),
+                val_trusted_l1_max=float(err[dis < sur.threshold].max()) if (dis < sur.threshold).any() else None)
+    return sur, info
+
+
+class FittingLoop:
+    """Metropolis walkers in parameter space fitting a target spectrum.
+
+    ``step()`` lets every walker that is not waiting propose a move.  With a
+    surrogate, trusted proposals are answered at once (one ensemble forward
+    pass for all of them) and the walker takes its Metropolis decision; an
+    untrusted proposal makes its walker wait.  Waiting proposals are evaluated
+    together by one batched exact-ED call once ``flush_at`` of them have
+    gathered (every step without a surrogate), so the fixed latency of an
+    exact-ED launch sequence is paid once per flush instead of once per step.
+    ``finish()`` evaluates what is still waiting.  Counters and the
+    surrogate-served (params, spectra) are kept for throughput and for scoring
+    against exact ED.
+    """
+
+    def __init__(self, family, target, surrogate=None, n_walkers=256, step=0.05, temperature=0.02, seed=0,
+                 flush_at=None, **xas_kw):
+        self.family, self.target, self.surrogate = family, np.asarray(target), surrogate
+        self.step_size, self.temperature, self.xas_kw = step, temperature, xas_kw
+        self.flush_at = n_walkers // 2 if flush_at is None else flush_at
+        self.rng = np.random.default_rng(seed)
+        self.lo, self.hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
+        self.reset_counters()
+        self.theta = sample_params(n_walkers, seed=seed + 1)
+        self.cur = l1_error(self._exact(self.theta), self.target[None])
+        self.best = float(self.cur.min())
+        self.waiting = np.zeros(n_walkers, bool)
+        self.proposal = self.theta.copy()
+
+    def reset_counters(self):
+        self.n_spectra = self.n_exact = self.n_surrogate = self.n_flushes = 0
+        self.sur_params, self.sur_spectra, self.exact_results = [], [], []
+
+    def _exact(self, P):
+        ex, res = exact_spectra(self.family, P, **self.xas_kw)
+        self.exact_results.append(res)
+        self.n_spectra += len(P)
+        self.n_exact += len(P)
+        return ex
+
+    def _decide(self, idx, spectra):
+        new = l1_error(spectra, self.target[None])
+        acc = self.rng.uniform(size=len(idx)) < np.exp(-(new - self.cur[idx]) / self.temperature)
+        take = idx[acc]
+        self.theta[take] = self.proposal[take]
+        self.cur[take] = new[acc]
+        self.best = min(self.best, float(self.cur.min()))
+
+    def flush(self):
+        idx = np.nonzero(self.waiting)[0]
+        if len(idx):
+            self._decide(idx, self._exact(self.proposal[idx]))
+            self.waiting[idx] = False
+            self.n_flushes += 1
+
+    def step(self):
+        lo, hi = self.lo, self.hi
+        free = np.nonzero(~self.waiting)[0]
+        prop = self.theta[free] + self.step_size * (hi - lo) * self.rng.normal(size=(len(free), len(lo)))
         prop = np.where(prop < lo, 2 * lo - prop, prop)
         prop = np.where(prop > hi, 2 * hi - prop, prop)
-        prop = np.clip(prop, lo, hi)
-        new = l1_error(evaluate(prop), target[None])
-        acc = rng.uniform(size=n_walkers) < np.exp(-(new - cur) / temperature)
-        theta[acc] = prop[acc]
-        cur[acc] = new[acc]
-        best = min(best, cur.min())
-    stats["seconds"] = time.perf_counter() - t0
-    stats["best_l1"] = float(best)
-    for k in ("sur_params", "sur_spectra"):
-        stats[k] = np.concatenate(stats[k]) if stats[k] else np.zeros((0, 8 if k == "sur_params" else len(OMEGA)))
-    return stats
+        self.proposal[free] = np.clip(prop, lo, hi)
+        if self.surrogate is not None and len(free):
+            pred, dis = self.surrogate.predict(self.proposal[free])
+            use = dis < self.surrogate.threshold
+            self.sur_params.append(self.proposal[free[use]])
+            self.sur_spectra.append(pred[use])
+            self.n_spectra += int(use.sum())
+            self.n_surrogate += int(use.sum())
+            self._decide(free[use], pred[use])
+            self.waiting[free[~use]] = True
+        else:
+            self.waiting[free] = True
+        if self.surrogate is None or self.waiting.sum() >= self.flush_at:
+            self.flush()
+
+    def finish(self):
+        self.flush()
+
+    def served(self):
+        """(params, spectra) the surrogate produced since the last counter reset."""
+        if not self.sur_params:
+            return np.zeros((0, len(PARAM_NAMES))), np.zeros((0, len(OMEGA)))
+        return np.concatenate(self.sur_params), np.concatenate(self.sur_spectra)
+
+
+def fitting_loop(family, target, surrogate=None, n_walkers=256, n_iter=8, step=0.05, temperature=0.02, seed=0,
+                 **xas_kw):
+    """Run ``n_iter`` steps of :class:`FittingLoop`; counts, wall time, best L1
+    and the surrogate-served (params, spectra) in a dict."""
+    import time
+
+    t0 = time.perf_counter()
+    loop = FittingLoop(family, target, surrogate, n_walkers, step, temperature, seed, **xas_kw)
+    for _ in range(n_iter):
+        loop.step()
+    loop.finish()
+    sp, ss = loop.served()
+    return dict(n_spectra=loop.n_spectra, n_exact=loop.n_exact, n_surrogate=loop.n_surrogate,
+                seconds=time.perf_counter() - t0, best_l1=loop.best, sur_params=sp, sur_spectra=ss)
 
 
 def exact_spectra(family, thetas, krylovSize=200, reorth=0, **kw):
     """Exact XAS of parameter sets by the batched GPU ED, area-normalized."""
     from .batched import ground_bases, xas_batch
 
-    models = family.models(thetas)
     from . import opcompile
 
+    models = family.models(thetas)
     W = opcompile.n_words(models[0].n_spin_orbitals)
-    basis = ground_bases(models[:1], W)[0]
+    if W not in family._bases:
+        family._bases[W] = ground_bases(models[:1], W)[0]
     res = xas_batch(models, omega=OMEGA, delta=DELTA, krylovSize=krylovSize, T=np.asarray(thetas)[:, 7],
-                    reorth=reorth, bases=[basis] * len(models), **kw)
+                    reorth=reorth, bases=[family._bases[W]] * len(models), **kw)
     return area_normalize(res.spectra), res

@@ -48,14 +48,19 @@ int imped_device_synchronize(void);
 /* Launch timing: when enabled, every kernel launch of the engine is bracketed
  * by HIP events on the library's stream; imped_timing_query sums the elapsed
  * time of the launches recorded under a kernel name ("apply_count",
- * "apply_fill", "eigen", "lanczos", "cf").  No reference counterpart. */
+ * "apply_fill", "eigen", "eigen_flat", "lanczos", "spmv", "solve",
+ * "solve_combine", "cf", "mlp_gemm", "mlp_adam").  No reference counterpart. */
 int imped_timing_enable(int on);
 int imped_timing_reset(void);
 int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
 
-/* Launch-shape parameters picked per chain-dimension bucket
- * ("lanczos_wave_max_n", "lanczos_wg256_max_n", "eigen_small_n",
- * "eigen_large_bt"); set by the autotuner.  No reference counterpart. */
+/* Launch-shape parameters picked per chain-dimension bucket; no reference
+ * counterpart.  Names: "lanczos_wave_max_n", "lanczos_wg256_max_n" (Lanczos
+ * threads per chain: 64 / 256 / 1024 by size), "eigen_wave_max_n" (eigensolver
+ * chains up to this size get one wave), "eigen_bt" (threads per larger
+ * eigensolver chain: 256, 512, 1024), "eigen_real", "eigen_dgks",
+ * "eigen_check_every", "flat_min_n" (chains above it use the grid-wide
+ * kernels), "spmv_lpr" (lanes per row of the grid-wide SpMV: 4..64). */
 int imped_set_tuning(const char* name, int64_t value);
 
 /* The engine's device radix sort (the one that orders BFS layers and
