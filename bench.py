@@ -135,6 +135,30 @@ def _roofline(kernels, bytes_by_kernel):
 KERNELS = ("eigen", "lanczos", "apply_count", "apply_fill", "cf", "mlp_gemm", "mlp_adam")
 
 
+class _Roctx:
+    """roctx ranges around the timed regions (seen by rocprofv3 --marker-trace); no-ops without the library."""
+
+    def __init__(self):
+        import ctypes
+
+        try:
+            self.lib = ctypes.CDLL("librocprofiler-sdk-roctx.so")
+            self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+        except OSError:
+            self.lib = None
+
+    def push(self, name):
+        if self.lib is not None:
+            self.lib.roctxRangePushA(name.encode())
+
+    def pop(self):
+        if self.lib is not None:
+            self.lib.roctxRangePop()
+
+
+ROCTX = _Roctx()
+
+
 # ------------------------------------------------------------------ workloads
 
 
@@ -151,7 +175,7 @@ def run_fit(args, torch):
     target_theta = S.sample_params(1, seed=99)[0]
     target = S.exact_spectra(fam, target_theta[None])[0][0]
 
-    def timed_loop(surrogate):
+    def timed_loop(surrogate, tag):
         loop = S.FittingLoop(fam, target, surrogate, n_walkers=args.walkers, seed=1)
         for _ in range(args.warmup):
             loop.step()
@@ -160,17 +184,19 @@ def run_fit(args, torch):
         engine.timing_reset()
         loop.reset_counters()
         torch.cuda.synchronize()
+        ROCTX.push(tag)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             loop.step()
         loop.finish()  # proposals still waiting for exact ED are evaluated inside the timed region
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        ROCTX.pop()
         return loop, dt, {k: engine.timing_query(k) for k in KERNELS}
 
     engine.timing_enable(True)
-    loop_s, dt_s, kern_s = timed_loop(sur)
-    loop_e, dt_e, _ = timed_loop(None)
+    loop_s, dt_s, kern_s = timed_loop(sur, "fit_timed_surrogate")
+    loop_e, dt_e, _ = timed_loop(None, "fit_timed_exact")
     engine.timing_enable(False)
     value = loop_s.n_spectra / dt_s
     batched_rate = loop_e.n_spectra / dt_e
@@ -266,11 +292,13 @@ def run_batched128(args, torch):
     engine.timing_enable(True)
     engine.timing_reset()
     torch.cuda.synchronize()
+    ROCTX.push("batched128_timed")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res = xas_batch(models, omega=omega, krylovSize=200)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    ROCTX.pop()
     kernels = {k: engine.timing_query(k) for k in KERNELS}
     engine.timing_enable(False)
     n_orb = max(m.n_spin_orbitals for m in models)
