@@ -277,6 +277,119 @@ def run_fit(args, torch):
     }
 
 
+def run_large(args, torch):
+    """Large single instance: 76 spin-orbitals (two words per state), sectors of ~10^6 states."""
+    from impuritymodel_amd import engine
+    from impuritymodel_amd.batched import POLARIZATIONS, ground_bases, state_spectra
+    from impuritymodel_amd.engine import Batch, Ops
+    from impuritymodel_amd.models import K_B, large_model
+    from impuritymodel_amd.operators import getDipoleOperators, getNIXSOperators, getPhotoEmissionOperators
+
+    m = large_model()
+    n, W = m.n_spin_orbitals, 2
+    ops_h = Ops([m.hOp], [m.restrictions], n, W)
+    ops_xas = Ops(getDipoleOperators(m.nBaths, POLARIZATIONS), [m.restrictions] * 3, n, W)
+    omega = np.linspace(-25.0, 25.0, 3000)
+    K = 300
+
+    def ground():
+        basis = ground_bases([m], W)[0]
+        gb = Batch(ops_h, [0])
+        gb.set_seeds([basis])
+        t0 = time.perf_counter()
+        gb.build(fixed_basis=True)
+        torch.cuda.synchronize()
+        t_build = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ev, _, steps, _ = gb.eigen(3, tol=1e-12, max_steps=600, want_vectors=False)
+        torch.cuda.synchronize()
+        t_eig = time.perf_counter() - t0
+        e = ev[0]
+        kept = [i for i in range(3) if e[i] - e[0] < 10.0 * K_B * 300.0]
+        psis = gb.vec_from_eigen(np.zeros(len(kept), np.int32), np.array(kept, np.int32), 1e-7)
+        return gb, e[kept], psis, t_build, t_eig, int(steps[0])
+
+    def xas_step():
+        t0 = time.perf_counter()
+        gb, es, psis, t_build, t_eig, steps = ground()
+        spec, info = state_spectra(ops_h, ops_xas, psis, es, omega, 0.2, K, T=300.0)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, gb, es, psis, t_build, t_eig, steps, spec, info
+
+    for _ in range(args.warmup):
+        xas_step()
+    engine.timing_enable(True)
+    engine.timing_reset()
+    torch.cuda.synchronize()
+    ROCTX.push("large_timed")
+    t0 = time.perf_counter()
+    runs = [xas_step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ROCTX.pop()
+    kernels = {k: engine.timing_query(k) for k in ("spmv", "eigen_flat", "apply_count", "apply_fill", "cf")}
+    engine.timing_enable(False)
+    _, gb, es, psis, t_build, t_eig, eig_steps, spec, info = runs[-1]
+    # algorithmic bytes of every flat SpMV launch in the timed steps: the
+    # Lanczos chains (K steps each) and the ground-state eigensolver
+    # (eig_steps SpMVs); 20 nnz + 8 (n+1) + 32 n per launch (complex128 values,
+    # int32 columns, int64 row pointers, x read once, y written)
+    spmv_bytes = lambda nn, zz: 20.0 * zz + 8.0 * (nn + 1) + 32.0 * nn
+    per_step = sum(int(k) * spmv_bytes(nn, zz) for nn, zz, k in zip(info["chain_n"], info["chain_nnz"], info["kout"]))
+    per_step += eig_steps * spmv_bytes(gb.rows, gb.nnz)
+    roofline = _roofline({"spmv": kernels["spmv"]}, {"spmv": per_step * args.steps})
+
+    # XPS (2p removal, photoemission convention) and NIXS (|q| = 2, 7 along (111), 3d -> 3d: k = 2, 4)
+    extra = {}
+    ops_xps = Ops(getPhotoEmissionOperators(m.nBaths, l=1), [m.restrictions] * 6, n, W)
+    t0 = time.perf_counter()
+    _, xinfo = state_spectra(ops_h, ops_xps, psis, es, omega, 0.2, K, T=300.0, sign=-1)
+    torch.cuda.synchronize()
+    extra["xps"] = dict(seconds=round(time.perf_counter() - t0, 4), chains=int(len(xinfo["chain_n"])),
+                        chain_n_max=int(xinfo["chain_n"].max()))
+    r, R = np.loadtxt(os.path.join(ROOT, "tests", "golden", "Ni3d.dat")).T
+    qs = [2.0 * np.ones(3) / np.sqrt(3.0), 7.0 * np.ones(3) / np.sqrt(3.0)]
+    ops_nixs = Ops(getNIXSOperators(m.nBaths, qs, 2, 2, R, R, r), [m.restrictions] * 2, n, W)
+    t0 = time.perf_counter()
+    _, ninfo = state_spectra(ops_h, ops_nixs, psis, es, omega, 0.2, K, T=300.0)
+    torch.cuda.synchronize()
+    extra["nixs"] = dict(seconds=round(time.perf_counter() - t0, 4), chains=int(len(ninfo["chain_n"])),
+                         chain_n_max=int(ninfo["chain_n"].max()), q_norms=[2.0, 7.0], l_i=2, l_j=2, k=[2, 4])
+    return {
+        "metric": METRIC,
+        "value": round(args.steps / dt, 4),
+        "unit": "spectra/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded parameters)",
+        "config": {
+            "workload": "large single instance: NiO-like model with 50 valence + 10 conduction bath spin-orbitals "
+                        "(76 spin-orbitals, 2 words per state); per step: ground basis, fixed-basis H, 3 lowest "
+                        "eigenstates (tol 1e-12), 3 dipole polarizations x kept states, BFS closure + CSR of "
+                        "every XAS sector, 300-step Lanczos, continued fractions on 3000 omega points; complex128",
+            "ground_states": int(gb.rows), "ground_nnz": int(gb.nnz),
+            "ground_build_s": round(t_build, 4),
+            "ground_build_states_per_s": round(gb.rows / t_build, 1),
+            "ground_eigen_s": round(t_eig, 4), "ground_eigen_steps": eig_steps,
+            "xas_chains": int(len(info["chain_n"])), "xas_chain_n": [int(x) for x in info["chain_n"]],
+            "xas_chain_nnz": [int(x) for x in info["chain_nnz"]],
+            "xas_build_s": round(info["build_s"], 4),
+            "xas_build_states_per_s": round(float(info["chain_n"].sum()) / info["build_s"], 1),
+            "xas_lanczos_s": round(info["lanczos_s"], 4), "xas_cf_s": round(info["cf_s"], 4),
+            "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kernels.items() if v[1]},
+            "extra_spectra": extra,
+        },
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+
+
 def run_batched128(args, torch):
     from impuritymodel_amd import engine
     from impuritymodel_amd.batched import ground_bases, xas_batch
@@ -346,7 +459,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("fit", "batched128"), default="fit")
+    ap.add_argument("--workload", choices=("fit", "batched128", "large"), default="fit")
     ap.add_argument("--models", type=int, default=128)
     ap.add_argument("--walkers", type=int, default=256)
     ap.add_argument("--n-train", type=int, default=100000)
@@ -364,7 +477,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    out = run_fit(args, torch) if args.workload == "fit" else run_batched128(args, torch)
+    run = {"fit": run_fit, "batched128": run_batched128, "large": run_large}[args.workload]
+    out = run(args, torch)
     print(json.dumps(out))
 
 

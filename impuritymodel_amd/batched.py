@@ -139,6 +139,49 @@ def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0,
                      xb.chain_nnz.copy(), steps, kout)
 
 
+def state_spectra(ops_h, ops_t, psis, energies, omega, delta, krylovSize=150, T=300.0, sign=1, reorth=0,
+                  weight_min=1e-7):
+    """Thermal, operator-summed spectrum of one model from device eigenstates.
+
+    The ``getSpectra`` + ``thermal_average`` + operator sum of
+    simulate_spectra (spectra.py:124-238) for transition operators ``ops_t``
+    (one Ops model per operator) on eigenstates ``psis`` (a Vec, one vector per
+    state, energies given): all (operator, state) chains in one batch.
+    ``sign = -1`` is the photoemission convention (-w, -delta, negated).
+    Returns ``(spectrum [n_w], info)`` with stage times and chain sizes.
+    """
+    info = {}
+    nt, ne = ops_t.M, len(energies)
+    omega = np.asarray(omega, np.float64)
+    t0 = time.perf_counter()
+    out_src = np.array([i for t in range(nt) for i in range(ne)], np.int32)
+    out_model = np.array([t for t in range(nt) for i in range(ne)], np.int32)
+    seeds = vec_apply(ops_t, psis, out_src, out_model, weight_min, normalize=True)
+    norm2 = seeds.norm2()
+    info["transition_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    xb = Batch(ops_h, np.zeros(len(out_src), np.int32))
+    xb.seed_from_vec(seeds)
+    xb.build(fixed_basis=False, weight_min=1e-12)
+    info["build_s"] = time.perf_counter() - t0
+    info["chain_n"] = xb.chain_n.copy()
+    info["chain_nnz"] = xb.chain_nnz.copy()
+    t0 = time.perf_counter()
+    K = int(min(krylovSize, max(int(xb.chain_n.max()), 1)))
+    _, _, kout, status = xb.lanczos(K, reorth=reorth)
+    if np.any(status != 0):
+        raise ValueError("Warning: beta==0, implementation absent!")
+    info["lanczos_s"] = time.perf_counter() - t0
+    info["kout"] = kout.copy()
+    t0 = time.perf_counter()
+    energy = np.asarray(energies, np.float64)[out_src]
+    zflag = (out_model == 0).astype(np.int32)
+    _, spec = xb.cf(energy, norm2, np.full(len(out_src), sign, np.int32), zflag, K_B * T,
+                    np.zeros(len(out_src), np.int32), 1, omega, delta)
+    info["cf_s"] = time.perf_counter() - t0
+    return spec[0], info
+
+
 def plain_xas(model, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0, energy_cut=10.0, reorth=0,
               eig_tol=1e-12, eig_max_steps=600, seed=12345, basis=None):
     """The same spectrum with one launch sequence per chain (unbatched GPU path)."""

@@ -11,6 +11,14 @@ from . import opcompile
 from ._lib import IMPED_ERR_BREAKDOWN, IMPED_ERR_NOCONV, C, check, lib, ptr
 
 
+def _release(destroy, obj):
+    """Destroy a handle once; at interpreter shutdown the binding may already be gone."""
+    h = getattr(obj, "h", None)
+    if h and lib is not None:
+        getattr(lib, destroy)(h)
+    obj.h = None
+
+
 def _c64(a):
     a = np.ascontiguousarray(a, dtype=np.complex128)
     return a.view(np.float64).reshape(-1) if a.size else np.zeros(2, np.float64)
@@ -37,9 +45,7 @@ class Ops:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib.imped_ops_destroy(self.h)
-            self.h = None
+        _release("imped_ops_destroy", self)
 
 
 def _nz(a):
@@ -101,9 +107,7 @@ class Vec:
         return keys[:N], amps[:N], norm2[: len(off) - 1]
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib.imped_vec_destroy(self.h)
-            self.h = None
+        _release("imped_vec_destroy", self)
 
 
 def vec_from_host(W, keys_list, amps_list):
@@ -142,9 +146,7 @@ class Batch:
         self.chain_n = None
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib.imped_batch_destroy(self.h)
-            self.h = None
+        _release("imped_batch_destroy", self)
 
     def set_seeds(self, keys_list, amps_list=None):
         off = np.zeros(self.C + 1, np.int64)

@@ -116,6 +116,26 @@ def get_restrictions(nBaths, nValBaths, n0imps, dnTols, dnValBaths, dnConBaths, 
     return r
 
 
+def large_model(n_val=50, n_con=10, seed=7):
+    """The large single instance: 6 2p + 10 3d + 50 valence + 10 conduction bath
+    spin-orbitals (76, two words per state), bath energies U(-6, 4) eV (valence)
+    and hoppings U(0.2, 1.5) eV, at most 2 valence holes and 2 conduction
+    electrons.  Restrictions for the excited sectors: 3d occupancy 7..10, 2p
+    core 5..6, valence >= n_val - 2, conduction <= 2."""
+    m = make_model(n_bath=n_val, n_con=n_con, seed=seed, dnValBath=2, dnConBath=2, eps_range=(-6.0, 4.0),
+                   v_range=(0.2, 1.5))
+    nb = m.nBaths
+    r = {
+        frozenset(ops.c2i(nb, (2, s, mm)) for s in range(2) for mm in range(-2, 3)): (7, 10),
+        frozenset(ops.c2i(nb, (1, s, mm)) for s in range(2) for mm in range(-1, 2)): (5, 6),
+        frozenset(ops.c2i(nb, (2, b)) for b in range(n_val)): (n_val - 2, n_val),
+        frozenset(ops.c2i(nb, (2, b)) for b in range(n_val, n_val + n_con)): (0, 2),
+    }
+    m.restrictions = r
+    m.__dict__.pop("_tables", None)
+    return m
+
+
 def batched_params(n_models, seed=0):
     """Parameter draws of the batched workload: 10Dq U(0.3, 1.5), F0 U(5, 9),
     charge transfer U(1, 8) eV (used as the MLFT correction c), V U(0.5, 3)

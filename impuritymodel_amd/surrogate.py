@@ -180,9 +180,11 @@ class Ensemble:
         self.set_params(self.init_params(seed))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            self._lib.imped_mlp_destroy(self.h)
-            self.h = None
+        h = getattr(self, "h", None)
+        lib = getattr(self, "_lib", None)
+        if h and lib is not None:
+            lib.imped_mlp_destroy(h)
+        self.h = None
 
     def init_params(self, seed):
         """He-normal weights (sqrt(2/fan_in); sqrt(1/fan_in) for the output layer), zero biases."""
