@@ -67,38 +67,12 @@ __device__ __forceinline__ double spmv_rows(const int64_t* rowptr, const int32_t
   return pa;
 }
 
-// Two passes of classical Gram-Schmidt of w against V[0..m) (each n long).
-// coef must hold m doubles*2 in LDS-sized scratch (global here).
-template <int BT>
-__device__ void cgs2(const double2* V, int m, int64_t n, double2* w, double2* coef) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int l = wave; l < m; l += BT / 64) {
-      const double2* vl = V + (size_t)l * n;
-      double cx = 0.0, cy = 0.0;
-      for (int64_t i = lane; i < n; i += 64) {
-        const double2 a = vl[i], b = w[i];
-        cx += a.x * b.x + a.y * b.y;  // conj(a) * b
-        cy += a.x * b.y - a.y * b.x;
-      }
-      cx = wave_sum(cx);
-      cy = wave_sum(cy);
-      if (lane == 0) coef[l] = make_double2(cx, cy);
-    }
-    __syncthreads();
-    for (int64_t i = threadIdx.x; i < n; i += BT) {
-      double wx = w[i].x, wy = w[i].y;
-      for (int l = 0; l < m; ++l) {
-        const double2 c = coef[l];
-        const double2 a = V[(size_t)l * n + i];
-        wx -= a.x * c.x - a.y * c.y;
-        wy -= a.x * c.y + a.y * c.x;
-      }
-      w[i] = make_double2(wx, wy);
-    }
-    __syncthreads();
-  }
-}
+// Classical Gram-Schmidt helpers (defined with the eigensolver below):
+// cl[l] = <V_l, w> into LDS, then w -= sum_l cl[l] V_l returning |w|^2.
+template <int BT, class T>
+__device__ void dots_lds(const T* V, int m, int64_t n, const T* w, double2* cl, bool self);
+template <int BT, class T>
+__device__ double update_norm(const T* V, int m, int64_t n, T* w, const double2* cl, double* red);
 
 struct LanczosArgs {
   const int64_t* row_base;
@@ -110,8 +84,7 @@ struct LanczosArgs {
   int K;
   int reorth;
   double2* ws;      // 3 vectors per chain at 3*row_base
-  double2* vstore;  // reorth: K vectors per chain at K*row_base
-  double2* coef;    // reorth: K per chain
+  double2* vstore;  // reorth: K vectors per chain at K*row_base (coefficients in LDS)
   double* alpha;
   double* beta;
   int32_t* kout;
@@ -122,6 +95,7 @@ struct LanczosArgs {
 template <int BT>
 __global__ __launch_bounds__(BT) void lanczos_kernel(LanczosArgs a) {
   __shared__ double red[BT / 64];
+  extern __shared__ double2 cl[];  // reorth: K + 2 Gram-Schmidt coefficients
   const int c = a.chain_list[blockIdx.x];
   const int64_t n = a.chain_n[c];
   const int64_t base = a.row_base[c];
@@ -139,7 +113,6 @@ __global__ __launch_bounds__(BT) void lanczos_kernel(LanczosArgs a) {
   double2* v1 = v0 + n;
   double2* w = v1 + n;
   double2* V = a.reorth ? a.vstore + (size_t)a.K * base : nullptr;
-  double2* coef = a.reorth ? a.coef + (size_t)a.K * c : nullptr;
   for (int64_t i = threadIdx.x; i < n; i += BT) {
     v0[i] = a.v0[base + i];
     if (V) V[i] = v0[i];
@@ -153,10 +126,21 @@ __global__ __launch_bounds__(BT) void lanczos_kernel(LanczosArgs a) {
   int status = IMPED_OK;
   int kdone = K;
   for (int j = 1; j < K; ++j) {
-    if (V) cgs2<BT>(V, j, n, w, coef);
     double p = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += BT) p += w[i].x * w[i].x + w[i].y * w[i].y;
-    const double b = sqrt(block_sum<BT>(p, red));
+    p = block_sum<BT>(p, red);
+    if (V) {
+      // full reorthogonalization against V_0..V_{j-1}; a second classical
+      // Gram-Schmidt pass when the first removed more than half of |w|^2
+      dots_lds<BT, double2>(V, j, n, w, cl, false);
+      double p1 = update_norm<BT, double2>(V, j, n, w, cl, red);
+      if (p1 < 0.5 * p) {
+        dots_lds<BT, double2>(V, j, n, w, cl, false);
+        p1 = update_norm<BT, double2>(V, j, n, w, cl, red);
+      }
+      p = p1;
+    }
+    const double b = sqrt(p);
     if (threadIdx.x == 0) beta[j - 1] = b;
     if (b == 0.0) {
       status = IMPED_ERR_BREAKDOWN;
@@ -926,11 +910,12 @@ struct CFArgs {
   double delta;
   double2* g_out;  // optional [C][n_w]
   double* spec_out;
+  int q0;          // first spectrum of this launch (grids hold at most 65535 rows)
 };
 
 // One thread per (spectrum, omega); the spectrum's chains are summed in order.
 __global__ __launch_bounds__(256) void cf_kernel(CFArgs a) {
-  const int q = blockIdx.y;
+  const int q = a.q0 + blockIdx.y;
   const int iw = blockIdx.x * blockDim.x + threadIdx.x;
   if (iw >= a.n_w) return;
   const double om = a.omega[iw];
@@ -1434,11 +1419,12 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
   a.reorth = reorth;
   a.ws = sc.ws.as<double2>();
   if (reorth) {
+    // the Gram-Schmidt coefficients (K + 2 complex) live in LDS
+    if (K > 3500) throw HipError{"lanczos: K must be <= 3500 with reorthogonalization"};
     sc.vstore.reserve(std::max<int64_t>(R * K, 1) * 16);
-    sc.coef.reserve((size_t)std::max(kb.C, 1) * K * 16);
     a.vstore = sc.vstore.as<double2>();
-    a.coef = sc.coef.as<double2>();
   }
+  const size_t shm = reorth ? (size_t)(K + 2) * sizeof(double2) : 0;
   a.alpha = alpha;
   a.beta = beta;
   a.kout = kout;
@@ -1469,11 +1455,11 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
     off += nb;
     TimedLaunch tl("lanczos", s);
     if (bkt == 0)
-      hipLaunchKernelGGL(lanczos_kernel<64>, dim3(nb), dim3(64), 0, s, a);
+      hipLaunchKernelGGL(lanczos_kernel<64>, dim3(nb), dim3(64), shm, s, a);
     else if (bkt == 1)
-      hipLaunchKernelGGL(lanczos_kernel<256>, dim3(nb), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(lanczos_kernel<256>, dim3(nb), dim3(256), shm, s, a);
     else
-      hipLaunchKernelGGL(lanczos_kernel<1024>, dim3(nb), dim3(1024), 0, s, a);
+      hipLaunchKernelGGL(lanczos_kernel<1024>, dim3(nb), dim3(1024), shm, s, a);
     HIP_CHECK(hipGetLastError());
   }
   // the chain list buffer must outlive the launches
@@ -1715,8 +1701,12 @@ void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t
            g_out ? sc.gout.as<double2>() : nullptr, sc.sout.as<double>()};
   if (n_spec > 0 && n_w > 0) {
     TimedLaunch tl("cf", s);
-    hipLaunchKernelGGL(cf_kernel, dim3(blocks_for(n_w, 256), n_spec), dim3(256), 0, s, a);
-    HIP_CHECK(hipGetLastError());
+    for (int q0 = 0; q0 < n_spec; q0 += 65535) {
+      a.q0 = q0;
+      const unsigned rows = (unsigned)std::min(65535, n_spec - q0);
+      hipLaunchKernelGGL(cf_kernel, dim3(blocks_for(n_w, 256), rows), dim3(256), 0, s, a);
+      HIP_CHECK(hipGetLastError());
+    }
   }
   if (g_out) d2h(g_out, sc.gout.p, (size_t)C * n_w * 2, s);
   d2h(spec_out, sc.sout.p, (size_t)n_spec * n_w, s);

@@ -96,8 +96,9 @@ def getSpectra(n_spin_orbitals, hOp, tOps, psis, es, w, delta, restrictions=None
     # shorter than K stop at their dimension inside the kernel.
     energy = np.array([es[i] for t in range(nt) for i in range(ne)], np.float64)
     # sign = +1: z = w + i delta + e with the caller's (possibly negated) w, delta
+    # one spectrum id per chain: one thread per (chain, omega)
     g, _ = b.cf(energy, norm2, np.ones(len(out_src), np.int32), np.ones(len(out_src), np.int32), 1.0,
-                np.zeros(len(out_src), np.int32), 1, w, delta, want_g=True)
+                np.arange(len(out_src), dtype=np.int32), len(out_src), w, delta, want_g=True)
     for o in range(len(out_src)):
         gs[out_src[o], out_model[o], :] = g[o]
     return gs
@@ -155,7 +156,9 @@ def _rixs_chains(n, hops, tin, tout, src, es, nin, nout, wIns, wLoss, delta1, de
         raise ValueError("Warning: beta==0, implementation absent!")
     energy = np.repeat(e_c, nwi * nout)
     ones = np.ones(len(o_src), np.int32)
-    g, _ = gb.cf(energy, norm2, ones, ones, 1.0, np.zeros(len(o_src), np.int32), 1, wLoss, delta2, want_g=True)
+    # one spectrum id per chain: one thread per (chain, loss energy)
+    g, _ = gb.cf(energy, norm2, ones, ones, 1.0, np.arange(len(o_src), dtype=np.int32), len(o_src), wLoss, delta2,
+                 want_g=True)
     tim["lanczos_cf"] = time.perf_counter() - t0
     # chain index = ((tIn * ne + iE) * nwi + iwIn) * nout + tOut
     g = g.reshape(nin, ne, nwi, nout, len(wLoss)).transpose(1, 0, 3, 2, 4)
