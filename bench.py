@@ -341,6 +341,8 @@ def run_large(args, torch):
 
     # XPS (2p removal, photoemission convention) and NIXS (|q| = 2, 7 along (111), 3d -> 3d: k = 2, 4)
     extra = {}
+    if args.no_extras:
+        return _large_result(args, dt, gb, t_build, t_eig, eig_steps, info, kernels, extra, roofline)
     ops_xps = Ops(getPhotoEmissionOperators(m.nBaths, l=1), [m.restrictions] * 6, n, W)
     t0 = time.perf_counter()
     _, xinfo = state_spectra(ops_h, ops_xps, psis, es, omega, 0.2, K, T=300.0, sign=-1)
@@ -355,6 +357,10 @@ def run_large(args, torch):
     torch.cuda.synchronize()
     extra["nixs"] = dict(seconds=round(time.perf_counter() - t0, 4), chains=int(len(ninfo["chain_n"])),
                          chain_n_max=int(ninfo["chain_n"].max()), q_norms=[2.0, 7.0], l_i=2, l_j=2, k=[2, 4])
+    return _large_result(args, dt, gb, t_build, t_eig, eig_steps, info, kernels, extra, roofline)
+
+
+def _large_result(args, dt, gb, t_build, t_eig, eig_steps, info, kernels, extra, roofline):
     return {
         "metric": METRIC,
         "value": round(args.steps / dt, 4),
@@ -565,6 +571,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=0)
+    ap.add_argument("--no-extras", action="store_true", help="large workload: skip the XPS/NIXS extras")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="copy this many MiB device-to-device first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the engine runs on a single MI355X; "
@@ -575,6 +584,17 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.calibration_copy_mib:
+        # one device copy of known size: the scale of the FETCH_SIZE/WRITE_SIZE
+        # counters against true bytes when the run is profiled with --pmc
+        nbytes = args.calibration_copy_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        # the same bytes through a plain streaming elementwise kernel
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     run = {"fit": run_fit, "batched128": run_batched128, "large": run_large, "rixs": run_rixs}[args.workload]
     out = run(args, torch)
     print(json.dumps(out))

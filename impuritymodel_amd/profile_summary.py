@@ -55,5 +55,38 @@ def summarize(prefix, range_name, top=12):
     return "\n".join(lines)
 
 
+def pmc_traffic(counter_csv, kernel, copy_kernel, copy_bytes):
+    """Per-dispatch counter averages of ``kernel`` scaled to bytes.
+
+    ``counter_csv`` is a rocprofv3 ``--pmc`` counter_collection.csv (one
+    counter per run: FETCH_SIZE or WRITE_SIZE, reported in KiB).  The copy
+    kernel of known size gives the factor bytes-per-counter-unit; the
+    kernel's average counter value times that factor is its traffic.
+    """
+    vals = defaultdict(list)
+    counter = None
+    with open(counter_csv) as f:
+        for row in csv.DictReader(f):
+            counter = row["Counter_Name"]
+            name = row["Kernel_Name"]
+            key = "copy" if copy_kernel in name else ("kernel" if kernel in name else None)
+            if key:
+                vals[(key, row["Dispatch_Id"])].append(float(row["Counter_Value"]))
+    per = defaultdict(list)
+    for (key, _), v in vals.items():
+        per[key].append(sum(v))  # sum over XCDs / instances of one dispatch
+    if not per["copy"] or not per["kernel"]:
+        raise SystemExit("no dispatches of %r or %r in %s" % (kernel, copy_kernel, counter_csv))
+    copy_units = max(per["copy"])
+    factor = copy_bytes / copy_units
+    avg = sum(per["kernel"]) / len(per["kernel"])
+    return dict(counter=counter, copy_bytes=copy_bytes, copy_counter=copy_units, bytes_per_unit=factor,
+                dispatches=len(per["kernel"]), counter_avg=avg, bytes_avg=avg * factor)
+
+
 if __name__ == "__main__":
-    print(summarize(sys.argv[1], sys.argv[2]))
+    if sys.argv[1] == "--pmc":
+        out = pmc_traffic(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5]))
+        print(" ".join("%s=%s" % kv for kv in out.items()))
+    else:
+        print(summarize(sys.argv[1], sys.argv[2]))
