@@ -974,30 +974,34 @@ __global__ __launch_bounds__(FT) void flat_spmv_kernel(const int64_t* rowptr, co
                                                        int64_t base, int64_t n, const T* x, T* y, double* partial) {
   __shared__ double red[FT / 64];
   const int g = threadIdx.x / LPR, r = threadIdx.x % LPR;
-  const int64_t row = (int64_t)blockIdx.x * (FT / LPR) + g;
-  double ax = 0.0, ay = 0.0;
-  if (row < n) {
-    const int64_t e0 = rowptr[base + row], e1 = rowptr[base + row + 1];
-    for (int64_t e = e0 + r; e < e1; e += LPR) {
-      const double2 h = val[e];
-      const T v = x[col[e]];
-      if constexpr (is_real_v<T>) {
-        ax += h.x * re_(v);
-      } else {
-        ax += h.x * re_(v) - h.y * im_(v);
-        ay += h.x * im_(v) + h.y * re_(v);
+  double pa = 0.0;
+  // grid-stride over groups of FT/LPR rows, so the partial sums number at
+  // most gridDim.x (the loop bound is uniform in the workgroup)
+  for (int64_t rb = blockIdx.x; rb * (FT / LPR) < n; rb += gridDim.x) {
+    const int64_t row = rb * (FT / LPR) + g;
+    double ax = 0.0, ay = 0.0;
+    if (row < n) {
+      const int64_t e0 = rowptr[base + row], e1 = rowptr[base + row + 1];
+      for (int64_t e = e0 + r; e < e1; e += LPR) {
+        const double2 h = val[e];
+        const T v = x[col[e]];
+        if constexpr (is_real_v<T>) {
+          ax += h.x * re_(v);
+        } else {
+          ax += h.x * re_(v) - h.y * im_(v);
+          ay += h.x * im_(v) + h.y * re_(v);
+        }
       }
     }
-  }
 #pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) {
-    ax += __shfl_xor(ax, off);
-    if constexpr (!is_real_v<T>) ay += __shfl_xor(ay, off);
-  }
-  double pa = 0.0;
-  if (row < n && r == 0) {
-    y[row] = mk<T>(ax, ay);
-    pa = ax * re_(x[row]) + ay * im_(x[row]);
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+      ax += __shfl_xor(ax, off);
+      if constexpr (!is_real_v<T>) ay += __shfl_xor(ay, off);
+    }
+    if (row < n && r == 0) {
+      y[row] = mk<T>(ax, ay);
+      pa += ax * re_(x[row]) + ay * im_(x[row]);
+    }
   }
   pa = block_sum<FT>(pa, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = pa;
@@ -1060,7 +1064,8 @@ __global__ __launch_bounds__(FT) void flat_scale_kernel(const T* src, T* dst, co
 template <class T>
 __global__ __launch_bounds__(FT) void flat_dots_kernel(const T* V, int m, int64_t n, const T* w, double2* partial,
                                                        int64_t chunk) {
-  __shared__ double red[FT / 64];
+  __shared__ double2 part[FT / 64][8];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t i0 = (int64_t)blockIdx.x * chunk;
   const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
   for (int l0 = 0; l0 < m; l0 += 8) {
@@ -1078,13 +1083,23 @@ __global__ __launch_bounds__(FT) void flat_dots_kernel(const T* V, int m, int64_
         }
       }
     }
+    // eight wave sums per wave into LDS, one barrier, eight threads finish
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      if (l0 + u >= m) break;
-      const double x = block_sum<FT>(cx[u], red);
-      const double y = is_real_v<T> ? 0.0 : block_sum<FT>(cy[u], red);
-      if (threadIdx.x == 0) partial[(size_t)(l0 + u) * gridDim.x + blockIdx.x] = make_double2(x, y);
+      const double x = wave_sum(cx[u]);
+      const double y = is_real_v<T> ? 0.0 : wave_sum(cy[u]);
+      if (lane == 0) part[wave][u] = make_double2(x, y);
     }
+    __syncthreads();
+    if ((int)threadIdx.x < 8 && l0 + (int)threadIdx.x < m) {
+      double sx = 0.0, sy = 0.0;
+      for (int wv = 0; wv < FT / 64; ++wv) {
+        sx += part[wv][threadIdx.x].x;
+        sy += part[wv][threadIdx.x].y;
+      }
+      partial[(size_t)(l0 + threadIdx.x) * gridDim.x + blockIdx.x] = make_double2(sx, sy);
+    }
+    __syncthreads();
   }
 }
 
@@ -1109,6 +1124,7 @@ __global__ __launch_bounds__(FT) void flat_gs_update_kernel(const T* V, int m, i
   double p = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * FT + threadIdx.x; i < n; i += (int64_t)gridDim.x * FT) {
     double wx = re_(w[i]), wy = im_(w[i]);
+#pragma unroll 8
     for (int l = 0; l < m; ++l) {
       const double2 c = coef[l];
       const T a = V[(size_t)l * n + i];
@@ -1124,6 +1140,15 @@ __global__ __launch_bounds__(FT) void flat_gs_update_kernel(const T* V, int m, i
   }
   p = block_sum<FT>(p, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = p;
+}
+
+// beta_j = sqrt(|w|^2) on the device: written to beta[j] and to the scale slot
+__global__ void flat_set_beta_kernel(const double* p, double* beta_j, double* scale_slot) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double b = sqrt(*p);
+    *beta_j = b;
+    *scale_slot = b;
+  }
 }
 
 // Ritz check of T_m (one workgroup): eigenvalues, inverse-iteration vectors,
@@ -1242,7 +1267,7 @@ unsigned vec_blocks(int64_t n) { return (unsigned)std::min<int64_t>(std::max<int
 template <class T>
 void launch_spmv(int lpr, const KrylovBatch& kb, int64_t base, int64_t n, const T* x, T* y, double* partial,
                  unsigned* nb_out, hipStream_t s) {
-  const unsigned nb = (unsigned)((n + FT / lpr - 1) / (FT / lpr));
+  const unsigned nb = (unsigned)std::min<int64_t>(std::max<int64_t>((n + FT / lpr - 1) / (FT / lpr), 1), 4096);
   *nb_out = nb;
   TimedLaunch tl("spmv", s);
 #define IMPED_SPMV(L) \
@@ -1309,7 +1334,8 @@ int eigen_flat_chain(const KrylovBatch& kb, int c, int nev, double tol, int max_
   sc.wbuf.reserve(n * sizeof(T));
   T* V = reinterpret_cast<T*>(sc.vstore.p);
   T* w = reinterpret_cast<T*>(sc.wbuf.p);
-  const int64_t chunk = std::max<int64_t>(FT * 16, (n + 255) / 256);
+  // up to 2048 workgroups stream the Krylov basis in the dot products
+  const int64_t chunk = std::max<int64_t>(FT * 4, (n + 2047) / 2048);
   const unsigned nbd = (unsigned)((n + chunk - 1) / chunk);
   sc.fpart.reserve(std::max<int64_t>((n + 3) / 4 + 1, 4096) * sizeof(double));
   sc.fcpart.reserve((size_t)(mmax + 1) * nbd * sizeof(double2));
@@ -1349,7 +1375,11 @@ int eigen_flat_chain(const KrylovBatch& kb, int c, int nev, double tol, int max_
                        alpha + j, j > 0 ? beta + j - 1 : nullptr, n, part);
     hipLaunchKernelGGL(flat_reduce_kernel, dim3(1), dim3(FT), 0, s, part, (int64_t)nbv, (int64_t)0, scal + 1, 0);
     gs_pass(j + 1);
+    // one host read per step (|w|^2 before and after the first pass, alpha_j)
+    // decides the DGKS second pass; beta_j is written on the device
+    double al = 0.0;
     HIP_CHECK(hipMemcpyAsync(hb, scal, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&al, alpha + j, sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     double p1 = hb[2];
     if (!g_tuning.eigen_dgks || p1 < 0.5 * hb[1]) {
@@ -1357,12 +1387,11 @@ int eigen_flat_chain(const KrylovBatch& kb, int c, int nev, double tol, int max_
       HIP_CHECK(hipMemcpyAsync(&p1, scal + 2, sizeof(double), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
     }
-    double al = 0.0;
-    HIP_CHECK(hipMemcpy(&al, alpha + j, sizeof(double), hipMemcpyDeviceToHost));
     const double b = std::sqrt(p1);
     const double bprev = j > 0 ? hb[3] : 0.0;
     hb[3] = b;
-    HIP_CHECK(hipMemcpy(beta + j, &b, sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(flat_set_beta_kernel, dim3(1), dim3(64), 0, s, scal + 2, beta + j, scal + 3);
+    HIP_CHECK(hipGetLastError());
     m = j + 1;
     const bool last = (m == mmax) || b <= 1e-13 * (std::fabs(al) + bprev + 1e-300);
     if (last || (m >= nev && m % g_tuning.eigen_check_every == 0)) {
@@ -1378,7 +1407,6 @@ int eigen_flat_chain(const KrylovBatch& kb, int c, int nev, double tol, int max_
         break;
       }
     }
-    HIP_CHECK(hipMemcpy(scal + 3, &b, sizeof(double), hipMemcpyHostToDevice));
     hipLaunchKernelGGL((flat_scale_kernel<T>), dim3(nbv), dim3(FT), 0, s, w, V + (size_t)(j + 1) * n, scal + 3, n,
                        sc.conv.as<int32_t>(), sc.conv.as<int32_t>(), 0);
     HIP_CHECK(hipGetLastError());
