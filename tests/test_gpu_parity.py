@@ -411,6 +411,33 @@ def test_rixs_map_vs_oracle(gpu, alias):
     np.testing.assert_allclose(gs_g, gs_o, rtol=0, atol=1e-7 * scale)
 
 
+def test_static_observables_vs_oracle(gpu, capsys):
+    from impuritymodel_amd import observables as gobs
+    from impuritymodel_amd.models import make_model
+    from oracle import observables as oobs
+
+    m = make_model(n_bath=10, seed=8)
+    n = m.n_spin_orbitals
+    bo = obasis.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    es, po = okry.eigensystem(n, m.hOp, bo, 3, groundDiagMode="full")
+    for p in po:
+        pg = dict(zip(ints_to_bytes(list(p), n), p.values()))
+        for name in ("getLz3d", "getSz3d", "getSsqr3d", "getLsqr3d", "getTraceDensityMatrix"):
+            assert abs(getattr(gobs, name)(m.nBaths, pg) - getattr(oobs, name)(m.nBaths, p)) < 1e-11, name
+        rg, ro = gobs.getDensityMatrix(m.nBaths, pg), oobs.getDensityMatrix(m.nBaths, p)
+        assert set(rg) == set(ro)
+        assert max(abs(rg[k] - ro[k]) for k in ro) < 1e-12
+        np.testing.assert_allclose(gobs.getEgT2gOccupation(m.nBaths, pg), oobs.getEgT2gOccupation(m.nBaths, p),
+                                   rtol=0, atol=1e-12)
+        cg, co = gobs.getDensityMatrixCubic(m.nBaths, pg), oobs.getDensityMatrixCubic(m.nBaths, p)
+        assert set(cg) == set(co) and max(abs(cg[k] - co[k]) for k in co) < 1e-12
+    pgs = [dict(zip(ints_to_bytes(list(p), n), p.values())) for p in po]
+    gobs.printExpValues(m.nBaths, np.array(es), pgs)
+    gobs.printThermalExpValues(m.nBaths, np.array(es), pgs)
+    out = capsys.readouterr().out
+    assert "E0 =" in out and "<L^2(3d)>" in out
+
+
 def test_batched_equals_single_chain_runs(gpu):
     """Many mixed-size models (W = 1 and 2) in one batch reproduce the per-model
     runs: same chain bases, CSR and spectra."""

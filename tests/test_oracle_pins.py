@@ -397,3 +397,54 @@ def test_mlp_oracle_bf16_mode_within_rounding():
     l16, g16 = mlp.loss_and_grad(params, dims, x, t, bf16=True)
     assert abs(l16 - l64) < 2e-2 * l64
     assert np.linalg.norm(g16 - g64) < 3e-2 * np.linalg.norm(g64)
+
+
+# ------------------------------------------------------------ static observables
+
+
+def test_observables_single_configurations():
+    """One 3d electron in (s, m): Lz = m, Sz = s - 1/2, L^2 = 6, S^2 = 3/4, N = 1;
+    a full shell: L^2 = S^2 = 0, N = 10, eg/t2g occupations (2, 2, 3, 3)."""
+    from oracle import observables as obs
+
+    nb = OrderedDict([(2, 0)])
+    for s in range(2):
+        for m in range(-2, 3):
+            st = psr.tuple2int((ops.c2i(nb, (2, s, m)),), 10)
+            psi = {st: 1.0}
+            assert obs.getLz3d(nb, psi) == m and obs.getSz3d(nb, psi) == s - 0.5
+            assert abs(obs.getLsqr3d(nb, psi) - 6) < 1e-14 and abs(obs.getSsqr3d(nb, psi) - 0.75) < 1e-14
+            assert obs.getTraceDensityMatrix(nb, psi) == 1
+            rho = obs.getDensityMatrix(nb, psi)
+            assert list(rho) == [((2, s, m), (2, s, m))] and rho[((2, s, m), (2, s, m))] == 1
+    full = {psr.tuple2int(tuple(range(10)), 10): 1.0}
+    assert abs(obs.getLsqr3d(nb, full)) < 1e-14 and abs(obs.getSsqr3d(nb, full)) < 1e-14
+    assert obs.getTraceDensityMatrix(nb, full) == 10
+    np.testing.assert_allclose(obs.getEgT2gOccupation(nb, full), [2, 2, 3, 3], atol=1e-14)
+    # m = 0 is the cubic z^2 (eg) orbital; m = 2 is half x^2-y^2 (eg), half xy (t2g)
+    up0 = {psr.tuple2int((ops.c2i(nb, (2, 1, 0)),), 10): 1.0}
+    np.testing.assert_allclose(obs.getEgT2gOccupation(nb, up0), [0, 1, 0, 0], atol=1e-14)
+    dn2 = {psr.tuple2int((ops.c2i(nb, (2, 0, 2)),), 10): 1.0}
+    np.testing.assert_allclose(obs.getEgT2gOccupation(nb, dn2), [0.5, 0, 0.5, 0], atol=1e-14)
+    cub = obs.getDensityMatrixCubic(nb, dn2)
+    assert abs(sum(v for (a, b), v in cub.items() if a == b) - 1) < 1e-14
+
+
+def test_observables_d2_hund_multiplets():
+    """The 21 lowest d^2 Slater-Condon states are 3F (L^2 = 12, S^2 = 2), the
+    highest is 1S (L^2 = S^2 = 0)."""
+    from oracle import observables as obs
+
+    nb = OrderedDict([(2, 0)])
+    uop = ops.getUop(2, 2, 2, 2, (3.0, 0, 7.0, 0, 5.0))
+    op = {tuple((ops.c2i(nb, so), a) for so, a in p): v for p, v in uop.items()}
+    states = [psr.tuple2int(t, 10) for t in itertools.combinations(range(10), 2)]
+    e, v = np.linalg.eigh(_dense_h(10, op, states))
+    for k in list(range(21)) + [44]:
+        psi = {s: complex(v[i, k]) for i, s in enumerate(states) if abs(v[i, k]) > 0}
+        L2, S2 = obs.getLsqr3d(nb, psi), obs.getSsqr3d(nb, psi)
+        if k < 21:
+            assert abs(L2 - 12) < 1e-10 and abs(S2 - 2) < 1e-10
+        else:
+            assert abs(L2) < 1e-10 and abs(S2) < 1e-10
+        assert abs(obs.getTraceDensityMatrix(nb, psi) - 2) < 1e-12
