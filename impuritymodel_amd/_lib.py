@@ -40,6 +40,8 @@ SIGNATURES = {
     "imped_enumerate_basis": [_i32, _i32, _i32, _p, _i64, _p, _p],
     "imped_ops_create": [_i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "imped_ops_destroy": [_p],
+    "imped_ops_set_coef": [_p, _p, _i64],
+    "imped_ops_set_affine": [_p, _i32, _i64, _p, _p, _i32, _p],
     "imped_vec_create": [_i32, _i32, _p, _p, _p, _p],
     "imped_vec_destroy": [_p],
     "imped_vec_offsets": [_p, _p],
@@ -66,6 +68,7 @@ SIGNATURES = {
     "imped_mlp_set_params": [_p, _p],
     "imped_mlp_get_params": [_p, _p],
     "imped_mlp_forward": [_p, _p, _i32, _p],
+    "imped_mlp_predict": [_p, _p, _i32, _p, _d, _p, _p],
     "imped_mlp_gradients": [_p, _p, _p, _i32, _p, _p],
     "imped_mlp_set_data": [_p, _p, _p, _i64],
     "imped_mlp_train": [_p, _p, _i32, _i32, _f, _f, _f, _f, _p],

@@ -33,6 +33,7 @@ DEFAULTS = {
     "eigen_check_every": 8,
     "eigen_wave_max_n": 0,
     "spmv_lpr": 8,
+    "lanczos_graph_steps": 16,
 }
 
 CANDIDATES = {

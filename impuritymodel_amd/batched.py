@@ -72,7 +72,6 @@ def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0,
     if omega is None:
         omega = np.linspace(-25.0, 25.0, 2000)
     omega = np.asarray(omega, dtype=np.float64)
-    M = len(models)
     n_orb = max(m.n_spin_orbitals for m in models)
     W = opcompile.n_words(n_orb)
     ops_h = Ops([m.term_table(W) for m in models], None, n_orb, W)
@@ -80,7 +79,19 @@ def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0,
     if bases is None:
         bases = ground_bases(models, W)
     tim["host_setup"] = time.perf_counter() - t0
+    return xas_batch_ops(ops_h, ops_t, bases, omega, delta, krylovSize, nPsiMax, T, energy_cut, reorth, eig_tol,
+                         eig_max_steps, seed, tim, t0)
 
+
+def xas_batch_ops(ops_h, ops_t, bases, omega, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0, energy_cut=10.0,
+                  reorth=0, eig_tol=1e-12, eig_max_steps=600, seed=12345, tim=None, t0=None):
+    """xas_batch on compiled operators: model m < len(bases) of ``ops_h`` is the
+    Hamiltonian, models 3m..3m+2 of ``ops_t`` its x, y, z dipole operators,
+    bases[m] its ground basis (the operator sets may hold more models)."""
+    t0 = time.perf_counter() if t0 is None else t0
+    tim = {} if tim is None else tim
+    omega = np.asarray(omega, dtype=np.float64)
+    M = len(bases)
     t1 = time.perf_counter()
     gb = Batch(ops_h, np.arange(M, dtype=np.int32))
     gb.set_seeds(bases)
@@ -93,28 +104,24 @@ def xas_batch(models, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0,
 
     t1 = time.perf_counter()
     T_m = np.broadcast_to(np.asarray(T, np.float64), (M,))
-    kept = []
-    for m in range(M):
-        e = evals[m]
-        lim = energy_cut * K_B * T_m[m]
-        kept.append([i for i in range(nev) if e[i] - e[0] < lim])
-    psi_chain = np.array([m for m in range(M) for i in kept[m]], np.int32)
-    psi_which = np.array([i for m in range(M) for i in kept[m]], np.int32)
-    psi_index = {}
-    for k, (m, i) in enumerate(zip(psi_chain, psi_which)):
-        psi_index[(int(m), int(i))] = k
+    ev = np.asarray(evals, np.float64).reshape(M, nev)
+    keep = (ev - ev[:, :1]) < (energy_cut * K_B * T_m)[:, None]
+    psi_chain, psi_which = (x.astype(np.int32) for x in np.nonzero(keep))  # model-major, state ascending
     psis = gb.vec_from_eigen(psi_chain, psi_which, 1e-7)
     nt = len(POLARIZATIONS)
-    out_src, out_model, chain_model, energy, spec_id, zflag = [], [], [], [], [], []
-    for m in range(M):
-        for t in range(nt):
-            for i in kept[m]:
-                out_src.append(psi_index[(m, i)])
-                out_model.append(nt * m + t)
-                chain_model.append(m)
-                energy.append(evals[m][i])
-                spec_id.append(m)
-                zflag.append(1 if t == 0 else 0)
+    # output vectors ordered by (model, polarization, kept state)
+    n_psi = len(psi_chain)
+    kk = np.tile(np.arange(n_psi), nt)
+    tt = np.repeat(np.arange(nt), n_psi)
+    order = np.lexsort((kk, tt, psi_chain[kk]))
+    out_src = kk[order].astype(np.int32)
+    t_o = tt[order]
+    chain_model = psi_chain[out_src]
+    out_model = (nt * chain_model + t_o).astype(np.int32)
+    energy = ev[chain_model, psi_which[out_src]]
+    spec_id = chain_model
+    zflag = (t_o == 0).astype(np.int32)
+    kept = [np.nonzero(keep[m])[0] for m in range(M)]
     seeds = vec_apply(ops_t, psis, out_src, out_model, 1e-7, normalize=True)
     norm2 = seeds.norm2()
     tim["transition"] = time.perf_counter() - t1

@@ -98,8 +98,12 @@ struct Tuning {
   int64_t eigen_check_every = 8;        // Ritz convergence check interval
   int64_t flat_min_n = 32768;           // chains above this size use the grid-wide (flat) kernels
   int64_t spmv_lpr = 8;                 // lanes per CSR row in the flat SpMV (4..64)
+  int64_t lanczos_graph_steps = 16;     // flat Lanczos steps per replayed HIP graph (0: direct launches)
 };
 extern Tuning g_tuning;
+
+// True while launch timing is switched on (every launch then goes out directly).
+bool timing_enabled();
 
 // Optional HIP-event timing of kernel launches, accumulated per kernel name.
 struct TimedLaunch {

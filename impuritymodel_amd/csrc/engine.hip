@@ -41,6 +41,8 @@ struct TimingRecord {
 static bool g_timing = false;
 static std::vector<std::pair<std::string, TimingRecord>> g_timings;
 
+bool timing_enabled() { return g_timing; }
+
 static TimingRecord& timing_record(const char* name) {
   for (auto& kv : g_timings)
     if (kv.first == name) return kv.second;
@@ -74,6 +76,7 @@ static hipStream_t g_stream = nullptr;
 
 struct Ops {
   int W = 1, M = 0, n_orb = 0;
+  int64_t T = 0;      // terms over all models
   bool real = false;  // every term value is real, so H is real in the product basis
   DevBuf model_group_off, group_term_off, group_flip, term_masks, term_meta, term_coef, model_restr_off, restr_mask,
       restr_lohi;
@@ -744,6 +747,12 @@ int imped_set_tuning(const char* name, int64_t value) {
       return IMPED_ERR_INVALID;
     }
     g_tuning.spmv_lpr = value;
+  } else if (n == "lanczos_graph_steps") {
+    if (value < 0 || value > 256) {
+      set_error("imped_set_tuning: lanczos_graph_steps must be in [0, 256]");
+      return IMPED_ERR_INVALID;
+    }
+    g_tuning.lanczos_graph_steps = value;
   } else if (n == "eigen_check_every") {
     if (value < 1) {
       set_error("imped_set_tuning: eigen_check_every must be >= 1");
@@ -773,6 +782,7 @@ int imped_ops_create(int W, int n_orb, int n_models, const int32_t* model_group_
   const int G = model_group_off[n_models];
   const int T = group_term_off[G];
   const int R = model_restr_off[n_models];
+  o->T = T;
   o->real = true;
   for (int t = 0; t < T; ++t)
     if (term_coef[2 * t + 1] != 0.0) o->real = false;
@@ -788,6 +798,72 @@ int imped_ops_create(int W, int n_orb, int n_models, const int32_t* model_group_
   h2d(o->restr_lohi, restr_lohi, (size_t)R * 2, s);
   HIP_CHECK(hipStreamSynchronize(s));
   *out = reinterpret_cast<imped_ops>(o.release());
+  API_END
+}
+
+int imped_ops_set_coef(imped_ops ops, const double* term_coef, int64_t n_terms) {
+  API_BEGIN
+  Ops* o = reinterpret_cast<Ops*>(ops);
+  if (!o || n_terms != o->T || (n_terms > 0 && !term_coef)) {
+    set_error("imped_ops_set_coef: the term count differs from the operator's");
+    return IMPED_ERR_INVALID;
+  }
+  bool real = true;
+  for (int64_t t = 0; t < n_terms; ++t)
+    if (term_coef[2 * t + 1] != 0.0) real = false;
+  hipStream_t s = stream();
+  // stream order: launches already queued finish with the old values
+  if (n_terms) HIP_CHECK(hipMemcpyAsync(o->term_coef.p, term_coef, n_terms * 16, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  o->real = real;
+  API_END
+}
+
+// coef[m][t] = base[t] + sum_p theta[m][p] dirs[p][t] for m < M, t < T1.
+__global__ void affine_coef_kernel(const double2* base, const double2* dirs, const double* theta, int P, int64_t T1,
+                                   int M, double2* coef) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)M * T1) return;
+  const int64_t m = i / T1, t = i % T1;
+  double ax = 0.0, ay = 0.0;
+  for (int p = 0; p < P; ++p) {
+    const double th = theta[m * P + p];
+    const double2 d = dirs[(int64_t)p * T1 + t];
+    ax += th * d.x;
+    ay += th * d.y;
+  }
+  coef[i] = make_double2(base[t].x + ax, base[t].y + ay);
+}
+
+int imped_ops_set_affine(imped_ops ops, int n_params, int64_t terms_per_model, const double* base, const double* dirs,
+                         int n_models, const double* theta) {
+  API_BEGIN
+  Ops* o = reinterpret_cast<Ops*>(ops);
+  if (!o || n_params < 0 || n_models < 1 || n_models > o->M || terms_per_model * o->M != o->T || !base ||
+      (n_params > 0 && (!dirs || !theta))) {
+    set_error("imped_ops_set_affine: every model must hold terms_per_model terms and n_models <= the model count");
+    return IMPED_ERR_INVALID;
+  }
+  const int64_t T1 = terms_per_model;
+  bool real = true;
+  for (int64_t i = 0; i < T1 * (n_params + 1); ++i)
+    if ((i < T1 ? base[2 * i + 1] : dirs[2 * (i - T1) + 1]) != 0.0) real = false;
+  static DevBuf d_base, d_dirs, d_theta;
+  hipStream_t s = stream();
+  h2d(d_base, base, (size_t)T1 * 2, s);
+  if (n_params > 0) {
+    h2d(d_dirs, dirs, (size_t)T1 * 2 * n_params, s);
+    h2d(d_theta, theta, (size_t)n_models * n_params, s);
+  }
+  if (T1 > 0) {
+    hipLaunchKernelGGL(affine_coef_kernel, dim3(blocks_for((int64_t)n_models * T1, 256)), dim3(256), 0, s,
+                       d_base.as<double2>(), d_dirs.as<double2>(), d_theta.as<double>(), n_params, T1, n_models,
+                       o->term_coef.as<double2>());
+    HIP_CHECK(hipGetLastError());
+  }
+  HIP_CHECK(hipStreamSynchronize(s));  // the host arrays may be released on return
+  // models past n_models keep their values: real only if they were real too
+  o->real = real && (n_models == o->M || o->real);
   API_END
 }
 
@@ -1259,6 +1335,17 @@ int imped_mlp_forward(imped_mlp net, const float* x, int batch, float* y) {
     return IMPED_ERR_INVALID;
   }
   mlp_forward(*reinterpret_cast<Mlp*>(net), x, batch, y, stream());
+  API_END
+}
+
+int imped_mlp_predict(imped_mlp net, const float* x, int batch, const double* weights, double scale, double* mean,
+                      double* disagreement) {
+  API_BEGIN
+  if (batch < 1 || !weights || !(scale > 0.0)) {
+    set_error("imped_mlp_predict: batch must be >= 1, weights given and scale > 0");
+    return IMPED_ERR_INVALID;
+  }
+  mlp_predict(*reinterpret_cast<Mlp*>(net), x, batch, weights, scale, mean, disagreement, stream());
   API_END
 }
 

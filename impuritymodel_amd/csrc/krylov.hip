@@ -970,9 +970,8 @@ __global__ __launch_bounds__(256) void cf_kernel(CFArgs a) {
 constexpr int FT = 256;  // threads per flat block
 
 template <int LPR, class T>
-__global__ __launch_bounds__(FT) void flat_spmv_kernel(const int64_t* rowptr, const int32_t* col, const double2* val,
-                                                       int64_t base, int64_t n, const T* x, T* y, double* partial) {
-  __shared__ double red[FT / 64];
+__device__ __forceinline__ void spmv_rows(const int64_t* rowptr, const int32_t* col, const double2* val, int64_t base,
+                                          int64_t n, const T* x, T* y, double* partial, double* red) {
   const int g = threadIdx.x / LPR, r = threadIdx.x % LPR;
   double pa = 0.0;
   // grid-stride over groups of FT/LPR rows, so the partial sums number at
@@ -1007,6 +1006,13 @@ __global__ __launch_bounds__(FT) void flat_spmv_kernel(const int64_t* rowptr, co
   if (threadIdx.x == 0) partial[blockIdx.x] = pa;
 }
 
+template <int LPR, class T>
+__global__ __launch_bounds__(FT) void flat_spmv_kernel(const int64_t* rowptr, const int32_t* col, const double2* val,
+                                                       int64_t base, int64_t n, const T* x, T* y, double* partial) {
+  __shared__ double red[FT / 64];
+  spmv_rows<LPR, T>(rowptr, col, val, base, n, x, y, partial, red);
+}
+
 // out[k] = sum of partial[k*stride .. k*stride + count) in a fixed order;
 // mode 1 takes the square root (beta), mode 2 stores into alpha and adds
 // the value to an accumulator slot.
@@ -1022,11 +1028,8 @@ __global__ __launch_bounds__(FT) void flat_reduce_kernel(const double* partial, 
 
 // w = w - alpha v - beta_prev v_prev (beta_prev may be null), partial |w|^2.
 template <class T>
-__global__ __launch_bounds__(FT) void flat_update_kernel(T* w, const T* v, const T* vprev, const double* alpha,
-                                                         const double* beta_prev, int64_t n, double* partial) {
-  __shared__ double red[FT / 64];
-  const double al = *alpha;
-  const double bp = beta_prev ? *beta_prev : 0.0;
+__device__ __forceinline__ void update_rows(T* w, const T* v, const T* vprev, double al, double bp, int64_t n,
+                                            double* partial, double* red) {
   double p = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * FT + threadIdx.x; i < n; i += (int64_t)gridDim.x * FT) {
     double xr = re_(w[i]) - al * re_(v[i]);
@@ -1040,6 +1043,13 @@ __global__ __launch_bounds__(FT) void flat_update_kernel(T* w, const T* v, const
   }
   p = block_sum<FT>(p, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = p;
+}
+
+template <class T>
+__global__ __launch_bounds__(FT) void flat_update_kernel(T* w, const T* v, const T* vprev, const double* alpha,
+                                                         const double* beta_prev, int64_t n, double* partial) {
+  __shared__ double red[FT / 64];
+  update_rows<T>(w, v, vprev, *alpha, beta_prev ? *beta_prev : 0.0, n, partial, red);
 }
 
 // dst = src / beta; a zero beta flags a breakdown at step `step`.
@@ -1283,8 +1293,173 @@ void launch_spmv(int lpr, const KrylovBatch& kb, int64_t base, int64_t n, const 
   HIP_CHECK(hipGetLastError());
 }
 
-// Plain Lanczos of one large chain, everything on the device (no per-step
-// host synchronization).  alpha/beta/kout/status point at this chain's slots.
+// ---- plain Lanczos of one large chain, replayed as a HIP graph ----
+//
+// Everything stays on the device (no per-step host synchronization), and
+// every per-step quantity is read on the device: the step index j from a
+// counter in the argument block, v_j from buffer (j+1)%2, v_{j-1} from
+// buffer j%2, w from buffer 2, alpha[j] and beta[j-1] by index.  A step is
+// five launches:
+//   beta reduce (+ counter advance) -> scale w into v_j -> SpMV (+ per-block
+//   Re<v, Hv>) -> alpha reduce -> three-term update (+ per-block |w|^2).
+// The launches of G steps are captured once into a HIP graph whose only
+// pointer argument is the argument block, and the graph is replayed K/G
+// times; the last K%G steps are launched directly.  The graph depends on
+// the grid shapes alone (powers of two), so one instantiation serves every
+// chain of that shape.  With launch timing on, every step is launched
+// directly so that each SpMV is timed.
+
+struct StepArgs {
+  double2* buf[3];
+  double* alpha;
+  double* beta;
+  int32_t* status;
+  int32_t* kout;
+  double* part;
+  const int64_t* rowptr;
+  const int32_t* col;
+  const double2* val;
+  int64_t base, n;
+  int32_t jc;  // index of the step opened last (-1 before the first)
+  int32_t kc;  // steps of this chain
+};
+
+// Per-chain set-up: argument block, counter, kout = K, status = ok.
+__global__ void step_init_kernel(StepArgs v, StepArgs* a) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *a = v;
+    a->jc = -1;
+    *v.kout = v.kc;
+    *v.status = IMPED_OK;
+  }
+}
+
+// Opens step j = jc + 1: beta[j-1] = sqrt(sum of the update's partials).
+__global__ __launch_bounds__(FT) void step_beta_kernel(StepArgs* a, int64_t count) {
+  __shared__ double red[FT / 64];
+  const int j = a->jc + 1;
+  double p = 0.0;
+  if (j > 0)
+    for (int64_t i = threadIdx.x; i < count; i += FT) p += a->part[i];
+  p = block_sum<FT>(p, red);  // every thread has read jc before thread 0 advances it
+  if (threadIdx.x == 0) {
+    a->jc = j;
+    if (j > 0) a->beta[j - 1] = sqrt(p);
+  }
+}
+
+// v_j = w / beta[j-1] into the buffer of v_{j-2}; a zero beta flags a breakdown.
+__global__ __launch_bounds__(FT) void step_scale_kernel(StepArgs* a) {
+  const int j = a->jc;
+  if (j == 0) return;
+  const double b = a->beta[j - 1];
+  if (b == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *a->status == IMPED_OK) {
+      *a->status = IMPED_ERR_BREAKDOWN;
+      *a->kout = j;
+    }
+    return;
+  }
+  const double2* src = a->buf[2];
+  double2* dst = a->buf[(j + 1) % 2];
+  const int64_t n = a->n;
+  for (int64_t i = (int64_t)blockIdx.x * FT + threadIdx.x; i < n; i += (int64_t)gridDim.x * FT)
+    dst[i] = make_double2(src[i].x / b, src[i].y / b);
+}
+
+template <int LPR>
+__global__ __launch_bounds__(FT) void step_spmv_kernel(StepArgs* a) {
+  __shared__ double red[FT / 64];
+  const int j = a->jc;
+  spmv_rows<LPR, double2>(a->rowptr, a->col, a->val, a->base, a->n, a->buf[(j + 1) % 2], a->buf[2], a->part, red);
+}
+
+__global__ __launch_bounds__(FT) void step_alpha_kernel(StepArgs* a, int64_t count) {
+  __shared__ double red[FT / 64];
+  double p = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += FT) p += a->part[i];
+  p = block_sum<FT>(p, red);
+  if (threadIdx.x == 0) a->alpha[a->jc] = p;
+}
+
+__global__ __launch_bounds__(FT) void step_update_kernel(StepArgs* a) {
+  __shared__ double red[FT / 64];
+  const int j = a->jc;
+  update_rows<double2>(a->buf[2], a->buf[(j + 1) % 2], j > 0 ? a->buf[j % 2] : nullptr, a->alpha[j],
+                       j > 0 ? a->beta[j - 1] : 0.0, a->n, a->part, red);
+}
+
+// Grid of a step kernel: the power of two covering `need` blocks, at most cap.
+unsigned pow2_blocks(int64_t need, unsigned cap) {
+  unsigned b = 1;
+  while ((int64_t)b < need && b < cap) b <<= 1;
+  return b;
+}
+
+void launch_lanczos_step(StepArgs* a, unsigned nbv, unsigned nbs, int lpr, hipStream_t s) {
+  hipLaunchKernelGGL(step_beta_kernel, dim3(1), dim3(FT), 0, s, a, (int64_t)nbv);
+  hipLaunchKernelGGL(step_scale_kernel, dim3(nbv), dim3(FT), 0, s, a);
+  {
+    TimedLaunch tl("spmv", s);
+    switch (lpr) {
+      case 4: hipLaunchKernelGGL((step_spmv_kernel<4>), dim3(nbs), dim3(FT), 0, s, a); break;
+      case 8: hipLaunchKernelGGL((step_spmv_kernel<8>), dim3(nbs), dim3(FT), 0, s, a); break;
+      case 16: hipLaunchKernelGGL((step_spmv_kernel<16>), dim3(nbs), dim3(FT), 0, s, a); break;
+      case 32: hipLaunchKernelGGL((step_spmv_kernel<32>), dim3(nbs), dim3(FT), 0, s, a); break;
+      default: hipLaunchKernelGGL((step_spmv_kernel<64>), dim3(nbs), dim3(FT), 0, s, a); break;
+    }
+  }
+  hipLaunchKernelGGL(step_alpha_kernel, dim3(1), dim3(FT), 0, s, a, (int64_t)nbs);
+  hipLaunchKernelGGL(step_update_kernel, dim3(nbv), dim3(FT), 0, s, a);
+  HIP_CHECK(hipGetLastError());
+}
+
+// The argument block: allocated once and never moved, since instantiated
+// graphs hold its address.
+StepArgs* step_args_block() {
+  static DevBuf blk;
+  if (!blk.p) blk.reserve(sizeof(StepArgs));
+  return blk.as<StepArgs>();
+}
+
+struct StepGraph {
+  unsigned nbv, nbs;
+  int lpr, G;
+  hipGraphExec_t exec;
+};
+
+// Instantiated G-step graph for a grid shape (captured on a private stream;
+// at most 32 shapes are kept, the oldest destroyed after `s` has drained).
+hipGraphExec_t lanczos_step_graph(StepArgs* a, unsigned nbv, unsigned nbs, int lpr, int G, hipStream_t s) {
+  static std::vector<StepGraph> cache;
+  static hipStream_t cap = nullptr;
+  for (const auto& g : cache)
+    if (g.nbv == nbv && g.nbs == nbs && g.lpr == lpr && g.G == G) return g.exec;
+  if (!cap) HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    for (int g = 0; g < G; ++g) launch_lanczos_step(a, nbv, nbs, lpr, cap);
+  } catch (...) {
+    (void)hipStreamEndCapture(cap, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HIP_CHECK(hipStreamEndCapture(cap, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIP_CHECK(ie);
+  if (cache.size() >= 32) {
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipGraphExecDestroy(cache.front().exec));
+    cache.erase(cache.begin());
+  }
+  cache.push_back(StepGraph{nbv, nbs, lpr, G, exec});
+  return exec;
+}
+
+// alpha/beta/kout/status point at this chain's slots.
 void lanczos_flat_chain(const KrylovBatch& kb, int c, const double2* v0, int K, double* alpha, double* beta,
                         int32_t* kout, int32_t* status, hipStream_t s) {
   KScratch& sc = kscratch();
@@ -1292,32 +1467,32 @@ void lanczos_flat_chain(const KrylovBatch& kb, int c, const double2* v0, int K, 
   const int Kc = (int)std::min<int64_t>(K, n);
   const int lpr = (int)g_tuning.spmv_lpr;
   sc.wbuf.reserve(3 * n * sizeof(double2));
-  double2* P[3] = {sc.wbuf.as<double2>(), sc.wbuf.as<double2>() + n, sc.wbuf.as<double2>() + 2 * n};
-  const unsigned nbv = vec_blocks(n);
+  const unsigned nbv = pow2_blocks((n + FT - 1) / FT, 2048);
+  const unsigned nbs = pow2_blocks((n + FT / lpr - 1) / (FT / lpr), 4096);
   sc.fpart.reserve(std::max<int64_t>((n + 3) / 4 + 1, 4096) * sizeof(double));
-  double* part = sc.fpart.as<double>();
-  const int32_t init[2] = {Kc, IMPED_OK};
-  HIP_CHECK(hipMemcpy(kout, &init[0], 4, hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(status, &init[1], 4, hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpyAsync(P[1], v0 + base, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
-  int iv = 1, ip = 0, iw = 2;  // v_j, v_{j-1}, w
-  for (int j = 0; j < Kc; ++j) {
-    if (j > 0) {
-      // beta_{j-1} = |w|, v_j = w / beta_{j-1} into the v_{j-2} buffer
-      hipLaunchKernelGGL(flat_reduce_kernel, dim3(1), dim3(FT), 0, s, part, (int64_t)nbv, (int64_t)0, beta + j - 1, 1);
-      hipLaunchKernelGGL((flat_scale_kernel<double2>), dim3(nbv), dim3(FT), 0, s, P[iw], P[ip], beta + j - 1, n,
-                         status, kout, j);
-      const int nv = ip;
-      ip = iv;
-      iv = nv;
-    }
-    unsigned nbs = 0;
-    launch_spmv<double2>(lpr, kb, base, n, P[iv], P[iw], part, &nbs, s);
-    hipLaunchKernelGGL(flat_reduce_kernel, dim3(1), dim3(FT), 0, s, part, (int64_t)nbs, (int64_t)0, alpha + j, 0);
-    hipLaunchKernelGGL((flat_update_kernel<double2>), dim3(nbv), dim3(FT), 0, s, P[iw], P[iv],
-                       j > 0 ? P[ip] : (const double2*)nullptr, alpha + j, j > 0 ? beta + j - 1 : nullptr, n, part);
-    HIP_CHECK(hipGetLastError());
+  StepArgs v{};
+  for (int i = 0; i < 3; ++i) v.buf[i] = sc.wbuf.as<double2>() + i * n;
+  v.alpha = alpha;
+  v.beta = beta;
+  v.status = status;
+  v.kout = kout;
+  v.part = sc.fpart.as<double>();
+  v.rowptr = kb.rowptr;
+  v.col = kb.col;
+  v.val = kb.val;
+  v.base = base;
+  v.n = n;
+  v.kc = Kc;
+  StepArgs* a = step_args_block();
+  hipLaunchKernelGGL(step_init_kernel, dim3(1), dim3(64), 0, s, v, a);
+  HIP_CHECK(hipMemcpyAsync(v.buf[1], v0 + base, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+  const int G = (int)g_tuning.lanczos_graph_steps;
+  int j = 0;
+  if (G > 0 && !timing_enabled() && Kc >= G) {
+    hipGraphExec_t exec = lanczos_step_graph(a, nbv, nbs, lpr, G, s);
+    for (; j + G <= Kc; j += G) HIP_CHECK(hipGraphLaunch(exec, s));
   }
+  for (; j < Kc; ++j) launch_lanczos_step(a, nbv, nbs, lpr, s);
 }
 
 // Lowest eigenpairs of one large chain: Lanczos with full reorthogonalization

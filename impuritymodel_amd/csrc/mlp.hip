@@ -174,6 +174,54 @@ __global__ __launch_bounds__(256) void mse_grad_kernel(const float* Y, const flo
   if (threadIdx.x == 0) atomicAdd(&loss[z], (red[0] + red[1] + red[2] + red[3]) / (double)BN);
 }
 
+// Sum over a 256-thread workgroup, broadcast to every thread.
+__device__ __forceinline__ double sum256(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  __syncthreads();  // red may still be read by the previous call
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+// Ensemble statistics of one sample per workgroup, from the outputs
+// y [E][B][D]: member spectra m_e = max(y_e / scale, 0) / area_e (area =
+// sum_d w_d (.), the trapezoid rule when w holds its weights), their mean
+// renormalized to unit area, and the disagreement
+// dis = (1/E) sum_e sum_d w_d |m_e,d - mean_d|.  E <= 64.
+__global__ __launch_bounds__(256) void ensemble_stats_kernel(const float* y, int E, int B, int D, const double* w,
+                                                             double scale, double* mean, double* dis) {
+  __shared__ double red[4];
+  __shared__ double area[64];
+  const int b = blockIdx.x;
+  for (int e = 0; e < E; ++e) {
+    const float* ye = y + ((int64_t)e * B + b) * D;
+    double p = 0.0;
+    for (int d = threadIdx.x; d < D; d += 256) p += w[d] * fmax((double)ye[d] / scale, 0.0);
+    p = sum256(p, red);
+    if (threadIdx.x == 0) area[e] = p;
+  }
+  __syncthreads();
+  double* mb = mean + (int64_t)b * D;
+  double pm = 0.0;
+  for (int d = threadIdx.x; d < D; d += 256) {
+    double acc = 0.0;
+    for (int e = 0; e < E; ++e) acc += fmax((double)y[((int64_t)e * B + b) * D + d] / scale, 0.0) / area[e];
+    acc /= (double)E;
+    mb[d] = acc;
+    pm += w[d] * acc;
+  }
+  const double marea = sum256(pm, red);
+  double pd = 0.0;
+  for (int d = threadIdx.x; d < D; d += 256) {
+    const double mn = mb[d] / marea;
+    mb[d] = mn;
+    for (int e = 0; e < E; ++e)
+      pd += w[d] * fabs(fmax((double)y[((int64_t)e * B + b) * D + d] / scale, 0.0) / area[e] - mn);
+  }
+  pd = sum256(pd, red);
+  if (threadIdx.x == 0) dis[b] = pd / (double)E;
+}
+
 // db[z][n] = sum_b dZ[z][b][n]
 __global__ void bias_grad_kernel(const uint16_t* dZ, int B, int N, int64_t dz_z, float* db, int64_t db_z) {
   const int z = blockIdx.y;
@@ -221,6 +269,7 @@ struct Mlp {
   bool acts = false;
   DevBuf xb, yb, out, loss, idx;    // minibatch input bf16, targets fp32, output fp32 [E][B][dL]
   DevBuf dX, dY;                    // data set (fp32)
+  DevBuf sw, smean, sdis;           // ensemble statistics: area weights, mean [B][dL], disagreement [B]
   int64_t n_data = 0;
 };
 
@@ -406,6 +455,23 @@ void mlp_forward(Mlp& net, const float* x_host, int B, float* y_host, hipStream_
   upload_batch_input(net, x_host, B, s);
   forward(net, B, s);
   d2h(y_host, net.out.p, (size_t)net.E * B * net.dims[net.L], s);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, double scale, double* mean_host,
+                 double* dis_host, hipStream_t s) {
+  alloc_acts(net, B);
+  upload_batch_input(net, x_host, B, s);
+  forward(net, B, s);
+  const int D = net.dims[net.L];
+  h2d(net.sw, w_host, D, s);
+  net.smean.reserve((size_t)B * D * sizeof(double));
+  net.sdis.reserve((size_t)B * sizeof(double));
+  hipLaunchKernelGGL(ensemble_stats_kernel, dim3(B), dim3(256), 0, s, net.out.as<float>(), net.E, B, D,
+                     net.sw.as<double>(), scale, net.smean.as<double>(), net.sdis.as<double>());
+  HIP_CHECK(hipGetLastError());
+  d2h(mean_host, net.smean.p, (size_t)B * D, s);
+  d2h(dis_host, net.sdis.p, B, s);
   HIP_CHECK(hipStreamSynchronize(s));
 }
 

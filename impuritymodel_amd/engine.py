@@ -44,6 +44,24 @@ class Ops:
         )
         self.h = h
 
+    @property
+    def n_terms(self):
+        return len(self._arrays["term_meta"])
+
+    def set_coef(self, values):
+        """Replace the term values (complex, compiled term order; same structure)."""
+        v = _c64(values)
+        check(lib.imped_ops_set_coef(self.h, ptr(v), len(v) // 2 if np.size(values) else 0))
+
+    def set_affine(self, base, dirs, theta):
+        """Values of the first len(theta) models: base [T1] + theta [m, P] @ dirs [P, T1],
+        computed on the device (every model holds the same T1-term structure)."""
+        base = np.asarray(base, np.complex128).reshape(-1)
+        dirs = np.asarray(dirs, np.complex128).reshape(-1, len(base))
+        theta = np.ascontiguousarray(theta, np.float64).reshape(-1, len(dirs))
+        check(lib.imped_ops_set_affine(self.h, len(dirs), len(base), ptr(_c64(base)), ptr(_c64(dirs)), len(theta),
+                                       ptr(_nz(theta.reshape(-1)))))
+
     def __del__(self):
         _release("imped_ops_destroy", self)
 

@@ -134,6 +134,7 @@ class XASFamily:
         self._tables = {}
         self._dip_tables = {}
         self._bases = {}
+        self._ops = {}
 
     def hop(self, theta):
         """The operator dict of one parameter set, assembled term by term."""
@@ -154,6 +155,29 @@ class XASFamily:
         if W not in self._tables:
             self._tables[W] = TermTable(dict(zip(self.keys, [1.0] * len(self.keys))), self.restrictions, W)
         return self._tables[W]
+
+    def batch_ops(self, thetas, W=1):
+        """Device operators for a batch of parameter sets: (H of every set, the
+        x/y/z dipoles of every set).  The structure is compiled once per batch
+        capacity (a power of two >= 16); a batch uploads only theta, and the
+        term values c0 + theta @ cmat are formed on the device.  The returned
+        operators are shared: their values belong to the last call."""
+        from .batched import _dipole_tables
+        from .engine import Ops
+
+        th = np.ascontiguousarray(np.asarray(thetas, np.float64).reshape(len(thetas), -1)[:, :7])
+        cap = max(16, 1 << (len(th) - 1).bit_length())
+        tab = self.table(W)
+        if (cap, W) not in self._ops:
+            if len(self._ops) >= 8:
+                self._ops.pop(next(iter(self._ops)))
+            n = ops.n_spin_orbitals(self.nBaths)
+            m0 = FamilyModel(self, np.zeros(len(PARAM_NAMES)), self.c0)
+            self._ops[(cap, W)] = (Ops([tab.with_coef(self.c0)] * cap, None, n, W),
+                                   Ops(_dipole_tables([m0], W) * cap, None, n, W))
+        ops_h, ops_t = self._ops[(cap, W)]
+        ops_h.set_affine(self.c0[tab.src], self.cmat[:, tab.src], th)
+        return ops_h, ops_t
 
     def models(self, thetas):
         thetas = np.asarray(thetas, np.float64)
@@ -216,6 +240,20 @@ class Ensemble:
         self._check(self._lib.imped_mlp_forward(self.h, self._ptr(x), len(x), self._ptr(y)))
         return y
 
+    def predict(self, x, weights, scale):
+        """Ensemble statistics on the device: (mean [B, D], disagreement [B]).
+
+        Members m_e = max(y_e / scale, 0) normalized to sum_d weights_d m_e,d = 1;
+        mean = their average renormalized the same way; disagreement = the
+        members' mean of sum_d weights_d |m_e,d - mean_d|."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
+        w = np.ascontiguousarray(weights, np.float64).reshape(self.dims[-1])
+        mean = np.zeros((len(x), self.dims[-1]), np.float64)
+        dis = np.zeros(len(x), np.float64)
+        self._check(self._lib.imped_mlp_predict(self.h, self._ptr(x), len(x), self._ptr(w), float(scale),
+                                                self._ptr(mean), self._ptr(dis)))
+        return mean, dis
+
     def gradients(self, x, t):
         x = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
         t = np.ascontiguousarray(t, np.float32).reshape(len(x), self.dims[-1])
@@ -265,16 +303,32 @@ def normalize_params(p):
     return (2.0 * (np.asarray(p) - lo) / (hi - lo) - 1.0).astype(np.float32)
 
 
+def trapezoid_weights(omega):
+    """w with sum_d w_d f_d = the trapezoid rule of f on the grid omega."""
+    x = np.asarray(omega, np.float64)
+    d = np.diff(x)
+    w = np.zeros_like(x)
+    w[:-1] += 0.5 * d
+    w[1:] += 0.5 * d
+    return w
+
+
+_W_OMEGA = trapezoid_weights(OMEGA)
+
+
+def _weights(omega):
+    return _W_OMEGA if omega is OMEGA else trapezoid_weights(omega)
+
+
 def area_normalize(spectra, omega=OMEGA):
     """Spectra scaled to unit trapezoid area on omega."""
     s = np.asarray(spectra, np.float64)
-    area = np.trapezoid(s, omega, axis=-1)
-    return s / area[..., None]
+    return s / (s @ _weights(omega))[..., None]
 
 
 def l1_error(a, b, omega=OMEGA):
-    """L1 distance of area-normalized spectra: sum |a - b| d omega."""
-    return np.trapezoid(np.abs(area_normalize(a, omega) - area_normalize(b, omega)), omega, axis=-1)
+    """L1 distance of area-normalized spectra: sum |a - b| d omega (trapezoid rule)."""
+    return np.abs(area_normalize(a, omega) - area_normalize(b, omega)) @ _weights(omega)
 
 
 SCALE = float(OMEGA[-1] - OMEGA[0])  # unit-area spectra times the window: mean value 1
@@ -298,12 +352,8 @@ class Surrogate:
                               seed)
 
     def predict(self, params):
-        y = self.net.forward(normalize_params(params)).astype(np.float64) / SCALE
-        y = np.maximum(y, 0.0)
-        members = area_normalize(y)
-        mean = area_normalize(members.mean(axis=0))
-        dis = np.trapezoid(np.abs(members - mean[None]), OMEGA, axis=-1).mean(axis=0)
-        return mean, dis
+        """(area-normalized ensemble mean [B, n_w], disagreement [B]), computed on the device."""
+        return self.net.predict(normalize_params(params), _W_OMEGA, SCALE)
 
     def calibrate(self, params, exact, bar=1e-2):
         """Largest threshold under which every validation prediction meets the L1 bar."""
@@ -451,14 +501,15 @@ def fitting_loop(family, target, surrogate=None, n_walkers=256, n_iter=8, step=0
 
 def exact_spectra(family, thetas, krylovSize=200, reorth=0, **kw):
     """Exact XAS of parameter sets by the batched GPU ED, area-normalized."""
-    from .batched import ground_bases, xas_batch
+    from .batched import ground_bases, xas_batch_ops
 
     from . import opcompile
 
-    models = family.models(thetas)
-    W = opcompile.n_words(models[0].n_spin_orbitals)
+    thetas = np.asarray(thetas, np.float64)
+    W = opcompile.n_words(ops.n_spin_orbitals(family.nBaths))
     if W not in family._bases:
-        family._bases[W] = ground_bases(models[:1], W)[0]
-    res = xas_batch(models, omega=OMEGA, delta=DELTA, krylovSize=krylovSize, T=np.asarray(thetas)[:, 7],
-                    reorth=reorth, bases=[family._bases[W]] * len(models), **kw)
+        family._bases[W] = ground_bases(family.models(thetas[:1]), W)[0]
+    ops_h, ops_t = family.batch_ops(thetas, W)
+    res = xas_batch_ops(ops_h, ops_t, [family._bases[W]] * len(thetas), OMEGA, delta=DELTA, krylovSize=krylovSize,
+                        T=thetas[:, 7], reorth=reorth, **kw)
     return area_normalize(res.spectra), res

@@ -60,7 +60,9 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * chains up to this size get one wave), "eigen_bt" (threads per larger
  * eigensolver chain: 256, 512, 1024), "eigen_real", "eigen_dgks",
  * "eigen_check_every", "flat_min_n" (chains above it use the grid-wide
- * kernels), "spmv_lpr" (lanes per row of the grid-wide SpMV: 4..64). */
+ * kernels), "spmv_lpr" (lanes per row of the grid-wide SpMV: 4..64),
+ * "lanczos_graph_steps" (grid-wide Lanczos steps per replayed HIP graph,
+ * 0..256; 0 launches every step directly, as does switched-on timing). */
 int imped_set_tuning(const char* name, int64_t value);
 
 /* The engine's device radix sort (the one that orders BFS layers and
@@ -91,6 +93,21 @@ int imped_ops_create(int W, int n_orb, int n_models, const int32_t* model_group_
                      const uint64_t* group_flip, const uint64_t* term_masks, const uint32_t* term_meta,
                      const double* term_coef, const int32_t* model_restr_off, const uint64_t* restr_mask,
                      const int32_t* restr_lohi, imped_ops* out);
+/* Replaces the term values of an operator set with the same term structure
+ * (n_terms = the term count given at creation, values in the same order): a
+ * parameter sweep over one model family compiles its structure once.  Launches
+ * already queued on the stream use the old values.  No reference
+ * counterpart: the reference rebuilds its operator dicts per parameter set. */
+int imped_ops_set_coef(imped_ops ops, const double* term_coef, int64_t n_terms);
+/* Term values of an affine model family on the device: every model holds
+ * terms_per_model terms of one structure, and model m < n_models gets
+ * value[t] = base[t] + sum_p theta[m][p] dirs[p][t] (complex base/dirs as
+ * interleaved doubles, real theta [n_models][n_params]); models from n_models
+ * on keep their values.  Only theta changes from one parameter batch to the
+ * next.  No reference counterpart (the reference assembles each hOp dict term
+ * by term: get_spectra.py:123-176 with finite.py:271-374). */
+int imped_ops_set_affine(imped_ops ops, int n_params, int64_t terms_per_model, const double* base, const double* dirs,
+                         int n_models, const double* theta);
 int imped_ops_destroy(imped_ops ops);
 
 /* ---- sparse states (multi-configurational dicts {state: amplitude}) ---- */
@@ -221,6 +238,14 @@ int imped_mlp_set_params(imped_mlp net, const float* params);
 int imped_mlp_get_params(imped_mlp net, float* params);
 /* y[e][b][dims[L]] = network e applied to x[b][dims[0]] (inputs rounded to BF16). */
 int imped_mlp_forward(imped_mlp net, const float* x, int batch, float* y);
+/* Forward pass plus the ensemble statistics of each sample b, on the device:
+ * member spectra m_e = max(y_e / scale, 0) / sum_d weights[d] max(y_e / scale, 0),
+ * mean[b][d] = their average renormalized to sum_d weights[d] mean = 1, and
+ * disagreement[b] = (1/n_members) sum_e sum_d weights[d] |m_e,d - mean[b][d]|
+ * (trapezoid weights of the output grid give unit-area spectra and mean L1
+ * distances).  No reference counterpart. */
+int imped_mlp_predict(imped_mlp net, const float* x, int batch, const double* weights, double scale, double* mean,
+                      double* disagreement);
 /* Gradients [n_members][P] of the mean squared error mean_{b,j} (y_e - t)^2
  * over one batch (x, t), and the loss of each member (either may be NULL). */
 int imped_mlp_gradients(imped_mlp net, const float* x, const float* t, int batch, float* grads, double* loss);

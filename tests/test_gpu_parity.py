@@ -267,6 +267,52 @@ def test_flat_kernels_match_workgroup_kernels(gpu):
         engine.set_tuning("eigen_real", 1)
 
 
+def test_flat_lanczos_graph_replay_matches_direct(gpu):
+    """The grid-wide Lanczos replayed as HIP graphs of G steps (with a direct
+    remainder) gives bit-identical alpha/beta to direct launches, for several
+    G and two chains that reuse one instantiated graph; a breakdown inside a
+    replayed graph is flagged at the same step as with direct launches."""
+    from impuritymodel_amd import engine, finite
+    from impuritymodel_amd.batched import ground_bases
+    from impuritymodel_amd.models import make_model
+
+    models = [make_model(n_bath=nb, seed=40 + nb, hField=(0.0, hy, 1e-4)) for nb, hy in ((20, 0.0), (30, 1e-3))]
+    ops = engine.Ops([m.hOp for m in models], [m.restrictions for m in models], max(m.n_spin_orbitals for m in models))
+    b = engine.Batch(ops, np.arange(2, dtype=np.int32))
+    b.set_seeds(ground_bases(models, ops.W))
+    b.build(fixed_basis=True)
+    rng = np.random.default_rng(2)
+    v0 = rng.normal(size=b.rows) + 1j * rng.normal(size=b.rows)
+    for c in range(2):
+        r0, r1 = b.row_base[c], b.row_base[c + 1]
+        v0[r0:r1] /= np.linalg.norm(v0[r0:r1])
+    K = 45
+    wa, wb, wk, ws = b.lanczos(K, v0=v0)  # workgroup kernels
+    try:
+        engine.set_tuning("flat_min_n", 0)
+        engine.set_tuning("lanczos_graph_steps", 0)
+        da, db, dk, ds = b.lanczos(K, v0=v0)
+        np.testing.assert_allclose(da, wa, rtol=1e-11, atol=0)
+        np.testing.assert_allclose(db[:, :K - 1], wb[:, :K - 1], rtol=1e-10, atol=0)
+        for G in (16, 7, 1, 45):
+            engine.set_tuning("lanczos_graph_steps", G)
+            for _ in range(2):  # the second call replays the cached graph
+                ga, gb_, gk, gs = b.lanczos(K, v0=v0)
+                assert np.array_equal(ga, da) and np.array_equal(gb_, db)
+                assert np.array_equal(gk, dk) and np.array_equal(gs, ds)
+        engine.set_tuning("lanczos_graph_steps", 1)
+        d = scipy.sparse.csr_matrix(np.diag([1.0, 2.0, 3.0]).astype(complex))
+        with pytest.raises(ValueError, match="beta==0"):
+            finite.get_tridiagonal_krylov_vectors(d, np.array([1, 0, 0], complex), 3)
+        a3, b3 = finite.get_tridiagonal_krylov_vectors(d, np.ones(3, complex) / np.sqrt(3), 3)
+        ao, bo = okry.lanczos(d, np.ones(3, complex) / np.sqrt(3), 3)
+        np.testing.assert_allclose(a3, ao, atol=1e-12)
+        np.testing.assert_allclose(b3, bo, atol=1e-12)
+    finally:
+        engine.set_tuning("flat_min_n", 32768)
+        engine.set_tuning("lanczos_graph_steps", 16)
+
+
 def test_lanczos_random_hermitian_and_breakdown(gpu):
     from impuritymodel_amd import finite
 

@@ -91,3 +91,62 @@ def test_family_spectra_match_assembled_models(gpu):
         r = xas_batch([m], omega=OMEGA, delta=DELTA, krylovSize=200, T=p[7], reorth=1)
         slow.append(area_normalize(r.spectra[0]))
     np.testing.assert_allclose(fast, np.array(slow), rtol=0, atol=1e-8 * np.abs(slow).max())
+
+
+def test_ensemble_predict_statistics_vs_numpy(gpu):
+    """The device ensemble statistics equal the same statistics computed in
+    numpy (trapezoid rule, np.trapezoid) from the members' raw outputs."""
+    from impuritymodel_amd.surrogate import OMEGA, SCALE, Ensemble, trapezoid_weights
+
+    dims = (8, 64, 64, len(OMEGA))
+    net = Ensemble(5, dims, max_batch=64, seed=2)
+    rng = np.random.default_rng(3)
+    p = net.init_params(3)
+    p += rng.normal(0, 0.02, p.shape).astype(np.float32)
+    net.set_params(p)
+    for B in (37, 5):
+        x = rng.uniform(-1, 1, (B, 8)).astype(np.float32)
+        y = np.maximum(net.forward(x).astype(np.float64) / SCALE, 0.0)
+        members = y / np.trapezoid(y, OMEGA, axis=-1)[..., None]
+        mean = members.mean(axis=0)
+        mean = mean / np.trapezoid(mean, OMEGA, axis=-1)[:, None]
+        dis = np.trapezoid(np.abs(members - mean[None]), OMEGA, axis=-1).mean(axis=0)
+        gm, gd = net.predict(x, trapezoid_weights(OMEGA), SCALE)
+        np.testing.assert_allclose(gm, mean, rtol=1e-12, atol=1e-14 * np.abs(mean).max())
+        np.testing.assert_allclose(gd, dis, rtol=1e-11, atol=1e-15)
+
+
+def test_affine_family_operators_on_device(gpu):
+    """Operators of a parameter batch formed on the device (c0 + theta @ cmat)
+    give the spectra of per-model compiled tables; a second, smaller batch
+    reuses the cached operators; set_coef replaces values in place."""
+    from impuritymodel_amd.batched import xas_batch
+    from impuritymodel_amd.engine import Batch
+    from impuritymodel_amd.surrogate import DELTA, OMEGA, XASFamily, area_normalize, exact_spectra, sample_params
+
+    fam = XASFamily()
+    for n, seed in ((20, 4), (9, 5)):
+        P = sample_params(n, seed=seed)
+        fast, _ = exact_spectra(fam, P, reorth=1)
+        r = xas_batch(fam.models(P), omega=OMEGA, delta=DELTA, krylovSize=200, T=P[:, 7], reorth=1)
+        ref = area_normalize(r.spectra)
+        np.testing.assert_allclose(fast, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+    # set_coef: the operators of the last batch, values rewritten from the host
+    P = sample_params(3, seed=6)
+    ops_h, _ = fam.batch_ops(P)
+    tab = fam.table(1)
+    cf = fam.coef(P)
+    vals = np.concatenate([cf[i % 3][tab.src] for i in range(ops_h.M)])
+    ops_h.set_coef(vals)
+    b = Batch(ops_h, np.arange(3, dtype=np.int32))
+    b.set_seeds([fam._bases[1]] * 3)
+    b.build(fixed_basis=True)
+    ev, _, _, ok = b.eigen(2, want_vectors=False)
+    b2 = Batch(fam.batch_ops(P)[0], np.arange(3, dtype=np.int32))
+    b2.set_seeds([fam._bases[1]] * 3)
+    b2.build(fixed_basis=True)
+    ev2, _, _, ok2 = b2.eigen(2, want_vectors=False)
+    assert ok and ok2
+    np.testing.assert_allclose(ev, ev2, rtol=0, atol=1e-10)
+    with pytest.raises(Exception):
+        ops_h.set_coef(vals[:-1])
