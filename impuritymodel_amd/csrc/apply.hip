@@ -14,7 +14,10 @@
 // the operator dict's order, the order in which applyOp adds them.
 //
 // A wave handles 64 sources of one model, so the group/term loops are
-// wave-uniform and the term tables are read through scalar loads.
+// wave-uniform.  The wave copies its model's group and term tables into its
+// own LDS slice in chunks (coalesced vector loads, TERM_CHUNK terms and
+// GROUP_CHUNK groups at a time), and the loops read them from there: a term
+// then costs an LDS broadcast instead of a dependent scalar-cache miss.
 #include "kernels.h"
 
 namespace imped {
@@ -96,10 +99,29 @@ __device__ void hash_discover(const HashView& h, const CandView& c, int32_t chai
   }
 }
 
+constexpr int TERM_CHUNK = 128;  // terms per staged chunk (per wave)
+constexpr int GROUP_CHUNK = 64;  // groups per staged chunk (per wave)
+constexpr int RESTR_LDS = 16;    // restrictions kept in LDS (more are read from memory)
+
+// LDS writes by some lanes of a wave, read by every lane of the same wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int W, int MODE, bool FILL>
 __global__ __launch_bounds__(256) void apply_kernel(ApplyArgs a) {
+  __shared__ uint64_t l_mask[4][TERM_CHUNK * 3 * W];
+  __shared__ double2 l_coef[4][TERM_CHUNK];
+  __shared__ uint32_t l_meta[4][TERM_CHUNK];
+  __shared__ int32_t l_gto[4][GROUP_CHUNK + 1];
+  __shared__ uint64_t l_flip[4][GROUP_CHUNK * W];
+  __shared__ uint64_t l_rmask[4][RESTR_LDS * W];
+  __shared__ int32_t l_rlohi[4][RESTR_LDS * 2];
   const int lane = threadIdx.x & 63;
-  const int64_t task_id = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int wv = threadIdx.x >> 6;
+  const int64_t task_id = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
   if (task_id >= a.n_tasks) return;
   const WaveTask task = a.tasks[task_id];
   const int model = __builtin_amdgcn_readfirstlane(task.model);
@@ -124,43 +146,103 @@ __global__ __launch_bounds__(256) void apply_kernel(ApplyArgs a) {
   const int g1 = o.model_group_off[model + 1];
   const int r0 = o.model_restr_off[model];
   const int r1 = o.model_restr_off[model + 1];
+  const int t_end = o.group_term_off[g1];  // the model's terms end here
+  uint64_t* lm = l_mask[wv];
+  double2* lc = l_coef[wv];
+  uint32_t* lt = l_meta[wv];
+  int32_t* lg = l_gto[wv];
+  uint64_t* lf = l_flip[wv];
+  int gc0 = g0, gc1 = g0;  // staged groups [gc0, gc1)
+  int tc0 = 0, tc1 = 0;    // staged terms [tc0, tc1)
+  const int nr = r1 - r0;
+  const int nrl = nr < RESTR_LDS ? nr : RESTR_LDS;
+  uint64_t* lrm = l_rmask[wv];
+  int32_t* lrl = l_rlohi[wv];
+  if (MODE != MODE_FIXED) {
+    for (int i = lane; i < nrl * W; i += 64) lrm[i] = o.restr_mask[(size_t)r0 * W + i];
+    for (int i = lane; i < nrl * 2; i += 64) lrl[i] = o.restr_lohi[2 * r0 + i];
+  }
 
   for (int g = g0; g < g1; ++g) {
-    const int tb = o.group_term_off[g];
-    const int te = o.group_term_off[g + 1];
+    if (g >= gc1) {
+      wave_lds_sync();  // every lane is done with the previous chunk
+      gc0 = g;
+      gc1 = g + GROUP_CHUNK < g1 ? g + GROUP_CHUNK : g1;
+      for (int i = lane; i <= gc1 - gc0; i += 64) lg[i] = o.group_term_off[gc0 + i];
+      for (int i = lane; i < (gc1 - gc0) * W; i += 64) lf[i] = o.group_flip[(size_t)gc0 * W + i];
+      wave_lds_sync();
+    }
+    const int tb = lg[g - gc0];
+    const int te = lg[g - gc0 + 1];
     double2 acc = make_double2(0.0, 0.0);
     bool any = false;
     int first = 0;
-    for (int t = tb; t < te; ++t) {
-      const uint64_t* m = o.term_masks + (size_t)t * 3 * W;
+    // one matching term: add its signed value (in term order)
+    auto take = [&](int t) {
+      const uint64_t* m = lm + (size_t)(t - tc0) * 3 * W;
+      const uint32_t meta = lt[t - tc0];
+      uint32_t par = meta & 1u;
+#pragma unroll
+      for (int w = 0; w < W; ++w) par ^= (uint32_t)__popcll(s[w] & m[2 * W + w]) & 1u;
+      const double2 c = lc[t - tc0];
+      if (par) {
+        acc.x -= c.x;
+        acc.y -= c.y;
+      } else {
+        acc.x += c.x;
+        acc.y += c.y;
+      }
+      if (!any) first = (int)(meta >> 1);
+      any = true;
+    };
+    auto matches = [&](int t) {
+      const uint64_t* m = lm + (size_t)(t - tc0) * 3 * W;
       bool ok = true;
 #pragma unroll
       for (int w = 0; w < W; ++w) ok = ok && ((s[w] & m[w]) == m[W + w]);
-      if (ok) {
-        const uint32_t meta = o.term_meta[t];
-        uint32_t par = meta & 1u;
-#pragma unroll
-        for (int w = 0; w < W; ++w) par ^= (uint32_t)__popcll(s[w] & m[2 * W + w]) & 1u;
-        const double2 c = o.term_coef[t];
-        if (par) {
-          acc.x -= c.x;
-          acc.y -= c.y;
-        } else {
-          acc.x += c.x;
-          acc.y += c.y;
+      return ok;
+    };
+    for (int t = tb; t < te;) {
+      if (t >= tc1 || te > tc1) {
+        // stage from this term on (a group is staged whole when it fits a chunk)
+        if (t >= tc1 || (t == tb && te - tb <= TERM_CHUNK)) {
+          wave_lds_sync();
+          tc0 = t;
+          tc1 = t + TERM_CHUNK < t_end ? t + TERM_CHUNK : t_end;
+          for (int i = lane; i < (tc1 - tc0) * 3 * W; i += 64) lm[i] = o.term_masks[(size_t)tc0 * 3 * W + i];
+          for (int i = lane; i < tc1 - tc0; i += 64) {
+            lc[i] = o.term_coef[tc0 + i];
+            lt[i] = o.term_meta[tc0 + i];
+          }
+          wave_lds_sync();
         }
-        if (!any) first = (int)(meta >> 1);
-        any = true;
       }
+      const int tend = te < tc1 ? te : tc1;
+      // four mask tests per round trip, the matches taken in term order
+      for (; t + 4 <= tend; t += 4) {
+        const bool k0 = matches(t), k1 = matches(t + 1), k2 = matches(t + 2), k3 = matches(t + 3);
+        if (k0) take(t);
+        if (k1) take(t + 1);
+        if (k2) take(t + 2);
+        if (k3) take(t + 3);
+      }
+      for (; t < tend; ++t)
+        if (matches(t)) take(t);
     }
     if (!(any && active)) continue;
     uint64_t tgt[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) tgt[w] = s[w] ^ o.group_flip[(size_t)g * W + w];
+    for (int w = 0; w < W; ++w) tgt[w] = s[w] ^ lf[(g - gc0) * W + w];
     bool keep = true;
     if (MODE != MODE_VECTOR) keep = abs2_ref(acc) >= a.weight_min;
     if (MODE != MODE_FIXED) {
-      for (int r = r0; r < r1 && keep; ++r) {
+      for (int r = 0; r < nrl && keep; ++r) {
+        int n = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) n += __popcll(tgt[w] & lrm[r * W + w]);
+        keep = n >= lrl[2 * r] && n <= lrl[2 * r + 1];
+      }
+      for (int r = r0 + nrl; r < r1 && keep; ++r) {
         int n = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) n += __popcll(tgt[w] & o.restr_mask[(size_t)r * W + w]);
@@ -179,9 +261,10 @@ __global__ __launch_bounds__(256) void apply_kernel(ApplyArgs a) {
       continue;
     }
     if (MODE == MODE_EXPAND) {
+      // new targets are entered into the hash by discover_kernel afterwards
       for (int w = 0; w < W; ++w) a.out_keys[out * W + w] = tgt[w];
       a.out_val[out] = acc;
-      hash_discover<W>(a.hash, a.cand, chain, tgt);
+      a.out_chain[out] = chain;
     } else if (MODE == MODE_FIXED) {
       a.out_col[out] = local;
       a.out_val[out] = cconj(acc);
@@ -212,6 +295,31 @@ static void launch_w(int mode, bool fill, const ApplyArgs& a, hipStream_t s) {
     if (fill) IMPED_LAUNCH(MODE_VECTOR, true); else IMPED_LAUNCH(MODE_VECTOR, false);
   }
 #undef IMPED_LAUNCH
+  HIP_CHECK(hipGetLastError());
+}
+
+// Every expand-fill entry (chain[e], keys[e]) becomes a candidate unless it
+// is a pool state or already a candidate: one thread per entry, so the
+// dependent hash probes of many entries overlap.
+template <int W>
+__global__ __launch_bounds__(256) void discover_kernel(const uint64_t* keys, const int32_t* chain, int64_t E, HashView h,
+                                                       CandView c) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  uint64_t k[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) k[w] = keys[e * W + w];
+  hash_discover<W>(h, c, chain[e], k);
+}
+
+void launch_discover(int W, const uint64_t* keys, const int32_t* chain, int64_t E, HashView h, CandView c,
+                     hipStream_t s) {
+  if (E <= 0) return;
+  TimedLaunch tl("discover", s);
+  if (W == 1)
+    hipLaunchKernelGGL(discover_kernel<1>, dim3(blocks_for(E, 256)), dim3(256), 0, s, keys, chain, E, h, c);
+  else
+    hipLaunchKernelGGL(discover_kernel<2>, dim3(blocks_for(E, 256)), dim3(256), 0, s, keys, chain, E, h, c);
   HIP_CHECK(hipGetLastError());
 }
 

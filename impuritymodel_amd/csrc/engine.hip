@@ -307,7 +307,7 @@ static HashView hash_view(Batch& b) {
 
 // Scratch shared by the build steps.
 struct BuildScratch {
-  DevBuf tasks, counts, offsets, cand_keys, cand_chain, cand_slot, cand_cnt, alive, perm, rowgid, err;
+  DevBuf tasks, counts, offsets, cand_keys, cand_chain, cand_slot, cand_cnt, alive, perm, rowgid, err, col_chain;
 };
 static BuildScratch& bscratch() {
   static BuildScratch s;
@@ -360,11 +360,16 @@ static int64_t apply_layer(Batch& b, int64_t f0, int64_t f1, int mode, double we
     HIP_CHECK(hipMemsetAsync(sc.cand_cnt.p, 0, 2 * sizeof(int32_t), stream()));
     a.out_keys = b.col_keys.as<uint64_t>() + base * b.W;
     a.out_val = b.col_val.as<double2>() + base;
+    sc.col_chain.reserve(std::max<int64_t>(E, 1) * 4);
+    a.out_chain = sc.col_chain.as<int32_t>();
     a.offsets = sc.offsets.as<int64_t>() - f0;
     a.cand = CandView{sc.cand_keys.as<uint64_t>(), sc.cand_chain.as<int32_t>(), sc.cand_slot.as<int32_t>(),
                       sc.cand_cnt.as<int32_t>(),   sc.alive.as<int32_t>(),      sc.cand_cnt.as<int32_t>() + 1,
                       (int32_t)std::min<int64_t>(E, INT32_MAX)};
-    if (E > 0) launch_apply(b.W, mode, true, a, stream());
+    if (E > 0) {
+      launch_apply(b.W, mode, true, a, stream());
+      launch_discover(b.W, a.out_keys, a.out_chain, E, a.hash, a.cand, stream());
+    }
     int32_t cc[2] = {0, 0};
     HIP_CHECK(hipMemcpyAsync(cc, sc.cand_cnt.p, sizeof(cc), hipMemcpyDeviceToHost, stream()));
     HIP_CHECK(hipStreamSynchronize(stream()));

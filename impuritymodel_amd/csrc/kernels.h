@@ -72,6 +72,7 @@ struct ApplyArgs {
   // expand fill
   uint64_t* out_keys;
   double2* out_val;
+  int32_t* out_chain;
   // fixed fill
   int32_t* out_col;
   const int32_t* pool_local;
@@ -84,6 +85,8 @@ struct ApplyArgs {
 };
 
 void launch_apply(int W, int mode, bool fill, const ApplyArgs& a, hipStream_t s);
+void launch_discover(int W, const uint64_t* keys, const int32_t* chain, int64_t E, HashView h, CandView c,
+                     hipStream_t s);
 
 void launch_hash_insert_pool(int W, HashView h, int64_t first, int64_t count, hipStream_t s);
 void launch_hash_clear(int32_t* table, uint32_t cap, hipStream_t s);

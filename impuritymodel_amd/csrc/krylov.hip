@@ -192,6 +192,41 @@ __device__ int sturm_count(const double* a, const double* b, int m, double x, do
   return cnt;
 }
 
+// 1/x to within an ulp: the hardware reciprocal estimate refined by two
+// Newton steps (a short dependent chain, where a correctly rounded division
+// is a long one).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Sturm counts at P shifts: P independent recurrences interleaved, so the
+// latency of one step is shared by all of them.  Same recurrence and pivot
+// guard as sturm_count, with b^2/q formed as b^2 * rcp_nr(q).
+template <int P>
+__device__ __forceinline__ void sturm_counts(const double* a, const double* b, int m, const double* x, double pivmin,
+                                             int* cnt) {
+  double q[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    q[p] = a[0] - x[p];
+    if (fabs(q[p]) < pivmin) q[p] = -pivmin;
+    cnt[p] = q[p] < 0 ? 1 : 0;
+  }
+  for (int i = 1; i < m; ++i) {
+    const double ai = a[i], bb = b[i - 1] * b[i - 1];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      q[p] = (ai - x[p]) - bb * rcp_nr(q[p]);
+      if (fabs(q[p]) < pivmin) q[p] = -pivmin;
+      cnt[p] += q[p] < 0 ? 1 : 0;
+    }
+  }
+}
+
 __device__ double kth_eigenvalue(const double* a, const double* b, int m, int k) {
   double lo = a[0], hi = a[0], scale = 0.0;
   for (int i = 0; i < m; ++i) {
@@ -215,20 +250,24 @@ __device__ double kth_eigenvalue(const double* a, const double* b, int m, int k)
 // pivoting of the shifted tridiagonal.  y and work (4m + m ints) are
 // thread-private (LDS or global).  Every recurrence carries its running
 // values in registers, so no iteration waits on a store it just made.
-__device__ void tri_inverse_iteration(const double* a, const double* b, int m, double lambda, double* y, double* dd,
-                                      double* uu, double* ww, double* ll, int* piv, uint64_t seed) {
+__device__ __forceinline__ void tri_inverse_iteration(const double* __restrict__ a, const double* __restrict__ b, int m, double lambda,
+                                      double* __restrict__ y, double* __restrict__ dd, double* __restrict__ uu,
+                                      double* __restrict__ ww, double* __restrict__ ll, int* __restrict__ piv,
+                                      uint64_t seed) {
   double scale = 0.0;
+#pragma unroll 8
   for (int i = 0; i < m; ++i) scale = fmax(scale, fabs(a[i]) + (i < m - 1 ? fabs(b[i]) : 0.0));
   const double tiny = 2.2e-16 * (scale > 0 ? scale : 1.0);
   // (d, u): diagonal and superdiagonal of the current row after elimination
   double d = a[0] - lambda, u = m > 1 ? b[0] : 0.0;
+#pragma unroll 8
   for (int i = 0; i < m - 1; ++i) {
     const double sub = b[i];
     const double dn = a[i + 1] - lambda;
     const double un = i + 1 < m - 1 ? b[i + 1] : 0.0;
     if (fabs(d) >= fabs(sub)) {
       if (d == 0.0) d = tiny;
-      const double mult = sub / d;
+      const double mult = sub * rcp_nr(d);
       dd[i] = d;
       uu[i] = u;
       ww[i] = 0.0;
@@ -238,7 +277,7 @@ __device__ void tri_inverse_iteration(const double* a, const double* b, int m, d
       u = un;
     } else {
       // rows i and i+1 interchanged
-      const double mult = d / sub;
+      const double mult = d * rcp_nr(sub);
       dd[i] = sub;
       uu[i] = dn;
       ww[i] = un;
@@ -252,10 +291,19 @@ __device__ void tri_inverse_iteration(const double* a, const double* b, int m, d
   dd[m - 1] = d;
   uu[m - 1] = 0.0;
   ww[m - 1] = 0.0;
+  // back substitution multiplies by the guarded reciprocal pivots
+#pragma unroll 8
+  for (int i = 0; i < m; ++i) {
+    double dv = dd[i];
+    if (fabs(dv) < tiny) dv = dv < 0 ? -tiny : tiny;
+    dd[i] = rcp_nr(dv);
+  }
+#pragma unroll 8
   for (int i = 0; i < m; ++i) y[i] = 0.5 + u01(seed + (uint64_t)i);
   for (int it = 0; it < 3; ++it) {
     // forward: L^-1 with the interchanges; yi is the running y[i]
     double yi = y[0];
+#pragma unroll 8
     for (int i = 0; i < m - 1; ++i) {
       double yn = y[i + 1];
       if (piv[i]) {
@@ -270,20 +318,20 @@ __device__ void tri_inverse_iteration(const double* a, const double* b, int m, d
     y[m - 1] = yi;
     // back substitution with U (bandwidth 3); y1, y2 = y[i+1], y[i+2]
     double y1 = 0.0, y2 = 0.0, nrm = 0.0;
+#pragma unroll 8
     for (int i = m - 1; i >= 0; --i) {
       double r = y[i];
       if (i + 1 < m) r -= uu[i] * y1;
       if (i + 2 < m) r -= ww[i] * y2;
-      double dv = dd[i];
-      if (fabs(dv) < tiny) dv = dv < 0 ? -tiny : tiny;
-      const double v = r / dv;
+      const double v = r * dd[i];
       y[i] = v;
       nrm += v * v;
       y2 = y1;
       y1 = v;
     }
-    nrm = sqrt(nrm);
-    for (int i = 0; i < m; ++i) y[i] /= nrm;
+    const double inv = 1.0 / sqrt(nrm);
+#pragma unroll 8
+    for (int i = 0; i < m; ++i) y[i] *= inv;
   }
 }
 
@@ -312,9 +360,10 @@ struct EigenArgs {
   const int32_t* chain_list;
   int dgks;          // 1: second Gram-Schmidt pass only when needed
   int check_every;   // Ritz check interval (steps)
-  int lstride;       // LDS arrays: Gram-Schmidt coefficients (lstride + 2), then optionally
-  int lds_ab;        //   alpha/beta (2 lstride) and, per eigenvalue, y (lstride),
-  int lds_work;      //   LU work (4 lstride) and pivots (lstride ints); lstride >= every chain's steps
+  int lstride;       // LDS arrays: Gram-Schmidt coefficients (lstride + 2), then, when
+  int lds_ab;        //   lds_ab = lds_work = 1, alpha/beta (2 lstride) and per eigenvalue y
+  int lds_work;      //   (lstride), LU work (4 lstride) and pivots (lstride ints); lstride >= every
+                     //   chain's steps
 };
 
 // Element helpers: the eigensolver runs on complex (double2) vectors, or on
@@ -405,19 +454,23 @@ __device__ __forceinline__ double norm2_t(const T* w, int64_t n, double* red) {
 
 constexpr int MAX_NEV = 16;
 
-// cl[l] = <V_l, w> for l < m (and cl[m] = <w, w> when self), four vectors per
-// wave at a time so w is read once per four dot products; results in LDS.
+// cl[l] = <V_l, w> for l < m (and cl[m] = <w, w> when self), eight vectors
+// per wave at a time so w is read once per eight dot products (and eight
+// loads are in flight together); results in LDS.
 template <int BT, class T>
 __device__ void dots_lds(const T* V, int m, int64_t n, const T* w, double2* cl, bool self) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   constexpr int NW = BT / 64;
+  constexpr int U = 8;
   const int total = m + (self ? 1 : 0);
-  for (int l0 = wave * 4; l0 < total; l0 += NW * 4) {
-    double cx[4] = {0.0, 0.0, 0.0, 0.0}, cy[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int l0 = wave * U; l0 < total; l0 += NW * U) {
+    double cx[U], cy[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) cx[u] = cy[u] = 0.0;
     for (int64_t i = lane; i < n; i += 64) {
       const T b = w[i];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) {
         const int l = l0 + u;
         if (l < total) {
           const T av = l < m ? V[(size_t)l * n + i] : b;
@@ -427,7 +480,7 @@ __device__ void dots_lds(const T* V, int m, int64_t n, const T* w, double2* cl, 
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const double x = wave_sum(cx[u]);
       const double y = is_real_v<T> ? 0.0 : wave_sum(cy[u]);
       if (lane == 0 && l0 + u < total) cl[l0 + u] = make_double2(x, y);
@@ -442,8 +495,8 @@ __device__ double update_norm(const T* V, int m, int64_t n, T* w, const double2*
   double p = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += BT) {
     double wx = re_(w[i]), wy = im_(w[i]);
-    // unrolled so the loads of eight Krylov entries are in flight together
-#pragma unroll 8
+    // unrolled so the loads of sixteen Krylov entries are in flight together
+#pragma unroll 16
     for (int l = 0; l < m; ++l) {
       const double2 c = cl[l];
       const T av = V[(size_t)l * n + i];
@@ -464,30 +517,78 @@ __device__ double update_norm(const T* V, int m, int64_t n, T* w, const double2*
 // multisection: the workgroup splits into nev groups; each evaluates Sturm
 // counts at G interior points of its interval per round and keeps the
 // sub-interval where the count crosses k, shrinking it G+1 times per round.
+// hint (or null): the same eigenvalues of a leading block of (a, b).  By
+// Cauchy interlacing they bound the new ones from above, so a first round of
+// points hint - scale 10^(-16 r / (G-2)) (r < G-1) and hint + 1e-14 scale
+// brackets each new value within about its distance from the hint, and the
+// ordinary rounds start from that bracket (the Gershgorin interval when the
+// hint does not bracket it).
 template <int BT>
-__device__ void ritz_values(const double* a, const double* b, int m, int nev, double* s_lam, double* s_lo,
-                            double* s_hi, int* s_first, int* s_flag) {
-  __shared__ double s_bounds[3];
-  if (threadIdx.x == 0) {
-    double lo = a[0], hi = a[0], scale = 0.0;
-    for (int i = 0; i < m; ++i) {
+__device__ __forceinline__ void ritz_values(const double* a, const double* b, int m, int nev, const double* hint, double* s_lam,
+                            double* s_lo, double* s_hi, int* s_first, int* s_flag) {
+  __shared__ double s_part[3][BT / 64];
+  __shared__ double s_bounds[4];
+  {
+    double lo = INFINITY, hi = -INFINITY, sc = 0.0;
+    for (int i = threadIdx.x; i < m; i += BT) {
       const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
       lo = fmin(lo, a[i] - r);
       hi = fmax(hi, a[i] + r);
-      scale = fmax(scale, fabs(a[i]) + r);
+      sc = fmax(sc, fabs(a[i]) + r);
     }
-    s_bounds[0] = lo - 1e-14 * scale - 1e-300;
-    s_bounds[1] = hi + 1e-14 * scale + 1e-300;
-    s_bounds[2] = 1e-300 + 2.2e-16 * 2.2e-16 * scale * scale;
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = fmin(lo, __shfl_xor(lo, off));
+      hi = fmax(hi, __shfl_xor(hi, off));
+      sc = fmax(sc, __shfl_xor(sc, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      s_part[0][threadIdx.x >> 6] = lo;
+      s_part[1][threadIdx.x >> 6] = hi;
+      s_part[2][threadIdx.x >> 6] = sc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int wv = 1; wv < BT / 64; ++wv) {
+        lo = fmin(lo, s_part[0][wv]);
+        hi = fmax(hi, s_part[1][wv]);
+        sc = fmax(sc, s_part[2][wv]);
+      }
+      s_bounds[0] = lo - 1e-14 * sc - 1e-300;
+      s_bounds[1] = hi + 1e-14 * sc + 1e-300;
+      s_bounds[2] = 1e-300 + 2.2e-16 * 2.2e-16 * sc * sc;
+      s_bounds[3] = sc;
+    }
   }
   __syncthreads();
   const int G = BT / nev;
   const int g = threadIdx.x / G, r = threadIdx.x % G;
+  const double pivmin = s_bounds[2];
+  const double scale = s_bounds[3];
   if ((int)threadIdx.x < nev) {
     s_lo[threadIdx.x] = s_bounds[0];
     s_hi[threadIdx.x] = s_bounds[1];
   }
-  const double pivmin = s_bounds[2];
+  if (hint != nullptr) {
+    auto hint_point = [&](int k, int rr) {
+      return rr < G - 1 ? hint[k] - scale * exp10(-16.0 * (double)rr / (double)(G - 2)) : hint[k] + 1e-14 * scale;
+    };
+    if ((int)threadIdx.x < nev) s_first[threadIdx.x] = G;
+    __syncthreads();
+    if (g < nev) {
+      double x = hint_point(g, r);
+      int cnt;
+      sturm_counts<1>(a, b, m, &x, pivmin, &cnt);
+      if (cnt > g) atomicMin(&s_first[g], r);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nev) {
+      const int k = threadIdx.x, f = s_first[k];
+      if (f < G) {  // otherwise the hint does not bound the value: Gershgorin interval
+        if (f > 0) s_lo[k] = fmax(s_lo[k], hint_point(k, f - 1));
+        s_hi[k] = fmin(s_hi[k], hint_point(k, f));
+      }
+    }
+  }
   __syncthreads();
   for (int round = 0; round < 64; ++round) {
     if ((int)threadIdx.x < nev) s_first[threadIdx.x] = G;
@@ -495,8 +596,10 @@ __device__ void ritz_values(const double* a, const double* b, int m, int nev, do
     __syncthreads();
     if (g < nev) {
       const double lo = s_lo[g], hi = s_hi[g];
-      const double x = lo + (hi - lo) * (double)(r + 1) / (double)(G + 1);
-      if (sturm_count(a, b, m, x, pivmin) > g) atomicMin(&s_first[g], r);
+      double x = lo + (hi - lo) * (double)(r + 1) / (double)(G + 1);
+      int cnt;
+      sturm_counts<1>(a, b, m, &x, pivmin, &cnt);
+      if (cnt > g) atomicMin(&s_first[g], r);
     }
     __syncthreads();
     if ((int)threadIdx.x < nev) {
@@ -518,13 +621,42 @@ __device__ void ritz_values(const double* a, const double* b, int m, int nev, do
   __syncthreads();
 }
 
-template <int BT, class T>
+// Orthonormalize the inverse-iteration vectors of clustered eigenvalues
+// (|lam_k - lam_q| <= 1e-3 max(1, |lam_k|)) in order, by wave 0 of the
+// workgroup (lane-parallel dot products and updates).
+__device__ __forceinline__ void ortho_clustered(double* Y, int ys, int m, int ne, const double* lam) {
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  for (int k = 1; k < ne; ++k) {
+    double* yk = Y + (size_t)k * ys;
+    for (int q = 0; q < k; ++q) {
+      if (fabs(lam[k] - lam[q]) > 1e-3 * fmax(1.0, fabs(lam[k]))) continue;
+      const double* yq = Y + (size_t)q * ys;
+      double d = 0.0;
+      for (int i = lane; i < m; i += 64) d += yq[i] * yk[i];
+      d = wave_sum(d);
+      double nn = 0.0;
+      for (int i = lane; i < m; i += 64) {
+        yk[i] -= d * yq[i];
+        nn += yk[i] * yk[i];
+      }
+      const double inv = 1.0 / sqrt(wave_sum(nn));
+      for (int i = lane; i < m; i += 64) yk[i] *= inv;
+    }
+  }
+}
+
+// LW: the tridiagonal and the Ritz work arrays live in LDS (a compile-time
+// choice, so every access to them is an LDS instruction), else in memory.
+template <int BT, class T, bool LW>
 __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   __shared__ double red[BT / 64];
   __shared__ int s_done;
   __shared__ double s_lam[MAX_NEV], s_lo[MAX_NEV], s_hi[MAX_NEV];
   __shared__ int s_first[MAX_NEV];
   __shared__ int s_flag;
+  __shared__ double s_hint[MAX_NEV];  // Ritz values of the previous check
+  int hint_ne = -1;
   extern __shared__ double2 cl[];  // lstride + 2 coefficients, then the optional arrays
   const int c = a.chain_list[blockIdx.x];
   const int64_t n = a.chain_n[c];
@@ -546,17 +678,27 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   // tridiagonal and Ritz work arrays: LDS when they fit (lstride >= mmax)
   const int L = a.lstride;
   double* tail = reinterpret_cast<double*>(cl + L + 2);
-  double* sa = alpha;
-  double* sb = beta;
-  if (a.lds_ab) {
+  double* sa;
+  double* sb;
+  double* Y;
+  double* twb;
+  int* pivb;
+  int ys;  // stride of eigenvalue k's arrays
+  if constexpr (LW) {
     sa = tail;
     sb = tail + L;
-    tail += 2 * L;
+    ys = L;
+    Y = tail + 2 * L;
+    twb = Y + (size_t)nev * L;
+    pivb = reinterpret_cast<int*>(twb + (size_t)nev * 4 * L);
+  } else {
+    sa = alpha;
+    sb = beta;
+    ys = ms;
+    Y = a.yk + (size_t)c * nev * ms;
+    twb = a.tw + (size_t)c * nev * 5 * ms;
+    pivb = a.tpiv + (size_t)c * nev * ms;
   }
-  const int ys = a.lds_work ? L : ms;  // stride of eigenvalue k's arrays
-  double* Y = a.lds_work ? tail : a.yk + (size_t)c * nev * ms;
-  double* twb = a.lds_work ? tail + (size_t)nev * L : a.tw + (size_t)c * nev * 5 * ms;
-  int* pivb = a.lds_work ? reinterpret_cast<int*>(twb + (size_t)nev * 4 * L) : a.tpiv + (size_t)c * nev * ms;
   // seeded random start vector (real when T is real)
   double p = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += BT) {
@@ -614,14 +756,17 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
     if (last || (m >= nev && m % a.check_every == 0)) {
       // Ritz pairs of T_m: eigenvalues by workgroup multisection, then
       // thread k refines eigenvector k by inverse iteration
-      ritz_values<BT>(sa, sb, m, nev < m ? nev : m, s_lam, s_lo, s_hi, s_first, &s_flag);
+      const int ne = nev < m ? nev : m;
+      ritz_values<BT>(sa, sb, m, ne, ne == hint_ne ? s_hint : nullptr, s_lam, s_lo, s_hi, s_first, &s_flag);
+      if ((int)threadIdx.x < ne) s_hint[threadIdx.x] = s_lam[threadIdx.x];
+      hint_ne = ne;
       if (threadIdx.x == 0) s_done = 1;
       __syncthreads();
       if ((int)threadIdx.x < nev && (int)threadIdx.x < m) {
         const int k = threadIdx.x;
         const double lam = s_lam[k];
         double* y = Y + (size_t)k * ys;
-        double* tw = twb + (size_t)k * (a.lds_work ? 4 * ys : 5 * ys);
+        double* tw = twb + (size_t)k * (LW ? 4 * ys : 5 * ys);
         int* piv = pivb + (size_t)k * ys;
         tri_inverse_iteration(sa, sb, m, lam, y, tw, tw + ys, tw + 2 * ys, tw + 3 * ys, piv,
                               a.seed + 7919 * (uint64_t)k);
@@ -630,26 +775,8 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
         if (!(res <= a.tol * fmax(1.0, fabs(lam)))) atomicAnd(&s_done, 0);
       }
       __syncthreads();
-      if (threadIdx.x == 0 && nev > 1) {
-        // orthogonalize inverse-iteration vectors of clustered eigenvalues
-        for (int k = 1; k < nev && k < m; ++k) {
-          double* yk = Y + (size_t)k * ys;
-          for (int q = 0; q < k; ++q) {
-            const double dl = fabs(a.evals[(size_t)c * nev + k] - a.evals[(size_t)c * nev + q]);
-            if (dl > 1e-3 * fmax(1.0, fabs(a.evals[(size_t)c * nev + k]))) continue;
-            const double* yq = Y + (size_t)q * ys;
-            double d = 0.0;
-            for (int i = 0; i < m; ++i) d += yq[i] * yk[i];
-            double nn = 0.0;
-            for (int i = 0; i < m; ++i) {
-              yk[i] -= d * yq[i];
-              nn += yk[i] * yk[i];
-            }
-            nn = sqrt(nn);
-            for (int i = 0; i < m; ++i) yk[i] /= nn;
-          }
-        }
-      }
+      // orthogonalize inverse-iteration vectors of clustered eigenvalues
+      ortho_clustered(Y, ys, m, ne, s_lam);
       __syncthreads();
       if (s_done || last) {
         converged = s_done;
@@ -1170,7 +1297,7 @@ __global__ __launch_bounds__(FT) void flat_ritz_kernel(const double* alpha, cons
   __shared__ int s_first[MAX_NEV];
   __shared__ int s_flag, s_done;
   const int ne = nev < m ? nev : m;
-  ritz_values<FT>(alpha, beta, m, ne, s_lam, s_lo, s_hi, s_first, &s_flag);
+  ritz_values<FT>(alpha, beta, m, ne, nullptr, s_lam, s_lo, s_hi, s_first, &s_flag);
   if (threadIdx.x == 0) s_done = 1;
   __syncthreads();
   if ((int)threadIdx.x < ne) {
@@ -1183,25 +1310,9 @@ __global__ __launch_bounds__(FT) void flat_ritz_kernel(const double* alpha, cons
     if (!(fabs(beta[m - 1] * y[m - 1]) <= tol * fmax(1.0, fabs(s_lam[k])))) atomicAnd(&s_done, 0);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    // orthogonalize inverse-iteration vectors of clustered eigenvalues
-    for (int k = 1; k < ne; ++k) {
-      double* yk = Y + (size_t)k * ms;
-      for (int q = 0; q < k; ++q) {
-        if (fabs(evals[k] - evals[q]) > 1e-3 * fmax(1.0, fabs(evals[k]))) continue;
-        const double* yq = Y + (size_t)q * ms;
-        double d = 0.0, nn = 0.0;
-        for (int i = 0; i < m; ++i) d += yq[i] * yk[i];
-        for (int i = 0; i < m; ++i) {
-          yk[i] -= d * yq[i];
-          nn += yk[i] * yk[i];
-        }
-        nn = sqrt(nn);
-        for (int i = 0; i < m; ++i) yk[i] /= nn;
-      }
-    }
-    *converged = s_done;
-  }
+  // orthogonalize inverse-iteration vectors of clustered eigenvalues
+  ortho_clustered(Y, ms, m, ne, s_lam);
+  if (threadIdx.x == 0) *converged = s_done;
 }
 
 // X_k = sum_l y_k[l] V_l for k < ne (complex output), partial |X_k|^2.
@@ -1743,35 +1854,42 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
     const size_t work = (size_t)nev * ((size_t)L * 8 + 4 * (size_t)L * 8 + (size_t)L * 4);
     constexpr size_t budget = 60 * 1024;
     a.lstride = L;
-    a.lds_ab = coef + ab <= budget;
-    a.lds_work = a.lds_ab && coef + ab + work <= budget;
-    shm = coef + (a.lds_ab ? ab : 0) + (a.lds_work ? work : 0);
+    a.lds_ab = a.lds_work = coef + ab + work <= budget;
+    shm = coef + (a.lds_work ? ab + work : 0);
+  };
+  // eigen_kernel<bt, T, LW> for the layout just chosen
+  auto launch = [&](int bt, dim3 g) {
+#define IMPED_EIG(BT_)                                                                                     \
+  if (real) {                                                                                              \
+    if (a.lds_work) hipLaunchKernelGGL((eigen_kernel<BT_, double, true>), g, dim3(BT_), shm, s, a);       \
+    else hipLaunchKernelGGL((eigen_kernel<BT_, double, false>), g, dim3(BT_), shm, s, a);                 \
+  } else {                                                                                                 \
+    if (a.lds_work) hipLaunchKernelGGL((eigen_kernel<BT_, double2, true>), g, dim3(BT_), shm, s, a);      \
+    else hipLaunchKernelGGL((eigen_kernel<BT_, double2, false>), g, dim3(BT_), shm, s, a);                \
+  }
+    if (bt == 64) {
+      IMPED_EIG(64)
+    } else if (bt == 256) {
+      IMPED_EIG(256)
+    } else if (bt == 512) {
+      IMPED_EIG(512)
+    } else {
+      IMPED_EIG(1024)
+    }
+#undef IMPED_EIG
+    HIP_CHECK(hipGetLastError());
   };
   if (n_wide < order.size()) {
     TimedLaunch tl("eigen", s);
     layout(n_wide);
     a.chain_list = sc.clist.as<int32_t>() + n_wide;
-    const dim3 g((unsigned)(order.size() - n_wide));
-    if (real) hipLaunchKernelGGL((eigen_kernel<64, double>), g, dim3(64), shm, s, a);
-    else hipLaunchKernelGGL((eigen_kernel<64, double2>), g, dim3(64), shm, s, a);
-    HIP_CHECK(hipGetLastError());
+    launch(64, dim3((unsigned)(order.size() - n_wide)));
   }
   a.chain_list = sc.clist.as<int32_t>();
   if (n_wide > 0) {
     TimedLaunch tl("eigen", s);
     layout(0);
-    const dim3 g((unsigned)n_wide);
-    if (g_tuning.eigen_bt == 256) {
-      if (real) hipLaunchKernelGGL((eigen_kernel<256, double>), g, dim3(256), shm, s, a);
-      else hipLaunchKernelGGL((eigen_kernel<256, double2>), g, dim3(256), shm, s, a);
-    } else if (g_tuning.eigen_bt == 512) {
-      if (real) hipLaunchKernelGGL((eigen_kernel<512, double>), g, dim3(512), shm, s, a);
-      else hipLaunchKernelGGL((eigen_kernel<512, double2>), g, dim3(512), shm, s, a);
-    } else {
-      if (real) hipLaunchKernelGGL((eigen_kernel<1024, double>), g, dim3(1024), shm, s, a);
-      else hipLaunchKernelGGL((eigen_kernel<1024, double2>), g, dim3(1024), shm, s, a);
-    }
-    HIP_CHECK(hipGetLastError());
+    launch((int)g_tuning.eigen_bt, dim3((unsigned)n_wide));
   }
   std::vector<int32_t> conv(C, 1);
   d2h(conv.data(), sc.conv.p, C, s);
