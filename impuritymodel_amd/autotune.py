@@ -34,6 +34,7 @@ DEFAULTS = {
     "eigen_wave_max_n": 0,
     "spmv_lpr": 8,
     "lanczos_graph_steps": 16,
+    "eigen_lds_vectors": -1,
 }
 
 CANDIDATES = {

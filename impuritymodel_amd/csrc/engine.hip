@@ -752,6 +752,8 @@ int imped_set_tuning(const char* name, int64_t value) {
       return IMPED_ERR_INVALID;
     }
     g_tuning.spmv_lpr = value;
+  } else if (n == "eigen_lds_vectors") {
+    g_tuning.eigen_lds_vectors = value < 0 ? -1 : value;
   } else if (n == "lanczos_graph_steps") {
     if (value < 0 || value > 256) {
       set_error("imped_set_tuning: lanczos_graph_steps must be in [0, 256]");

@@ -25,12 +25,36 @@ namespace imped {
 
 namespace {
 
+// Cross-lane move of a double by a DPP pattern (both 32-bit halves), lanes
+// outside row_mask reading 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave, identical in every lane, with every lane
+// active: DPP moves inside registers (quad swaps, half-row and row mirrors,
+// then row broadcasts 15 and 31 collect the four rows in lane 63), read back
+// from lane 63.  Fixed order, so runs are reproducible.
+__device__ __forceinline__ double wave_sum(double x) {
+  x += dpp_move<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+  x += dpp_move<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+  x += dpp_move<0x141, 0xF>(x);  // row_half_mirror
+  x += dpp_move<0x140, 0xF>(x);  // row_mirror
+  x += dpp_move<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_move<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+  return __hiloint2double(hi, lo);
+}
+
 // Sum over the workgroup, identical in every thread; fixed order, so runs
 // are reproducible.  A one-wave workgroup needs no LDS round trip.
 template <int BT>
 __device__ __forceinline__ double block_sum(double x, double* lds) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  x = wave_sum(x);
   if (BT == 64) return x;
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
   __syncthreads();
@@ -39,12 +63,6 @@ __device__ __forceinline__ double block_sum(double x, double* lds) {
   for (int k = 0; k < BT / 64; ++k) r += lds[k];
   __syncthreads();
   return r;
-}
-
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-  return x;
 }
 
 // y = H x for the rows of one chain; returns the partial Re<x, y>.
@@ -364,6 +382,7 @@ struct EigenArgs {
   int lds_ab;        //   lds_ab = lds_work = 1, alpha/beta (2 lstride) and per eigenvalue y
   int lds_work;      //   (lstride), LU work (4 lstride) and pivots (lstride ints); lstride >= every
                      //   chain's steps
+  int kl;            // with the LDS Krylov prefix: vectors kept in LDS (after w, 16-byte aligned)
 };
 
 // Element helpers: the eigensolver runs on complex (double2) vectors, or on
@@ -414,7 +433,7 @@ __device__ __forceinline__ double spmv_t(const int64_t* rowptr, const int32_t* c
 // wave-level dot products, then one update sweep.
 template <int BT, class T>
 __device__ void cgs_pass(const T* V, int m, int64_t n, T* w, double2* coef) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   for (int l = wave; l < m; l += BT / 64) {
     const T* vl = V + (size_t)l * n;
     double cx = 0.0, cy = 0.0;
@@ -459,24 +478,30 @@ constexpr int MAX_NEV = 16;
 // loads are in flight together); results in LDS.
 template <int BT, class T>
 __device__ void dots_lds(const T* V, int m, int64_t n, const T* w, double2* cl, bool self) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   constexpr int NW = BT / 64;
   constexpr int U = 8;
   const int total = m + (self ? 1 : 0);
   for (int l0 = wave * U; l0 < total; l0 += NW * U) {
+    // loads of all eight vectors issue together: indices past m read V_{m-1}
+    // (or w for the self product), sums past total are discarded
     double cx[U], cy[U];
+    size_t off[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) cx[u] = cy[u] = 0.0;
+    for (int u = 0; u < U; ++u) {
+      cx[u] = cy[u] = 0.0;
+      off[u] = (size_t)(l0 + u < m ? l0 + u : (m > 0 ? m - 1 : 0)) * n;
+    }
     for (int64_t i = lane; i < n; i += 64) {
       const T b = w[i];
+      T av[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) av[u] = m > 0 ? V[off[u] + i] : b;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int l = l0 + u;
-        if (l < total) {
-          const T av = l < m ? V[(size_t)l * n + i] : b;
-          cx[u] += re_(av) * re_(b) + im_(av) * im_(b);
-          if constexpr (!is_real_v<T>) cy[u] += re_(av) * im_(b) - im_(av) * re_(b);
-        }
+        if (l0 + u >= m) av[u] = b;  // the self product (or a discarded sum)
+        cx[u] += re_(av[u]) * re_(b) + im_(av[u]) * im_(b);
+        if constexpr (!is_real_v<T>) cy[u] += re_(av[u]) * im_(b) - im_(av[u]) * re_(b);
       }
     }
 #pragma unroll
@@ -646,9 +671,140 @@ __device__ __forceinline__ void ortho_clustered(double* Y, int ys, int m, int ne
   }
 }
 
+// ---- eigensolver step pieces over a split Krylov basis: vectors l < kl at
+// VL + l n (LDS), the others at VG + l n (memory).  Each piece is inlined at
+// call sites whose pointers have one known address space, and sums in the
+// same order whatever the split, so the results do not depend on it.
+
+// w = H vj - alpha vj - bprev vp (vp unused when !has_prev); returns |w|^2, alpha in *al.
+template <int BT, class T>
+__device__ __forceinline__ double three_term(const EigenArgs& a, int64_t base, int64_t n, const T* vj, const T* vp,
+                                             bool has_prev, double bprev, T* w, double* red, double* al_out) {
+  const double al = block_sum<BT>(spmv_t<BT, T>(a.rowptr, a.col, a.val, base, n, vj, w), red);
+  double p0 = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
+    double xr = re_(w[i]) - al * re_(vj[i]);
+    double xi = im_(w[i]) - al * im_(vj[i]);
+    if (has_prev) {
+      xr -= bprev * re_(vp[i]);
+      xi -= bprev * im_(vp[i]);
+    }
+    w[i] = mk<T>(xr, xi);
+    p0 += xr * xr + xi * xi;
+  }
+  *al_out = al;
+  return block_sum<BT>(p0, red);
+}
+
+// cl[l] = <V_l, w> for l in [l0, l0 + 8) ∩ [l0, lend), one wave (l0 and
+// lend wave-uniform).  Every lane loads all eight vectors' entries (indices
+// past lend clamped to lend - 1, their sums discarded), so the loads issue
+// together instead of one bounds-checked branch at a time.
+template <class T>
+__device__ __forceinline__ void dots_wave(const T* V, int l0, int lend, int64_t n, const T* w, double2* cl) {
+  constexpr int U = 8;
+  const int lane = threadIdx.x & 63;
+  double cx[U], cy[U];
+  size_t off[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    cx[u] = cy[u] = 0.0;
+    off[u] = (size_t)(l0 + u < lend ? l0 + u : lend - 1) * n;
+  }
+  for (int64_t i = lane; i < n; i += 64) {
+    const T b = w[i];
+    T av[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) av[u] = V[off[u] + i];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cx[u] += re_(av[u]) * re_(b) + im_(av[u]) * im_(b);
+      if constexpr (!is_real_v<T>) cy[u] += re_(av[u]) * im_(b) - im_(av[u]) * re_(b);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const double x = wave_sum(cx[u]);
+    const double y = is_real_v<T> ? 0.0 : wave_sum(cy[u]);
+    if (lane == 0 && l0 + u < lend) cl[l0 + u] = make_double2(x, y);
+  }
+}
+
+// cl[l] = <V_l, w> for l < m over the split basis; results in LDS.
+template <int BT, class T>
+__device__ __forceinline__ void dots_split(const T* VL, int kl, const T* VG, int m, int64_t n, const T* w,
+                                           double2* cl) {
+  constexpr int NW = BT / 64, U = 8;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ma = m < kl ? m : kl;
+  for (int l0 = wave * U; l0 < ma; l0 += NW * U) dots_wave<T>(VL, l0, ma, n, w, cl);
+  for (int l0 = ma + wave * U; l0 < m; l0 += NW * U) dots_wave<T>(VG, l0, m, n, w, cl);
+  __syncthreads();
+}
+
+// w -= sum_l cl[l] V_l over l < m of the split basis; returns |w|^2.
+template <int BT, class T>
+__device__ __forceinline__ double update_split(const T* VL, int kl, const T* VG, int m, int64_t n, T* w,
+                                               const double2* cl, double* red) {
+  const int ma = m < kl ? m : kl;
+  double p = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
+    double wx = re_(w[i]), wy = im_(w[i]);
+#pragma unroll 16
+    for (int l = 0; l < ma; ++l) {
+      const double2 c = cl[l];
+      const T av = VL[(size_t)l * n + i];
+      if constexpr (is_real_v<T>) {
+        wx -= re_(av) * c.x;
+      } else {
+        wx -= re_(av) * c.x - im_(av) * c.y;
+        wy -= re_(av) * c.y + im_(av) * c.x;
+      }
+    }
+#pragma unroll 16
+    for (int l = ma; l < m; ++l) {
+      const double2 c = cl[l];
+      const T av = VG[(size_t)l * n + i];
+      if constexpr (is_real_v<T>) {
+        wx -= re_(av) * c.x;
+      } else {
+        wx -= re_(av) * c.x - im_(av) * c.y;
+        wy -= re_(av) * c.y + im_(av) * c.x;
+      }
+    }
+    w[i] = mk<T>(wx, wy);
+    p += wx * wx + wy * wy;
+  }
+  return block_sum<BT>(p, red);
+}
+
+// x = sum_l y[l] V_l over l < m of the split basis, for one row i.
+template <class T>
+__device__ __forceinline__ void combine_split(const T* VL, int kl, const T* VG, int m, int64_t n, int64_t i,
+                                              const double* y, double* xr, double* xi) {
+  const int ma = m < kl ? m : kl;
+  double r = 0.0, q = 0.0;
+#pragma unroll 8
+  for (int l = 0; l < ma; ++l) {
+    const T v = VL[(size_t)l * n + i];
+    r += y[l] * re_(v);
+    q += y[l] * im_(v);
+  }
+#pragma unroll 8
+  for (int l = ma; l < m; ++l) {
+    const T v = VG[(size_t)l * n + i];
+    r += y[l] * re_(v);
+    q += y[l] * im_(v);
+  }
+  *xr = r;
+  *xi = q;
+}
+
 // LW: the tridiagonal and the Ritz work arrays live in LDS (a compile-time
 // choice, so every access to them is an LDS instruction), else in memory.
-template <int BT, class T, bool LW>
+// LV (with LW): the residual w and the first a.kl Krylov vectors live in LDS
+// too, so the Gram-Schmidt passes over them wait on LDS, not on memory.
+template <int BT, class T, bool LW, bool LV>
 __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   __shared__ double red[BT / 64];
   __shared__ int s_done;
@@ -656,6 +812,7 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   __shared__ int s_first[MAX_NEV];
   __shared__ int s_flag;
   __shared__ double s_hint[MAX_NEV];  // Ritz values of the previous check
+  __shared__ int s_full;
   int hint_ne = -1;
   extern __shared__ double2 cl[];  // lstride + 2 coefficients, then the optional arrays
   const int c = a.chain_list[blockIdx.x];
@@ -671,8 +828,7 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
     return;
   }
   const int mmax = (int)(ms < n ? ms : n);
-  T* V = reinterpret_cast<T*>(a.vstore) + a.voff[c];
-  T* w = reinterpret_cast<T*>(a.wbuf) + base;
+  T* VG = reinterpret_cast<T*>(a.vstore) + a.voff[c];
   double* alpha = a.ab + (size_t)c * 2 * ms;
   double* beta = alpha + ms;
   // tridiagonal and Ritz work arrays: LDS when they fit (lstride >= mmax)
@@ -699,47 +855,59 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
     twb = a.tw + (size_t)c * nev * 5 * ms;
     pivb = a.tpiv + (size_t)c * nev * ms;
   }
-  // seeded random start vector (real when T is real)
-  double p = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += BT) {
-    const uint64_t h = a.seed * 0x100000001B3ull + (uint64_t)c * 0x9E3779B97F4A7C15ull + 2 * (uint64_t)i;
-    const double xr = u01(h) - 0.5, xi = is_real_v<T> ? 0.0 : u01(h + 1) - 0.5;
-    V[i] = mk<T>(xr, xi);
-    p += xr * xr + xi * xi;
+  // split Krylov basis and residual: LDS prefix of kl vectors when LV
+  T* VL;
+  T* w;
+  int kl;
+  if constexpr (LV) {
+    const size_t off = ((size_t)((char*)(pivb + (size_t)nev * L) - (char*)cl) + 15) & ~(size_t)15;
+    w = reinterpret_cast<T*>(reinterpret_cast<char*>(cl) + off);
+    VL = w + n;
+    kl = a.kl;
+  } else {
+    w = reinterpret_cast<T*>(a.wbuf) + base;
+    VL = VG;
+    kl = 0;
   }
-  const double nrm = sqrt(block_sum<BT>(p, red));
-  for (int64_t i = threadIdx.x; i < n; i += BT) V[i] = mk<T>(re_(V[i]) / nrm, im_(V[i]) / nrm);
+  // seeded random start vector (real when T is real)
+  {
+    T* V0 = kl > 0 ? VL : VG;
+    double p = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += BT) {
+      const uint64_t h = a.seed * 0x100000001B3ull + (uint64_t)c * 0x9E3779B97F4A7C15ull + 2 * (uint64_t)i;
+      const double xr = u01(h) - 0.5, xi = is_real_v<T> ? 0.0 : u01(h + 1) - 0.5;
+      V0[i] = mk<T>(xr, xi);
+      p += xr * xr + xi * xi;
+    }
+    const double nrm = sqrt(block_sum<BT>(p, red));
+    for (int64_t i = threadIdx.x; i < n; i += BT) V0[i] = mk<T>(re_(V0[i]) / nrm, im_(V0[i]) / nrm);
+  }
   __syncthreads();
   int m = 0;
   int converged = 0;
   for (int j = 0; j < mmax; ++j) {
-    const T* vj = V + (size_t)j * n;
     // three-term step: w = H v_j - alpha_j v_j - beta_{j-1} v_{j-1}, with the
     // big cancellation done before any Gram-Schmidt coefficient is formed
     // (coefficients taken from H v_j itself would carry errors of order
     // eps*|H v_j| and lose orthogonality geometrically)
-    double al = block_sum<BT>(spmv_t<BT, T>(a.rowptr, a.col, a.val, base, n, vj, w), red);
     const double bprev = j > 0 ? sb[j - 1] : 0.0;
-    double p0 = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += BT) {
-      double xr = re_(w[i]) - al * re_(vj[i]);
-      double xi = im_(w[i]) - al * im_(vj[i]);
-      if (j > 0) {
-        xr -= bprev * re_(vj[i - n]);
-        xi -= bprev * im_(vj[i - n]);
-      }
-      w[i] = mk<T>(xr, xi);
-      p0 += xr * xr + xi * xi;
-    }
-    p0 = block_sum<BT>(p0, red);
+    double al, p0;
+    if (j + 1 <= kl)
+      p0 = three_term<BT, T>(a, base, n, VL + (size_t)j * n, VL + (size_t)(j > 0 ? j - 1 : 0) * n, j > 0, bprev, w,
+                             red, &al);
+    else if (j == kl && j > 0)
+      p0 = three_term<BT, T>(a, base, n, VG + (size_t)j * n, VL + (size_t)(j - 1) * n, true, bprev, w, red, &al);
+    else
+      p0 = three_term<BT, T>(a, base, n, VG + (size_t)j * n, VG + (size_t)(j > 0 ? j - 1 : 0) * n, j > 0, bprev,
+                             w, red, &al);
     // full reorthogonalization against V_0..V_j; a second pass when the
     // first removed more than half of |w|^2 (Daniel-Gragg-Kaufman-Stewart),
     // always when the criterion is switched off
-    dots_lds<BT, T>(V, j + 1, n, w, cl, false);
-    double p1 = update_norm<BT, T>(V, j + 1, n, w, cl, red);
+    dots_split<BT, T>(VL, kl, VG, j + 1, n, w, cl);
+    double p1 = update_split<BT, T>(VL, kl, VG, j + 1, n, w, cl, red);
     if (!a.dgks || p1 < 0.5 * p0) {
-      dots_lds<BT, T>(V, j + 1, n, w, cl, false);
-      p1 = update_norm<BT, T>(V, j + 1, n, w, cl, red);
+      dots_split<BT, T>(VL, kl, VG, j + 1, n, w, cl);
+      p1 = update_split<BT, T>(VL, kl, VG, j + 1, n, w, cl, red);
     }
     if (threadIdx.x == 0) {
       alpha[j] = al;
@@ -757,34 +925,53 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
       // Ritz pairs of T_m: eigenvalues by workgroup multisection, then
       // thread k refines eigenvector k by inverse iteration
       const int ne = nev < m ? nev : m;
-      ritz_values<BT>(sa, sb, m, ne, ne == hint_ne ? s_hint : nullptr, s_lam, s_lo, s_hi, s_first, &s_flag);
+      const bool have_prev = ne == hint_ne;
+      ritz_values<BT>(sa, sb, m, ne, have_prev ? s_hint : nullptr, s_lam, s_lo, s_hi, s_first, &s_flag);
+      // The eigenvector work is skipped while some Ritz value still moved by
+      // more than tol max(1, |lam|) since the previous check: its residual
+      // cannot be below tol yet (the Ritz value error is of the order of the
+      // squared residual over the gap), so the pair is checked next time.
+      if (threadIdx.x == 0) s_full = (last || have_prev) ? 1 : 0;
+      __syncthreads();
+      if (!last && have_prev && (int)threadIdx.x < ne &&
+          !(fabs(s_lam[threadIdx.x] - s_hint[threadIdx.x]) <= a.tol * fmax(1.0, fabs(s_lam[threadIdx.x]))))
+        atomicAnd(&s_full, 0);
+      __syncthreads();
       if ((int)threadIdx.x < ne) s_hint[threadIdx.x] = s_lam[threadIdx.x];
       hint_ne = ne;
       if (threadIdx.x == 0) s_done = 1;
       __syncthreads();
-      if ((int)threadIdx.x < nev && (int)threadIdx.x < m) {
-        const int k = threadIdx.x;
-        const double lam = s_lam[k];
-        double* y = Y + (size_t)k * ys;
-        double* tw = twb + (size_t)k * (LW ? 4 * ys : 5 * ys);
-        int* piv = pivb + (size_t)k * ys;
-        tri_inverse_iteration(sa, sb, m, lam, y, tw, tw + ys, tw + 2 * ys, tw + 3 * ys, piv,
-                              a.seed + 7919 * (uint64_t)k);
-        a.evals[(size_t)c * nev + k] = lam;
-        const double res = fabs(b * y[m - 1]);
-        if (!(res <= a.tol * fmax(1.0, fabs(lam)))) atomicAnd(&s_done, 0);
-      }
-      __syncthreads();
-      // orthogonalize inverse-iteration vectors of clustered eigenvalues
-      ortho_clustered(Y, ys, m, ne, s_lam);
-      __syncthreads();
-      if (s_done || last) {
-        converged = s_done;
-        break;
+      if (s_full) {
+        if ((int)threadIdx.x < nev && (int)threadIdx.x < m) {
+          const int k = threadIdx.x;
+          const double lam = s_lam[k];
+          double* y = Y + (size_t)k * ys;
+          double* tw = twb + (size_t)k * (LW ? 4 * ys : 5 * ys);
+          int* piv = pivb + (size_t)k * ys;
+          tri_inverse_iteration(sa, sb, m, lam, y, tw, tw + ys, tw + 2 * ys, tw + 3 * ys, piv,
+                                a.seed + 7919 * (uint64_t)k);
+          a.evals[(size_t)c * nev + k] = lam;
+          const double res = fabs(b * y[m - 1]);
+          if (!(res <= a.tol * fmax(1.0, fabs(lam)))) atomicAnd(&s_done, 0);
+        }
+        __syncthreads();
+        // orthogonalize inverse-iteration vectors of clustered eigenvalues
+        ortho_clustered(Y, ys, m, ne, s_lam);
+        __syncthreads();
+        if (s_done || last) {
+          converged = s_done;
+          break;
+        }
       }
     }
     if (last) break;
-    for (int64_t i = threadIdx.x; i < n; i += BT) V[(size_t)(j + 1) * n + i] = mk<T>(re_(w[i]) / b, im_(w[i]) / b);
+    if (j + 1 < kl) {
+      T* vn = VL + (size_t)(j + 1) * n;
+      for (int64_t i = threadIdx.x; i < n; i += BT) vn[i] = mk<T>(re_(w[i]) / b, im_(w[i]) / b);
+    } else {
+      T* vn = VG + (size_t)(j + 1) * n;
+      for (int64_t i = threadIdx.x; i < n; i += BT) vn[i] = mk<T>(re_(w[i]) / b, im_(w[i]) / b);
+    }
     __syncthreads();
   }
   // Ritz vectors x_k = sum_l y_k[l] V_l, normalized, stored complex
@@ -794,13 +981,8 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
     const double* y = Y + (size_t)k * ys;
     double pn = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += BT) {
-      double xr = 0.0, xi = 0.0;
-#pragma unroll 8
-      for (int l = 0; l < m; ++l) {
-        const T v = V[(size_t)l * n + i];
-        xr += y[l] * re_(v);
-        xi += y[l] * im_(v);
-      }
+      double xr, xi;
+      combine_split<T>(VL, kl, VG, m, n, i, y, &xr, &xi);
       X[(size_t)k * n + i] = make_double2(xr, xi);
       pn += xr * xr + xi * xi;
     }
@@ -1206,18 +1388,23 @@ __global__ __launch_bounds__(FT) void flat_dots_kernel(const T* V, int m, int64_
   const int64_t i0 = (int64_t)blockIdx.x * chunk;
   const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
   for (int l0 = 0; l0 < m; l0 += 8) {
+    // all eight loads issue together (indices past m clamped, sums discarded)
     double cx[8], cy[8];
+    size_t off[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) cx[u] = cy[u] = 0.0;
+    for (int u = 0; u < 8; ++u) {
+      cx[u] = cy[u] = 0.0;
+      off[u] = (size_t)(l0 + u < m ? l0 + u : m - 1) * n;
+    }
     for (int64_t i = i0 + threadIdx.x; i < i1; i += FT) {
       const T b = w[i];
+      T av[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) av[u] = V[off[u] + i];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        if (l0 + u < m) {
-          const T a = V[(size_t)(l0 + u) * n + i];
-          cx[u] += re_(a) * re_(b) + im_(a) * im_(b);
-          if constexpr (!is_real_v<T>) cy[u] += re_(a) * im_(b) - im_(a) * re_(b);
-        }
+        cx[u] += re_(av[u]) * re_(b) + im_(av[u]) * im_(b);
+        if constexpr (!is_real_v<T>) cy[u] += re_(av[u]) * im_(b) - im_(av[u]) * re_(b);
       }
     }
     // eight wave sums per wave into LDS, one barrier, eight threads finish
@@ -1848,24 +2035,60 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   // always, alpha/beta and the Ritz work arrays when they fit in 60 KiB (the
   // 64 KiB per-workgroup limit less the kernel's static arrays)
   size_t shm = 0;
-  auto layout = [&](size_t first) {
-    const int L = (int)std::min<int64_t>(max_steps, kb.h_n[order[first]]);
+  bool lv = false;
+  const size_t esz = real ? 8 : 16;
+  // LDS per workgroup: the whole 160 KiB when every chain of the launch gets
+  // its own CU, shared out when several must sit on one CU at once
+  static int n_cu = 0;
+  if (!n_cu) HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, 0));
+  auto layout = [&](size_t first, int bt, size_t n_chains) {
+    const int64_t nmax = kb.h_n[order[first]];
+    const int L = (int)std::min<int64_t>(max_steps, nmax);
     const size_t coef = (size_t)(L + 2) * sizeof(double2), ab = 2 * (size_t)L * 8;
     const size_t work = (size_t)nev * ((size_t)L * 8 + 4 * (size_t)L * 8 + (size_t)L * 4);
-    constexpr size_t budget = 60 * 1024;
     a.lstride = L;
-    a.lds_ab = a.lds_work = coef + ab + work <= budget;
+    a.lds_ab = a.lds_work = coef + ab + work <= 60 * 1024;
     shm = coef + (a.lds_work ? ab + work : 0);
+    // Krylov prefix in LDS (after w): as many vectors as the share allows
+    const int per_cu = (int)std::min<size_t>((n_chains + n_cu - 1) / n_cu, std::max(1, 2048 / bt));
+    const size_t share = (size_t)(160 * 1024) / (size_t)per_cu;
+    const size_t static_lds = 6 * 1024;  // the kernel's own arrays, with room to spare
+    const size_t woff = (shm + 15) & ~(size_t)15;
+    lv = false;
+    a.kl = 0;
+    if (a.lds_work && share > woff + static_lds) {
+      int64_t kl = std::min<int64_t>((int64_t)((share - static_lds - woff) / (esz * nmax)) - 1,
+                                     std::min<int64_t>(max_steps, nmax));
+      const int64_t cap = g_tuning.eigen_lds_vectors;
+      if (cap >= 0) kl = std::min(kl, cap);
+      if (kl >= 8 || (cap >= 0 && kl >= 1)) {
+        lv = true;
+        a.kl = (int)kl;
+        shm = woff + (size_t)(kl + 1) * nmax * esz;
+      }
+    }
   };
-  // eigen_kernel<bt, T, LW> for the layout just chosen
+  // eigen_kernel<bt, T, LW, LV> for the layout just chosen
   auto launch = [&](int bt, dim3 g) {
-#define IMPED_EIG(BT_)                                                                                     \
-  if (real) {                                                                                              \
-    if (a.lds_work) hipLaunchKernelGGL((eigen_kernel<BT_, double, true>), g, dim3(BT_), shm, s, a);       \
-    else hipLaunchKernelGGL((eigen_kernel<BT_, double, false>), g, dim3(BT_), shm, s, a);                 \
+#define IMPED_EIG_T(BT_, T_)                                                                               \
+  if (lv) {                                                                                                \
+    static bool attr = false;                                                                              \
+    if (!attr) {                                                                                           \
+      HIP_CHECK(hipFuncSetAttribute((const void*)eigen_kernel<BT_, T_, true, true>,                        \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 6 * 1024));   \
+      attr = true;                                                                                         \
+    }                                                                                                      \
+    hipLaunchKernelGGL((eigen_kernel<BT_, T_, true, true>), g, dim3(BT_), shm, s, a);                     \
+  } else if (a.lds_work) {                                                                                 \
+    hipLaunchKernelGGL((eigen_kernel<BT_, T_, true, false>), g, dim3(BT_), shm, s, a);                    \
   } else {                                                                                                 \
-    if (a.lds_work) hipLaunchKernelGGL((eigen_kernel<BT_, double2, true>), g, dim3(BT_), shm, s, a);      \
-    else hipLaunchKernelGGL((eigen_kernel<BT_, double2, false>), g, dim3(BT_), shm, s, a);                \
+    hipLaunchKernelGGL((eigen_kernel<BT_, T_, false, false>), g, dim3(BT_), shm, s, a);                   \
+  }
+#define IMPED_EIG(BT_)      \
+  if (real) {               \
+    IMPED_EIG_T(BT_, double)  \
+  } else {                  \
+    IMPED_EIG_T(BT_, double2) \
   }
     if (bt == 64) {
       IMPED_EIG(64)
@@ -1877,18 +2100,19 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
       IMPED_EIG(1024)
     }
 #undef IMPED_EIG
+#undef IMPED_EIG_T
     HIP_CHECK(hipGetLastError());
   };
   if (n_wide < order.size()) {
     TimedLaunch tl("eigen", s);
-    layout(n_wide);
+    layout(n_wide, 64, order.size() - n_wide);
     a.chain_list = sc.clist.as<int32_t>() + n_wide;
     launch(64, dim3((unsigned)(order.size() - n_wide)));
   }
   a.chain_list = sc.clist.as<int32_t>();
   if (n_wide > 0) {
     TimedLaunch tl("eigen", s);
-    layout(0);
+    layout(0, (int)g_tuning.eigen_bt, n_wide);
     launch((int)g_tuning.eigen_bt, dim3((unsigned)n_wide));
   }
   std::vector<int32_t> conv(C, 1);

@@ -62,7 +62,9 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * "eigen_check_every", "flat_min_n" (chains above it use the grid-wide
  * kernels), "spmv_lpr" (lanes per row of the grid-wide SpMV: 4..64),
  * "lanczos_graph_steps" (grid-wide Lanczos steps per replayed HIP graph,
- * 0..256; 0 launches every step directly, as does switched-on timing). */
+ * 0..256; 0 launches every step directly, as does switched-on timing),
+ * "eigen_lds_vectors" (cap on the Krylov vectors the eigensolver keeps in
+ * LDS; -1 as many as fit, 0 none). */
 int imped_set_tuning(const char* name, int64_t value);
 
 /* The engine's device radix sort (the one that orders BFS layers and

@@ -228,6 +228,45 @@ def test_eigensolver_variants_vs_dense(gpu, hy):
         engine.set_tuning("eigen_dgks", 1)
 
 
+@pytest.mark.parametrize("hy", [0.0, 2e-3])
+def test_eigensolver_lds_krylov_prefix_split(gpu, hy):
+    """The Krylov basis split between LDS (first k vectors) and memory gives
+    the same eigenpairs whatever k: none, 1, 5, 20, as many as fit; against
+    dense diagonalization too."""
+    from impuritymodel_amd import engine
+    from impuritymodel_amd.batched import ground_bases
+    from impuritymodel_amd.models import make_model
+
+    models = [make_model(n_bath=nb, seed=30 + nb, hField=(0.0, hy, 1e-4)) for nb in (10, 20)]
+    ops = engine.Ops([m.hOp for m in models], [m.restrictions for m in models], max(m.n_spin_orbitals for m in models))
+    b = engine.Batch(ops, np.arange(2, dtype=np.int32))
+    b.set_seeds(ground_bases(models, ops.W))
+    b.build(fixed_basis=True)
+    rowptr, col, val = b.csr()
+    ref = []
+    for c in range(2):
+        r0, r1 = b.row_base[c], b.row_base[c + 1]
+        dense = np.zeros((r1 - r0, r1 - r0), complex)
+        for r in range(r1 - r0):
+            for e in range(rowptr[r0 + r], rowptr[r0 + r + 1]):
+                dense[r, col[e]] = val[e]
+        ref.append(np.linalg.eigvalsh(dense)[:3])
+    try:
+        engine.set_tuning("eigen_lds_vectors", 0)
+        ev0, vec0, st0, ok = b.eigen(3, tol=1e-12)
+        assert ok
+        for c in range(2):
+            np.testing.assert_allclose(ev0[c], ref[c], rtol=0, atol=1e-10)
+        for cap in (1, 5, 20, -1):
+            engine.set_tuning("eigen_lds_vectors", cap)
+            ev, vec, st, ok = b.eigen(3, tol=1e-12)
+            assert ok and np.array_equal(st, st0)
+            np.testing.assert_allclose(ev, ev0, rtol=0, atol=1e-13)
+            np.testing.assert_allclose(vec, vec0, rtol=0, atol=1e-11)
+    finally:
+        engine.set_tuning("eigen_lds_vectors", -1)
+
+
 def test_flat_kernels_match_workgroup_kernels(gpu):
     """The grid-wide (flat) Lanczos and eigensolver used for large sectors give
     the workgroup kernels' results; every SpMV lanes-per-row shape."""
