@@ -38,6 +38,16 @@ struct HipError {
     }                                                                                    \
   } while (0)
 
+// Device memory in size classes (powers of two up to 64 MiB, then 2 MiB
+// steps) with a cache of idle blocks: a block given back is handed out again
+// for the same class instead of returning to the runtime, so the handles a
+// batched call creates and destroys cost no hipMalloc/hipFree after the
+// first call.  All engine work runs in order on one stream, so a block's next
+// owner is ordered after its previous owner's queued work.
+size_t devmem_class(size_t bytes);
+void* devmem_alloc(size_t class_bytes);
+void devmem_free(void* p, size_t class_bytes);
+
 // Device buffer owned by a handle; grows on demand, never shrinks.
 struct DevBuf {
   void* p = nullptr;
@@ -46,27 +56,24 @@ struct DevBuf {
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) devmem_free(p, bytes);
   }
   void reserve(size_t nb) {
     if (nb <= bytes) return;
-    if (p) HIP_CHECK(hipFree(p));
+    if (p) devmem_free(p, bytes);
     p = nullptr;
-    size_t alloc = nb < 256 ? 256 : nb;
-    HIP_CHECK(hipMalloc(&p, alloc));
+    bytes = 0;
+    const size_t alloc = devmem_class(nb);
+    p = devmem_alloc(alloc);
     bytes = alloc;
   }
   // Grow while keeping the first `keep` bytes.
   void grow_keep(size_t nb, size_t keep, hipStream_t s) {
     if (nb <= bytes) return;
-    size_t alloc = bytes * 2 > nb ? bytes * 2 : nb;
-    void* q = nullptr;
-    HIP_CHECK(hipMalloc(&q, alloc));
+    const size_t alloc = devmem_class(bytes * 2 > nb ? bytes * 2 : nb);
+    void* q = devmem_alloc(alloc);
     if (p && keep) HIP_CHECK(hipMemcpyAsync(q, p, keep, hipMemcpyDeviceToDevice, s));
-    if (p) {
-      HIP_CHECK(hipStreamSynchronize(s));
-      HIP_CHECK(hipFree(p));
-    }
+    if (p) devmem_free(p, bytes);  // reused only by work queued after the copy
     p = q;
     bytes = alloc;
   }

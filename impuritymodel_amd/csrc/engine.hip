@@ -20,6 +20,8 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <unordered_map>
 
 #include "../../include/imped.h"
 #include "kernels.h"
@@ -32,6 +34,74 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 Tuning g_tuning;
+
+// ------------------------------------------------------------------ device memory cache
+
+namespace {
+struct MemCache {
+  std::mutex mu;
+  std::unordered_map<size_t, std::vector<void*>> idle;
+  size_t idle_bytes = 0;
+};
+// never destroyed: handles released during process exit still find it
+MemCache& mem_cache() {
+  static MemCache* c = new MemCache;
+  return *c;
+}
+constexpr size_t kIdleCap = size_t(16) << 30;  // idle blocks kept at most
+}  // namespace
+
+size_t devmem_class(size_t n) {
+  if (n <= 256) return 256;
+  if (n <= (size_t(64) << 20)) {
+    size_t c = 256;
+    while (c < n) c <<= 1;
+    return c;
+  }
+  const size_t g = size_t(2) << 20;
+  return (n + g - 1) / g * g;
+}
+
+void* devmem_alloc(size_t bytes) {
+  MemCache& mc = mem_cache();
+  {
+    std::lock_guard<std::mutex> lk(mc.mu);
+    auto it = mc.idle.find(bytes);
+    if (it != mc.idle.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      mc.idle_bytes -= bytes;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e == hipErrorOutOfMemory) {
+    // give the idle blocks back to the runtime and try once more
+    (void)hipGetLastError();
+    std::lock_guard<std::mutex> lk(mc.mu);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : mc.idle)
+      for (void* q : kv.second) (void)hipFree(q);
+    mc.idle.clear();
+    mc.idle_bytes = 0;
+    e = hipMalloc(&p, bytes);
+  }
+  HIP_CHECK(e);
+  return p;
+}
+
+void devmem_free(void* p, size_t bytes) {
+  if (!p) return;
+  MemCache& mc = mem_cache();
+  std::lock_guard<std::mutex> lk(mc.mu);
+  if (mc.idle_bytes + bytes > kIdleCap) {
+    (void)hipFree(p);
+    return;
+  }
+  mc.idle[bytes].push_back(p);
+  mc.idle_bytes += bytes;
+}
 
 // ------------------------------------------------------------------ launch timing
 
