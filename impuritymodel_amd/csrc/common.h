@@ -89,9 +89,18 @@ inline void h2d(DevBuf& b, const T* src, size_t n, hipStream_t s) {
   if (n) HIP_CHECK(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
+// Device-to-host reads.  Small ones (<= 64 KiB) go through a pinned ring:
+// the copy is queued like a kernel and dst is filled by the next
+// stream_sync on the same stream (a pageable destination would make every
+// small read a blocking staged copy of its own).  Larger reads go straight
+// to dst; they too are complete only after stream_sync.
+void d2h_bytes(void* dst, const void* src, size_t bytes, hipStream_t s);
+// hipStreamSynchronize, then completes the staged reads of stream s.
+void stream_sync(hipStream_t s);
+
 template <class T>
 inline void d2h(T* dst, const void* src, size_t n, hipStream_t s) {
-  if (n) HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  if (n) d2h_bytes(dst, src, n * sizeof(T), s);
 }
 
 // Launch shapes chosen per chain-dimension bucket (set by the autotuner).

@@ -35,6 +35,70 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 Tuning g_tuning;
 
+// ------------------------------------------------------------------ staged small reads
+
+namespace {
+struct PendingRead {
+  void* dst;
+  const char* slot;
+  size_t bytes;
+  hipStream_t s;
+};
+struct Staging {
+  std::mutex mu;
+  char* ring = nullptr;
+  size_t cap = 0, pos = 0;
+  std::vector<PendingRead> pending;
+};
+Staging& staging() {
+  static Staging* st = new Staging;
+  return *st;
+}
+constexpr size_t kStageMax = 64 * 1024;   // reads up to this size are staged
+constexpr size_t kRingBytes = 4 << 20;    // pinned ring
+}  // namespace
+
+void d2h_bytes(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (!bytes) return;
+  if (bytes > kStageMax) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    return;
+  }
+  Staging& st = staging();
+  {
+    std::lock_guard<std::mutex> lk(st.mu);
+    if (!st.ring) {
+      HIP_CHECK(hipHostMalloc((void**)&st.ring, kRingBytes, hipHostMallocDefault));
+      st.cap = kRingBytes;
+    }
+  }
+  const size_t need = (bytes + 63) & ~(size_t)63;
+  if (st.pos + need > st.cap) stream_sync(s);
+  std::lock_guard<std::mutex> lk(st.mu);
+  if (st.pos + need > st.cap) {
+    // reads of other streams still hold the ring: read this one directly
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    return;
+  }
+  char* slot = st.ring + st.pos;
+  st.pos += need;
+  HIP_CHECK(hipMemcpyAsync(slot, src, bytes, hipMemcpyDeviceToHost, s));
+  st.pending.push_back(PendingRead{dst, slot, bytes, s});
+}
+
+void stream_sync(hipStream_t s) {
+  HIP_CHECK(hipStreamSynchronize(s));
+  Staging& st = staging();
+  std::lock_guard<std::mutex> lk(st.mu);
+  size_t keep = 0;
+  for (const PendingRead& r : st.pending) {
+    if (r.s == s) std::memcpy(r.dst, r.slot, r.bytes);
+    else st.pending[keep++] = r;
+  }
+  st.pending.resize(keep);
+  if (st.pending.empty()) st.pos = 0;
+}
+
 // ------------------------------------------------------------------ device memory cache
 
 namespace {
@@ -441,8 +505,8 @@ static int64_t apply_layer(Batch& b, int64_t f0, int64_t f1, int mode, double we
       launch_discover(b.W, a.out_keys, a.out_chain, E, a.hash, a.cand, stream());
     }
     int32_t cc[2] = {0, 0};
-    HIP_CHECK(hipMemcpyAsync(cc, sc.cand_cnt.p, sizeof(cc), hipMemcpyDeviceToHost, stream()));
-    HIP_CHECK(hipStreamSynchronize(stream()));
+    d2h_bytes(cc, sc.cand_cnt.p, sizeof(cc), stream());
+    stream_sync(stream());
     b.E = base + E;
     *n_alive = cc[1];
   } else {
@@ -450,13 +514,13 @@ static int64_t apply_layer(Batch& b, int64_t f0, int64_t f1, int mode, double we
     b.rowptr.reserve((n + 1) * sizeof(int64_t));
     HIP_CHECK(hipMemcpyAsync(b.rowptr.p, sc.offsets.p, n * sizeof(int64_t), hipMemcpyDeviceToDevice, stream()));
     HIP_CHECK(hipMemcpyAsync(b.rowptr.as<int64_t>() + n, &E, sizeof(int64_t), hipMemcpyHostToDevice, stream()));
-    HIP_CHECK(hipStreamSynchronize(stream()));
+    stream_sync(stream());
     b.col.reserve(std::max<int64_t>(E, 1) * 4);
     b.val.reserve(std::max<int64_t>(E, 1) * 16);
     a.out_col = b.col.as<int32_t>();
     a.out_val = b.val.as<double2>();
     if (E > 0) launch_apply(b.W, mode, true, a, stream());
-    HIP_CHECK(hipStreamSynchronize(stream()));
+    stream_sync(stream());
     *n_alive = 0;
   }
   return E;
@@ -464,8 +528,8 @@ static int64_t apply_layer(Batch& b, int64_t f0, int64_t f1, int mode, double we
 
 static void finalize_sizes(Batch& b) {
   b.h_n.assign(b.C, 0);
-  HIP_CHECK(hipMemcpyAsync(b.h_n.data(), b.chain_size.p, b.C * sizeof(int64_t), hipMemcpyDeviceToHost, stream()));
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  d2h_bytes(b.h_n.data(), b.chain_size.p, b.C * sizeof(int64_t), stream());
+  stream_sync(stream());
   b.h_row_base.assign(b.C + 1, 0);
   for (int c = 0; c < b.C; ++c) b.h_row_base[c + 1] = b.h_row_base[c] + b.h_n[c];
   b.R = b.h_row_base[b.C];
@@ -487,7 +551,7 @@ static void compute_chain_nnz(Batch& b) {
                      b.rowptr.as<int64_t>(), b.row_base.as<int64_t>(), b.C, tmp.as<int64_t>());
   HIP_CHECK(hipGetLastError());
   d2h(rp.data(), tmp.p, b.C + 1, stream());
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   for (int c = 0; c < b.C; ++c) b.h_nnz[c] = rp[c + 1] - rp[c];
   b.NNZ = rp[b.C];
 }
@@ -591,7 +655,7 @@ static void build(Batch& b, int mode, double weight_min, int max_layers) {
   b.rowptr.reserve((b.R + 1) * sizeof(int64_t));
   int64_t nnz = scan_counts(sc.counts.as<int32_t>(), b.R, b.rowptr.as<int64_t>(), s);
   HIP_CHECK(hipMemcpyAsync(b.rowptr.as<int64_t>() + b.R, &nnz, sizeof(int64_t), hipMemcpyHostToDevice, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   b.col.reserve(std::max<int64_t>(nnz, 1) * 4);
   b.val.reserve(std::max<int64_t>(nnz, 1) * 16);
   sc.err.reserve(4);
@@ -617,8 +681,8 @@ static void build(Batch& b, int mode, double weight_min, int max_layers) {
                      b.R, b.pool_keys.as<uint64_t>(), tmp.as<uint64_t>());
   HIP_CHECK(hipMemcpyAsync(b.pool_keys.p, tmp.p, b.R * b.W * 8, hipMemcpyDeviceToDevice, s));
   int32_t err = 0;
-  HIP_CHECK(hipMemcpyAsync(&err, sc.err.p, 4, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  d2h_bytes(&err, sc.err.p, 4, s);
+  stream_sync(s);
   if (err) throw HipError{"internal error: CSR target missing from basis"};
   compute_chain_nnz(b);
   b.built = true;
@@ -656,7 +720,7 @@ int imped_set_stream(void* stream) {
 
 int imped_device_synchronize(void) {
   API_BEGIN
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   API_END
 }
 
@@ -669,14 +733,14 @@ int imped_timing_enable(int on) {
 
 int imped_timing_reset(void) {
   API_BEGIN
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   timing_clear();
   API_END
 }
 
 int imped_timing_query(const char* name, double* total_ms, int64_t* launches) {
   API_BEGIN
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   double t = 0.0;
   int64_t k = 0;
   for (auto& kv : g_timings) {
@@ -791,7 +855,7 @@ int imped_sort_u64(const uint64_t* keys, int64_t n, int begin_bit, int end_bit, 
   h2d(dp, perm, std::max<int64_t>(n, 1), stream());
   sort_u64_perm(dk.as<uint64_t>(), dp.as<int32_t>(), n, begin_bit, end_bit, stream());
   d2h(perm, dp.p, n, stream());
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   API_END
 }
 
@@ -873,7 +937,7 @@ int imped_ops_create(int W, int n_orb, int n_models, const int32_t* model_group_
   h2d(o->model_restr_off, model_restr_off, n_models + 1, s);
   h2d(o->restr_mask, restr_mask, (size_t)R * W, s);
   h2d(o->restr_lohi, restr_lohi, (size_t)R * 2, s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   *out = reinterpret_cast<imped_ops>(o.release());
   API_END
 }
@@ -891,7 +955,7 @@ int imped_ops_set_coef(imped_ops ops, const double* term_coef, int64_t n_terms) 
   hipStream_t s = stream();
   // stream order: launches already queued finish with the old values
   if (n_terms) HIP_CHECK(hipMemcpyAsync(o->term_coef.p, term_coef, n_terms * 16, hipMemcpyHostToDevice, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   o->real = real;
   API_END
 }
@@ -938,7 +1002,7 @@ int imped_ops_set_affine(imped_ops ops, int n_params, int64_t terms_per_model, c
                        o->term_coef.as<double2>());
     HIP_CHECK(hipGetLastError());
   }
-  HIP_CHECK(hipStreamSynchronize(s));  // the host arrays may be released on return
+  stream_sync(s);  // the host arrays may be released on return
   // models past n_models keep their values: real only if they were real too
   o->real = real && (n_models == o->M || o->real);
   API_END
@@ -1065,7 +1129,7 @@ int imped_batch_set_csr(imped_batch batch, const int64_t* chain_n, const int64_t
   h2d(b->chain_n, b->h_n.data(), std::max(b->C, 1), s);
   b->h_nnz.assign(b->C, 0);
   for (int c = 0; c < b->C; ++c) b->h_nnz[c] = rowptr[b->h_row_base[c + 1]] - rowptr[b->h_row_base[c]];
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   b->real = true;
   for (int64_t e = 0; e < b->NNZ; ++e)
     if (val[2 * e + 1] != 0.0) b->real = false;
@@ -1098,7 +1162,7 @@ int imped_batch_get_basis(imped_batch batch, uint64_t* keys) {
     return IMPED_ERR_INVALID;
   }
   d2h(keys, b->pool_keys.p, (size_t)b->R * b->W, stream());
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   API_END
 }
 
@@ -1108,7 +1172,7 @@ int imped_batch_get_csr(imped_batch batch, int64_t* rowptr, int32_t* col, double
   if (rowptr) d2h(rowptr, b->rowptr.p, b->R + 1, stream());
   if (col) d2h(col, b->col.p, b->NNZ, stream());
   if (val) d2h(val, b->val.p, b->NNZ * 2, stream());
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   API_END
 }
 
@@ -1159,7 +1223,7 @@ int imped_batch_lanczos(imped_batch batch, const double* v0_host, int K, int reo
   if (beta) d2h(beta, b->beta.p, (size_t)b->C * K, stream());
   if (kout) d2h(kout, b->kout.p, b->C, stream());
   if (status) d2h(status, b->status.p, b->C, stream());
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   API_END
 }
 
@@ -1190,7 +1254,7 @@ int imped_batch_eigen(imped_batch batch, int nev, double tol, int max_steps, uin
   if (evals) d2h(evals, b->evals.p, (size_t)b->C * nev, stream());
   if (evecs_host) d2h(evecs_host, b->evecs.p, (size_t)b->R * nev * 2, stream());
   if (steps_out) d2h(steps_out, steps.p, b->C, stream());
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   if (rc != IMPED_OK) {
     set_error("imped_batch_eigen: not converged within max_steps");
     return rc;
@@ -1234,7 +1298,7 @@ int imped_batch_solve(imped_batch batch, int n_shift, const double* shifts, doub
   b->nev = n_shift;
   if (x_out) {
     d2h(x_out, b->evecs.p, (size_t)b->R * n_shift * 2, stream());
-    HIP_CHECK(hipStreamSynchronize(stream()));
+    stream_sync(stream());
   }
   if (rc != IMPED_OK) {
     set_error("imped_batch_solve: residual above tolerance after max_steps");
@@ -1273,7 +1337,7 @@ int imped_vec_create(int W, int n_vec, const int64_t* off, const uint64_t* keys,
   const int64_t n = v->total();
   h2d(v->keys, keys, std::max<int64_t>(n, 1) * W, stream());
   h2d(v->amps, amps, std::max<int64_t>(n, 1) * 2, stream());
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   *out = reinterpret_cast<imped_vec>(v.release());
   API_END
 }
@@ -1296,7 +1360,7 @@ int imped_vec_get(imped_vec vec, uint64_t* keys, double* amps, double* norm2) {
   if (keys) d2h(keys, v->keys.p, (size_t)n * v->W, stream());
   if (amps) d2h(amps, v->amps.p, (size_t)n * 2, stream());
   if (norm2 && v->norm2.p) d2h(norm2, v->norm2.p, v->V, stream());
-  HIP_CHECK(hipStreamSynchronize(stream()));
+  stream_sync(stream());
   API_END
 }
 
@@ -1334,7 +1398,7 @@ int imped_vec_prune(imped_vec src, double weight_min, imped_vec* out) {
   if (v->norm2.p && v->V) {
     r->norm2.reserve((size_t)v->V * 8);
     HIP_CHECK(hipMemcpyAsync(r->norm2.p, v->norm2.p, (size_t)v->V * 8, hipMemcpyDeviceToDevice, stream()));
-    HIP_CHECK(hipStreamSynchronize(stream()));
+    stream_sync(stream());
   }
   *out = reinterpret_cast<imped_vec>(r.release());
   API_END

@@ -1749,7 +1749,7 @@ hipGraphExec_t lanczos_step_graph(StepArgs* a, unsigned nbv, unsigned nbs, int l
   (void)hipGraphDestroy(graph);
   HIP_CHECK(ie);
   if (cache.size() >= 32) {
-    HIP_CHECK(hipStreamSynchronize(s));
+    stream_sync(s);
     HIP_CHECK(hipGraphExecDestroy(cache.front().exec));
     cache.erase(cache.begin());
   }
@@ -1851,14 +1851,14 @@ int eigen_flat_chain(const KrylovBatch& kb, int c, int nev, double tol, int max_
     // one host read per step (|w|^2 before and after the first pass, alpha_j)
     // decides the DGKS second pass; beta_j is written on the device
     double al = 0.0;
-    HIP_CHECK(hipMemcpyAsync(hb, scal, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(&al, alpha + j, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    d2h_bytes(hb, scal, 3 * sizeof(double), s);
+    d2h_bytes(&al, alpha + j, sizeof(double), s);
+    stream_sync(s);
     double p1 = hb[2];
     if (!g_tuning.eigen_dgks || p1 < 0.5 * hb[1]) {
       gs_pass(j + 1);
-      HIP_CHECK(hipMemcpyAsync(&p1, scal + 2, sizeof(double), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
+      d2h_bytes(&p1, scal + 2, sizeof(double), s);
+      stream_sync(s);
     }
     const double b = std::sqrt(p1);
     const double bprev = j > 0 ? hb[3] : 0.0;
@@ -1873,8 +1873,8 @@ int eigen_flat_chain(const KrylovBatch& kb, int c, int nev, double tol, int max_
                          seed);
       HIP_CHECK(hipGetLastError());
       int32_t conv = 0;
-      HIP_CHECK(hipMemcpyAsync(&conv, sc.conv.p, 4, hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
+      d2h_bytes(&conv, sc.conv.p, 4, s);
+      stream_sync(s);
       if (conv || last) {
         converged = conv;
         break;
@@ -1897,7 +1897,7 @@ int eigen_flat_chain(const KrylovBatch& kb, int c, int nev, double tol, int max_
     HIP_CHECK(hipMemcpyAsync(evals + ne, nanv.data(), nanv.size() * sizeof(double), hipMemcpyHostToDevice, s));
   }
   HIP_CHECK(hipMemcpyAsync(steps_out, &m, 4, hipMemcpyHostToDevice, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   return converged ? IMPED_OK : IMPED_ERR_NOCONV;
 }
 
@@ -1964,7 +1964,7 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
     HIP_CHECK(hipGetLastError());
   }
   // the chain list buffer must outlive the launches
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_t seed, double* evals,
@@ -2117,7 +2117,7 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   }
   std::vector<int32_t> conv(C, 1);
   d2h(conv.data(), sc.conv.p, C, s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   int rc = IMPED_OK;
   for (int c : order)
     if (!conv[c]) rc = IMPED_ERR_NOCONV;
@@ -2212,7 +2212,7 @@ int run_solve(const KrylovBatch& kb, const double2* y, int S, const double* z_ho
   d2h(conv.data(), sc.conv.p, C, s);
   if (steps_host) d2h(steps_host, sc.ssteps.p, C, s);
   if (resid_host) d2h(resid_host, sc.sres.p, (size_t)C * S, s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   for (int c = 0; c < C; ++c)
     if (!conv[c]) return IMPED_ERR_NOCONV;
   return IMPED_OK;
@@ -2255,7 +2255,7 @@ void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t
   }
   if (g_out) d2h(g_out, sc.gout.p, (size_t)C * n_w * 2, s);
   d2h(spec_out, sc.sout.p, (size_t)n_spec * n_w, s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 }  // namespace imped

@@ -390,7 +390,7 @@ void upload_batch_input(Mlp& net, const float* x_host, int B, hipStream_t s) {
   hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, tmp.as<float>(), n,
                      net.xb.as<uint16_t>());
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 }  // namespace
@@ -435,7 +435,7 @@ int mlp_layers(const Mlp& net, int32_t* dims) {
 
 void mlp_get_params(Mlp& net, float* params, hipStream_t s) {
   d2h(params, net.w.p, (size_t)net.E * net.P, s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 void mlp_set_params(Mlp& net, const float* params, hipStream_t s) {
@@ -446,7 +446,7 @@ void mlp_set_params(Mlp& net, const float* params, hipStream_t s) {
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemsetAsync(net.m.p, 0, tot * 4, s));
   HIP_CHECK(hipMemsetAsync(net.v.p, 0, tot * 4, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   net.step = 0;
 }
 
@@ -455,7 +455,7 @@ void mlp_forward(Mlp& net, const float* x_host, int B, float* y_host, hipStream_
   upload_batch_input(net, x_host, B, s);
   forward(net, B, s);
   d2h(y_host, net.out.p, (size_t)net.E * B * net.dims[net.L], s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, double scale, double* mean_host,
@@ -472,7 +472,7 @@ void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, dou
   HIP_CHECK(hipGetLastError());
   d2h(mean_host, net.smean.p, (size_t)B * D, s);
   d2h(dis_host, net.sdis.p, B, s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 void mlp_gradients(Mlp& net, const float* x_host, const float* y_host, int B, float* g_host, double* loss,
@@ -485,13 +485,13 @@ void mlp_gradients(Mlp& net, const float* x_host, const float* y_host, int B, fl
   backward(net, B, s);
   if (g_host) d2h(g_host, net.g.p, (size_t)net.E * net.P, s);
   if (loss) d2h(loss, net.loss.p, net.E, s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 void mlp_set_data(Mlp& net, const float* X, const float* Y, int64_t n, hipStream_t s) {
   h2d(net.dX, X, (size_t)n * net.dims[0], s);
   h2d(net.dY, Y, (size_t)n * net.dims[net.L], s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   net.n_data = n;
 }
 
@@ -520,7 +520,7 @@ void mlp_train(Mlp& net, const int32_t* batches, int steps, int B, float lr, flo
     HIP_CHECK(hipGetLastError());
   }
   if (losses) d2h(losses, lbuf.p, (size_t)steps * net.E, s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 }  // namespace imped

@@ -204,9 +204,9 @@ int64_t scan_counts(const int32_t* counts, int64_t n, int64_t* offsets, hipStrea
   scan_counts_async(counts, n, offsets, s);
   int64_t last_off = 0;
   int32_t last_cnt = 0;
-  HIP_CHECK(hipMemcpyAsync(&last_off, offsets + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipMemcpyAsync(&last_cnt, counts + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  d2h_bytes(&last_off, offsets + n - 1, sizeof(int64_t), s);
+  d2h_bytes(&last_cnt, counts + n - 1, sizeof(int32_t), s);
+  stream_sync(s);
   return last_off + last_cnt;
 }
 

@@ -193,7 +193,7 @@ void segment_positions(const std::vector<int64_t>& seg, int64_t n, int64_t kept,
   HIP_CHECK(hipGetLastError());
   pos_at.resize(m);
   d2h(pos_at.data(), sc.cnt.p, m, s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 }  // namespace
@@ -221,7 +221,7 @@ void vector_apply(OpsView ops, int W, const std::vector<int64_t>& src_off, const
     out_keys.reserve(8);
     out_amps.reserve(16);
     if (n_out) HIP_CHECK(hipMemsetAsync(out_norm2.p, 0, n_out * 8, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    stream_sync(s);
     return;
   }
   h2d(sc.so_off, so_off.data(), n_out + 1, s);
@@ -251,7 +251,7 @@ void vector_apply(OpsView ops, int W, const std::vector<int64_t>& src_off, const
     out_keys.reserve(8);
     out_amps.reserve(16);
     HIP_CHECK(hipMemsetAsync(out_norm2.p, 0, std::max(n_out, 1) * 8, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    stream_sync(s);
     return;
   }
   if (E >= INT32_MAX) throw HipError{"vector_apply: too many entries"};
@@ -292,7 +292,7 @@ void vector_apply(OpsView ops, int W, const std::vector<int64_t>& src_off, const
     out_keys.reserve(8);
     out_amps.reserve(16);
     HIP_CHECK(hipMemsetAsync(out_norm2.p, 0, std::max(n_out, 1) * 8, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    stream_sync(s);
     return;
   }
   sc.ck.reserve(NU * W * 8);
@@ -320,13 +320,13 @@ void vector_apply(OpsView ops, int W, const std::vector<int64_t>& src_off, const
                      sc.cnt.as<int32_t>());
   std::vector<int32_t> cnt(n_out);
   d2h(cnt.data(), sc.cnt.p, n_out, s);
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   for (int o = 0; o < n_out; ++o) out_off[o + 1] = out_off[o] + cnt[o];
   h2d(sc.off, out_off.data(), n_out + 1, s);
   hipLaunchKernelGGL(norm_kernel, dim3(blocks_for(n_out, 64)), dim3(64), 0, s, sc.off.as<int64_t>(), n_out,
                      out_amps.as<double2>(), out_norm2.as<double>(), normalize ? 1 : 0);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 void truncate_eigvecs(int W, const uint64_t* basis_keys, const double2* evecs, int nev,
