@@ -11,7 +11,7 @@ its proposal (3 lowest eigenstates, energy cut, 3 dipole polarizations,
 the ensemble agrees on are answered by the surrogate network at once; the
 others wait and are evaluated together by one batched exact-ED call once
 half the walkers wait (and at the end of the timed steps), so the timed
-region contains every proposal's spectrum.  Set-up (untimed, reported): a 100k-spectrum
+region contains every proposal's spectrum.  Set-up (untimed, reported): a 300k-spectrum
 training set from the batched exact ED, 5-member ensemble training with the
 library's BF16 matrix-core kernels, trust-threshold calibration on 2048
 validation spectra.  After the timed steps every surrogate-served spectrum
@@ -117,15 +117,27 @@ def cpu_baseline(fn, items, workers, what):
                        % (len(sample), what, dt))
 
 
-def _roofline(kernels, bytes_by_kernel):
-    """Roofline object of the kernel with the largest timed total."""
-    cand = {k: v for k, v in kernels.items() if k in bytes_by_kernel and v[1] > 0}
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # dense bf16 matrix-core peak of one MI355X
+
+
+def _roofline(kernels, bytes_by_kernel, flops_by_kernel=None):
+    """Roofline object of the kernel with the largest timed total: HBM-bound
+    kernels priced by algorithmic bytes against 8 TB/s, matrix-core kernels
+    by algorithmic flops against the dense bf16 peak."""
+    flops_by_kernel = flops_by_kernel or {}
+    cand = {k: v for k, v in kernels.items() if (k in bytes_by_kernel or k in flops_by_kernel) and v[1] > 0}
     if not cand:
         return None
     dom = max(cand, key=lambda k: cand[k][0])
     ms_total, launches = cand[dom]
-    per_launch = bytes_by_kernel[dom] / launches
     avg_ms = ms_total / launches
+    if dom in flops_by_kernel:
+        per_launch = flops_by_kernel[dom] / launches
+        achieved = per_launch / (avg_ms * 1e-3) / 1e12
+        return dict(bound="mfma", kernel=dom + "_kernel", achieved=round(achieved, 3), peak=BF16_MFMA_PEAK_TFLOPS,
+                    unit="TFLOP/s", frac=round(achieved / BF16_MFMA_PEAK_TFLOPS, 5), traffic=None,
+                    avg_launch_ms=round(avg_ms, 4), launches=launches, algorithmic_flops_per_launch=per_launch)
+    per_launch = bytes_by_kernel[dom] / launches
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     return dict(bound="hbm", kernel=dom + "_kernel", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, avg_launch_ms=round(avg_ms, 4),
@@ -196,7 +208,7 @@ def run_fit(args, torch):
 
     engine.timing_enable(True)
     loop_s, dt_s, kern_s = timed_loop(sur, "fit_timed_surrogate")
-    loop_e, dt_e, _ = timed_loop(None, "fit_timed_exact")
+    loop_e, dt_e, kern_e = timed_loop(None, "fit_timed_exact")
     engine.timing_enable(False)
     value = loop_s.n_spectra / dt_s
     batched_rate = loop_e.n_spectra / dt_e
@@ -228,10 +240,18 @@ def run_fit(args, torch):
     gb.set_seeds([fam._bases[1]])
     gb.build(fixed_basis=True)
     n_g, nnz_g = int(gb.chain_n[0]), int(gb.chain_nnz[0])
-    eig_bytes = sum(algorithmic_bytes_eigen([n_g] * len(r.eig_steps), [nnz_g] * len(r.eig_steps), r.eig_steps, 3)
-                    for r in loop_s.exact_results)
-    lan_bytes = sum(algorithmic_bytes_lanczos(r.chain_dims, r.chain_nnz, r.kout) for r in loop_s.exact_results)
-    roofline = _roofline(kern_s, {"eigen": eig_bytes, "lanczos": lan_bytes})
+    def ed_bytes(loop):
+        eig = sum(algorithmic_bytes_eigen([n_g] * len(r.eig_steps), [nnz_g] * len(r.eig_steps), r.eig_steps, 3)
+                  for r in loop.exact_results)
+        lan = sum(algorithmic_bytes_lanczos(r.chain_dims, r.chain_nnz, r.kout) for r in loop.exact_results)
+        return {"eigen": eig, "lanczos": lan}
+
+    # surrogate GEMMs: 2 flops per multiply-add, every layer, every member, every predicted sample
+    dims = sur.net.dims
+    gemm_flops = 2.0 * sur.net.E * sum(dims[l] * dims[l + 1] for l in range(len(dims) - 1)) * loop_s.n_predicted
+    roofline = _roofline(kern_s, ed_bytes(loop_s), {"mlp_gemm": gemm_flops})
+    # the exact-ED eigensolver where it dominates: the batched-ED-only loop
+    roofline_ed = _roofline({"eigen": kern_e["eigen"]}, ed_bytes(loop_e))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -271,6 +291,8 @@ def run_fit(args, torch):
             "speedup_batched_ed_vs_plain": round(batched_rate / plain_rate, 2),
             "setup": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()},
             "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kern_s.items() if v[1]},
+            "kernel_ms_per_step_exact_loop": {k: round(v[0] / args.steps, 4) for k, v in kern_e.items() if v[1]},
+            "roofline_eigen_exact_loop": roofline_ed,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -569,13 +591,14 @@ def run_batched128(args, torch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 50 fitting-loop steps, 20 steps of the other workloads)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=("fit", "batched128", "large", "rixs"), default="fit")
     ap.add_argument("--models", type=int, default=128)
     ap.add_argument("--walkers", type=int, default=256)
-    ap.add_argument("--n-train", type=int, default=100000)
-    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--n-train", type=int, default=300000)
+    ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=0)
@@ -583,6 +606,8 @@ def main():
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="copy this many MiB device-to-device first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 50 if args.workload == "fit" else 20
     if args.gpus != 1:
         print("bench.py measures one GPU only: the engine runs on a single MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)

@@ -35,6 +35,7 @@ DEFAULTS = {
     "spmv_lpr": 8,
     "lanczos_graph_steps": 16,
     "eigen_lds_vectors": -1,
+    "small_sort": 1,
 }
 
 CANDIDATES = {

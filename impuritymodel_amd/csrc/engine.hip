@@ -886,6 +886,8 @@ int imped_set_tuning(const char* name, int64_t value) {
       return IMPED_ERR_INVALID;
     }
     g_tuning.spmv_lpr = value;
+  } else if (n == "small_sort") {
+    g_tuning.small_sort = value != 0;
   } else if (n == "eigen_lds_vectors") {
     g_tuning.eigen_lds_vectors = value < 0 ? -1 : value;
   } else if (n == "lanczos_graph_steps") {
@@ -1476,6 +1478,22 @@ int imped_mlp_forward(imped_mlp net, const float* x, int batch, float* y) {
     return IMPED_ERR_INVALID;
   }
   mlp_forward(*reinterpret_cast<Mlp*>(net), x, batch, y, stream());
+  API_END
+}
+
+int imped_pinned_alloc(int64_t bytes, void** out) {
+  API_BEGIN
+  if (bytes < 1 || !out) {
+    set_error("imped_pinned_alloc: bytes must be >= 1");
+    return IMPED_ERR_INVALID;
+  }
+  HIP_CHECK(hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault));
+  API_END
+}
+
+int imped_pinned_free(void* p) {
+  API_BEGIN
+  if (p) HIP_CHECK(hipHostFree(p));
   API_END
 }
 

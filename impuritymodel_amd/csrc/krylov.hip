@@ -413,14 +413,33 @@ __device__ __forceinline__ double spmv_t(const int64_t* rowptr, const int32_t* c
   for (int64_t i = threadIdx.x; i < n; i += BT) {
     const int64_t e0 = rowptr[base + i], e1 = rowptr[base + i + 1];
     double ax = 0.0, ay = 0.0;
-    for (int64_t e = e0; e < e1; ++e) {
-      const double2 h = val[e];
-      const T v = x[col[e]];
-      if constexpr (is_real_v<T>) {
-        ax += h.x * re_(v);
-      } else {
-        ax += h.x * re_(v) - h.y * im_(v);
-        ay += h.x * im_(v) + h.y * re_(v);
+    // four nonzeros per round trip: the column and value loads issue
+    // together (indices past the row clamped, their products skipped), the
+    // products are added in row order
+    for (int64_t e = e0; e < e1; e += 4) {
+      int32_t cc[4];
+      double2 hh[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t k = e + u < e1 ? e + u : e1 - 1;
+        cc[u] = col[k];
+        hh[u] = val[k];
+      }
+      T vv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) vv[u] = x[cc[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (e + u < e1) {
+          const double2 h = hh[u];
+          const T v = vv[u];
+          if constexpr (is_real_v<T>) {
+            ax += h.x * re_(v);
+          } else {
+            ax += h.x * re_(v) - h.y * im_(v);
+            ay += h.x * im_(v) + h.y * re_(v);
+          }
+        }
       }
     }
     y[i] = mk<T>(ax, ay);

@@ -12,6 +12,7 @@
 //   gradient, bias gradients, the Adam update (FP32 master weights, BF16
 //   copies for the GEMMs) and minibatch assembly.
 #include <cmath>
+#include <cstring>
 #include <memory>
 
 #include "common.h"
@@ -141,6 +142,74 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   }
 }
 
+// Forward GEMM C = X W^T (+ bias, GELU) with K staged FK at a time for both
+// 64-row operand tiles: every global load of a stage is in flight together
+// and one barrier precedes its MFMAs, so a 512-deep layer pays two load
+// latencies instead of sixteen.  2 x 64 x (FK + 8) bf16 = 66 KiB of LDS, so
+// two workgroups share a CU (the 1000-wide output layer has 320 of them).
+constexpr int FK = 256, FLK = FK + 8;
+constexpr size_t FWD_LDS = (size_t)2 * TM * FLK * sizeof(uint16_t);
+
+// 8 consecutive bf16 of row gr from column gk (zeros outside [0, R) x [0, K)).
+__device__ __forceinline__ uint4 load8(const uint16_t* src, int64_t ld, int R, int K, int gr, int gk) {
+  const uint16_t* p = src + (int64_t)gr * ld + gk;
+  if (gr < R && gk + 8 <= K && (reinterpret_cast<uintptr_t>(p) & 15) == 0) return *reinterpret_cast<const uint4*>(p);
+  uint16_t v[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = (gr < R && gk + i < K) ? src[(int64_t)gr * ld + gk + i] : (uint16_t)0;
+  return *reinterpret_cast<const uint4*>(v);
+}
+
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_fwd_kernel(GemmArgs g) {
+  extern __shared__ __align__(16) uint16_t fsm[];
+  uint16_t(*As)[FLK] = reinterpret_cast<uint16_t(*)[FLK]>(fsm);
+  uint16_t(*Bs)[FLK] = reinterpret_cast<uint16_t(*)[FLK]>(fsm + TM * FLK);
+  const int z = blockIdx.z;
+  const uint16_t* A = g.A + (int64_t)z * g.a_z;
+  const uint16_t* B = g.B + (int64_t)z * g.b_z;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int k0 = 0; k0 < g.K; k0 += FK) {
+    const int kc8 = ((g.K - k0 < FK ? g.K - k0 : FK) + 7) & ~7;  // MFMA steps of 8, zero padded
+    const int per_row = kc8 / 8;
+    const int chunks = TM * per_row;
+#pragma unroll 8
+    for (int c = t; c < chunks; c += 256) {
+      const int r = c / per_row, k8 = (c - r * per_row) * 8;
+      const uint4 va = load8(A, g.lda, g.M, g.K, m0 + r, k0 + k8);
+      const uint4 vb = load8(B, g.ldb, g.N, g.K, n0 + r, k0 + k8);
+      *reinterpret_cast<uint4*>(&As[r][k8]) = va;
+      *reinterpret_cast<uint4*>(&Bs[r][k8]) = vb;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < kc8; kk += 8) {
+      // lane l supplies row l%32 and k = kk + 4*(l/32) .. +3 of each operand
+      const int kq = kk + 4 * (lane >> 5);
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(&As[wm + (lane & 31)][kq]);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(&Bs[wn + (lane & 31)][kq]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // accumulator r of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32
+  const int64_t cz = (int64_t)z * g.c_z;
+  const int n = n0 + wn + (lane & 31);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int m = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (m >= g.M || n >= g.N) continue;
+    const int64_t o = cz + (int64_t)m * g.ldc + n;
+    const float zv = acc[r] + g.bias[(int64_t)z * g.bias_z + n];
+    g.Cf[o] = zv;
+    if constexpr (EPI == EPI_BIAS_GELU) g.Cb[o] = f2bf(gelu(zv));
+  }
+}
+
 __global__ void to_bf16_kernel(const float* x, int64_t n, uint16_t* y) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = f2bf(x[i]);
@@ -253,6 +322,20 @@ void gemm(const GemmArgs& g, int E, hipStream_t s) {
   HIP_CHECK(hipGetLastError());
 }
 
+// forward layer (X W^T + bias, GELU or not) by the whole-K kernel
+template <int EPI>
+void gemm_fwd(const GemmArgs& g, int E, hipStream_t s) {
+  static bool attr = false;
+  if (!attr) {
+    HIP_CHECK(hipFuncSetAttribute((const void*)gemm_fwd_kernel<EPI>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)FWD_LDS));
+    attr = true;
+  }
+  const dim3 grid(blocks_for(g.N, TN), blocks_for(g.M, TM), E);
+  hipLaunchKernelGGL((gemm_fwd_kernel<EPI>), grid, dim3(256), FWD_LDS, s, g);
+  HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ network state
@@ -270,6 +353,7 @@ struct Mlp {
   DevBuf xb, yb, out, loss, idx;    // minibatch input bf16, targets fp32, output fp32 [E][B][dL]
   DevBuf dX, dY;                    // data set (fp32)
   DevBuf sw, smean, sdis;           // ensemble statistics: area weights, mean [B][dL], disagreement [B]
+  std::vector<double> sw_host;      // the area weights last uploaded
   int64_t n_data = 0;
 };
 
@@ -317,12 +401,12 @@ void forward(Mlp& net, int B, hipStream_t s) {
     if (l + 1 < net.L) {
       g.Cf = net.zf[l].as<float>();
       g.Cb = net.hb[l].as<uint16_t>();
-      gemm<true, true, EPI_BIAS_GELU>(g, net.E, s);
+      gemm_fwd<EPI_BIAS_GELU>(g, net.E, s);
       x = net.hb[l].as<uint16_t>();
       x_z = (int64_t)B * outd;
     } else {
       g.Cf = net.out.as<float>();
-      gemm<true, true, EPI_BIAS>(g, net.E, s);
+      gemm_fwd<EPI_BIAS>(g, net.E, s);
     }
   }
 }
@@ -390,7 +474,7 @@ void upload_batch_input(Mlp& net, const float* x_host, int B, hipStream_t s) {
   hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, tmp.as<float>(), n,
                      net.xb.as<uint16_t>());
   HIP_CHECK(hipGetLastError());
-  stream_sync(s);
+  // tmp goes back to the device-memory cache; later owners are ordered after the conversion
 }
 
 }  // namespace
@@ -464,7 +548,11 @@ void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, dou
   upload_batch_input(net, x_host, B, s);
   forward(net, B, s);
   const int D = net.dims[net.L];
-  h2d(net.sw, w_host, D, s);
+  // the weights are the same grid's from call to call: upload on change only
+  if (net.sw_host.size() != (size_t)D || std::memcmp(net.sw_host.data(), w_host, D * sizeof(double)) != 0) {
+    net.sw_host.assign(w_host, w_host + D);
+    h2d(net.sw, net.sw_host.data(), D, s);
+  }
   net.smean.reserve((size_t)B * D * sizeof(double));
   net.sdis.reserve((size_t)B * sizeof(double));
   hipLaunchKernelGGL(ensemble_stats_kernel, dim3(B), dim3(256), 0, s, net.out.as<float>(), net.E, B, D,

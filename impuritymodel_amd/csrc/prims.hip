@@ -177,9 +177,77 @@ __global__ void gather_i32_as_u64_kernel(const int32_t* src, const int32_t* perm
   if (i < n) out[i] = (uint64_t)(uint32_t)src[perm[i]];
 }
 
+// One-workgroup sort of up to SMALL_SORT_MAX records whose (prim, key,
+// position) fits one 63-bit code: code = prim << (n_orb + ib) | key << ib | i
+// with i the record's position in perm, so every code is distinct and the
+// order is the stable (prim, key) order of the digit passes.  Bitonic network
+// in LDS (padding codes all ones sort last), then perm[i] = perm[code_i & mask].
+constexpr int SMALL_SORT_MAX = 16384;
+constexpr int SMALL_SORT_THREADS = 1024;
+
+__global__ __launch_bounds__(SMALL_SORT_THREADS) void small_sort_kernel(const uint64_t* keys, const int32_t* prim,
+                                                                        int32_t* perm, int n, int npow2, int n_orb,
+                                                                        int ib) {
+  extern __shared__ uint64_t code[];
+  const int shift_prim = n_orb + ib;
+  for (int i = threadIdx.x; i < npow2; i += SMALL_SORT_THREADS) {
+    uint64_t c = ~0ull;
+    if (i < n) {
+      const int32_t r = perm[i];
+      const uint64_t key = n_orb > 0 ? keys[r] >> (64 - n_orb) : 0ull;
+      const uint64_t pr = shift_prim < 64 ? (uint64_t)(uint32_t)prim[r] << shift_prim : 0ull;
+      c = pr | (key << ib) | (uint64_t)i;
+    }
+    code[i] = c;
+  }
+  __syncthreads();
+  for (int k = 2; k <= npow2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < npow2; i += SMALL_SORT_THREADS) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = code[i], b = code[ixj];
+          if ((a > b) == ((i & k) == 0)) {
+            code[i] = b;
+            code[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const uint64_t mask = (1ull << ib) - 1;
+  int32_t v[SMALL_SORT_MAX / SMALL_SORT_THREADS];
+  int q = 0;
+  for (int i = threadIdx.x; i < n; i += SMALL_SORT_THREADS) v[q++] = perm[(int)(code[i] & mask)];
+  __syncthreads();  // every read of the old perm precedes the writes
+  q = 0;
+  for (int i = threadIdx.x; i < n; i += SMALL_SORT_THREADS) perm[i] = v[q++];
+}
+
+bool small_sort(int W, int n_orb, int n_prim_bits, const uint64_t* keys, const int32_t* prim, int32_t* perm,
+                int64_t n, hipStream_t s) {
+  if (W != 1 || n > SMALL_SORT_MAX || n_orb > 64) return false;
+  const int ib = bits_for(n);
+  if (n_prim_bits + n_orb + ib > 63) return false;
+  int npow2 = 1;
+  while (npow2 < n) npow2 <<= 1;
+  static bool attr = false;
+  if (!attr) {
+    HIP_CHECK(hipFuncSetAttribute((const void*)small_sort_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  SMALL_SORT_MAX * (int)sizeof(uint64_t)));
+    attr = true;
+  }
+  hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(SMALL_SORT_THREADS), (size_t)npow2 * sizeof(uint64_t), s, keys,
+                     prim, perm, (int)n, npow2, n_orb, ib);
+  HIP_CHECK(hipGetLastError());
+  return true;
+}
+
 void sort_words_then(int W, int n_orb, int n_prim_bits, const uint64_t* keys, const int32_t* prim, int32_t* perm,
                      int64_t n, hipStream_t s) {
   if (n <= 1) return;
+  if (g_tuning.small_sort && small_sort(W, n_orb, n_prim_bits, keys, prim, perm, n, s)) return;
   Scratch& sc = scratch();
   sc.k0.reserve(n * sizeof(uint64_t));
   for (int w = W - 1; w >= 0; --w) {

@@ -11,11 +11,14 @@ from . import opcompile
 from ._lib import IMPED_ERR_BREAKDOWN, IMPED_ERR_NOCONV, C, check, lib, ptr
 
 
-def _release(destroy, obj):
-    """Destroy a handle once; at interpreter shutdown the binding may already be gone."""
+def _release(destroy, obj, _lib=lib):
+    """Destroy a handle once.  The library is bound at definition time: at
+    interpreter shutdown the module's globals may already be gone."""
     h = getattr(obj, "h", None)
-    if h and lib is not None:
-        getattr(lib, destroy)(h)
+    if h and _lib is not None:
+        fn = getattr(_lib, destroy, None)
+        if fn is not None:
+            fn(h)
     obj.h = None
 
 
@@ -62,7 +65,7 @@ class Ops:
         check(lib.imped_ops_set_affine(self.h, len(dirs), len(base), ptr(_c64(base)), ptr(_c64(dirs)), len(theta),
                                        ptr(_nz(theta.reshape(-1)))))
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release("imped_ops_destroy", self)
 
 
@@ -124,7 +127,7 @@ class Vec:
         check(lib.imped_vec_get(self.h, ptr(keys), ptr(amps.view(np.float64)), ptr(norm2)))
         return keys[:N], amps[:N], norm2[: len(off) - 1]
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release("imped_vec_destroy", self)
 
 
@@ -163,7 +166,7 @@ class Batch:
         self.h = h
         self.chain_n = None
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release("imped_batch_destroy", self)
 
     def set_seeds(self, keys_list, amps_list=None):

@@ -204,6 +204,7 @@ class Ensemble:
         self.set_params(self.init_params(seed))
 
     def __del__(self):
+        self._free_pinned()
         h = getattr(self, "h", None)
         lib = getattr(self, "_lib", None)
         if h and lib is not None:
@@ -240,19 +241,42 @@ class Ensemble:
         self._check(self._lib.imped_mlp_forward(self.h, self._ptr(x), len(x), self._ptr(y)))
         return y
 
+    def _pinned(self, B):
+        """Page-locked output arrays for B samples (mean [B, D], disagreement [B]), grown on demand."""
+        import ctypes
+
+        D = self.dims[-1]
+        if getattr(self, "_pin_rows", 0) < B:
+            self._free_pinned()
+            rows = max(B, 256)
+            ptr = ctypes.c_void_p()
+            self._check(self._lib.imped_pinned_alloc(rows * (D + 1) * 8, ctypes.byref(ptr)))
+            self._pin_ptr, self._pin_rows = ptr, rows
+            buf = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_double)), shape=(rows * (D + 1),))
+            self._pin_mean = buf[: rows * D].reshape(rows, D)
+            self._pin_dis = buf[rows * D:]
+        return self._pin_mean[:B], self._pin_dis[:B]
+
+    def _free_pinned(self):
+        p = getattr(self, "_pin_ptr", None)
+        lib = getattr(self, "_lib", None)
+        if p is not None and lib is not None:
+            lib.imped_pinned_free(p)
+        self._pin_ptr, self._pin_rows = None, 0
+
     def predict(self, x, weights, scale):
         """Ensemble statistics on the device: (mean [B, D], disagreement [B]).
 
         Members m_e = max(y_e / scale, 0) normalized to sum_d weights_d m_e,d = 1;
         mean = their average renormalized the same way; disagreement = the
-        members' mean of sum_d weights_d |m_e,d - mean_d|."""
+        members' mean of sum_d weights_d |m_e,d - mean_d|.  The results are
+        read back into page-locked memory and returned as copies."""
         x = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
         w = np.ascontiguousarray(weights, np.float64).reshape(self.dims[-1])
-        mean = np.zeros((len(x), self.dims[-1]), np.float64)
-        dis = np.zeros(len(x), np.float64)
+        mean, dis = self._pinned(len(x))
         self._check(self._lib.imped_mlp_predict(self.h, self._ptr(x), len(x), self._ptr(w), float(scale),
                                                 self._ptr(mean), self._ptr(dis)))
-        return mean, dis
+        return mean.copy(), dis.copy()
 
     def gradients(self, x, t):
         x = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
@@ -366,7 +390,7 @@ class Surrogate:
         return err, dis
 
 
-def train_surrogate(family, n_train=100000, epochs=200, n_val=2048, members=5, seed=0, bar=1e-2, margin=0.8,
+def train_surrogate(family, n_train=300000, epochs=150, n_val=2048, members=5, seed=0, bar=1e-2, margin=0.8,
                     chunk=4096, **fit_kw):
     """Training set by batched exact ED, ensemble training, threshold calibration.
 
@@ -415,6 +439,7 @@ class FittingLoop:
     def __init__(self, family, target, surrogate=None, n_walkers=256, step=0.05, temperature=0.02, seed=0,
                  flush_at=None, **xas_kw):
         self.family, self.target, self.surrogate = family, np.asarray(target), surrogate
+        self.target_n = area_normalize(self.target)
         self.step_size, self.temperature, self.xas_kw = step, temperature, xas_kw
         self.flush_at = n_walkers // 2 if flush_at is None else flush_at
         self.rng = np.random.default_rng(seed)
@@ -427,7 +452,7 @@ class FittingLoop:
         self.proposal = self.theta.copy()
 
     def reset_counters(self):
-        self.n_spectra = self.n_exact = self.n_surrogate = self.n_flushes = 0
+        self.n_spectra = self.n_exact = self.n_surrogate = self.n_flushes = self.n_predicted = 0
         self.sur_params, self.sur_spectra, self.exact_results = [], [], []
 
     def _exact(self, P):
@@ -438,7 +463,8 @@ class FittingLoop:
         return ex
 
     def _decide(self, idx, spectra):
-        new = l1_error(spectra, self.target[None])
+        # spectra (surrogate means and exact results) come with unit area already
+        new = np.abs(spectra - self.target_n[None]) @ _W_OMEGA
         acc = self.rng.uniform(size=len(idx)) < np.exp(-(new - self.cur[idx]) / self.temperature)
         take = idx[acc]
         self.theta[take] = self.proposal[take]
@@ -461,6 +487,7 @@ class FittingLoop:
         self.proposal[free] = np.clip(prop, lo, hi)
         if self.surrogate is not None and len(free):
             pred, dis = self.surrogate.predict(self.proposal[free])
+            self.n_predicted += len(free)
             use = dis < self.surrogate.threshold
             self.sur_params.append(self.proposal[free[use]])
             self.sur_spectra.append(pred[use])

@@ -64,7 +64,9 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * "lanczos_graph_steps" (grid-wide Lanczos steps per replayed HIP graph,
  * 0..256; 0 launches every step directly, as does switched-on timing),
  * "eigen_lds_vectors" (cap on the Krylov vectors the eigensolver keeps in
- * LDS; -1 as many as fit, 0 none). */
+ * LDS; -1 as many as fit, 0 none), "small_sort" (1: key sets of at most
+ * 16384 records whose chain, key and position fit 63 bits are sorted by one
+ * workgroup; 0: always the digit passes). */
 int imped_set_tuning(const char* name, int64_t value);
 
 /* The engine's device radix sort (the one that orders BFS layers and
@@ -248,6 +250,11 @@ int imped_mlp_forward(imped_mlp net, const float* x, int batch, float* y);
  * distances).  No reference counterpart. */
 int imped_mlp_predict(imped_mlp net, const float* x, int batch, const double* weights, double scale, double* mean,
                       double* disagreement);
+/* Page-locked host memory (hipHostMalloc) for outputs read back every call:
+ * device-to-host copies into it run at full link speed.  No reference
+ * counterpart. */
+int imped_pinned_alloc(int64_t bytes, void** out);
+int imped_pinned_free(void* p);
 /* Gradients [n_members][P] of the mean squared error mean_{b,j} (y_e - t)^2
  * over one batch (x, t), and the loss of each member (either may be NULL). */
 int imped_mlp_gradients(imped_mlp net, const float* x, const float* t, int batch, float* grads, double* loss);

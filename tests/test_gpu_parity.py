@@ -537,3 +537,25 @@ def test_batched_equals_single_chain_runs(gpu):
         one = xas_batch([models[k]], omega=w, krylovSize=200, reorth=1)
         scale = np.abs(one.spectra[0]).max()
         np.testing.assert_allclose(big.spectra[k], one.spectra[0], rtol=0, atol=1e-10 * scale)
+
+
+def test_small_sort_matches_digit_passes(gpu):
+    """The one-workgroup sort of small key sets orders BFS layers and vector
+    entries exactly as the digit passes do: identical chain sizes, CSR and
+    spectra with it on and off (W = 1 and W = 2 models in one batch)."""
+    from impuritymodel_amd import engine
+    from impuritymodel_amd.batched import xas_batch
+    from impuritymodel_amd.models import batched_params, model_from_params
+
+    models = [model_from_params(p) for p in batched_params(12, seed=8)][:6]
+    out = {}
+    try:
+        for flag in (0, 1):
+            engine.set_tuning("small_sort", flag)
+            r = xas_batch(models, omega=np.linspace(-10, 20, 400), delta=0.3, krylovSize=60, reorth=1)
+            out[flag] = r
+    finally:
+        engine.set_tuning("small_sort", 1)
+    assert np.array_equal(out[0].chain_dims, out[1].chain_dims)
+    assert np.array_equal(out[0].chain_nnz, out[1].chain_nnz)
+    assert np.array_equal(out[0].spectra, out[1].spectra)
