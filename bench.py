@@ -11,8 +11,8 @@ its proposal (3 lowest eigenstates, energy cut, 3 dipole polarizations,
 the ensemble agrees on are answered by the surrogate network at once; the
 others wait and are evaluated together by one batched exact-ED call once
 half the walkers wait (and at the end of the timed steps), so the timed
-region contains every proposal's spectrum.  Set-up (untimed, reported): a 300k-spectrum
-training set from the batched exact ED, 5-member ensemble training with the
+region contains every proposal's spectrum.  Set-up (untimed, reported): a 20k-spectrum
+training set from the batched exact ED (the configuration's size), 5-member ensemble training with the
 library's BF16 matrix-core kernels, trust-threshold calibration on 2048
 validation spectra.  After the timed steps every surrogate-served spectrum
 is recomputed by exact ED and its L1 error reported.  The same loop without
@@ -597,8 +597,8 @@ def main():
     ap.add_argument("--workload", choices=("fit", "batched128", "large", "rixs"), default="fit")
     ap.add_argument("--models", type=int, default=128)
     ap.add_argument("--walkers", type=int, default=256)
-    ap.add_argument("--n-train", type=int, default=300000)
-    ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--n-train", type=int, default=20000)
+    ap.add_argument("--epochs", type=int, default=2500)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=0)

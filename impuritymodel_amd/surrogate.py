@@ -390,7 +390,7 @@ class Surrogate:
         return err, dis
 
 
-def train_surrogate(family, n_train=300000, epochs=150, n_val=2048, members=5, seed=0, bar=1e-2, margin=0.8,
+def train_surrogate(family, n_train=20000, epochs=2500, n_val=2048, members=5, seed=0, bar=1e-2, margin=0.8,
                     chunk=4096, **fit_kw):
     """Training set by batched exact ED, ensemble training, threshold calibration.
 
