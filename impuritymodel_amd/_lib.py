@@ -69,6 +69,7 @@ SIGNATURES = {
     "imped_mlp_get_params": [_p, _p],
     "imped_mlp_forward": [_p, _p, _i32, _p],
     "imped_mlp_predict": [_p, _p, _i32, _p, _d, _p, _p],
+    "imped_mlp_misfit": [_p, _p, _i32, _p, _d, _p, _p, _p, _p],
     "imped_pinned_alloc": [_i64, _p],
     "imped_pinned_free": [_p],
     "imped_mlp_gradients": [_p, _p, _p, _i32, _p, _p],

@@ -1504,7 +1504,18 @@ int imped_mlp_predict(imped_mlp net, const float* x, int batch, const double* we
     set_error("imped_mlp_predict: batch must be >= 1, weights given and scale > 0");
     return IMPED_ERR_INVALID;
   }
-  mlp_predict(*reinterpret_cast<Mlp*>(net), x, batch, weights, scale, mean, disagreement, stream());
+  mlp_predict(*reinterpret_cast<Mlp*>(net), x, batch, weights, scale, nullptr, mean, disagreement, nullptr, stream());
+  API_END
+}
+
+int imped_mlp_misfit(imped_mlp net, const float* x, int batch, const double* weights, double scale,
+                     const double* target, double* disagreement, double* misfit, double* mean) {
+  API_BEGIN
+  if (batch < 1 || !weights || !target || !disagreement || !misfit || !(scale > 0.0)) {
+    set_error("imped_mlp_misfit: batch must be >= 1, weights, target and outputs given and scale > 0");
+    return IMPED_ERR_INVALID;
+  }
+  mlp_predict(*reinterpret_cast<Mlp*>(net), x, batch, weights, scale, target, mean, disagreement, misfit, stream());
   API_END
 }
 

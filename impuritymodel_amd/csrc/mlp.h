@@ -17,8 +17,9 @@ int mlp_layers(const Mlp& net, int32_t* dims);
 void mlp_set_params(Mlp& net, const float* params, hipStream_t s);
 void mlp_get_params(Mlp& net, float* params, hipStream_t s);
 void mlp_forward(Mlp& net, const float* x_host, int B, float* y_host, hipStream_t s);
-void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, double scale, double* mean_host,
-                 double* dis_host, hipStream_t s);
+// target_host, mean_host and l1_host may be NULL (no misfit; no read-back).
+void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, double scale, const double* target_host,
+                 double* mean_host, double* dis_host, double* l1_host, hipStream_t s);
 void mlp_gradients(Mlp& net, const float* x_host, const float* y_host, int B, float* g_host, double* loss,
                    hipStream_t s);
 void mlp_set_data(Mlp& net, const float* X, const float* Y, int64_t n, hipStream_t s);

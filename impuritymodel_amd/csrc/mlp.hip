@@ -255,10 +255,13 @@ __device__ __forceinline__ double sum256(double v, double* red) {
 // Ensemble statistics of one sample per workgroup, from the outputs
 // y [E][B][D]: member spectra m_e = max(y_e / scale, 0) / area_e (area =
 // sum_d w_d (.), the trapezoid rule when w holds its weights), their mean
-// renormalized to unit area, and the disagreement
-// dis = (1/E) sum_e sum_d w_d |m_e,d - mean_d|.  E <= 64.
+// renormalized to unit area, the disagreement
+// dis = (1/E) sum_e sum_d w_d |m_e,d - mean_d| and, when target is given,
+// the misfit l1 = sum_d w_d |mean_d - target_d|.  E <= 64; the general
+// path (ensemble_stats_wave_kernel covers E <= 8, D <= 1024).
 __global__ __launch_bounds__(256) void ensemble_stats_kernel(const float* y, int E, int B, int D, const double* w,
-                                                             double scale, double* mean, double* dis) {
+                                                             double scale, const double* target, double* mean,
+                                                             double* dis, double* l1) {
   __shared__ double red[4];
   __shared__ double area[64];
   const int b = blockIdx.x;
@@ -280,15 +283,94 @@ __global__ __launch_bounds__(256) void ensemble_stats_kernel(const float* y, int
     pm += w[d] * acc;
   }
   const double marea = sum256(pm, red);
-  double pd = 0.0;
+  double pd = 0.0, pl = 0.0;
   for (int d = threadIdx.x; d < D; d += 256) {
     const double mn = mb[d] / marea;
     mb[d] = mn;
     for (int e = 0; e < E; ++e)
       pd += w[d] * fabs(fmax((double)y[((int64_t)e * B + b) * D + d] / scale, 0.0) / area[e] - mn);
+    if (target) pl += w[d] * fabs(mn - target[d]);
   }
   pd = sum256(pd, red);
-  if (threadIdx.x == 0) dis[b] = pd / (double)E;
+  if (target) pl = sum256(pl, red);
+  if (threadIdx.x == 0) {
+    dis[b] = pd / (double)E;
+    if (target) l1[b] = pl;
+  }
+}
+
+__device__ __forceinline__ double wave_sum64(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// The same statistics with one wave per sample (E <= 8, D <= 64 K): lane l
+// holds y_e[l + 64 k] of every member in registers, loaded once, so each
+// statistic is register arithmetic and one wave reduction (three rounds in
+// all) instead of a pass over memory and a workgroup reduction per member.
+template <int K>
+__global__ __launch_bounds__(64) void ensemble_stats_wave_kernel(const float* y, int E, int B, int D,
+                                                                 const double* w, double scale,
+                                                                 const double* target, double* mean, double* dis,
+                                                                 double* l1) {
+  constexpr int EM = 8;
+  const int lane = threadIdx.x;
+  const int b = blockIdx.x;
+  float v[EM][K];
+  double wd[K];
+#pragma unroll
+  for (int e = 0; e < EM; ++e)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int d = lane + 64 * k;
+      v[e][k] = (e < E && d < D) ? y[((int64_t)e * B + b) * D + d] : 0.0f;
+    }
+#pragma unroll
+  for (int k = 0; k < K; ++k) wd[k] = lane + 64 * k < D ? w[lane + 64 * k] : 0.0;
+  double area[EM];
+#pragma unroll
+  for (int e = 0; e < EM; ++e) {
+    double p = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) p += wd[k] * fmax((double)v[e][k] / scale, 0.0);
+    area[e] = p;
+  }
+#pragma unroll
+  for (int e = 0; e < EM; ++e) area[e] = e < E ? wave_sum64(area[e]) : 1.0;
+  double mn[K];
+  double pm = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double acc = 0.0;
+#pragma unroll
+    for (int e = 0; e < EM; ++e)
+      if (e < E) acc += fmax((double)v[e][k] / scale, 0.0) / area[e];
+    mn[k] = acc / (double)E;
+    pm += wd[k] * mn[k];
+  }
+  const double marea = wave_sum64(pm);
+  double pd = 0.0, pl = 0.0;
+  double* mb = mean + (int64_t)b * D;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int d = lane + 64 * k;
+    const double m = mn[k] / marea;
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < EM; ++e)
+      if (e < E) s += fabs(fmax((double)v[e][k] / scale, 0.0) / area[e] - m);
+    pd += wd[k] * s;
+    if (d < D) {
+      mb[d] = m;
+      if (target) pl += wd[k] * fabs(m - target[d]);
+    }
+  }
+  pd = wave_sum64(pd);
+  if (target) pl = wave_sum64(pl);
+  if (lane == 0) {
+    dis[b] = pd / (double)E;
+    if (target) l1[b] = pl;
+  }
 }
 
 // db[z][n] = sum_b dZ[z][b][n]
@@ -353,7 +435,8 @@ struct Mlp {
   DevBuf xb, yb, out, loss, idx;    // minibatch input bf16, targets fp32, output fp32 [E][B][dL]
   DevBuf dX, dY;                    // data set (fp32)
   DevBuf sw, smean, sdis;           // ensemble statistics: area weights, mean [B][dL], disagreement [B]
-  std::vector<double> sw_host;      // the area weights last uploaded
+  DevBuf st, sl1;                   // misfit: target [dL], l1 [B]
+  std::vector<double> sw_host, st_host;  // the area weights and target last uploaded
   int64_t n_data = 0;
 };
 
@@ -542,24 +625,50 @@ void mlp_forward(Mlp& net, const float* x_host, int B, float* y_host, hipStream_
   stream_sync(s);
 }
 
-void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, double scale, double* mean_host,
-                 double* dis_host, hipStream_t s) {
+namespace {
+
+// Uploads a host vector of D doubles to dev when it differs from the copy
+// last uploaded (the grid's weights and a fit's target stay the same from
+// call to call).
+void upload_on_change(DevBuf& dev, std::vector<double>& last, const double* host, int D, hipStream_t s) {
+  if (last.size() != (size_t)D || std::memcmp(last.data(), host, D * sizeof(double)) != 0) {
+    last.assign(host, host + D);
+    h2d(dev, last.data(), D, s);
+  }
+}
+
+}  // namespace
+
+void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, double scale, const double* target_host,
+                 double* mean_host, double* dis_host, double* l1_host, hipStream_t s) {
   alloc_acts(net, B);
   upload_batch_input(net, x_host, B, s);
   forward(net, B, s);
   const int D = net.dims[net.L];
-  // the weights are the same grid's from call to call: upload on change only
-  if (net.sw_host.size() != (size_t)D || std::memcmp(net.sw_host.data(), w_host, D * sizeof(double)) != 0) {
-    net.sw_host.assign(w_host, w_host + D);
-    h2d(net.sw, net.sw_host.data(), D, s);
-  }
+  upload_on_change(net.sw, net.sw_host, w_host, D, s);
+  if (target_host) upload_on_change(net.st, net.st_host, target_host, D, s);
+  const double* tgt = target_host ? net.st.as<double>() : nullptr;
   net.smean.reserve((size_t)B * D * sizeof(double));
   net.sdis.reserve((size_t)B * sizeof(double));
-  hipLaunchKernelGGL(ensemble_stats_kernel, dim3(B), dim3(256), 0, s, net.out.as<float>(), net.E, B, D,
-                     net.sw.as<double>(), scale, net.smean.as<double>(), net.sdis.as<double>());
+  net.sl1.reserve((size_t)B * sizeof(double));
+  const float* y = net.out.as<float>();
+  double *m = net.smean.as<double>(), *dis = net.sdis.as<double>(), *l1 = net.sl1.as<double>();
+  const double* w = net.sw.as<double>();
+  if (net.E <= 8 && D <= 256)
+    hipLaunchKernelGGL(ensemble_stats_wave_kernel<4>, dim3(B), dim3(64), 0, s, y, net.E, B, D, w, scale, tgt, m, dis,
+                       l1);
+  else if (net.E <= 8 && D <= 512)
+    hipLaunchKernelGGL(ensemble_stats_wave_kernel<8>, dim3(B), dim3(64), 0, s, y, net.E, B, D, w, scale, tgt, m, dis,
+                       l1);
+  else if (net.E <= 8 && D <= 1024)
+    hipLaunchKernelGGL(ensemble_stats_wave_kernel<16>, dim3(B), dim3(64), 0, s, y, net.E, B, D, w, scale, tgt, m,
+                       dis, l1);
+  else
+    hipLaunchKernelGGL(ensemble_stats_kernel, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis, l1);
   HIP_CHECK(hipGetLastError());
-  d2h(mean_host, net.smean.p, (size_t)B * D, s);
+  if (mean_host) d2h(mean_host, net.smean.p, (size_t)B * D, s);
   d2h(dis_host, net.sdis.p, B, s);
+  if (target_host && l1_host) d2h(l1_host, net.sl1.p, B, s);
   stream_sync(s);
 }
 

@@ -242,7 +242,7 @@ class Ensemble:
         return y
 
     def _pinned(self, B):
-        """Page-locked output arrays for B samples (mean [B, D], disagreement [B]), grown on demand."""
+        """Page-locked output arrays for B samples (mean [B, D], disagreement [B], misfit [B]), grown on demand."""
         import ctypes
 
         D = self.dims[-1]
@@ -250,12 +250,13 @@ class Ensemble:
             self._free_pinned()
             rows = max(B, 256)
             ptr = ctypes.c_void_p()
-            self._check(self._lib.imped_pinned_alloc(rows * (D + 1) * 8, ctypes.byref(ptr)))
+            self._check(self._lib.imped_pinned_alloc(rows * (D + 2) * 8, ctypes.byref(ptr)))
             self._pin_ptr, self._pin_rows = ptr, rows
-            buf = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_double)), shape=(rows * (D + 1),))
+            buf = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_double)), shape=(rows * (D + 2),))
             self._pin_mean = buf[: rows * D].reshape(rows, D)
-            self._pin_dis = buf[rows * D:]
-        return self._pin_mean[:B], self._pin_dis[:B]
+            self._pin_dis = buf[rows * D: rows * (D + 1)]
+            self._pin_l1 = buf[rows * (D + 1):]
+        return self._pin_mean[:B], self._pin_dis[:B], self._pin_l1[:B]
 
     def _free_pinned(self):
         p = getattr(self, "_pin_ptr", None)
@@ -273,10 +274,24 @@ class Ensemble:
         read back into page-locked memory and returned as copies."""
         x = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
         w = np.ascontiguousarray(weights, np.float64).reshape(self.dims[-1])
-        mean, dis = self._pinned(len(x))
+        mean, dis, _ = self._pinned(len(x))
         self._check(self._lib.imped_mlp_predict(self.h, self._ptr(x), len(x), self._ptr(w), float(scale),
                                                 self._ptr(mean), self._ptr(dis)))
         return mean.copy(), dis.copy()
+
+    def misfit(self, x, weights, scale, target, want_mean=False):
+        """(disagreement [B], misfit [B]) with misfit_b = sum_d weights_d |mean_b,d - target_d|,
+        all on the device; the means [B, D] come back as a third item only when asked for."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
+        w = np.ascontiguousarray(weights, np.float64).reshape(self.dims[-1])
+        t = np.ascontiguousarray(target, np.float64).reshape(self.dims[-1])
+        mean, dis, l1 = self._pinned(len(x))
+        self._check(self._lib.imped_mlp_misfit(self.h, self._ptr(x), len(x), self._ptr(w), float(scale),
+                                               self._ptr(t), self._ptr(dis), self._ptr(l1),
+                                               self._ptr(mean) if want_mean else None))
+        if want_mean:
+            return dis.copy(), l1.copy(), mean.copy()
+        return dis.copy(), l1.copy()
 
     def gradients(self, x, t):
         x = np.ascontiguousarray(x, np.float32).reshape(-1, self.dims[0])
@@ -379,6 +394,11 @@ class Surrogate:
         """(area-normalized ensemble mean [B, n_w], disagreement [B]), computed on the device."""
         return self.net.predict(normalize_params(params), _W_OMEGA, SCALE)
 
+    def misfit(self, params, target_n):
+        """(disagreement [B], L1 distance of the ensemble mean from the unit-area
+        spectrum target_n [B]), computed on the device; the means stay there."""
+        return self.net.misfit(normalize_params(params), _W_OMEGA, SCALE, target_n)
+
     def calibrate(self, params, exact, bar=1e-2):
         """Largest threshold under which every validation prediction meets the L1 bar."""
         pred, dis = self.predict(params)
@@ -432,14 +452,23 @@ class FittingLoop:
     gathered (every step without a surrogate), so the fixed latency of an
     exact-ED launch sequence is paid once per flush instead of once per step.
     ``finish()`` evaluates what is still waiting.  Counters and the
-    surrogate-served (params, spectra) are kept for throughput and for scoring
+    surrogate-served parameters are kept for throughput and for scoring
     against exact ED.
+
+    A trusted proposal's misfit (the L1 distance of the ensemble mean from the
+    target) is computed on the device, next to the prediction; only the
+    disagreement and the misfit come back to the host.  With
+    ``keep_spectra`` the served spectra are read back too and kept;
+    otherwise ``served()`` predicts them again from the kept parameters (the
+    network computes each sample's row on its own, so the rows are the same
+    bit for bit).
     """
 
     def __init__(self, family, target, surrogate=None, n_walkers=256, step=0.05, temperature=0.02, seed=0,
-                 flush_at=None, **xas_kw):
+                 flush_at=None, keep_spectra=False, **xas_kw):
         self.family, self.target, self.surrogate = family, np.asarray(target), surrogate
         self.target_n = area_normalize(self.target)
+        self.keep_spectra = keep_spectra
         self.step_size, self.temperature, self.xas_kw = step, temperature, xas_kw
         self.flush_at = n_walkers // 2 if flush_at is None else flush_at
         self.rng = np.random.default_rng(seed)
@@ -462,9 +491,11 @@ class FittingLoop:
         self.n_exact += len(P)
         return ex
 
-    def _decide(self, idx, spectra):
+    def _misfit(self, spectra):
         # spectra (surrogate means and exact results) come with unit area already
-        new = np.abs(spectra - self.target_n[None]) @ _W_OMEGA
+        return np.abs(spectra - self.target_n[None]) @ _W_OMEGA
+
+    def _decide(self, idx, new):
         acc = self.rng.uniform(size=len(idx)) < np.exp(-(new - self.cur[idx]) / self.temperature)
         take = idx[acc]
         self.theta[take] = self.proposal[take]
@@ -474,7 +505,7 @@ class FittingLoop:
     def flush(self):
         idx = np.nonzero(self.waiting)[0]
         if len(idx):
-            self._decide(idx, self._exact(self.proposal[idx]))
+            self._decide(idx, self._misfit(self._exact(self.proposal[idx])))
             self.waiting[idx] = False
             self.n_flushes += 1
 
@@ -486,14 +517,19 @@ class FittingLoop:
         prop = np.where(prop > hi, 2 * hi - prop, prop)
         self.proposal[free] = np.clip(prop, lo, hi)
         if self.surrogate is not None and len(free):
-            pred, dis = self.surrogate.predict(self.proposal[free])
+            if self.keep_spectra:
+                pred, dis = self.surrogate.predict(self.proposal[free])
+                new = self._misfit(pred)
+            else:
+                dis, new = self.surrogate.misfit(self.proposal[free], self.target_n)
             self.n_predicted += len(free)
             use = dis < self.surrogate.threshold
             self.sur_params.append(self.proposal[free[use]])
-            self.sur_spectra.append(pred[use])
+            if self.keep_spectra:
+                self.sur_spectra.append(pred[use])
             self.n_spectra += int(use.sum())
             self.n_surrogate += int(use.sum())
-            self._decide(free[use], pred[use])
+            self._decide(free[use], new[use])
             self.waiting[free[~use]] = True
         else:
             self.waiting[free] = True
@@ -507,7 +543,12 @@ class FittingLoop:
         """(params, spectra) the surrogate produced since the last counter reset."""
         if not self.sur_params:
             return np.zeros((0, len(PARAM_NAMES))), np.zeros((0, len(OMEGA)))
-        return np.concatenate(self.sur_params), np.concatenate(self.sur_spectra)
+        params = np.concatenate(self.sur_params)
+        if self.keep_spectra:
+            return params, np.concatenate(self.sur_spectra)
+        chunk = 1024
+        return params, np.concatenate([self.surrogate.predict(params[i:i + chunk])[0]
+                                       for i in range(0, len(params), chunk)])
 
 
 def fitting_loop(family, target, surrogate=None, n_walkers=256, n_iter=8, step=0.05, temperature=0.02, seed=0,

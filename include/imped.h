@@ -250,6 +250,14 @@ int imped_mlp_forward(imped_mlp net, const float* x, int batch, float* y);
  * distances).  No reference counterpart. */
 int imped_mlp_predict(imped_mlp net, const float* x, int batch, const double* weights, double scale, double* mean,
                       double* disagreement);
+/* The same statistics plus each sample's misfit to a target spectrum,
+ * misfit[b] = sum_d weights[d] |mean[b][d] - target[d]| (the L1 distance of
+ * unit-area spectra when target has unit area), computed on the device: a
+ * fitting loop's Metropolis decisions read back 2 x batch doubles instead of
+ * batch x dims[L].  mean may be NULL (the means stay on the device).  No
+ * reference counterpart. */
+int imped_mlp_misfit(imped_mlp net, const float* x, int batch, const double* weights, double scale,
+                     const double* target, double* disagreement, double* misfit, double* mean);
 /* Page-locked host memory (hipHostMalloc) for outputs read back every call:
  * device-to-host copies into it run at full link speed.  No reference
  * counterpart. */

@@ -93,27 +93,79 @@ def test_family_spectra_match_assembled_models(gpu):
     np.testing.assert_allclose(fast, np.array(slow), rtol=0, atol=1e-8 * np.abs(slow).max())
 
 
-def test_ensemble_predict_statistics_vs_numpy(gpu):
-    """The device ensemble statistics equal the same statistics computed in
-    numpy (trapezoid rule, np.trapezoid) from the members' raw outputs."""
-    from impuritymodel_amd.surrogate import OMEGA, SCALE, Ensemble, trapezoid_weights
+@pytest.mark.parametrize("E,D", [(5, 1000), (3, 200), (9, 1000), (2, 1100)])
+def test_ensemble_predict_statistics_vs_numpy(gpu, E, D):
+    """The device ensemble statistics and misfits equal the same quantities
+    computed in numpy (trapezoid rule, np.trapezoid) from the members' raw
+    outputs.  The cases cover the one-wave kernel (64 and 16 values per lane)
+    and the workgroup kernel (more than 8 members; more than 1024 outputs)."""
+    from impuritymodel_amd.surrogate import Ensemble, trapezoid_weights
 
-    dims = (8, 64, 64, len(OMEGA))
-    net = Ensemble(5, dims, max_batch=64, seed=2)
+    omega = np.linspace(-5.0, 15.0, D)
+    scale = float(omega[-1] - omega[0])
+    dims = (8, 64, 64, D)
+    net = Ensemble(E, dims, max_batch=64, seed=2)
     rng = np.random.default_rng(3)
     p = net.init_params(3)
     p += rng.normal(0, 0.02, p.shape).astype(np.float32)
     net.set_params(p)
+    w = trapezoid_weights(omega)
     for B in (37, 5):
         x = rng.uniform(-1, 1, (B, 8)).astype(np.float32)
-        y = np.maximum(net.forward(x).astype(np.float64) / SCALE, 0.0)
-        members = y / np.trapezoid(y, OMEGA, axis=-1)[..., None]
+        y = np.maximum(net.forward(x).astype(np.float64) / scale, 0.0)
+        members = y / np.trapezoid(y, omega, axis=-1)[..., None]
         mean = members.mean(axis=0)
-        mean = mean / np.trapezoid(mean, OMEGA, axis=-1)[:, None]
-        dis = np.trapezoid(np.abs(members - mean[None]), OMEGA, axis=-1).mean(axis=0)
-        gm, gd = net.predict(x, trapezoid_weights(OMEGA), SCALE)
+        mean = mean / np.trapezoid(mean, omega, axis=-1)[:, None]
+        dis = np.trapezoid(np.abs(members - mean[None]), omega, axis=-1).mean(axis=0)
+        gm, gd = net.predict(x, w, scale)
         np.testing.assert_allclose(gm, mean, rtol=1e-12, atol=1e-14 * np.abs(mean).max())
         np.testing.assert_allclose(gd, dis, rtol=1e-11, atol=1e-15)
+        target = np.abs(np.sin(omega)) + 0.1
+        target /= np.trapezoid(target, omega)
+        l1 = np.trapezoid(np.abs(mean - target[None]), omega, axis=-1)
+        md, ml = net.misfit(x, w, scale, target)
+        np.testing.assert_allclose(md, dis, rtol=1e-11, atol=1e-15)
+        np.testing.assert_allclose(ml, l1, rtol=1e-11, atol=1e-15)
+        md2, ml2, mm = net.misfit(x, w, scale, target, want_mean=True)
+        assert np.array_equal(md2, md) and np.array_equal(ml2, ml) and np.array_equal(mm, gm)
+
+
+def test_surrogate_rows_independent_of_batch(gpu):
+    """A sample's predicted spectrum does not depend on the other samples of
+    its batch (bit for bit), so re-predicting served parameters reproduces
+    the spectra a fitting loop used."""
+    from impuritymodel_amd.surrogate import OMEGA, SCALE, Ensemble, trapezoid_weights
+
+    net = Ensemble(5, (8, 512, 512, 512, len(OMEGA)), max_batch=1024, seed=6)
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-1, 1, (300, 8)).astype(np.float32)
+    w = trapezoid_weights(OMEGA)
+    mean, dis = net.predict(x, w, SCALE)
+    rows = np.sort(rng.choice(300, 41, replace=False))
+    m2, d2 = net.predict(x[rows], w, SCALE)
+    assert np.array_equal(m2, mean[rows]) and np.array_equal(d2, dis[rows])
+
+
+def test_fitting_loop_device_misfit_matches_host(gpu):
+    """The fitting loop's decisions from device misfits follow the same walk
+    as decisions from spectra read back and compared on the host."""
+    from impuritymodel_amd import surrogate as S
+
+    fam = S.XASFamily()
+    sur = S.Surrogate(fam, S.Ensemble(5, (8, 512, 512, 512, len(S.OMEGA)), max_batch=1024, seed=0))
+    sur.threshold = np.inf  # every proposal served by the (untrained) network
+    target = np.abs(np.sin(S.OMEGA)) + 0.1
+    runs = []
+    for keep in (True, False):
+        loop = S.FittingLoop(fam, target, sur, n_walkers=64, seed=3, keep_spectra=keep)
+        for _ in range(6):
+            loop.step()
+        loop.finish()
+        runs.append((loop.theta.copy(), loop.cur.copy(), loop.served()))
+    (t1, c1, (p1, s1)), (t2, c2, (p2, s2)) = runs
+    assert np.array_equal(t1, t2) and np.array_equal(p1, p2)
+    np.testing.assert_allclose(c2, c1, rtol=1e-12, atol=0)
+    assert np.array_equal(s1, s2)
 
 
 def test_affine_family_operators_on_device(gpu):
