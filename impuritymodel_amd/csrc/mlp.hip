@@ -258,7 +258,7 @@ __device__ __forceinline__ double sum256(double v, double* red) {
 // renormalized to unit area, the disagreement
 // dis = (1/E) sum_e sum_d w_d |m_e,d - mean_d| and, when target is given,
 // the misfit l1 = sum_d w_d |mean_d - target_d|.  E <= 64; the general
-// path (ensemble_stats_wave_kernel covers E <= 8, D <= 1024).
+// path (ensemble_stats_reg_kernel covers E <= 8, D <= 1024).
 __global__ __launch_bounds__(256) void ensemble_stats_kernel(const float* y, int E, int B, int D, const double* w,
                                                              double scale, const double* target, double* mean,
                                                              double* dis, double* l1) {
@@ -304,72 +304,87 @@ __device__ __forceinline__ double wave_sum64(double v) {
   return v;
 }
 
-// The same statistics with one wave per sample (E <= 8, D <= 64 K): lane l
-// holds y_e[l + 64 k] of every member in registers, loaded once, so each
-// statistic is register arithmetic and one wave reduction (three rounds in
-// all) instead of a pass over memory and a workgroup reduction per member.
+// Sums of NV values over a 256-thread workgroup, broadcast to every thread;
+// red [4][NV] is this round's own LDS, so one barrier suffices.
+template <int NV>
+__device__ __forceinline__ void sums256(double (&v)[NV], double (*red)[NV]) {
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum64(v[i]);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[threadIdx.x >> 6][i] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = red[0][i] + red[1][i] + red[2][i] + red[3][i];
+}
+
+// The same statistics with the sample's member outputs loaded once into
+// registers (E <= 8, D <= 256 K; thread t holds d = t + 256 k): each
+// statistic is register arithmetic and one workgroup reduction, three
+// rounds in all, and every division is by a reciprocal formed once.
 template <int K>
-__global__ __launch_bounds__(64) void ensemble_stats_wave_kernel(const float* y, int E, int B, int D,
+__global__ __launch_bounds__(256) void ensemble_stats_reg_kernel(const float* y, int E, int B, int D,
                                                                  const double* w, double scale,
                                                                  const double* target, double* mean, double* dis,
                                                                  double* l1) {
   constexpr int EM = 8;
-  const int lane = threadIdx.x;
+  __shared__ double red_a[4][EM], red_m[4][1], red_d[4][2];
+  const int t = threadIdx.x;
   const int b = blockIdx.x;
-  float v[EM][K];
-  double wd[K];
+  const double inv_scale = 1.0 / scale;
+  double v[EM][K], wd[K];
 #pragma unroll
   for (int e = 0; e < EM; ++e)
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int d = lane + 64 * k;
-      v[e][k] = (e < E && d < D) ? y[((int64_t)e * B + b) * D + d] : 0.0f;
+      const int d = t + 256 * k;
+      v[e][k] = (e < E && d < D) ? fmax((double)y[((int64_t)e * B + b) * D + d] * inv_scale, 0.0) : 0.0;
     }
 #pragma unroll
-  for (int k = 0; k < K; ++k) wd[k] = lane + 64 * k < D ? w[lane + 64 * k] : 0.0;
-  double area[EM];
+  for (int k = 0; k < K; ++k) wd[k] = t + 256 * k < D ? w[t + 256 * k] : 0.0;
+  double inv_area[EM];
 #pragma unroll
   for (int e = 0; e < EM; ++e) {
     double p = 0.0;
 #pragma unroll
-    for (int k = 0; k < K; ++k) p += wd[k] * fmax((double)v[e][k] / scale, 0.0);
-    area[e] = p;
+    for (int k = 0; k < K; ++k) p += wd[k] * v[e][k];
+    inv_area[e] = p;
   }
+  sums256<EM>(inv_area, red_a);
 #pragma unroll
-  for (int e = 0; e < EM; ++e) area[e] = e < E ? wave_sum64(area[e]) : 1.0;
-  double mn[K];
-  double pm = 0.0;
+  for (int e = 0; e < EM; ++e) inv_area[e] = e < E ? 1.0 / inv_area[e] : 0.0;
+  double mn[K], pm[1] = {0.0};
+  const double inv_e = 1.0 / (double)E;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     double acc = 0.0;
 #pragma unroll
-    for (int e = 0; e < EM; ++e)
-      if (e < E) acc += fmax((double)v[e][k] / scale, 0.0) / area[e];
-    mn[k] = acc / (double)E;
-    pm += wd[k] * mn[k];
+    for (int e = 0; e < EM; ++e) acc += v[e][k] * inv_area[e];
+    mn[k] = acc * inv_e;
+    pm[0] += wd[k] * mn[k];
   }
-  const double marea = wave_sum64(pm);
-  double pd = 0.0, pl = 0.0;
+  sums256<1>(pm, red_m);
+  const double inv_marea = 1.0 / pm[0];
+  double pd[2] = {0.0, 0.0};
   double* mb = mean + (int64_t)b * D;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int d = lane + 64 * k;
-    const double m = mn[k] / marea;
+    const int d = t + 256 * k;
+    const double m = mn[k] * inv_marea;
     double s = 0.0;
 #pragma unroll
     for (int e = 0; e < EM; ++e)
-      if (e < E) s += fabs(fmax((double)v[e][k] / scale, 0.0) / area[e] - m);
-    pd += wd[k] * s;
+      if (e < E) s += fabs(v[e][k] * inv_area[e] - m);
+    pd[0] += wd[k] * s;
     if (d < D) {
       mb[d] = m;
-      if (target) pl += wd[k] * fabs(m - target[d]);
+      if (target) pd[1] += wd[k] * fabs(m - target[d]);
     }
   }
-  pd = wave_sum64(pd);
-  if (target) pl = wave_sum64(pl);
-  if (lane == 0) {
-    dis[b] = pd / (double)E;
-    if (target) l1[b] = pl;
+  sums256<2>(pd, red_d);
+  if (t == 0) {
+    dis[b] = pd[0] * inv_e;
+    if (target) l1[b] = pd[1];
   }
 }
 
@@ -422,6 +437,32 @@ void gemm_fwd(const GemmArgs& g, int E, hipStream_t s) {
 
 // ------------------------------------------------------------------ network state
 
+// Page-locked host memory mapped into the device's address space: kernels
+// read small per-call inputs from it and write small results into it, so
+// they move without a copy dispatch.  Every call that uses it synchronizes
+// its stream before returning, so the next call may overwrite it.
+struct HostIO {
+  void* h = nullptr;  // host address
+  void* d = nullptr;  // device address of the same memory
+  size_t bytes = 0;
+  void reserve(size_t n) {
+    if (n <= bytes) return;
+    release();
+    HIP_CHECK(hipHostMalloc(&h, n, hipHostMallocDefault));
+    HIP_CHECK(hipHostGetDevicePointer(&d, h, 0));
+    bytes = n;
+  }
+  void release() {
+    if (h) (void)hipHostFree(h);
+    h = d = nullptr;
+    bytes = 0;
+  }
+  HostIO() = default;
+  HostIO(const HostIO&) = delete;
+  HostIO& operator=(const HostIO&) = delete;
+  ~HostIO() { release(); }
+};
+
 struct Mlp {
   int E = 0, L = 0, max_batch = 0;
   std::vector<int> dims;            // L + 1
@@ -434,8 +475,8 @@ struct Mlp {
   bool acts = false;
   DevBuf xb, yb, out, loss, idx;    // minibatch input bf16, targets fp32, output fp32 [E][B][dL]
   DevBuf dX, dY;                    // data set (fp32)
-  DevBuf sw, smean, sdis;           // ensemble statistics: area weights, mean [B][dL], disagreement [B]
-  DevBuf st, sl1;                   // misfit: target [dL], l1 [B]
+  DevBuf sw, smean, st;             // ensemble statistics: area weights, mean [B][dL], misfit target [dL]
+  HostIO xin, sout;                 // batch input x [B][d0] (fp32); disagreement [B] and misfit [B]
   std::vector<double> sw_host, st_host;  // the area weights and target last uploaded
   int64_t n_data = 0;
 };
@@ -550,14 +591,14 @@ void backward(Mlp& net, int B, hipStream_t s) {
   }
 }
 
+// The conversion kernel reads x from the mapped input buffer (no copy dispatch).
 void upload_batch_input(Mlp& net, const float* x_host, int B, hipStream_t s) {
-  DevBuf tmp;
-  h2d(tmp, x_host, (size_t)B * net.dims[0], s);
   const int64_t n = (int64_t)B * net.dims[0];
-  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, tmp.as<float>(), n,
-                     net.xb.as<uint16_t>());
+  net.xin.reserve(n * sizeof(float));
+  std::memcpy(net.xin.h, x_host, n * sizeof(float));
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, static_cast<const float*>(net.xin.d),
+                     n, net.xb.as<uint16_t>());
   HIP_CHECK(hipGetLastError());
-  // tmp goes back to the device-memory cache; later owners are ordered after the conversion
 }
 
 }  // namespace
@@ -649,27 +690,28 @@ void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, dou
   if (target_host) upload_on_change(net.st, net.st_host, target_host, D, s);
   const double* tgt = target_host ? net.st.as<double>() : nullptr;
   net.smean.reserve((size_t)B * D * sizeof(double));
-  net.sdis.reserve((size_t)B * sizeof(double));
-  net.sl1.reserve((size_t)B * sizeof(double));
+  net.sout.reserve((size_t)2 * B * sizeof(double));
   const float* y = net.out.as<float>();
-  double *m = net.smean.as<double>(), *dis = net.sdis.as<double>(), *l1 = net.sl1.as<double>();
+  // disagreement and misfit are written straight into the mapped result buffer
+  double *m = net.smean.as<double>(), *dis = static_cast<double*>(net.sout.d), *l1 = dis + B;
   const double* w = net.sw.as<double>();
   if (net.E <= 8 && D <= 256)
-    hipLaunchKernelGGL(ensemble_stats_wave_kernel<4>, dim3(B), dim3(64), 0, s, y, net.E, B, D, w, scale, tgt, m, dis,
+    hipLaunchKernelGGL(ensemble_stats_reg_kernel<1>, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis,
                        l1);
   else if (net.E <= 8 && D <= 512)
-    hipLaunchKernelGGL(ensemble_stats_wave_kernel<8>, dim3(B), dim3(64), 0, s, y, net.E, B, D, w, scale, tgt, m, dis,
+    hipLaunchKernelGGL(ensemble_stats_reg_kernel<2>, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis,
                        l1);
   else if (net.E <= 8 && D <= 1024)
-    hipLaunchKernelGGL(ensemble_stats_wave_kernel<16>, dim3(B), dim3(64), 0, s, y, net.E, B, D, w, scale, tgt, m,
-                       dis, l1);
+    hipLaunchKernelGGL(ensemble_stats_reg_kernel<4>, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis,
+                       l1);
   else
     hipLaunchKernelGGL(ensemble_stats_kernel, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis, l1);
   HIP_CHECK(hipGetLastError());
   if (mean_host) d2h(mean_host, net.smean.p, (size_t)B * D, s);
-  d2h(dis_host, net.sdis.p, B, s);
-  if (target_host && l1_host) d2h(l1_host, net.sl1.p, B, s);
   stream_sync(s);
+  const double* res = static_cast<const double*>(net.sout.h);
+  std::memcpy(dis_host, res, B * sizeof(double));
+  if (target_host && l1_host) std::memcpy(l1_host, res + B, B * sizeof(double));
 }
 
 void mlp_gradients(Mlp& net, const float* x_host, const float* y_host, int B, float* g_host, double* loss,
