@@ -36,6 +36,7 @@ DEFAULTS = {
     "lanczos_graph_steps": 16,
     "eigen_lds_vectors": -1,
     "small_sort": 1,
+    "mlp_graph": 0,
 }
 
 CANDIDATES = {

@@ -22,6 +22,7 @@ namespace imped {
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // round-to-nearest-even float -> bfloat16 (NaN stays NaN)
@@ -55,6 +56,7 @@ struct GemmArgs {
   const float* bias;  // [N] per member
   int64_t bias_z;
   const float* aux;  // EPI_DGELU: pre-activation, layout of C
+  int vec;           // gemm_fwd_kernel: K, lda, ldb, a_z, b_z multiples of 8, A and B 16-byte aligned
 };
 
 constexpr int TM = 64, TN = 64, TK = 32, LK = TK + 8;
@@ -178,17 +180,51 @@ __global__ __launch_bounds__(256) void gemm_fwd_kernel(GemmArgs g) {
     const int kc8 = ((g.K - k0 < FK ? g.K - k0 : FK) + 7) & ~7;  // MFMA steps of 8, zero padded
     const int per_row = kc8 / 8;
     const int chunks = TM * per_row;
+    if (g.vec) {
+      // every load of the stage issued before the first LDS write; rows past
+      // M or N read the last row instead (their outputs are not stored), so
+      // no load waits behind a branch
+      // (chunk indices past the stage repeat its last chunk: those threads
+      // write the same values to the same LDS slot, so nothing branches)
+      uint4 ra[8], rb[8];
+      int slot[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int c = t + 256 * i < chunks ? t + 256 * i : chunks - 1;
+        const int r = c / per_row, k8 = (c - r * per_row) * 8;
+        slot[i] = r * FLK + k8;
+        const int ga = m0 + r < g.M ? m0 + r : g.M - 1, gb = n0 + r < g.N ? n0 + r : g.N - 1;
+        ra[i] = *reinterpret_cast<const uint4*>(A + (int64_t)ga * g.lda + k0 + k8);
+        rb[i] = *reinterpret_cast<const uint4*>(B + (int64_t)gb * g.ldb + k0 + k8);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        *reinterpret_cast<uint4*>(&As[0][0] + slot[i]) = ra[i];
+        *reinterpret_cast<uint4*>(&Bs[0][0] + slot[i]) = rb[i];
+      }
+    } else {
 #pragma unroll 8
-    for (int c = t; c < chunks; c += 256) {
-      const int r = c / per_row, k8 = (c - r * per_row) * 8;
-      const uint4 va = load8(A, g.lda, g.M, g.K, m0 + r, k0 + k8);
-      const uint4 vb = load8(B, g.ldb, g.N, g.K, n0 + r, k0 + k8);
-      *reinterpret_cast<uint4*>(&As[r][k8]) = va;
-      *reinterpret_cast<uint4*>(&Bs[r][k8]) = vb;
+      for (int c = t; c < chunks; c += 256) {
+        const int r = c / per_row, k8 = (c - r * per_row) * 8;
+        const uint4 va = load8(A, g.lda, g.M, g.K, m0 + r, k0 + k8);
+        const uint4 vb = load8(B, g.ldb, g.N, g.K, n0 + r, k0 + k8);
+        *reinterpret_cast<uint4*>(&As[r][k8]) = va;
+        *reinterpret_cast<uint4*>(&Bs[r][k8]) = vb;
+      }
     }
     __syncthreads();
-    for (int kk = 0; kk < kc8; kk += 8) {
-      // lane l supplies row l%32 and k = kk + 4*(l/32) .. +3 of each operand
+    int kk = 0;
+    for (; kk + 16 <= kc8; kk += 16) {
+      // 16-deep steps (gfx950): lane l supplies row l%32 and k = kk + 8*(l/32) .. +7
+      // of each operand; both operands use the same k assignment
+      const int kq = kk + 8 * (lane >> 5);
+      const uint4 a = *reinterpret_cast<const uint4*>(&As[wm + (lane & 31)][kq]);
+      const uint4 b = *reinterpret_cast<const uint4*>(&Bs[wn + (lane & 31)][kq]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                    0, 0, 0);
+    }
+    if (kk < kc8) {
+      // an odd 8-deep step: lane l supplies row l%32 and k = kk + 4*(l/32) .. +3
       const int kq = kk + 4 * (lane >> 5);
       const bf16x4 a = *reinterpret_cast<const bf16x4*>(&As[wm + (lane & 31)][kq]);
       const bf16x4 b = *reinterpret_cast<const bf16x4*>(&Bs[wn + (lane & 31)][kq]);
@@ -421,13 +457,21 @@ void gemm(const GemmArgs& g, int E, hipStream_t s) {
 
 // forward layer (X W^T + bias, GELU or not) by the whole-K kernel
 template <int EPI>
-void gemm_fwd(const GemmArgs& g, int E, hipStream_t s) {
+void fwd_attr() {
   static bool attr = false;
   if (!attr) {
     HIP_CHECK(hipFuncSetAttribute((const void*)gemm_fwd_kernel<EPI>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)FWD_LDS));
     attr = true;
   }
+}
+
+template <int EPI>
+void gemm_fwd(const GemmArgs& g0, int E, hipStream_t s) {
+  fwd_attr<EPI>();
+  GemmArgs g = g0;
+  g.vec = g.K % 8 == 0 && g.lda % 8 == 0 && g.ldb % 8 == 0 && g.a_z % 8 == 0 && g.b_z % 8 == 0 &&
+          (reinterpret_cast<uintptr_t>(g.A) & 15) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0;
   const dim3 grid(blocks_for(g.N, TN), blocks_for(g.M, TM), E);
   hipLaunchKernelGGL((gemm_fwd_kernel<EPI>), grid, dim3(256), FWD_LDS, s, g);
   HIP_CHECK(hipGetLastError());
@@ -478,6 +522,19 @@ struct Mlp {
   DevBuf sw, smean, st;             // ensemble statistics: area weights, mean [B][dL], misfit target [dL]
   HostIO xin, sout;                 // batch input x [B][d0] (fp32); disagreement [B] and misfit [B]
   std::vector<double> sw_host, st_host;  // the area weights and target last uploaded
+  // Prediction graphs (input conversion, forward GEMMs, statistics): one
+  // launch instead of six, so the GPU does not wait for the host between
+  // kernels.  Keyed by everything baked into the captured launches.
+  struct PredictGraph {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<PredictGraph> graphs;
+  hipStream_t cap = nullptr;  // capture stream
+  ~Mlp() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
   int64_t n_data = 0;
 };
 
@@ -591,14 +648,24 @@ void backward(Mlp& net, int B, hipStream_t s) {
   }
 }
 
-// The conversion kernel reads x from the mapped input buffer (no copy dispatch).
-void upload_batch_input(Mlp& net, const float* x_host, int B, hipStream_t s) {
+// x into the mapped input buffer; convert_batch_input launches the
+// conversion, which reads it there (no copy dispatch).
+void stage_batch_input(Mlp& net, const float* x_host, int B) {
   const int64_t n = (int64_t)B * net.dims[0];
   net.xin.reserve(n * sizeof(float));
   std::memcpy(net.xin.h, x_host, n * sizeof(float));
+}
+
+void convert_batch_input(Mlp& net, int B, hipStream_t s) {
+  const int64_t n = (int64_t)B * net.dims[0];
   hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, static_cast<const float*>(net.xin.d),
                      n, net.xb.as<uint16_t>());
   HIP_CHECK(hipGetLastError());
+}
+
+void upload_batch_input(Mlp& net, const float* x_host, int B, hipStream_t s) {
+  stage_batch_input(net, x_host, B);
+  convert_batch_input(net, B, s);
 }
 
 }  // namespace
@@ -680,17 +747,14 @@ void upload_on_change(DevBuf& dev, std::vector<double>& last, const double* host
 
 }  // namespace
 
-void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, double scale, const double* target_host,
-                 double* mean_host, double* dis_host, double* l1_host, hipStream_t s) {
-  alloc_acts(net, B);
-  upload_batch_input(net, x_host, B, s);
+namespace {
+
+// Conversion, forward pass and ensemble statistics of B staged samples.
+void enqueue_predict(Mlp& net, int B, double scale, bool with_target, hipStream_t s) {
+  convert_batch_input(net, B, s);
   forward(net, B, s);
   const int D = net.dims[net.L];
-  upload_on_change(net.sw, net.sw_host, w_host, D, s);
-  if (target_host) upload_on_change(net.st, net.st_host, target_host, D, s);
-  const double* tgt = target_host ? net.st.as<double>() : nullptr;
-  net.smean.reserve((size_t)B * D * sizeof(double));
-  net.sout.reserve((size_t)2 * B * sizeof(double));
+  const double* tgt = with_target ? net.st.as<double>() : nullptr;
   const float* y = net.out.as<float>();
   // disagreement and misfit are written straight into the mapped result buffer
   double *m = net.smean.as<double>(), *dis = static_cast<double*>(net.sout.d), *l1 = dis + B;
@@ -707,6 +771,64 @@ void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, dou
   else
     hipLaunchKernelGGL(ensemble_stats_kernel, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis, l1);
   HIP_CHECK(hipGetLastError());
+}
+
+// The captured enqueue_predict for this call's sizes and buffers.
+hipGraphExec_t predict_graph(Mlp& net, int B, double scale, bool with_target) {
+  uint64_t sbits;
+  std::memcpy(&sbits, &scale, sizeof(sbits));
+  std::vector<uintptr_t> key = {(uintptr_t)B, (uintptr_t)sbits, (uintptr_t)with_target,
+                                (uintptr_t)net.xin.d, (uintptr_t)net.sout.d, (uintptr_t)net.smean.p,
+                                (uintptr_t)net.sw.p, (uintptr_t)net.st.p, (uintptr_t)net.xb.p,
+                                (uintptr_t)net.wb.p, (uintptr_t)net.w.p, (uintptr_t)net.out.p};
+  for (int l = 0; l < net.L; ++l) {
+    key.push_back((uintptr_t)net.zf[l].p);
+    key.push_back((uintptr_t)net.hb[l].p);
+  }
+  for (const auto& g : net.graphs)
+    if (g.key == key) return g.exec;
+  if (!net.cap) HIP_CHECK(hipStreamCreateWithFlags(&net.cap, hipStreamNonBlocking));
+  fwd_attr<EPI_BIAS_GELU>();  // kernel attributes are set outside the capture
+  fwd_attr<EPI_BIAS>();
+  HIP_CHECK(hipStreamBeginCapture(net.cap, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    enqueue_predict(net, B, scale, with_target, net.cap);
+  } catch (...) {
+    (void)hipStreamEndCapture(net.cap, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HIP_CHECK(hipStreamEndCapture(net.cap, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIP_CHECK(ie);
+  if (net.graphs.size() >= 8) {
+    // every call synchronizes before returning: no launch of the oldest graph is in flight
+    HIP_CHECK(hipGraphExecDestroy(net.graphs.front().exec));
+    net.graphs.erase(net.graphs.begin());
+  }
+  net.graphs.push_back(Mlp::PredictGraph{key, exec});
+  return exec;
+}
+
+}  // namespace
+
+void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, double scale, const double* target_host,
+                 double* mean_host, double* dis_host, double* l1_host, hipStream_t s) {
+  alloc_acts(net, B);
+  stage_batch_input(net, x_host, B);
+  const int D = net.dims[net.L];
+  upload_on_change(net.sw, net.sw_host, w_host, D, s);
+  if (target_host) upload_on_change(net.st, net.st_host, target_host, D, s);
+  net.smean.reserve((size_t)B * D * sizeof(double));
+  net.sout.reserve((size_t)2 * B * sizeof(double));
+  // per-kernel timing needs direct launches
+  if (timing_enabled() || !g_tuning.mlp_graph)
+    enqueue_predict(net, B, scale, target_host != nullptr, s);
+  else
+    HIP_CHECK(hipGraphLaunch(predict_graph(net, B, scale, target_host != nullptr), s));
   if (mean_host) d2h(mean_host, net.smean.p, (size_t)B * D, s);
   stream_sync(s);
   const double* res = static_cast<const double*>(net.sout.h);

@@ -66,7 +66,10 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * "eigen_lds_vectors" (cap on the Krylov vectors the eigensolver keeps in
  * LDS; -1 as many as fit, 0 none), "small_sort" (1: key sets of at most
  * 16384 records whose chain, key and position fit 63 bits are sorted by one
- * workgroup; 0: always the digit passes). */
+ * workgroup; 0: always the digit passes), "mlp_graph" (1: a surrogate
+ * prediction's kernels run as one replayed HIP graph, which shortens the
+ * GPU's span but costs the calling thread more than the launches it
+ * replaces; 0, the default: direct launches, as with switched-on timing). */
 int imped_set_tuning(const char* name, int64_t value);
 
 /* The engine's device radix sort (the one that orders BFS layers and

@@ -146,6 +146,33 @@ def test_surrogate_rows_independent_of_batch(gpu):
     assert np.array_equal(m2, mean[rows]) and np.array_equal(d2, dis[rows])
 
 
+def test_predict_graph_replays_match_direct_launches(gpu):
+    """The prediction runs as a replayed HIP graph when kernel timing is off
+    and as direct launches when it is on: every replay, with other work and
+    batch-size changes in between, equals the direct launches' results."""
+    from impuritymodel_amd import engine
+    from impuritymodel_amd.surrogate import OMEGA, SCALE, Ensemble, trapezoid_weights
+
+    net = Ensemble(5, (8, 512, 512, 512, len(OMEGA)), max_batch=1024, seed=9)
+    rng = np.random.default_rng(10)
+    w = trapezoid_weights(OMEGA)
+    target = np.abs(np.cos(OMEGA)) + 0.2
+    target /= target @ w
+    try:
+        for B in (256, 37, 256, 256, 300):
+            x = rng.uniform(-1, 1, (B, 8)).astype(np.float32)
+            engine.timing_enable(True)
+            d0, l0, m0 = net.misfit(x, w, SCALE, target, want_mean=True)
+            engine.timing_enable(False)
+            net.forward(x[:5])  # unrelated work between the replays
+            d1, l1, m1 = net.misfit(x, w, SCALE, target, want_mean=True)
+            d2, l2 = net.misfit(x, w, SCALE, target)
+            assert np.array_equal(d0, d1) and np.array_equal(l0, l1) and np.array_equal(m0, m1)
+            assert np.array_equal(d0, d2) and np.array_equal(l0, l2)
+    finally:
+        engine.timing_enable(False)
+
+
 def test_fitting_loop_device_misfit_matches_host(gpu):
     """The fitting loop's decisions from device misfits follow the same walk
     as decisions from spectra read back and compared on the host."""
