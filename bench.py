@@ -22,8 +22,15 @@ the same proposals give the plain GPU ED rate.
 ``--workload batched128``: 128 seeded models with bath sizes 10-50 (26-66
 spin-orbitals) per step, exact path only (the batched-ED configuration).
 
+The timed loops run as the product does, without kernel timing; each is
+then repeated with the same seeds (the same walk) with a HIP-event pair
+around every kernel launch, which gives the kernels' average durations.
+A trusted proposal's misfit is computed on the device, so the served
+spectra stay there during the loop; they are predicted again for scoring
+(a sample's row does not depend on the rest of its batch).
+
 Prints one JSON line with a roofline figure for the dominant kernel
-(HIP-event timed inside the timed steps) and a CPU baseline: the oracle (a
+(from the instrumented repeat) and a CPU baseline: the oracle (a
 pure Python/numpy restatement of the reference algorithm) on a bounded
 sample of the same spectra, one model per worker process.
 """
@@ -190,7 +197,10 @@ def run_fit(args, torch):
     def timed_loop(surrogate, tag, timing):
         # timing: per-kernel HIP events around direct launches (no graphs)
         engine.timing_enable(timing)
-        loop = S.FittingLoop(fam, target, surrogate, n_walkers=args.walkers, seed=1)
+        if surrogate is not None:  # walkers stepped inside the library, one call per step
+            loop = S.NativeFittingLoop(fam, target, surrogate, n_walkers=args.walkers, seed=1)
+        else:
+            loop = S.FittingLoop(fam, target, None, n_walkers=args.walkers, seed=1)
         for _ in range(args.warmup):
             loop.step()
         loop.finish()
@@ -283,7 +293,8 @@ def run_fit(args, torch):
             "workload": "fitting loop: %d Metropolis walkers over 8 parameters of a NiO-like 10-bath model "
                         "(26 spin-orbitals), one XAS per proposal (3 eigenstates, 3 polarizations, Lanczos 200, "
                         "%d omega points, delta %.2f eV, complex128); surrogate = 5 x MLP 8-512-512-512-%d "
-                        "(BF16 MFMA, FP32 accumulate), exact fallback by batched ED"
+                        "(BF16 MFMA, FP32 accumulate), exact fallback by batched ED; walkers stepped in libimped "
+                        "(imped_walkers_step: proposals, one prediction with device misfits, decisions)"
                         % (args.walkers, len(S.OMEGA), S.DELTA, len(S.OMEGA)),
             "walkers": args.walkers,
             "spectra_timed": int(loop_s.n_spectra),
@@ -603,7 +614,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default: 50 fitting-loop steps, 20 steps of the other workloads)")
+                    help="timed steps (default: 200 fitting-loop steps, 20 steps of the other workloads)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=("fit", "batched128", "large", "rixs"), default="fit")
     ap.add_argument("--models", type=int, default=128)
@@ -618,7 +629,7 @@ def main():
                     help="copy this many MiB device-to-device first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = 50 if args.workload == "fit" else 20
+        args.steps = 200 if args.workload == "fit" else 20
     if args.gpus != 1:
         print("bench.py measures one GPU only: the engine runs on a single MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)

@@ -70,6 +70,14 @@ SIGNATURES = {
     "imped_mlp_forward": [_p, _p, _i32, _p],
     "imped_mlp_predict": [_p, _p, _i32, _p, _d, _p, _p],
     "imped_mlp_misfit": [_p, _p, _i32, _p, _d, _p, _p, _p, _p],
+    "imped_walkers_create": [_p, _i32, _i32, _p, _p, _p, _p, _d, _d, _d, _u64, _p],
+    "imped_walkers_destroy": [_p],
+    "imped_walkers_step": [_p, _p, _d, _p, _p],
+    "imped_walkers_waiting": [_p, _p, _p, _p],
+    "imped_walkers_decide": [_p, _i32, _p, _p],
+    "imped_walkers_state": [_p, _p, _p, _p, _p],
+    "imped_walkers_served": [_p, _i64, _i64, _p],
+    "imped_walkers_reset_counters": [_p],
     "imped_pinned_alloc": [_i64, _p],
     "imped_pinned_free": [_p],
     "imped_mlp_gradients": [_p, _p, _p, _i32, _p, _p],

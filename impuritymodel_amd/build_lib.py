@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libimped.so")
-SOURCES = ["apply.hip", "prims.hip", "krylov.hip", "vecops.hip", "mlp.hip", "engine.hip"]
+SOURCES = ["apply.hip", "prims.hip", "krylov.hip", "vecops.hip", "mlp.hip", "walkers.hip", "engine.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3",

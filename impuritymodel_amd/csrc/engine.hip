@@ -27,6 +27,7 @@
 #include "kernels.h"
 #include "krylov.h"
 #include "mlp.h"
+#include "walkers.h"
 
 namespace imped {
 
@@ -1508,6 +1509,126 @@ int imped_mlp_predict(imped_mlp net, const float* x, int batch, const double* we
   }
   mlp_predict(*reinterpret_cast<Mlp*>(net), x, batch, weights, scale, nullptr, mean, disagreement, nullptr, stream());
   API_END
+}
+
+int imped_walkers_create(imped_mlp net, int n_walkers, int n_params, const double* theta, const double* cur,
+                         const double* lo, const double* hi, double step, double temperature, double threshold,
+                         uint64_t seed, imped_walkers* out) {
+  API_BEGIN
+  if (!net || n_walkers < 1 || n_params < 1 || !theta || !cur || !lo || !hi || !(temperature > 0.0) || !out) {
+    set_error("imped_walkers_create: need a network, n_walkers, n_params >= 1, states, a box and temperature > 0");
+    return IMPED_ERR_INVALID;
+  }
+  Mlp* m = reinterpret_cast<Mlp*>(net);
+  int32_t dims[17];
+  const int L = mlp_layers(*m, dims);
+  if (dims[0] != n_params) {
+    set_error("imped_walkers_create: n_params must equal the network's input width");
+    return IMPED_ERR_INVALID;
+  }
+  for (int j = 0; j < n_params; ++j)
+    if (!(hi[j] > lo[j])) {
+      set_error("imped_walkers_create: every box needs hi > lo");
+      return IMPED_ERR_INVALID;
+    }
+  (void)L;
+  auto w = std::make_unique<Walkers>();
+  w->net = m;
+  w->W = n_walkers;
+  w->P = n_params;
+  w->theta.assign(theta, theta + (size_t)n_walkers * n_params);
+  w->prop = w->theta;
+  w->cur.assign(cur, cur + n_walkers);
+  w->lo.assign(lo, lo + n_params);
+  w->hi.assign(hi, hi + n_params);
+  w->span.resize(n_params);
+  for (int j = 0; j < n_params; ++j) w->span[j] = step * (hi[j] - lo[j]);
+  w->waiting.assign(n_walkers, 0);
+  w->temperature = temperature;
+  w->threshold = threshold;
+  w->best = *std::min_element(w->cur.begin(), w->cur.end());
+  w->rng.state = seed;
+  *out = reinterpret_cast<imped_walkers>(w.release());
+  API_END
+}
+
+int imped_walkers_destroy(imped_walkers w) {
+  delete reinterpret_cast<Walkers*>(w);
+  return IMPED_OK;
+}
+
+int imped_walkers_step(imped_walkers wk, const double* weights, double scale, const double* target,
+                       int32_t* n_waiting) {
+  API_BEGIN
+  if (!wk || !weights || !target || !(scale > 0.0)) {
+    set_error("imped_walkers_step: walkers, weights, target and scale > 0 required");
+    return IMPED_ERR_INVALID;
+  }
+  Walkers& w = *reinterpret_cast<Walkers*>(wk);
+  const int nw = walkers_step(w, weights, scale, target, stream());
+  if (n_waiting) *n_waiting = nw;
+  API_END
+}
+
+int imped_walkers_waiting(imped_walkers wk, int32_t* idx, double* proposals, int32_t* count) {
+  API_BEGIN
+  const Walkers& w = *reinterpret_cast<const Walkers*>(wk);
+  int k = 0;
+  for (int i = 0; i < w.W; ++i)
+    if (w.waiting[i]) {
+      if (idx) idx[k] = i;
+      if (proposals)
+        std::copy(w.prop.begin() + (size_t)i * w.P, w.prop.begin() + (size_t)(i + 1) * w.P,
+                  proposals + (size_t)k * w.P);
+      ++k;
+    }
+  *count = k;
+  API_END
+}
+
+int imped_walkers_decide(imped_walkers wk, int32_t count, const int32_t* idx, const double* misfit) {
+  API_BEGIN
+  Walkers& w = *reinterpret_cast<Walkers*>(wk);
+  for (int k = 0; k < count; ++k)
+    if (idx[k] < 0 || idx[k] >= w.W || !w.waiting[idx[k]] || (k > 0 && idx[k] <= idx[k - 1])) {
+      set_error("imped_walkers_decide: indices must be waiting walkers in ascending order");
+      return IMPED_ERR_INVALID;
+    }
+  walkers_decide(w, count, idx, misfit);
+  API_END
+}
+
+int imped_walkers_state(imped_walkers wk, double* theta, double* cur, double* best, int64_t* counters) {
+  API_BEGIN
+  const Walkers& w = *reinterpret_cast<const Walkers*>(wk);
+  if (theta) std::copy(w.theta.begin(), w.theta.end(), theta);
+  if (cur) std::copy(w.cur.begin(), w.cur.end(), cur);
+  if (best) *best = w.best;
+  if (counters) {
+    counters[0] = w.n_predicted;
+    counters[1] = w.n_served;
+    counters[2] = w.n_accepted;
+    counters[3] = w.n_waiting;
+  }
+  API_END
+}
+
+int imped_walkers_served(imped_walkers wk, int64_t start, int64_t count, double* params) {
+  API_BEGIN
+  const Walkers& w = *reinterpret_cast<const Walkers*>(wk);
+  if (start < 0 || count < 0 || start + count > w.n_served) {
+    set_error("imped_walkers_served: rows out of range");
+    return IMPED_ERR_INVALID;
+  }
+  std::copy(w.served.begin() + start * w.P, w.served.begin() + (start + count) * w.P, params);
+  API_END
+}
+
+int imped_walkers_reset_counters(imped_walkers wk) {
+  Walkers& w = *reinterpret_cast<Walkers*>(wk);
+  w.n_predicted = w.n_served = w.n_accepted = 0;
+  w.served.clear();
+  return IMPED_OK;
 }
 
 int imped_mlp_misfit(imped_mlp net, const float* x, int batch, const double* weights, double scale,

@@ -465,13 +465,14 @@ class FittingLoop:
     """
 
     def __init__(self, family, target, surrogate=None, n_walkers=256, step=0.05, temperature=0.02, seed=0,
-                 flush_at=None, keep_spectra=False, **xas_kw):
+                 flush_at=None, keep_spectra=False, rng=None, **xas_kw):
         self.family, self.target, self.surrogate = family, np.asarray(target), surrogate
         self.target_n = area_normalize(self.target)
         self.keep_spectra = keep_spectra
         self.step_size, self.temperature, self.xas_kw = step, temperature, xas_kw
         self.flush_at = n_walkers // 2 if flush_at is None else flush_at
-        self.rng = np.random.default_rng(seed)
+        # rng: anything with normal(size=) and uniform(size=) (numpy's generator by default)
+        self.rng = np.random.default_rng(seed) if rng is None else rng
         self.lo, self.hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
         self.reset_counters()
         self.theta = sample_params(n_walkers, seed=seed + 1)
@@ -549,6 +550,132 @@ class FittingLoop:
         chunk = 1024
         return params, np.concatenate([self.surrogate.predict(params[i:i + chunk])[0]
                                        for i in range(0, len(params), chunk)])
+
+
+class NativeFittingLoop:
+    """:class:`FittingLoop` with a surrogate, its walkers stepped inside
+    libimped (``imped_walkers_*``): one library call per step proposes for
+    every walker that is not waiting, predicts all proposals with their
+    misfits on the device and takes the Metropolis decisions.  Waiting
+    proposals are evaluated by batched exact ED here, as in FittingLoop.
+
+    The rules, the order in which random numbers are used and the arithmetic
+    are FittingLoop's; the numbers come from the library's SplitMix64 stream
+    (seeded with ``seed``): standard normals by Box-Muller from two uniforms
+    each, one uniform per decision.  FittingLoop driven by the same stream
+    (its ``rng`` argument) walks the same way.
+    """
+
+    def __init__(self, family, target, surrogate, n_walkers=256, step=0.05, temperature=0.02, seed=0,
+                 flush_at=None, **xas_kw):
+        from ._lib import C, check, lib, ptr
+
+        self._lib, self._check, self._ptr = lib, check, ptr
+        self.family, self.target, self.surrogate = family, np.asarray(target), surrogate
+        self.target_n = area_normalize(self.target)
+        self.xas_kw = xas_kw
+        self.n_walkers = int(n_walkers)
+        self.flush_at = n_walkers // 2 if flush_at is None else flush_at
+        self.n_exact = self.n_flushes = 0
+        self.exact_results = []
+        theta = np.ascontiguousarray(sample_params(n_walkers, seed=seed + 1))
+        cur = np.ascontiguousarray(l1_error(self._exact(theta), self.target[None]))
+        lo = np.ascontiguousarray(PARAM_RANGES[:, 0])
+        hi = np.ascontiguousarray(PARAM_RANGES[:, 1])
+        h = C.c_void_p()
+        check(lib.imped_walkers_create(surrogate.net.h, self.n_walkers, len(PARAM_NAMES), ptr(theta), ptr(cur),
+                                       ptr(lo), ptr(hi), float(step), float(temperature),
+                                       float(surrogate.threshold), int(seed) % (1 << 64), C.byref(h)))
+        self.h = h
+        self._wt = np.ascontiguousarray(_W_OMEGA, np.float64)
+        self._tg = np.ascontiguousarray(self.target_n, np.float64)
+        self._nw = np.zeros(1, np.int32)
+        self._step_args = (self.h, ptr(self._wt), float(SCALE), ptr(self._tg), ptr(self._nw))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        lib = getattr(self, "_lib", None)
+        if h and lib is not None:
+            lib.imped_walkers_destroy(h)
+        self.h = None
+
+    def _exact(self, P):
+        ex, res = exact_spectra(self.family, P, **self.xas_kw)
+        self.exact_results.append(res)
+        self.n_exact += len(P)
+        return ex
+
+    def reset_counters(self):
+        self.n_exact = self.n_flushes = 0
+        self.exact_results = []
+        self._check(self._lib.imped_walkers_reset_counters(self.h))
+
+    def _state(self):
+        theta = np.zeros((self.n_walkers, len(PARAM_NAMES)))
+        cur = np.zeros(self.n_walkers)
+        best = np.zeros(1)
+        cnt = np.zeros(4, np.int64)
+        p = self._ptr
+        self._check(self._lib.imped_walkers_state(self.h, p(theta), p(cur), p(best), p(cnt)))
+        return theta, cur, float(best[0]), cnt
+
+    @property
+    def theta(self):
+        return self._state()[0]
+
+    @property
+    def cur(self):
+        return self._state()[1]
+
+    @property
+    def best(self):
+        return self._state()[2]
+
+    @property
+    def n_predicted(self):
+        return int(self._state()[3][0])
+
+    @property
+    def n_surrogate(self):
+        return int(self._state()[3][1])
+
+    @property
+    def n_spectra(self):
+        return self.n_surrogate + self.n_exact
+
+    def step(self):
+        self._check(self._lib.imped_walkers_step(*self._step_args))
+        if self._nw[0] >= self.flush_at:
+            self.flush()
+
+    def flush(self):
+        W, P = self.n_walkers, len(PARAM_NAMES)
+        idx = np.zeros(W, np.int32)
+        props = np.zeros((W, P))
+        cnt = np.zeros(1, np.int32)
+        p = self._ptr
+        self._check(self._lib.imped_walkers_waiting(self.h, p(idx), p(props), p(cnt)))
+        k = int(cnt[0])
+        if k:
+            # exact spectra come with unit area
+            mis = np.ascontiguousarray(np.abs(self._exact(props[:k]) - self.target_n[None]) @ _W_OMEGA)
+            self._check(self._lib.imped_walkers_decide(self.h, k, p(np.ascontiguousarray(idx[:k])), p(mis)))
+            self.n_flushes += 1
+
+    def finish(self):
+        self.flush()
+
+    def served(self):
+        """(params, spectra) the surrogate produced since the last counter reset;
+        the spectra are predicted again (rows do not depend on their batch)."""
+        n = self.n_surrogate
+        params = np.zeros((n, len(PARAM_NAMES)))
+        if n:
+            self._check(self._lib.imped_walkers_served(self.h, 0, n, self._ptr(params)))
+            chunk = 1024
+            return params, np.concatenate([self.surrogate.predict(params[i:i + chunk])[0]
+                                           for i in range(0, n, chunk)])
+        return params, np.zeros((0, len(OMEGA)))
 
 
 def fitting_loop(family, target, surrogate=None, n_walkers=256, n_iter=8, step=0.05, temperature=0.02, seed=0,

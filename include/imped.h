@@ -261,6 +261,37 @@ int imped_mlp_predict(imped_mlp net, const float* x, int batch, const double* we
  * reference counterpart. */
 int imped_mlp_misfit(imped_mlp net, const float* x, int batch, const double* weights, double scale,
                      const double* target, double* disagreement, double* misfit, double* mean);
+/* Metropolis walkers of a surrogate fitting loop over n_params parameters
+ * (the network's input width) in the box [lo, hi].  A step lets every walker
+ * that is not waiting propose p = theta + step (hi - lo) z (z standard
+ * normal; reflected at the box, then clipped), predicts all proposals in one
+ * imped_mlp_misfit call (inputs 2 (p - lo) / (hi - lo) - 1, misfit to
+ * target), and each trusted proposal (disagreement < threshold) is accepted
+ * with probability min(1, exp(-(misfit - cur) / temperature)); untrusted
+ * proposals make their walker wait until imped_walkers_decide answers them
+ * (misfits from exact evaluation, walkers in ascending order).  Random
+ * numbers: SplitMix64 from seed, two uniforms (top 53 bits) per normal by
+ * Box-Muller (cosine branch), one uniform per decision, consumed in walker
+ * order.  The host loop of a fitting run with one library call per step; no
+ * reference counterpart (the reference has no surrogate or fitting loop). */
+typedef struct imped_walkers_s* imped_walkers;
+int imped_walkers_create(imped_mlp net, int n_walkers, int n_params, const double* theta, const double* cur,
+                         const double* lo, const double* hi, double step, double temperature, double threshold,
+                         uint64_t seed, imped_walkers* out);
+int imped_walkers_destroy(imped_walkers w);
+/* One step; *n_waiting = walkers waiting for exact evaluation afterwards. */
+int imped_walkers_step(imped_walkers w, const double* weights, double scale, const double* target,
+                       int32_t* n_waiting);
+/* The waiting walkers (ascending) and their proposals [count][n_params]. */
+int imped_walkers_waiting(imped_walkers w, int32_t* idx, double* proposals, int32_t* count);
+int imped_walkers_decide(imped_walkers w, int32_t count, const int32_t* idx, const double* misfit);
+/* theta [n_walkers][n_params], cur [n_walkers], best misfit so far and the
+ * counters {predicted, served, accepted, waiting} (any may be NULL). */
+int imped_walkers_state(imped_walkers w, double* theta, double* cur, double* best, int64_t* counters);
+/* Rows [start, start + count) of the served proposals' parameters. */
+int imped_walkers_served(imped_walkers w, int64_t start, int64_t count, double* params);
+/* Zeroes the counters and forgets the served proposals. */
+int imped_walkers_reset_counters(imped_walkers w);
 /* Page-locked host memory (hipHostMalloc) for outputs read back every call:
  * device-to-host copies into it run at full link speed.  No reference
  * counterpart. */

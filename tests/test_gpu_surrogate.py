@@ -195,6 +195,38 @@ def test_fitting_loop_device_misfit_matches_host(gpu):
     assert np.array_equal(s1, s2)
 
 
+def test_native_walkers_follow_python_loop(gpu):
+    """The library's walkers (one call per step) take the walk of the Python
+    fitting loop driven by the same random stream: the same proposals served,
+    waiting and accepted, the same exact flushes, the same final states."""
+    from walker_rng import SplitMixStream
+
+    from impuritymodel_amd import surrogate as S
+
+    fam = S.XASFamily()
+    sur = S.Surrogate(fam, S.Ensemble(5, (8, 512, 512, 512, len(S.OMEGA)), max_batch=1024, seed=0))
+    _, dis = sur.predict(S.sample_params(256, seed=7))
+    sur.threshold = float(np.median(dis))  # about half the proposals wait for exact ED
+    target = np.abs(np.sin(S.OMEGA)) + 0.1
+    py = S.FittingLoop(fam, target, sur, n_walkers=48, seed=5, flush_at=12, rng=SplitMixStream(5))
+    nat = S.NativeFittingLoop(fam, target, sur, n_walkers=48, seed=5, flush_at=12)
+    for _ in range(6):
+        py.step()
+        nat.step()
+    py.finish()
+    nat.finish()
+    assert (nat.n_surrogate, nat.n_exact, nat.n_flushes, nat.n_predicted) == \
+        (py.n_surrogate, py.n_exact, py.n_flushes, py.n_predicted)
+    assert nat.n_surrogate > 0 and nat.n_flushes > 1
+    # Box-Muller through numpy's and the C library's log and cos may differ in the last bit
+    np.testing.assert_allclose(nat.theta, py.theta, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(nat.cur, py.cur, rtol=1e-9, atol=1e-15)
+    assert abs(nat.best - py.best) <= 1e-9 * py.best
+    (pp, ps), (np_, ns) = py.served(), nat.served()
+    np.testing.assert_allclose(np_, pp, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(ns, ps, rtol=1e-5, atol=1e-7 * np.abs(ps).max())
+
+
 def test_affine_family_operators_on_device(gpu):
     """Operators of a parameter batch formed on the device (c0 + theta @ cmat)
     give the spectra of per-model compiled tables; a second, smaller batch

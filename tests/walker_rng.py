@@ -1,0 +1,36 @@
+"""The random stream of libimped's fitting-loop walkers, restated in numpy
+with the interface of numpy's generator (normal(size=), uniform(size=)), so
+the Python fitting loop can be driven by the same numbers.
+
+SplitMix64 (Steele, Lea and Flood, OOPSLA 2014): state += 0x9E3779B97F4A7C15,
+output = the state through the 64-bit finalizer; uniforms are the top 53
+bits times 2^-53; each normal is Box-Muller's cosine branch of two uniforms
+(u1 = 1 - first, u2 = second).
+"""
+
+import numpy as np
+
+GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+class SplitMixStream:
+    def __init__(self, seed):
+        self.state = np.uint64(int(seed) % (1 << 64))
+
+    def next_u64(self, n):
+        s = self.state + GAMMA * np.arange(1, n + 1, dtype=np.uint64)
+        if n:
+            self.state = s[-1]
+        z = (s ^ (s >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+    def uniform(self, size):
+        n = int(np.prod(size))
+        return ((self.next_u64(n) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53).reshape(size)
+
+    def normal(self, size):
+        n = int(np.prod(size))
+        u = self.uniform(2 * n)
+        u1, u2 = 1.0 - u[0::2], u[1::2]
+        return (np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)).reshape(size)
