@@ -1548,6 +1548,7 @@ int imped_walkers_create(imped_mlp net, int n_walkers, int n_params, const doubl
   w->threshold = threshold;
   w->best = *std::min_element(w->cur.begin(), w->cur.end());
   w->rng.state = seed;
+  w->urng.state = ~seed;
   *out = reinterpret_cast<imped_walkers>(w.release());
   API_END
 }

@@ -20,6 +20,12 @@ void mlp_forward(Mlp& net, const float* x_host, int B, float* y_host, hipStream_
 // target_host, mean_host and l1_host may be NULL (no misfit; no read-back).
 void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, double scale, const double* target_host,
                  double* mean_host, double* dis_host, double* l1_host, hipStream_t s);
+// mlp_predict in two halves: _begin enqueues (x is consumed before it
+// returns), _end waits and reads the results; the caller may work between
+// them but must not start another call on the network.
+void mlp_predict_begin(Mlp& net, const float* x_host, int B, const double* w_host, double scale,
+                       const double* target_host, double* mean_host, hipStream_t s);
+void mlp_predict_end(Mlp& net, int B, bool with_target, double* dis_host, double* l1_host, hipStream_t s);
 void mlp_gradients(Mlp& net, const float* x_host, const float* y_host, int B, float* g_host, double* loss,
                    hipStream_t s);
 void mlp_set_data(Mlp& net, const float* X, const float* Y, int64_t n, hipStream_t s);

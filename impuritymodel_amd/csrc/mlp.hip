@@ -16,6 +16,7 @@
 #include <memory>
 
 #include "common.h"
+#include "mlp.h"
 
 namespace imped {
 
@@ -817,6 +818,12 @@ hipGraphExec_t predict_graph(Mlp& net, int B, double scale, bool with_target) {
 
 void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, double scale, const double* target_host,
                  double* mean_host, double* dis_host, double* l1_host, hipStream_t s) {
+  mlp_predict_begin(net, x_host, B, w_host, scale, target_host, mean_host, s);
+  mlp_predict_end(net, B, target_host != nullptr, dis_host, l1_host, s);
+}
+
+void mlp_predict_begin(Mlp& net, const float* x_host, int B, const double* w_host, double scale,
+                       const double* target_host, double* mean_host, hipStream_t s) {
   alloc_acts(net, B);
   stage_batch_input(net, x_host, B);
   const int D = net.dims[net.L];
@@ -830,10 +837,13 @@ void mlp_predict(Mlp& net, const float* x_host, int B, const double* w_host, dou
   else
     HIP_CHECK(hipGraphLaunch(predict_graph(net, B, scale, target_host != nullptr), s));
   if (mean_host) d2h(mean_host, net.smean.p, (size_t)B * D, s);
+}
+
+void mlp_predict_end(Mlp& net, int B, bool with_target, double* dis_host, double* l1_host, hipStream_t s) {
   stream_sync(s);
   const double* res = static_cast<const double*>(net.sout.h);
   std::memcpy(dis_host, res, B * sizeof(double));
-  if (target_host && l1_host) std::memcpy(l1_host, res + B, B * sizeof(double));
+  if (with_target && l1_host) std::memcpy(l1_host, res + B, B * sizeof(double));
 }
 
 void mlp_gradients(Mlp& net, const float* x_host, const float* y_host, int B, float* g_host, double* loss,

@@ -35,7 +35,12 @@ struct Walkers {
   std::vector<double> lo, hi, span;      // box and step * (hi - lo), [P]
   std::vector<uint8_t> waiting;          // [W]
   double temperature = 0.0, threshold = 0.0, best = 0.0;
-  SplitMix64 rng{0};
+  SplitMix64 rng{0};   // normals (proposals)
+  SplitMix64 urng{0};  // uniforms (decisions)
+  // normals drawn ahead, first in first out: the pool is refilled while the
+  // GPU predicts, and the k-th normal used is always the stream's k-th
+  std::vector<double> pool;
+  size_t pool_head = 0;
   std::vector<double> served;  // parameters of every surrogate-served proposal, [n][P]
   int64_t n_predicted = 0, n_served = 0, n_accepted = 0;
   int n_waiting = 0;

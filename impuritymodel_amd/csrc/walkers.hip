@@ -19,13 +19,22 @@ double SplitMix64::normal() {
 namespace {
 
 void decide_one(Walkers& w, int i, double misfit) {
-  const double u = w.rng.uniform();
+  const double u = w.urng.uniform();
   if (u < std::exp(-(misfit - w.cur[i]) / w.temperature)) {
     for (int j = 0; j < w.P; ++j) w.theta[(size_t)i * w.P + j] = w.prop[(size_t)i * w.P + j];
     w.cur[i] = misfit;
     w.best = std::min(w.best, misfit);
     ++w.n_accepted;
   }
+}
+
+// Tops the pool up to n unused normals.
+void refill(Walkers& w, size_t n) {
+  if (w.pool_head > 0 && w.pool_head >= w.pool.size() / 2) {
+    w.pool.erase(w.pool.begin(), w.pool.begin() + w.pool_head);
+    w.pool_head = 0;
+  }
+  while (w.pool.size() - w.pool_head < n) w.pool.push_back(w.rng.normal());
 }
 
 }  // namespace
@@ -38,13 +47,14 @@ int walkers_step(Walkers& w, const double* weights, double scale, const double* 
   const int nf = (int)w.free_idx.size();
   if (nf == 0) return w.n_waiting;
   w.x.resize((size_t)nf * P);
+  refill(w, (size_t)nf * P);
   // proposals theta + step (hi - lo) z, reflected into the box and clipped;
   // network inputs 2 (p - lo) / (hi - lo) - 1
   for (int k = 0; k < nf; ++k) {
     const int i = w.free_idx[k];
     for (int j = 0; j < P; ++j) {
       const double lo = w.lo[j], hi = w.hi[j];
-      const double z = w.rng.normal();
+      const double z = w.pool[w.pool_head++];
       double p = w.theta[(size_t)i * P + j] + w.span[j] * z;
       if (p < lo) p = 2.0 * lo - p;
       if (p > hi) p = 2.0 * hi - p;
@@ -55,7 +65,9 @@ int walkers_step(Walkers& w, const double* weights, double scale, const double* 
   }
   w.dis.resize(nf);
   w.mis.resize(nf);
-  mlp_predict(*w.net, w.x.data(), nf, weights, scale, target, nullptr, w.dis.data(), w.mis.data(), s);
+  mlp_predict_begin(*w.net, w.x.data(), nf, weights, scale, target, nullptr, s);
+  refill(w, (size_t)w.W * P);  // the next step's normals, while the GPU predicts
+  mlp_predict_end(*w.net, nf, true, w.dis.data(), w.mis.data(), s);
   w.n_predicted += nf;
   for (int k = 0; k < nf; ++k) {
     const int i = w.free_idx[k];

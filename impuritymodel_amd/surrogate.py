@@ -560,10 +560,11 @@ class NativeFittingLoop:
     proposals are evaluated by batched exact ED here, as in FittingLoop.
 
     The rules, the order in which random numbers are used and the arithmetic
-    are FittingLoop's; the numbers come from the library's SplitMix64 stream
-    (seeded with ``seed``): standard normals by Box-Muller from two uniforms
-    each, one uniform per decision.  FittingLoop driven by the same stream
-    (its ``rng`` argument) walks the same way.
+    are FittingLoop's; the numbers come from two SplitMix64 streams in the
+    library: standard normals (Box-Muller, two uniforms each) from the stream
+    seeded with ``seed``, drawn ahead while the GPU predicts, and one uniform
+    per decision from the stream seeded with ``~seed``.  FittingLoop driven
+    by the same streams (its ``rng`` argument) walks the same way.
     """
 
     def __init__(self, family, target, surrogate, n_walkers=256, step=0.05, temperature=0.02, seed=0,

@@ -270,10 +270,12 @@ int imped_mlp_misfit(imped_mlp net, const float* x, int batch, const double* wei
  * with probability min(1, exp(-(misfit - cur) / temperature)); untrusted
  * proposals make their walker wait until imped_walkers_decide answers them
  * (misfits from exact evaluation, walkers in ascending order).  Random
- * numbers: SplitMix64 from seed, two uniforms (top 53 bits) per normal by
- * Box-Muller (cosine branch), one uniform per decision, consumed in walker
- * order.  The host loop of a fitting run with one library call per step; no
- * reference counterpart (the reference has no surrogate or fitting loop). */
+ * numbers: two SplitMix64 streams, seeded with seed (normals: two uniforms,
+ * the top 53 bits of an output each, per normal by Box-Muller's cosine
+ * branch) and with ~seed (one uniform per decision), each consumed in walker
+ * order; normals are drawn ahead while the GPU predicts.  The host loop of a
+ * fitting run with one library call per step; no reference counterpart (the
+ * reference has no surrogate or fitting loop). */
 typedef struct imped_walkers_s* imped_walkers;
 int imped_walkers_create(imped_mlp net, int n_walkers, int n_params, const double* theta, const double* cur,
                          const double* lo, const double* hi, double step, double temperature, double threshold,

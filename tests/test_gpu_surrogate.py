@@ -199,7 +199,7 @@ def test_native_walkers_follow_python_loop(gpu):
     """The library's walkers (one call per step) take the walk of the Python
     fitting loop driven by the same random stream: the same proposals served,
     waiting and accepted, the same exact flushes, the same final states."""
-    from walker_rng import SplitMixStream
+    from walker_rng import WalkerStreams
 
     from impuritymodel_amd import surrogate as S
 
@@ -208,7 +208,7 @@ def test_native_walkers_follow_python_loop(gpu):
     _, dis = sur.predict(S.sample_params(256, seed=7))
     sur.threshold = float(np.median(dis))  # about half the proposals wait for exact ED
     target = np.abs(np.sin(S.OMEGA)) + 0.1
-    py = S.FittingLoop(fam, target, sur, n_walkers=48, seed=5, flush_at=12, rng=SplitMixStream(5))
+    py = S.FittingLoop(fam, target, sur, n_walkers=48, seed=5, flush_at=12, rng=WalkerStreams(5))
     nat = S.NativeFittingLoop(fam, target, sur, n_walkers=48, seed=5, flush_at=12)
     for _ in range(6):
         py.step()

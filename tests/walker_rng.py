@@ -1,6 +1,7 @@
-"""The random stream of libimped's fitting-loop walkers, restated in numpy
+"""The random streams of libimped's fitting-loop walkers, restated in numpy
 with the interface of numpy's generator (normal(size=), uniform(size=)), so
-the Python fitting loop can be driven by the same numbers.
+the Python fitting loop can be driven by the same numbers: normals from the
+stream seeded with seed, decision uniforms from the one seeded with ~seed.
 
 SplitMix64 (Steele, Lea and Flood, OOPSLA 2014): state += 0x9E3779B97F4A7C15,
 output = the state through the 64-bit finalizer; uniforms are the top 53
@@ -34,3 +35,17 @@ class SplitMixStream:
         u = self.uniform(2 * n)
         u1, u2 = 1.0 - u[0::2], u[1::2]
         return (np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)).reshape(size)
+
+
+class WalkerStreams:
+    """normal() from SplitMix64(seed), uniform() from SplitMix64(~seed)."""
+
+    def __init__(self, seed):
+        self._n = SplitMixStream(seed)
+        self._u = SplitMixStream(~int(seed) % (1 << 64))
+
+    def normal(self, size):
+        return self._n.normal(size)
+
+    def uniform(self, size):
+        return self._u.uniform(size)
