@@ -58,6 +58,7 @@ struct GemmArgs {
   int64_t bias_z;
   const float* aux;  // EPI_DGELU: pre-activation, layout of C
   int vec;           // gemm_fwd_kernel: K, lda, ldb, a_z, b_z multiples of 8, A and B 16-byte aligned
+  const float* Af;   // gemm_fwd_kernel<EPI, true>: A in FP32 (shared by the members), rounded to BF16 when staged
 };
 
 constexpr int TM = 64, TN = 64, TK = 32, LK = TK + 8;
@@ -163,13 +164,26 @@ __device__ __forceinline__ uint4 load8(const uint16_t* src, int64_t ld, int R, i
   return *reinterpret_cast<const uint4*>(v);
 }
 
-template <int EPI>
+// Eight FP32 values rounded to BF16, packed as the BF16 row chunk.
+__device__ __forceinline__ uint4 pack_bf16x8(const float4& a, const float4& b) {
+  uint4 r;
+  r.x = (uint32_t)f2bf(a.x) | ((uint32_t)f2bf(a.y) << 16);
+  r.y = (uint32_t)f2bf(a.z) | ((uint32_t)f2bf(a.w) << 16);
+  r.z = (uint32_t)f2bf(b.x) | ((uint32_t)f2bf(b.y) << 16);
+  r.w = (uint32_t)f2bf(b.z) | ((uint32_t)f2bf(b.w) << 16);
+  return r;
+}
+
+// AF: the A operand is FP32 (g.Af, vec path only; the predictions' first
+// layer reads the batch inputs straight from mapped host memory, so no
+// conversion launch precedes it).
+template <int EPI, bool AF>
 __global__ __launch_bounds__(256) void gemm_fwd_kernel(GemmArgs g) {
   extern __shared__ __align__(16) uint16_t fsm[];
   uint16_t(*As)[FLK] = reinterpret_cast<uint16_t(*)[FLK]>(fsm);
   uint16_t(*Bs)[FLK] = reinterpret_cast<uint16_t(*)[FLK]>(fsm + TM * FLK);
   const int z = blockIdx.z;
-  const uint16_t* A = g.A + (int64_t)z * g.a_z;
+  const uint16_t* A = AF ? nullptr : g.A + (int64_t)z * g.a_z;
   const uint16_t* B = g.B + (int64_t)z * g.b_z;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
@@ -188,6 +202,7 @@ __global__ __launch_bounds__(256) void gemm_fwd_kernel(GemmArgs g) {
       // (chunk indices past the stage repeat its last chunk: those threads
       // write the same values to the same LDS slot, so nothing branches)
       uint4 ra[8], rb[8];
+      float4 fa[AF ? 8 : 1][2];
       int slot[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
@@ -195,15 +210,22 @@ __global__ __launch_bounds__(256) void gemm_fwd_kernel(GemmArgs g) {
         const int r = c / per_row, k8 = (c - r * per_row) * 8;
         slot[i] = r * FLK + k8;
         const int ga = m0 + r < g.M ? m0 + r : g.M - 1, gb = n0 + r < g.N ? n0 + r : g.N - 1;
-        ra[i] = *reinterpret_cast<const uint4*>(A + (int64_t)ga * g.lda + k0 + k8);
+        if constexpr (AF) {
+          const float* pa = g.Af + (int64_t)ga * g.lda + k0 + k8;
+          fa[i][0] = *reinterpret_cast<const float4*>(pa);
+          fa[i][1] = *reinterpret_cast<const float4*>(pa + 4);
+        } else {
+          ra[i] = *reinterpret_cast<const uint4*>(A + (int64_t)ga * g.lda + k0 + k8);
+        }
         rb[i] = *reinterpret_cast<const uint4*>(B + (int64_t)gb * g.ldb + k0 + k8);
       }
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
+        if constexpr (AF) ra[i] = pack_bf16x8(fa[i][0], fa[i][1]);
         *reinterpret_cast<uint4*>(&As[0][0] + slot[i]) = ra[i];
         *reinterpret_cast<uint4*>(&Bs[0][0] + slot[i]) = rb[i];
       }
-    } else {
+    } else if constexpr (!AF) {
 #pragma unroll 8
       for (int c = t; c < chunks; c += 256) {
         const int r = c / per_row, k8 = (c - r * per_row) * 8;
@@ -457,11 +479,11 @@ void gemm(const GemmArgs& g, int E, hipStream_t s) {
 }
 
 // forward layer (X W^T + bias, GELU or not) by the whole-K kernel
-template <int EPI>
+template <int EPI, bool AF = false>
 void fwd_attr() {
   static bool attr = false;
   if (!attr) {
-    HIP_CHECK(hipFuncSetAttribute((const void*)gemm_fwd_kernel<EPI>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIP_CHECK(hipFuncSetAttribute((const void*)gemm_fwd_kernel<EPI, AF>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)FWD_LDS));
     attr = true;
   }
@@ -469,12 +491,19 @@ void fwd_attr() {
 
 template <int EPI>
 void gemm_fwd(const GemmArgs& g0, int E, hipStream_t s) {
-  fwd_attr<EPI>();
   GemmArgs g = g0;
+  const void* a = g.Af ? (const void*)g.Af : (const void*)g.A;
   g.vec = g.K % 8 == 0 && g.lda % 8 == 0 && g.ldb % 8 == 0 && g.a_z % 8 == 0 && g.b_z % 8 == 0 &&
-          (reinterpret_cast<uintptr_t>(g.A) & 15) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0;
+          (reinterpret_cast<uintptr_t>(a) & 15) == 0 && (reinterpret_cast<uintptr_t>(g.B) & 15) == 0;
   const dim3 grid(blocks_for(g.N, TN), blocks_for(g.M, TM), E);
-  hipLaunchKernelGGL((gemm_fwd_kernel<EPI>), grid, dim3(256), FWD_LDS, s, g);
+  if (g.Af) {
+    if (!g.vec) throw std::runtime_error("gemm_fwd: an FP32 operand needs K, strides and alignment in steps of 8");
+    fwd_attr<EPI, true>();
+    hipLaunchKernelGGL((gemm_fwd_kernel<EPI, true>), grid, dim3(256), FWD_LDS, s, g);
+  } else {
+    fwd_attr<EPI>();
+    hipLaunchKernelGGL((gemm_fwd_kernel<EPI, false>), grid, dim3(256), FWD_LDS, s, g);
+  }
   HIP_CHECK(hipGetLastError());
 }
 
@@ -560,13 +589,15 @@ void alloc_acts(Mlp& net, int B) {
 }
 
 // forward pass of every member on the BF16 minibatch in xb; output in out
-void forward(Mlp& net, int B, hipStream_t s) {
+// x_f32: the first layer reads FP32 inputs there (else the BF16 copy in xb)
+void forward(Mlp& net, int B, hipStream_t s, const float* x_f32 = nullptr) {
   const uint16_t* x = net.xb.as<uint16_t>();
   int64_t x_z = 0;  // the input is shared by all members
   for (int l = 0; l < net.L; ++l) {
     const int in = net.dims[l], outd = net.dims[l + 1];
     GemmArgs g{};
     g.A = x;
+    if (l == 0) g.Af = x_f32;
     g.lda = in;
     g.a_z = x_z;
     g.B = net.wb.as<uint16_t>() + net.w_off[l];
@@ -752,8 +783,12 @@ namespace {
 
 // Conversion, forward pass and ensemble statistics of B staged samples.
 void enqueue_predict(Mlp& net, int B, double scale, bool with_target, hipStream_t s) {
-  convert_batch_input(net, B, s);
-  forward(net, B, s);
+  if (net.dims[0] % 8 == 0) {
+    forward(net, B, s, static_cast<const float*>(net.xin.d));  // inputs read from the mapped buffer
+  } else {
+    convert_batch_input(net, B, s);
+    forward(net, B, s);
+  }
   const int D = net.dims[net.L];
   const double* tgt = with_target ? net.st.as<double>() : nullptr;
   const float* y = net.out.as<float>();
@@ -791,6 +826,8 @@ hipGraphExec_t predict_graph(Mlp& net, int B, double scale, bool with_target) {
   if (!net.cap) HIP_CHECK(hipStreamCreateWithFlags(&net.cap, hipStreamNonBlocking));
   fwd_attr<EPI_BIAS_GELU>();  // kernel attributes are set outside the capture
   fwd_attr<EPI_BIAS>();
+  fwd_attr<EPI_BIAS_GELU, true>();
+  fwd_attr<EPI_BIAS, true>();
   HIP_CHECK(hipStreamBeginCapture(net.cap, hipStreamCaptureModeThreadLocal));
   hipGraph_t graph = nullptr;
   try {

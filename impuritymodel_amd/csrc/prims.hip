@@ -201,16 +201,32 @@ __global__ __launch_bounds__(SMALL_SORT_THREADS) void small_sort_kernel(const ui
     code[i] = c;
   }
   __syncthreads();
+  // each stage's npow2/2 compare-exchange pairs are disjoint: a thread loads
+  // all of its pairs, then compares, then stores (one LDS round trip per
+  // stage).  Pair q of a stage: i = its index with a zero inserted at bit
+  // log2(j), partner i + j, ascending when (i & k) == 0.
+  constexpr int PPT = SMALL_SORT_MAX / 2 / SMALL_SORT_THREADS;  // pairs per thread at most
+  const int half = npow2 >> 1;
   for (int k = 2; k <= npow2; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < npow2; i += SMALL_SORT_THREADS) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const uint64_t a = code[i], b = code[ixj];
-          if ((a > b) == ((i & k) == 0)) {
-            code[i] = b;
-            code[ixj] = a;
-          }
+      uint64_t av[PPT], bv[PPT];
+      int iv[PPT];
+#pragma unroll
+      for (int u = 0; u < PPT; ++u) {
+        const int q = threadIdx.x + u * SMALL_SORT_THREADS;
+        const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+        iv[u] = i;
+        if (q < half) {
+          av[u] = code[i];
+          bv[u] = code[i + j];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < PPT; ++u) {
+        const int q = threadIdx.x + u * SMALL_SORT_THREADS;
+        if (q < half && (av[u] > bv[u]) == ((iv[u] & k) == 0)) {
+          code[iv[u]] = bv[u];
+          code[iv[u] + j] = av[u];
         }
       }
       __syncthreads();

@@ -24,8 +24,11 @@ struct SplitMix64 {
   }
   // the top 53 bits as a double in [0, 1)
   double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
-  // Box-Muller, cosine branch only: two uniforms per normal
+  // Box-Muller: two uniforms give two normals, the cosine one first, then
+  // the sine one
   double normal();
+  double spare = 0.0;
+  bool has_spare = false;
 };
 
 struct Walkers {

@@ -11,9 +11,16 @@
 namespace imped {
 
 double SplitMix64::normal() {
+  if (has_spare) {
+    has_spare = false;
+    return spare;
+  }
   const double u1 = 1.0 - uniform();  // (0, 1]
   const double u2 = uniform();
-  return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+  const double r = std::sqrt(-2.0 * std::log(u1)), a = 2.0 * M_PI * u2;
+  spare = r * std::sin(a);
+  has_spare = true;
+  return r * std::cos(a);
 }
 
 namespace {

@@ -561,7 +561,7 @@ class NativeFittingLoop:
 
     The rules, the order in which random numbers are used and the arithmetic
     are FittingLoop's; the numbers come from two SplitMix64 streams in the
-    library: standard normals (Box-Muller, two uniforms each) from the stream
+    library: standard normals (Box-Muller, two from each pair of uniforms) from the stream
     seeded with ``seed``, drawn ahead while the GPU predicts, and one uniform
     per decision from the stream seeded with ``~seed``.  FittingLoop driven
     by the same streams (its ``rng`` argument) walks the same way.

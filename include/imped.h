@@ -271,8 +271,8 @@ int imped_mlp_misfit(imped_mlp net, const float* x, int batch, const double* wei
  * proposals make their walker wait until imped_walkers_decide answers them
  * (misfits from exact evaluation, walkers in ascending order).  Random
  * numbers: two SplitMix64 streams, seeded with seed (normals: two uniforms,
- * the top 53 bits of an output each, per normal by Box-Muller's cosine
- * branch) and with ~seed (one uniform per decision), each consumed in walker
+ * the top 53 bits of an output each, give two normals by Box-Muller, the
+ * cosine one first) and with ~seed (one uniform per decision), each consumed in walker
  * order; normals are drawn ahead while the GPU predicts.  The host loop of a
  * fitting run with one library call per step; no reference counterpart (the
  * reference has no surrogate or fitting loop). */

@@ -22,7 +22,10 @@ def test_stream_uniforms_and_normals():
     z = SplitMixStream(43).normal((100000, 2))
     assert z.shape == (100000, 2)
     assert abs(z.mean()) < 0.02 and abs(z.std() - 1.0) < 0.02
-    # one normal per two uniforms, consumed in order
+    # two normals per two uniforms (cosine, then sine), the same sequence however it is drawn
     a, b = SplitMixStream(7), SplitMixStream(7)
     u = b.uniform(6)
-    np.testing.assert_array_equal(a.normal(3), np.sqrt(-2.0 * np.log(1.0 - u[0::2])) * np.cos(2.0 * np.pi * u[1::2]))
+    r, t = np.sqrt(-2.0 * np.log(1.0 - u[0::2])), 2.0 * np.pi * u[1::2]
+    ref = np.stack([r * np.cos(t), r * np.sin(t)], axis=1).ravel()
+    got = np.concatenate([a.normal(1), a.normal((2, 1)).ravel(), a.normal(3)])
+    np.testing.assert_array_equal(got, ref)

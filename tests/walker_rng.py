@@ -5,8 +5,9 @@ stream seeded with seed, decision uniforms from the one seeded with ~seed.
 
 SplitMix64 (Steele, Lea and Flood, OOPSLA 2014): state += 0x9E3779B97F4A7C15,
 output = the state through the 64-bit finalizer; uniforms are the top 53
-bits times 2^-53; each normal is Box-Muller's cosine branch of two uniforms
-(u1 = 1 - first, u2 = second).
+bits times 2^-53; two uniforms (u1 = 1 - first, u2 = second) give two
+normals by Box-Muller, r cos(2 pi u2) first and r sin(2 pi u2) second, with
+r = sqrt(-2 ln u1).
 """
 
 import numpy as np
@@ -17,6 +18,7 @@ GAMMA = np.uint64(0x9E3779B97F4A7C15)
 class SplitMixStream:
     def __init__(self, seed):
         self.state = np.uint64(int(seed) % (1 << 64))
+        self._spare = None  # the sine normal of the last pair, not yet used
 
     def next_u64(self, n):
         s = self.state + GAMMA * np.arange(1, n + 1, dtype=np.uint64)
@@ -32,9 +34,23 @@ class SplitMixStream:
 
     def normal(self, size):
         n = int(np.prod(size))
-        u = self.uniform(2 * n)
-        u1, u2 = 1.0 - u[0::2], u[1::2]
-        return (np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)).reshape(size)
+        out = []
+        if n and self._spare is not None:
+            out.append(np.array([self._spare]))
+            self._spare = None
+        m = n - sum(len(o) for o in out)
+        if m:
+            pairs = (m + 1) // 2
+            u = self.uniform(2 * pairs)
+            r = np.sqrt(-2.0 * np.log(1.0 - u[0::2]))
+            a = 2.0 * np.pi * u[1::2]
+            z = np.empty(2 * pairs)
+            z[0::2] = r * np.cos(a)
+            z[1::2] = r * np.sin(a)
+            out.append(z[:m])
+            if m % 2:
+                self._spare = z[-1]
+        return (np.concatenate(out) if out else np.zeros(0)).reshape(size)
 
 
 class WalkerStreams:
