@@ -127,6 +127,22 @@ def cpu_baseline(fn, items, workers, what):
 BF16_MFMA_PEAK_TFLOPS = 2500.0  # dense bf16 matrix-core peak of one MI355X
 
 
+def _pmc_traffic(kernel):
+    """HBM bytes per dispatch of ``kernel`` measured by the rocprofv3 --pmc
+    runs (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size)
+    whose summaries are kept in profiles/, or None."""
+    import glob
+    import re
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc_traffic.md"))):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"traffic_bytes_per_dispatch kernel=(\S+) bytes=(\d+)", line.strip())
+                if m and m.group(1) == kernel:
+                    return int(m.group(2))
+    return None
+
+
 def _roofline(kernels, bytes_by_kernel, flops_by_kernel=None):
     """Roofline object of the kernel with the largest timed total: HBM-bound
     kernels priced by algorithmic bytes against 8 TB/s, matrix-core kernels
@@ -142,12 +158,12 @@ def _roofline(kernels, bytes_by_kernel, flops_by_kernel=None):
         per_launch = flops_by_kernel[dom] / launches
         achieved = per_launch / (avg_ms * 1e-3) / 1e12
         return dict(bound="mfma", kernel=dom + "_kernel", achieved=round(achieved, 3), peak=BF16_MFMA_PEAK_TFLOPS,
-                    unit="TFLOP/s", frac=round(achieved / BF16_MFMA_PEAK_TFLOPS, 5), traffic=None,
+                    unit="TFLOP/s", frac=round(achieved / BF16_MFMA_PEAK_TFLOPS, 5), traffic=_pmc_traffic(dom + "_kernel"),
                     avg_launch_ms=round(avg_ms, 4), launches=launches, algorithmic_flops_per_launch=per_launch)
     per_launch = bytes_by_kernel[dom] / launches
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     return dict(bound="hbm", kernel=dom + "_kernel", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, avg_launch_ms=round(avg_ms, 4),
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=_pmc_traffic(dom + "_kernel"), avg_launch_ms=round(avg_ms, 4),
                 launches=launches, algorithmic_bytes_per_launch=per_launch)
 
 

@@ -55,13 +55,15 @@ def summarize(prefix, range_name, top=12):
     return "\n".join(lines)
 
 
-def pmc_traffic(counter_csv, kernel, copy_kernel, copy_bytes):
+def pmc_traffic(counter_csv, kernel, copy_kernel, copy_bytes, grids=None):
     """Per-dispatch counter averages of ``kernel`` scaled to bytes.
 
     ``counter_csv`` is a rocprofv3 ``--pmc`` counter_collection.csv (one
     counter per run: FETCH_SIZE or WRITE_SIZE, reported in KiB).  The copy
     kernel of known size gives the factor bytes-per-counter-unit; the
     kernel's average counter value times that factor is its traffic.
+    ``grids``: only dispatches of ``kernel`` with these grid sizes (work
+    items) count, e.g. one batch size of a kernel that also runs at others.
     """
     vals = defaultdict(list)
     counter = None
@@ -70,6 +72,8 @@ def pmc_traffic(counter_csv, kernel, copy_kernel, copy_bytes):
             counter = row["Counter_Name"]
             name = row["Kernel_Name"]
             key = "copy" if copy_kernel in name else ("kernel" if kernel in name else None)
+            if key == "kernel" and grids is not None and int(row["Grid_Size"]) not in grids:
+                key = None
             if key:
                 vals[(key, row["Dispatch_Id"])].append(float(row["Counter_Value"]))
     per = defaultdict(list)
@@ -86,7 +90,8 @@ def pmc_traffic(counter_csv, kernel, copy_kernel, copy_bytes):
 
 if __name__ == "__main__":
     if sys.argv[1] == "--pmc":
-        out = pmc_traffic(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5]))
+        grids = {int(g) for g in sys.argv[6].split(",")} if len(sys.argv) > 6 else None
+        out = pmc_traffic(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5]), grids)
         print(" ".join("%s=%s" % kv for kv in out.items()))
     else:
         print(summarize(sys.argv[1], sys.argv[2]))
