@@ -715,13 +715,13 @@ __device__ __forceinline__ double three_term(const EigenArgs& a, int64_t base, i
   return block_sum<BT>(p0, red);
 }
 
-// cl[l] = <V_l, w> for l in [l0, l0 + 8) ∩ [l0, lend), one wave (l0 and
-// lend wave-uniform).  Every lane loads all eight vectors' entries (indices
+// cl[l] = <V_l, w> for l in [l0, l0 + U) ∩ [l0, lend), one wave (l0 and
+// lend wave-uniform).  Every lane loads all U vectors' entries (indices
 // past lend clamped to lend - 1, their sums discarded), so the loads issue
-// together instead of one bounds-checked branch at a time.
-template <class T>
+// together instead of one bounds-checked branch at a time.  Each vector's
+// sum is formed the same way whatever U.
+template <int U, class T>
 __device__ __forceinline__ void dots_wave(const T* V, int l0, int lend, int64_t n, const T* w, double2* cl) {
-  constexpr int U = 8;
   const int lane = threadIdx.x & 63;
   double cx[U], cy[U];
   size_t off[U];
@@ -749,15 +749,29 @@ __device__ __forceinline__ void dots_wave(const T* V, int l0, int lend, int64_t 
   }
 }
 
-// cl[l] = <V_l, w> for l < m over the split basis; results in LDS.
+template <int U, int NW, class T>
+__device__ __forceinline__ void dots_chunks(const T* VL, int ma, const T* VG, int m, int64_t n, const T* w,
+                                            double2* cl, int wave) {
+  for (int l0 = wave * U; l0 < ma; l0 += NW * U) dots_wave<U, T>(VL, l0, ma, n, w, cl);
+  for (int l0 = ma + wave * U; l0 < m; l0 += NW * U) dots_wave<U, T>(VG, l0, m, n, w, cl);
+}
+
+// cl[l] = <V_l, w> for l < m over the split basis; results in LDS.  Chunks
+// of 2, 4 or 8 vectors per wave, the smallest that covers the m vectors in
+// one round of the waves, so short bases keep every wave busy (each wave's
+// cost is its loads plus two reductions per vector of its chunk).
 template <int BT, class T>
 __device__ __forceinline__ void dots_split(const T* VL, int kl, const T* VG, int m, int64_t n, const T* w,
                                            double2* cl) {
-  constexpr int NW = BT / 64, U = 8;
+  constexpr int NW = BT / 64;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ma = m < kl ? m : kl;
-  for (int l0 = wave * U; l0 < ma; l0 += NW * U) dots_wave<T>(VL, l0, ma, n, w, cl);
-  for (int l0 = ma + wave * U; l0 < m; l0 += NW * U) dots_wave<T>(VG, l0, m, n, w, cl);
+  if (m <= 2 * NW)
+    dots_chunks<2, NW, T>(VL, ma, VG, m, n, w, cl, wave);
+  else if (m <= 4 * NW)
+    dots_chunks<4, NW, T>(VL, ma, VG, m, n, w, cl, wave);
+  else
+    dots_chunks<8, NW, T>(VL, ma, VG, m, n, w, cl, wave);
   __syncthreads();
 }
 

@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void gemm_fwd_kernel(GemmArgs g) {
     if (m >= g.M || n >= g.N) continue;
     const int64_t o = cz + (int64_t)m * g.ldc + n;
     const float zv = acc[r] + g.bias[(int64_t)z * g.bias_z + n];
-    g.Cf[o] = zv;
+    if (EPI != EPI_BIAS_GELU || g.Cf) g.Cf[o] = zv;  // hidden layers of a prediction keep no pre-activation
     if constexpr (EPI == EPI_BIAS_GELU) g.Cb[o] = f2bf(gelu(zv));
   }
 }
@@ -589,8 +589,10 @@ void alloc_acts(Mlp& net, int B) {
 }
 
 // forward pass of every member on the BF16 minibatch in xb; output in out
-// x_f32: the first layer reads FP32 inputs there (else the BF16 copy in xb)
-void forward(Mlp& net, int B, hipStream_t s, const float* x_f32 = nullptr) {
+// x_f32: the first layer reads FP32 inputs there (else the BF16 copy in xb);
+// inference: the hidden layers' FP32 pre-activations (kept for the backward
+// pass) are not written
+void forward(Mlp& net, int B, hipStream_t s, const float* x_f32 = nullptr, bool inference = false) {
   const uint16_t* x = net.xb.as<uint16_t>();
   int64_t x_z = 0;  // the input is shared by all members
   for (int l = 0; l < net.L; ++l) {
@@ -612,7 +614,7 @@ void forward(Mlp& net, int B, hipStream_t s, const float* x_f32 = nullptr) {
     g.bias_z = net.P;
     TimedLaunch tl("mlp_gemm", s);
     if (l + 1 < net.L) {
-      g.Cf = net.zf[l].as<float>();
+      g.Cf = inference ? nullptr : net.zf[l].as<float>();
       g.Cb = net.hb[l].as<uint16_t>();
       gemm_fwd<EPI_BIAS_GELU>(g, net.E, s);
       x = net.hb[l].as<uint16_t>();
@@ -784,10 +786,10 @@ namespace {
 // Conversion, forward pass and ensemble statistics of B staged samples.
 void enqueue_predict(Mlp& net, int B, double scale, bool with_target, hipStream_t s) {
   if (net.dims[0] % 8 == 0) {
-    forward(net, B, s, static_cast<const float*>(net.xin.d));  // inputs read from the mapped buffer
+    forward(net, B, s, static_cast<const float*>(net.xin.d), true);  // inputs read from the mapped buffer
   } else {
     convert_batch_input(net, B, s);
-    forward(net, B, s);
+    forward(net, B, s, nullptr, true);
   }
   const int D = net.dims[net.L];
   const double* tgt = with_target ? net.st.as<double>() : nullptr;
