@@ -140,6 +140,32 @@ inline unsigned blocks_for(int64_t n, int threads) {
 
 // ------------------------------------------------------------------ device helpers
 
+// Cross-lane move of a double by a DPP pattern (both 32-bit halves), lanes
+// outside row_mask reading 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave, identical in every lane, with every lane
+// active: DPP moves inside registers (quad swaps, half-row and row mirrors,
+// then row broadcasts 15 and 31 collect the four rows in lane 63), read back
+// from lane 63.  Fixed order, so runs are reproducible.
+__device__ __forceinline__ double wave_sum(double x) {
+  x += dpp_move<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+  x += dpp_move<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+  x += dpp_move<0x141, 0xF>(x);  // row_half_mirror
+  x += dpp_move<0x140, 0xF>(x);  // row_mirror
+  x += dpp_move<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_move<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+  return __hiloint2double(hi, lo);
+}
+
+
 struct Key {
   uint64_t w[IMPED_MAXW];
 };

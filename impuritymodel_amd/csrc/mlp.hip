@@ -368,7 +368,7 @@ __device__ __forceinline__ double wave_sum64(double v) {
 template <int NV>
 __device__ __forceinline__ void sums256(double (&v)[NV], double (*red)[NV]) {
 #pragma unroll
-  for (int i = 0; i < NV; ++i) v[i] = wave_sum64(v[i]);
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
     for (int i = 0; i < NV; ++i) red[threadIdx.x >> 6][i] = v[i];
@@ -378,15 +378,15 @@ __device__ __forceinline__ void sums256(double (&v)[NV], double (*red)[NV]) {
 }
 
 // The same statistics with the sample's member outputs loaded once into
-// registers (E <= 8, D <= 256 K; thread t holds d = t + 256 k): each
-// statistic is register arithmetic and one workgroup reduction, three
-// rounds in all, and every division is by a reciprocal formed once.
-template <int K>
+// registers (EM members, D <= 256 K; thread t holds d = t + 256 k): each
+// statistic is register arithmetic and one workgroup reduction (DPP wave
+// sums, then LDS), three rounds in all, and every division is by a
+// reciprocal formed once.
+template <int K, int EM>
 __global__ __launch_bounds__(256) void ensemble_stats_reg_kernel(const float* y, int E, int B, int D,
                                                                  const double* w, double scale,
                                                                  const double* target, double* mean, double* dis,
                                                                  double* l1) {
-  constexpr int EM = 8;
   __shared__ double red_a[4][EM], red_m[4][1], red_d[4][2];
   const int t = threadIdx.x;
   const int b = blockIdx.x;
@@ -783,6 +783,32 @@ void upload_on_change(DevBuf& dev, std::vector<double>& last, const double* host
 
 namespace {
 
+template <int K>
+bool launch_stats_reg_k(int E, int B, const float* y, int D, const double* w, double scale, const double* tgt,
+                        double* m, double* dis, double* l1, hipStream_t s) {
+  const dim3 g(B), t(256);
+  switch (E) {
+    case 1: hipLaunchKernelGGL((ensemble_stats_reg_kernel<K, 1>), g, t, 0, s, y, E, B, D, w, scale, tgt, m, dis, l1); break;
+    case 2: hipLaunchKernelGGL((ensemble_stats_reg_kernel<K, 2>), g, t, 0, s, y, E, B, D, w, scale, tgt, m, dis, l1); break;
+    case 3: hipLaunchKernelGGL((ensemble_stats_reg_kernel<K, 3>), g, t, 0, s, y, E, B, D, w, scale, tgt, m, dis, l1); break;
+    case 4: hipLaunchKernelGGL((ensemble_stats_reg_kernel<K, 4>), g, t, 0, s, y, E, B, D, w, scale, tgt, m, dis, l1); break;
+    case 5: hipLaunchKernelGGL((ensemble_stats_reg_kernel<K, 5>), g, t, 0, s, y, E, B, D, w, scale, tgt, m, dis, l1); break;
+    case 6: hipLaunchKernelGGL((ensemble_stats_reg_kernel<K, 6>), g, t, 0, s, y, E, B, D, w, scale, tgt, m, dis, l1); break;
+    case 7: hipLaunchKernelGGL((ensemble_stats_reg_kernel<K, 7>), g, t, 0, s, y, E, B, D, w, scale, tgt, m, dis, l1); break;
+    case 8: hipLaunchKernelGGL((ensemble_stats_reg_kernel<K, 8>), g, t, 0, s, y, E, B, D, w, scale, tgt, m, dis, l1); break;
+    default: return false;
+  }
+  return true;
+}
+
+// The register kernel for E <= 8 members and K values per thread; false if none fits.
+bool launch_stats_reg(int K, int E, int B, const float* y, int D, const double* w, double scale, const double* tgt,
+                      double* m, double* dis, double* l1, hipStream_t s) {
+  if (K == 1) return launch_stats_reg_k<1>(E, B, y, D, w, scale, tgt, m, dis, l1, s);
+  if (K == 2) return launch_stats_reg_k<2>(E, B, y, D, w, scale, tgt, m, dis, l1, s);
+  return launch_stats_reg_k<4>(E, B, y, D, w, scale, tgt, m, dis, l1, s);
+}
+
 // Conversion, forward pass and ensemble statistics of B staged samples.
 void enqueue_predict(Mlp& net, int B, double scale, bool with_target, hipStream_t s) {
   if (net.dims[0] % 8 == 0) {
@@ -797,16 +823,8 @@ void enqueue_predict(Mlp& net, int B, double scale, bool with_target, hipStream_
   // disagreement and misfit are written straight into the mapped result buffer
   double *m = net.smean.as<double>(), *dis = static_cast<double*>(net.sout.d), *l1 = dis + B;
   const double* w = net.sw.as<double>();
-  if (net.E <= 8 && D <= 256)
-    hipLaunchKernelGGL(ensemble_stats_reg_kernel<1>, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis,
-                       l1);
-  else if (net.E <= 8 && D <= 512)
-    hipLaunchKernelGGL(ensemble_stats_reg_kernel<2>, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis,
-                       l1);
-  else if (net.E <= 8 && D <= 1024)
-    hipLaunchKernelGGL(ensemble_stats_reg_kernel<4>, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis,
-                       l1);
-  else
+  if (net.E > 8 || D > 1024 ||
+      !launch_stats_reg(D <= 256 ? 1 : D <= 512 ? 2 : 4, net.E, B, y, D, w, scale, tgt, m, dis, l1, s))
     hipLaunchKernelGGL(ensemble_stats_kernel, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis, l1);
   HIP_CHECK(hipGetLastError());
 }
