@@ -1520,8 +1520,8 @@ int imped_walkers_create(imped_mlp net, int n_walkers, int n_params, const doubl
     return IMPED_ERR_INVALID;
   }
   Mlp* m = reinterpret_cast<Mlp*>(net);
-  int32_t dims[17];
-  const int L = mlp_layers(*m, dims);
+  int32_t dims[17];  // at most 16 layers
+  mlp_layers(*m, dims);
   if (dims[0] != n_params) {
     set_error("imped_walkers_create: n_params must equal the network's input width");
     return IMPED_ERR_INVALID;
@@ -1531,7 +1531,6 @@ int imped_walkers_create(imped_mlp net, int n_walkers, int n_params, const doubl
       set_error("imped_walkers_create: every box needs hi > lo");
       return IMPED_ERR_INVALID;
     }
-  (void)L;
   auto w = std::make_unique<Walkers>();
   w->net = m;
   w->W = n_walkers;
@@ -1573,6 +1572,10 @@ int imped_walkers_step(imped_walkers wk, const double* weights, double scale, co
 
 int imped_walkers_waiting(imped_walkers wk, int32_t* idx, double* proposals, int32_t* count) {
   API_BEGIN
+  if (!wk || !count) {
+    set_error("imped_walkers_waiting: walkers and count required");
+    return IMPED_ERR_INVALID;
+  }
   const Walkers& w = *reinterpret_cast<const Walkers*>(wk);
   int k = 0;
   for (int i = 0; i < w.W; ++i)

@@ -281,10 +281,13 @@ int imped_walkers_create(imped_mlp net, int n_walkers, int n_params, const doubl
                          const double* lo, const double* hi, double step, double temperature, double threshold,
                          uint64_t seed, imped_walkers* out);
 int imped_walkers_destroy(imped_walkers w);
-/* One step; *n_waiting = walkers waiting for exact evaluation afterwards. */
+/* One step; weights and target hold the network's output width, and
+ * *n_waiting = walkers waiting for exact evaluation afterwards.  The walkers
+ * keep a pointer to their network: destroy them first. */
 int imped_walkers_step(imped_walkers w, const double* weights, double scale, const double* target,
                        int32_t* n_waiting);
-/* The waiting walkers (ascending) and their proposals [count][n_params]. */
+/* The waiting walkers (ascending) and their proposals [count][n_params];
+ * idx and proposals (either may be NULL) must hold n_walkers rows. */
 int imped_walkers_waiting(imped_walkers w, int32_t* idx, double* proposals, int32_t* count);
 int imped_walkers_decide(imped_walkers w, int32_t count, const int32_t* idx, const double* misfit);
 /* theta [n_walkers][n_params], cur [n_walkers], best misfit so far and the
