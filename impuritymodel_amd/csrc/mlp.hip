@@ -399,8 +399,13 @@ __global__ __launch_bounds__(256) void ensemble_stats_reg_kernel(const float* y,
       const int d = t + 256 * k;
       v[e][k] = (e < E && d < D) ? fmax((double)y[((int64_t)e * B + b) * D + d] * inv_scale, 0.0) : 0.0;
     }
+  double tg[K];  // the target's values, loaded with the rest (mean may be NULL: not written)
 #pragma unroll
-  for (int k = 0; k < K; ++k) wd[k] = t + 256 * k < D ? w[t + 256 * k] : 0.0;
+  for (int k = 0; k < K; ++k) {
+    const int d = t + 256 * k;
+    wd[k] = d < D ? w[d] : 0.0;
+    tg[k] = target && d < D ? target[d] : 0.0;
+  }
   double inv_area[EM];
 #pragma unroll
   for (int e = 0; e < EM; ++e) {
@@ -425,7 +430,6 @@ __global__ __launch_bounds__(256) void ensemble_stats_reg_kernel(const float* y,
   sums256<1>(pm, red_m);
   const double inv_marea = 1.0 / pm[0];
   double pd[2] = {0.0, 0.0};
-  double* mb = mean + (int64_t)b * D;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int d = t + 256 * k;
@@ -436,8 +440,8 @@ __global__ __launch_bounds__(256) void ensemble_stats_reg_kernel(const float* y,
       if (e < E) s += fabs(v[e][k] * inv_area[e] - m);
     pd[0] += wd[k] * s;
     if (d < D) {
-      mb[d] = m;
-      if (target) pd[1] += wd[k] * fabs(m - target[d]);
+      if (mean) mean[(int64_t)b * D + d] = m;
+      if (target) pd[1] += wd[k] * fabs(m - tg[k]);
     }
   }
   sums256<2>(pd, red_d);
@@ -810,7 +814,7 @@ bool launch_stats_reg(int K, int E, int B, const float* y, int D, const double* 
 }
 
 // Conversion, forward pass and ensemble statistics of B staged samples.
-void enqueue_predict(Mlp& net, int B, double scale, bool with_target, hipStream_t s) {
+void enqueue_predict(Mlp& net, int B, double scale, bool with_target, bool with_mean, hipStream_t s) {
   if (net.dims[0] % 8 == 0) {
     forward(net, B, s, static_cast<const float*>(net.xin.d), true);  // inputs read from the mapped buffer
   } else {
@@ -823,6 +827,7 @@ void enqueue_predict(Mlp& net, int B, double scale, bool with_target, hipStream_
   // disagreement and misfit are written straight into the mapped result buffer
   double *m = net.smean.as<double>(), *dis = static_cast<double*>(net.sout.d), *l1 = dis + B;
   const double* w = net.sw.as<double>();
+  if (net.E <= 8 && D <= 1024 && !with_mean) m = nullptr;  // the register kernel writes no means then
   if (net.E > 8 || D > 1024 ||
       !launch_stats_reg(D <= 256 ? 1 : D <= 512 ? 2 : 4, net.E, B, y, D, w, scale, tgt, m, dis, l1, s))
     hipLaunchKernelGGL(ensemble_stats_kernel, dim3(B), dim3(256), 0, s, y, net.E, B, D, w, scale, tgt, m, dis, l1);
@@ -830,10 +835,10 @@ void enqueue_predict(Mlp& net, int B, double scale, bool with_target, hipStream_
 }
 
 // The captured enqueue_predict for this call's sizes and buffers.
-hipGraphExec_t predict_graph(Mlp& net, int B, double scale, bool with_target) {
+hipGraphExec_t predict_graph(Mlp& net, int B, double scale, bool with_target, bool with_mean) {
   uint64_t sbits;
   std::memcpy(&sbits, &scale, sizeof(sbits));
-  std::vector<uintptr_t> key = {(uintptr_t)B, (uintptr_t)sbits, (uintptr_t)with_target,
+  std::vector<uintptr_t> key = {(uintptr_t)B, (uintptr_t)sbits, (uintptr_t)with_target, (uintptr_t)with_mean,
                                 (uintptr_t)net.xin.d, (uintptr_t)net.sout.d, (uintptr_t)net.smean.p,
                                 (uintptr_t)net.sw.p, (uintptr_t)net.st.p, (uintptr_t)net.xb.p,
                                 (uintptr_t)net.wb.p, (uintptr_t)net.w.p, (uintptr_t)net.out.p};
@@ -851,7 +856,7 @@ hipGraphExec_t predict_graph(Mlp& net, int B, double scale, bool with_target) {
   HIP_CHECK(hipStreamBeginCapture(net.cap, hipStreamCaptureModeThreadLocal));
   hipGraph_t graph = nullptr;
   try {
-    enqueue_predict(net, B, scale, with_target, net.cap);
+    enqueue_predict(net, B, scale, with_target, with_mean, net.cap);
   } catch (...) {
     (void)hipStreamEndCapture(net.cap, &graph);
     if (graph) (void)hipGraphDestroy(graph);
@@ -889,10 +894,11 @@ void mlp_predict_begin(Mlp& net, const float* x_host, int B, const double* w_hos
   net.smean.reserve((size_t)B * D * sizeof(double));
   net.sout.reserve((size_t)2 * B * sizeof(double));
   // per-kernel timing needs direct launches
+  const bool with_target = target_host != nullptr, with_mean = mean_host != nullptr;
   if (timing_enabled() || !g_tuning.mlp_graph)
-    enqueue_predict(net, B, scale, target_host != nullptr, s);
+    enqueue_predict(net, B, scale, with_target, with_mean, s);
   else
-    HIP_CHECK(hipGraphLaunch(predict_graph(net, B, scale, target_host != nullptr), s));
+    HIP_CHECK(hipGraphLaunch(predict_graph(net, B, scale, with_target, with_mean), s));
   if (mean_host) d2h(mean_host, net.smean.p, (size_t)B * D, s);
 }
 
