@@ -93,17 +93,20 @@ def test_family_spectra_match_assembled_models(gpu):
     np.testing.assert_allclose(fast, np.array(slow), rtol=0, atol=1e-8 * np.abs(slow).max())
 
 
-@pytest.mark.parametrize("E,D", [(5, 1000), (3, 200), (9, 1000), (2, 1100)])
-def test_ensemble_predict_statistics_vs_numpy(gpu, E, D):
+@pytest.mark.parametrize("E,D,H", [(5, 1000, 64), (3, 200, 64), (9, 1000, 64), (2, 1100, 64), (5, 1000, 512)])
+def test_ensemble_predict_statistics_vs_numpy(gpu, E, D, H):
     """The device ensemble statistics and misfits equal the same quantities
     computed in numpy (trapezoid rule, np.trapezoid) from the members' raw
-    outputs.  The cases cover the one-wave kernel (64 and 16 values per lane)
-    and the workgroup kernel (more than 8 members; more than 1024 outputs)."""
+    outputs (two-launch forward pass).  The cases cover the register kernel
+    (one and four values per thread), the workgroup kernel (more than 8
+    members; more than 1024 outputs), and, with 512 hidden units, the
+    prediction that forms the first hidden layer inside the second layer's
+    launch."""
     from impuritymodel_amd.surrogate import Ensemble, trapezoid_weights
 
     omega = np.linspace(-5.0, 15.0, D)
     scale = float(omega[-1] - omega[0])
-    dims = (8, 64, 64, D)
+    dims = (8, H, H, D)
     net = Ensemble(E, dims, max_batch=64, seed=2)
     rng = np.random.default_rng(3)
     p = net.init_params(3)
