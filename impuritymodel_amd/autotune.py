@@ -29,7 +29,7 @@ from . import engine
 DEFAULTS = {
     "lanczos_wave_max_n": 4096,
     "lanczos_wg256_max_n": 65536,
-    "eigen_bt": 512,
+    "eigen_bt": 0,
     "eigen_check_every": 8,
     "eigen_wave_max_n": 0,
     "spmv_lpr": 8,

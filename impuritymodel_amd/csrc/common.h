@@ -107,7 +107,7 @@ inline void d2h(T* dst, const void* src, size_t n, hipStream_t s) {
 struct Tuning {
   int64_t lanczos_wave_max_n = 4096;    // chains up to this size: one wave each
   int64_t lanczos_wg256_max_n = 65536;  // then 256 threads, above: 1024
-  int64_t eigen_bt = 512;               // eigensolver threads per chain (256, 512, 1024)
+  int64_t eigen_bt = 0;                 // eigensolver threads per chain (256, 512, 1024; 0: by the largest chain)
   int64_t eigen_wave_max_n = 0;         // eigensolver chains up to this size: one wave each (off)
   int64_t eigen_real = 1;               // real Krylov vectors when H is real
   int64_t eigen_dgks = 1;               // second Gram-Schmidt pass only when needed

@@ -868,8 +868,8 @@ int imped_set_tuning(const char* name, int64_t value) {
   else if (n == "lanczos_wg256_max_n")
     g_tuning.lanczos_wg256_max_n = value;
   else if (n == "eigen_bt") {
-    if (value != 256 && value != 512 && value != 1024) {
-      set_error("imped_set_tuning: eigen_bt must be 256, 512 or 1024");
+    if (value != 0 && value != 256 && value != 512 && value != 1024) {
+      set_error("imped_set_tuning: eigen_bt must be 0 (automatic), 256, 512 or 1024");
       return IMPED_ERR_INVALID;
     }
     g_tuning.eigen_bt = value;

@@ -2119,9 +2119,13 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   }
   a.chain_list = sc.clist.as<int32_t>();
   if (n_wide > 0) {
+    // eigen_bt 0 (automatic): a launch lasts as long as its largest chain,
+    // which gets 1024 threads above 512 rows (fewer rows per thread), else 512
+    // (fewer waves at each barrier)
+    const int bt = g_tuning.eigen_bt ? (int)g_tuning.eigen_bt : (kb.h_n[order[0]] > 512 ? 1024 : 512);
     TimedLaunch tl("eigen", s);
-    layout(0, (int)g_tuning.eigen_bt, n_wide);
-    launch((int)g_tuning.eigen_bt, dim3((unsigned)n_wide));
+    layout(0, bt, n_wide);
+    launch(bt, dim3((unsigned)n_wide));
   }
   std::vector<int32_t> conv(C, 1);
   d2h(conv.data(), sc.conv.p, C, s);
@@ -2203,7 +2207,7 @@ int run_solve(const KrylovBatch& kb, const double2* y, int S, const double* z_ho
     TimedLaunch tl("solve", s);
     if (g_tuning.eigen_bt == 256)
       hipLaunchKernelGGL(solve_kernel<256>, dim3(C), dim3(256), shm, s, a);
-    else if (g_tuning.eigen_bt == 512)
+    else if (g_tuning.eigen_bt == 512 || g_tuning.eigen_bt == 0)  // automatic: 512
       hipLaunchKernelGGL(solve_kernel<512>, dim3(C), dim3(512), shm, s, a);
     else
       hipLaunchKernelGGL(solve_kernel<1024>, dim3(C), dim3(1024), shm, s, a);

@@ -58,7 +58,8 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * counterpart.  Names: "lanczos_wave_max_n", "lanczos_wg256_max_n" (Lanczos
  * threads per chain: 64 / 256 / 1024 by size), "eigen_wave_max_n" (eigensolver
  * chains up to this size get one wave), "eigen_bt" (threads per larger
- * eigensolver chain: 256, 512, 1024), "eigen_real", "eigen_dgks",
+ * eigensolver chain: 256, 512, 1024; 0, the default: 1024 when the launch's
+ * largest chain has more than 512 rows, else 512), "eigen_real", "eigen_dgks",
  * "eigen_check_every", "flat_min_n" (chains above it use the grid-wide
  * kernels), "spmv_lpr" (lanes per row of the grid-wide SpMV: 4..64),
  * "lanczos_graph_steps" (grid-wide Lanczos steps per replayed HIP graph,

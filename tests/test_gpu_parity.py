@@ -212,7 +212,7 @@ def test_eigensolver_variants_vs_dense(gpu, hy):
                 dense[r, col[e]] = val[e]
         ref.append(np.linalg.eigvalsh(dense)[:3])
     try:
-        for bt in (256, 1024):
+        for bt in (0, 256, 1024):
             for real in (0, 1):
                 for dgks in (0, 1):
                     engine.set_tuning("eigen_bt", bt)
@@ -223,7 +223,7 @@ def test_eigensolver_variants_vs_dense(gpu, hy):
                     for c in range(3):
                         np.testing.assert_allclose(ev[c], ref[c], rtol=0, atol=1e-10)
     finally:
-        engine.set_tuning("eigen_bt", 512)
+        engine.set_tuning("eigen_bt", 0)
         engine.set_tuning("eigen_real", 1)
         engine.set_tuning("eigen_dgks", 1)
 
