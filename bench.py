@@ -303,7 +303,8 @@ def run_fit(args, torch):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        # surrogate GEMMs in BF16 (FP32 accumulate), statistics and every exact-ED kernel in FP64
+        "dtype": "bf16+f64",
         "data": "synthetic (seeded parameters; exact spectra from the engine itself)",
         "config": {
             "workload": "fitting loop: %d Metropolis walkers over 8 parameters of a NiO-like 10-bath model "
