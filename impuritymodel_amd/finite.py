@@ -7,11 +7,13 @@ expansion, Hamiltonian build, Lanczos recurrence and eigensolve runs in
 libimped.so's HIP kernels; this module converts dicts to packed words and
 back.
 
-Deliberate differences (documented per function): the ``h_dict`` memo is a
-cache the device path does not need -- H|PS> is recomputed on the GPU and
-``h_dict`` is left as passed (the reference states new elements "might" be
-added); ``parallelization_mode``/MPI arguments are accepted and ignored (one
-GPU).
+The ``h_dict`` memo of the BFS functions is filled as the reference fills
+it: after a closure on the device, every basis state's H|PS> (restricted,
+pruned at 1e-12) is written into the caller's dict from the device CSR, so
+the reference's own composition ``expand_basis`` ->
+``get_hamiltonian_matrix_from_h_dict`` works unchanged.  The device path does
+not read the memo back (H|PS> is recomputed on the GPU).
+``parallelization_mode``/MPI arguments are accepted and ignored (one GPU).
 """
 
 from collections import OrderedDict  # noqa: F401  (re-exported for callers)
@@ -224,12 +226,29 @@ def get_hamiltonian_matrix(n_spin_orbitals, hOp, basis, mode="sparse_MPI"):
     return h
 
 
+def _fill_h_dict(b, basis, h_dict):
+    """h_dict[basis[j]] = {basis[k]: <k|H|j>} from the device rows of chain 0
+    (finite.py:1744-1745: the memo holds H|PS> for every expanded state)."""
+    rowptr, col, val = b.csr()
+    vals = np.conj(val).tolist()
+    for j, s in enumerate(basis):
+        if s not in h_dict:
+            lo, hi = int(rowptr[j]), int(rowptr[j + 1])
+            h_dict[s] = {basis[k]: v for k, v in zip(col[lo:hi].tolist(), vals[lo:hi])}
+
+
 def expand_basis(n_spin_orbitals, h_dict, hOp, basis0, restrictions, parallelization_mode="serial"):
-    """BFS closure of basis0 under hOp with restrictions (finite.py:1696-1803), on the GPU."""
+    """BFS closure of basis0 under hOp with restrictions (finite.py:1696-1803), on the GPU.
+
+    Fills ``h_dict`` (when not None) with H|PS> of every basis state not yet
+    in it, as the reference does."""
     if parallelization_mode not in ("serial", "H_build"):
         raise Exception("Wrong parallelization parameter.")
     b = _expanded_batch(n_spin_orbitals, hOp, list(basis0), restrictions)
-    return tuple(opcompile.words_to_bytes(b.basis_words(), n_spin_orbitals))
+    basis = tuple(opcompile.words_to_bytes(b.basis_words(), n_spin_orbitals))
+    if h_dict is not None:
+        _fill_h_dict(b, basis, h_dict)
+    return basis
 
 
 def _expanded_batch(n, hOp, seeds, restrictions, seed_amps=None):
@@ -244,18 +263,24 @@ def get_hamiltonian_matrix_from_h_dict(h_dict, basis, parallelization_mode="seri
                                        mode="sparse"):
     """CSR (or dense) matrix from an H|PS> memo (finite.py:1613-1693).
 
-    A host format conversion of a dict the caller already holds.
+    A host format conversion of a dict the caller already holds.  As in the
+    reference, the serial mode raises ``KeyError`` for a basis state missing
+    from ``h_dict``; "H_build" uses the states present (finite.py:1671-1682).
     """
     if parallelization_mode == "serial":
         assert return_h_local is False
-    if mode not in ("sparse", "dense") or parallelization_mode not in ("serial", "H_build"):
+    if mode not in ("sparse", "dense") or parallelization_mode not in ("serial", "H_build") or (
+            mode == "dense" and parallelization_mode == "H_build"):
         raise Exception("Wrong input parameters")
     n = len(basis)
     basis_index = {basis[i]: i for i in range(n)}
     data, row, col = [], [], []
-    for j in range(n):
-        if basis[j] not in h_dict:
-            continue
+    if parallelization_mode == "serial":
+        columns = range(n)
+    else:
+        present = set(basis).intersection(h_dict.keys())
+        columns = [j for j in range(n) if basis[j] in present]
+    for j in columns:
         for k, v in h_dict[basis[j]].items():
             data.append(v)
             col.append(j)
@@ -272,6 +297,8 @@ def expand_basis_and_hamiltonian(n_spin_orbitals, h_dict, hOp, basis0, restricti
         raise Exception("Wrong parallelization parameter.")
     b = _expanded_batch(n_spin_orbitals, hOp, list(basis0), restrictions)
     basis = opcompile.words_to_bytes(b.basis_words(), n_spin_orbitals)
+    if h_dict is not None:
+        _fill_h_dict(b, basis, h_dict)
     return _reference_csr(b), {s: i for i, s in enumerate(basis)}
 
 

@@ -343,6 +343,83 @@ def get_bath_h0(e_imp, tenDq, eps_bath, v_bath, l=2):
     return op
 
 
+def get_CF_hamiltonian(nBaths, nValBaths, cf, bath_state_basis="spherical"):
+    """Non-interacting 3d + bath operator from crystal-field parameters
+    (get_spectra_using_CF.py:433-563).
+
+    ``cf`` is the ten-tuple of ``readers.read_h0_CF_file`` (or a JSON file
+    name).  In cubic harmonics every 3d orbital couples to one valence bath
+    state (and, with 10 conduction bath states, one conduction state): on-site
+    e_imp + 3/5 10Dq (eg) or e_imp - 2/5 10Dq (t2g), bath energies and
+    hoppings per eg/t2g.  The impurity block is rotated to spherical
+    harmonics; the bath states are spherical (``u_bath = u``) or kept cubic.
+    Bath state of spin s, orbital i: valence s*5 + i, conduction 10 + s*5 + i.
+    Returns the sum impurity + hopping + bath-energy operators, keyed by
+    (l, s, m) / (l, b) spin-orbitals.
+    """
+    if isinstance(cf, str):
+        from .readers import read_h0_CF_file
+
+        cf = read_h0_CF_file(cf)
+    (e_imp, tenDq, e_val_eg, e_val_t2g, e_con_eg, e_con_t2g, v_val_eg, v_val_t2g, v_con_eg, v_con_t2g) = cf
+    l = 2
+    nl = 2 * l + 1
+
+    def cubic(eg, t2g):
+        return np.diag([eg, eg, t2g, t2g, t2g]).astype(float)
+
+    u = get_spherical_2_cubic_matrix(spinpol=False, l=l)
+    if bath_state_basis == "spherical":
+        u_bath = u
+    elif bath_state_basis == "cubic":
+        u_bath = np.eye(nl)
+    else:
+        raise Exception("Design of this basis is not (yet) implemented.")
+    def herm(h):
+        # u D u^dagger is Hermitian up to the rounding of the products; the
+        # mean with its adjoint (a change of at most one ulp) makes the
+        # operator pass the exact-equality hermiticity check of
+        # get_hamiltonian_operator whatever order the products sum in
+        return 0.5 * (h + np.conj(h.T))
+
+    h_imp = herm(u @ cubic(e_imp + 3 / 5 * tenDq, e_imp - 2 / 5 * tenDq) @ np.conj(u.T))
+    v_val = u_bath @ cubic(v_val_eg, v_val_t2g) @ np.conj(u.T)
+    v_con = u_bath @ cubic(v_con_eg, v_con_t2g) @ np.conj(u.T)
+    e_val = herm(u_bath @ cubic(e_val_eg, e_val_t2g) @ np.conj(u_bath.T))
+    e_con = herm(u_bath @ cubic(e_con_eg, e_con_t2g) @ np.conj(u_bath.T))
+    with_con = nBaths[l] - nValBaths[l] == 10
+    ms = list(range(-l, l + 1))
+    # impurity block, spin by spin (i, j over m ascending)
+    h_imp_op = {}
+    for i, mi in enumerate(ms):
+        for j, mj in enumerate(ms):
+            if h_imp[i, j] != 0:
+                for s in range(2):
+                    h_imp_op[(((l, s, mi), "c"), ((l, s, mj), "a"))] = h_imp[i, j]
+    # hoppings and bath energies in the reference's insertion order:
+    # spin, then bath row i, then impurity column j; conduction terms follow
+    # the valence terms of the same (s, i, j)
+    hopp, ebath = {}, {}
+    for s in range(2):
+        for i in range(nl):
+            bi_val = s * nl + i
+            bi_con = 2 * nl + bi_val
+            for j, mj in enumerate(ms):
+                bj_val = s * nl + j
+                bj_con = 2 * nl + bj_val
+                blocks = [(v_val, e_val, bi_val, bj_val)]
+                if with_con:
+                    blocks.append((v_con, e_con, bi_con, bj_con))
+                for vmat, emat, bi, bj in blocks:
+                    v = vmat[i, j]
+                    if v != 0:
+                        hopp[(((l, bi), "c"), ((l, s, mj), "a"))] = v
+                        hopp[(((l, s, mj), "c"), ((l, bi), "a"))] = v.conjugate()
+                    if emat[i, j] != 0:
+                        ebath[(((l, bi), "c"), ((l, bj), "a"))] = emat[i, j]
+    return addOps([h_imp_op, hopp, ebath])
+
+
 # ---------------------------------------------------------------- transition operators
 
 

@@ -695,7 +695,7 @@ def fitting_loop(family, target, surrogate=None, n_walkers=256, n_iter=8, step=0
                 seconds=time.perf_counter() - t0, best_l1=loop.best, sur_params=sp, sur_spectra=ss)
 
 
-def exact_spectra(family, thetas, krylovSize=200, reorth=0, **kw):
+def exact_spectra(family, thetas, krylovSize=200, reorth=1, **kw):
     """Exact XAS of parameter sets by the batched GPU ED, area-normalized."""
     from .batched import ground_bases, xas_batch_ops
 

@@ -34,8 +34,11 @@ def thermal_average(energies, observable, T=300):
 
 
 def get_green(n, e, psi, hOp, omega, delta, krylovSize, slaterWeightMin, restrictions=None, h_dict=None,
-              reorth="none", return_ab=False):
-    """<psi|(omega + i delta + e - H)^-1|psi> by Lanczos (spectra.py:551-681)."""
+              reorth="none", return_ab=False, sizes=None):
+    """<psi|(omega + i delta + e - H)^-1|psi> by Lanczos (spectra.py:551-681).
+
+    ``sizes`` (a list) receives the BFS basis size, as the reference prints
+    it (finite.py:1879-1886)."""
     g = np.zeros(len(omega), dtype=complex)
     if len(psi) == 0:
         return (g, None, None) if return_ab else g
@@ -43,6 +46,8 @@ def get_green(n, e, psi, hOp, omega, delta, krylovSize, slaterWeightMin, restric
         h_dict = {}
     h, index = expand_basis_and_hamiltonian(n, h_dict, hOp, psi.keys(), restrictions)
     nb = len(index)
+    if sizes is not None:
+        sizes.append(nb)
     psi0 = np.zeros(nb, dtype=complex)
     for ps, amp in psi.items():
         psi0[index[ps]] = amp
@@ -53,7 +58,7 @@ def get_green(n, e, psi, hOp, omega, delta, krylovSize, slaterWeightMin, restric
 
 
 def get_spectra(n, hOp, tOps, psis, es, w, delta, restrictions=None, krylovSize=150, slaterWeightMin=1e-7,
-                reorth="none"):
+                reorth="none", sizes=None):
     """gs[i, t, :] = |T_t psi_i|^2 G_i(normalized T_t psi_i) (spectra.py:684-806).
 
     One ``h_dict`` memo is shared by all chains; each transition operator has
@@ -70,8 +75,8 @@ def get_spectra(n, hOp, tOps, psis, es, w, delta, restrictions=None, krylovSize=
             for s in psiR.keys():
                 psiR[s] /= normalization
             gs[i, t, :] = normalization**2 * get_green(
-                n, es[i], psiR, hOp, w, delta, krylovSize, slaterWeightMin, restrictions, h, reorth=reorth
-            )
+                n, es[i], psiR, hOp, w, delta, krylovSize, slaterWeightMin, restrictions, h, reorth=reorth,
+                sizes=sizes)
     return gs
 
 
@@ -166,25 +171,29 @@ def energy_cut(es, T=300, cut=10.0):
     return tuple(e for e in es if e - es[0] < lim)
 
 
-def xas(n, hOp, tOps, psis, es, w, delta, restrictions, T=300, krylovSize=150, reorth="none"):
+def xas(n, hOp, tOps, psis, es, w, delta, restrictions, T=300, krylovSize=150, reorth="none", sizes=None):
     """XAS branch of simulate_spectra (spectra.py:222-238).
 
     Returns ``(gs, a, aSum)``: raw Green's functions, thermally averaged
     -Im per polarization, and their sum.
     """
-    gs = get_spectra(n, hOp, tOps, psis, es, w, delta, restrictions, krylovSize, reorth=reorth)
+    gs = get_spectra(n, hOp, tOps, psis, es, w, delta, restrictions, krylovSize, reorth=reorth, sizes=sizes)
     a = thermal_average(es[: np.shape(gs)[0]], -gs.imag, T=T)
     return gs, a, np.sum(a, axis=0)
 
 
 def photoemission(n, hOp, tOpsPS, tOpsIPS, psis, es, w, delta, restrictions, T=300, krylovSize=150,
-                  reorth="none"):
+                  reorth="none", sizes=None):
     """PS + IPS branch of simulate_spectra (spectra.py:124-149).
 
     ``tOpsIPS`` may be None for the core XPS branch (spectra.py:163-179).
     """
-    gs = -get_spectra(n, hOp, tOpsPS, psis, es, -w, -delta, restrictions, krylovSize, reorth=reorth)
-    if tOpsIPS is not None:
-        gs = gs + get_spectra(n, hOp, tOpsIPS, psis, es, w, delta, restrictions, krylovSize, reorth=reorth)
+    if tOpsIPS is None:
+        gs = -get_spectra(n, hOp, tOpsPS, psis, es, -w, -delta, restrictions, krylovSize, reorth=reorth, sizes=sizes)
+    else:
+        # the reference's order: inverse photoemission first (spectra.py:129-138)
+        g_ips = get_spectra(n, hOp, tOpsIPS, psis, es, w, delta, restrictions, krylovSize, reorth=reorth, sizes=sizes)
+        gs = -get_spectra(n, hOp, tOpsPS, psis, es, -w, -delta, restrictions, krylovSize, reorth=reorth,
+                          sizes=sizes) + g_ips
     a = thermal_average(es[: np.shape(gs)[0]], -gs.imag, T=T)
     return gs, a, np.sum(a, axis=0)

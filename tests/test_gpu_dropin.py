@@ -1,0 +1,164 @@
+"""Drop-in compositions on the GPU against the oracle.
+
+* The reference's own composition ``expand_basis`` ->
+  ``get_hamiltonian_matrix_from_h_dict`` (finite.py:1865-1870): the drop-in
+  ``expand_basis`` fills the caller's ``h_dict`` from the device rows.
+* The sections of ``simulate_spectra`` as library calls (spectra.py:124-238):
+  PS/IPS, XPS, NIXS and XAS with full reorthogonalization against the
+  oracle at 1e-8 of the peak.
+* The timed exact path of the surrogate fitting loop
+  (``surrogate.exact_spectra`` -> ``batched.xas_batch_ops``, fully
+  reorthogonalized Lanczos) against the oracle's fully reorthogonalized
+  spectra at 1e-8, on family members at 300 K and 10 K.
+* The reference's plain recurrence (``reorth="none"``, the drop-in default)
+  against the oracle's plain recurrence.  These chains run K = n steps, past
+  loss of orthogonality, where two executions of the same recurrence that
+  sum in different orders differ by about 1e-5 of the peak (a rounding
+  property of the recurrence, shown on the CPU by permuting the basis); the
+  bar is 1e-4.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import applyop
+from oracle import basis as obasis
+from oracle import hamiltonian as oham
+from oracle import krylov as okry
+from oracle import spectra as ospec
+from oracle import states as psr
+
+pytestmark = pytest.mark.gpu
+
+
+def _bytes(states, n):
+    return [psr.int2bytes(s, n) for s in states]
+
+
+def _ground(m, nev=2):
+    n = m.n_spin_orbitals
+    b = obasis.get_basis(m.nBaths, m.nValBaths, m.dnValBaths, m.dnConBaths, m.dnTols, m.n0imps)
+    es, po = okry.eigensystem(n, m.hOp, b, nev, groundDiagMode="full")
+    pg = [dict(zip(_bytes(list(p), n), p.values())) for p in po]
+    return n, es, po, pg
+
+
+def test_expand_basis_fills_h_dict_for_reference_composition(gpu):
+    from impuritymodel_amd import finite
+    from impuritymodel_amd.models import make_model
+    from impuritymodel_amd.operators import getDipoleOperators
+
+    m = make_model(n_bath=10, seed=6)
+    n, es, po, pg = _ground(m, 1)
+    t = getDipoleOperators(m.nBaths, ([1, 0, 0],))[0]
+    seed = applyop.apply_op(n, t, po[0], 1e-7, m.restrictions)
+    h_o = {}
+    ho, idx_o = oham.expand_basis_and_hamiltonian(n, h_o, m.hOp, seed.keys(), m.restrictions)
+    h_dict = {}
+    basis = finite.expand_basis(n, h_dict, m.hOp, _bytes(list(seed), n), m.restrictions)
+    h, idx = finite.get_hamiltonian_matrix_from_h_dict(h_dict, basis)
+    order_o = sorted(idx_o, key=idx_o.get)
+    assert list(basis) == _bytes(order_o, n)
+    assert set(h_dict) == set(basis)
+    for s_int in order_o[:50]:
+        got = h_dict[psr.int2bytes(s_int, n)]
+        ref = h_o[s_int]
+        assert set(got) == set(_bytes(list(ref), n))
+        for k, v in ref.items():
+            assert abs(got[psr.int2bytes(k, n)] - v) <= 1e-12
+    d = (h - ho).tocoo()
+    assert (abs(d.data).max() if d.nnz else 0.0) <= 1e-12
+    assert np.array_equal((h != 0).toarray(), (ho != 0).toarray())
+    # the same composition through expand_basis_and_hamiltonian
+    h_dict2 = {}
+    h2, idx2 = finite.expand_basis_and_hamiltonian(n, h_dict2, m.hOp, _bytes(list(seed), n), m.restrictions)
+    assert idx2 == idx and set(h_dict2) == set(h_dict)
+    assert abs(h2 - h).max() <= 1e-12
+
+
+@pytest.mark.parametrize("section", ["PS", "XPS", "NIXS", "XAS"])
+def test_composition_sections_vs_oracle(gpu, section):
+    from impuritymodel_amd import spectra
+    from impuritymodel_amd.operators import (getDipoleOperators, getInversePhotoEmissionOperators,
+                                             getNIXSOperators, getPhotoEmissionOperators)
+    from impuritymodel_amd.models import make_model
+    import refout
+
+    m = make_model(n_bath=10, seed=4)
+    n, es, po, pg = _ground(m, 2)
+    es = tuple(es)
+    w = np.linspace(-15, 15, 401)
+    r = m.restrictions
+    kw = dict(T=300, krylovSize=150, reorth="full")
+    if section == "PS":
+        _, a, s = spectra.ps_spectra(n, m.hOp, pg, es, w, 0.2, r, m.nBaths, **kw)
+        _, ao, so = ospec.photoemission(n, m.hOp, getPhotoEmissionOperators(m.nBaths, l=2),
+                                        getInversePhotoEmissionOperators(m.nBaths, l=2), po, es, w, 0.2, r, **kw)
+        assert np.all(s >= -1e-12)  # both halves give non-negative spectral weight
+    elif section == "XPS":
+        _, a, s = spectra.xps_spectra(n, m.hOp, pg, es, w, 0.2, r, m.nBaths, **kw)
+        _, ao, so = ospec.photoemission(n, m.hOp, getPhotoEmissionOperators(m.nBaths, l=1), None, po, es, w, 0.2,
+                                        r, **kw)
+    elif section == "NIXS":
+        rm, ri = np.loadtxt(refout.radial_path()).T
+        qs = refout.grids()["qsNIXS"]
+        wl = np.linspace(-2, 12, 401)
+        _, a, s = spectra.nixs_spectra(n, m.hOp, pg, es, wl, 0.1, qs, 2, 2, ri, ri, rm, r, m.nBaths, **kw)
+        tops = getNIXSOperators(m.nBaths, qs, 2, 2, ri, ri, rm)
+        gso = ospec.get_spectra(n, m.hOp, tops, po, es, wl, 0.1, r, 150, reorth="full")
+        ao = ospec.thermal_average(es, -gso.imag)
+        so = ao.sum(axis=0)
+    else:
+        eps = [[1, 0, 0], [0, 1, 0], [0, 0, 1]]
+        _, a, s = spectra.xas_spectra(n, m.hOp, pg, es, w, 0.2, eps, r, m.nBaths, **kw)
+        _, ao, so = ospec.xas(n, m.hOp, getDipoleOperators(m.nBaths, eps), po, es, w, 0.2, r, **kw)
+    scale = np.abs(ao).max()
+    assert scale > 0
+    np.testing.assert_allclose(a, ao, rtol=0, atol=1e-8 * scale)
+    np.testing.assert_allclose(s, so, rtol=0, atol=1e-8 * np.abs(so).max())
+
+
+@pytest.mark.parametrize("T", [300.0, 10.0])
+def test_timed_exact_path_vs_oracle(gpu, T):
+    """surrogate.exact_spectra -- the exact evaluations of the fitting loop,
+    its training set and its L1 judge -- equals the oracle's fully
+    reorthogonalized XAS of the same parameter sets."""
+    from impuritymodel_amd import surrogate as S
+    from impuritymodel_amd.operators import getDipoleOperators
+
+    fam = S.XASFamily()
+    rng = np.random.default_rng(11)
+    lo, hi = S.PARAM_RANGES[:, 0], S.PARAM_RANGES[:, 1]
+    th = lo + (hi - lo) * rng.random((3, 8))
+    th[:, 7] = T
+    got, _ = S.exact_spectra(fam, th)
+    tops = getDipoleOperators(fam.nBaths, ([1, 0, 0], [0, 1, 0], [0, 0, 1]))
+    for i in range(len(th)):
+        hOp = fam.hop(th[i])
+        n = 26
+        b = obasis.get_basis(fam.nBaths, fam.nValBaths, fam.dnValBaths, fam.dnConBaths, fam.dnTols, fam.n0imps)
+        es, po = okry.eigensystem(n, hOp, b, 3, groundDiagMode="full")
+        es = ospec.energy_cut(tuple(es), T=T)
+        _, _, ref = ospec.xas(n, hOp, tops, po[:len(es)], es, S.OMEGA, S.DELTA, fam.restrictions, T=T,
+                              krylovSize=200, reorth="full")
+        ref = S.area_normalize(ref[None])[0]
+        np.testing.assert_allclose(got[i], ref, rtol=0, atol=1e-8 * np.abs(ref).max())
+
+
+def test_plain_recurrence_xas_vs_oracle(gpu):
+    from impuritymodel_amd import spectra
+    from impuritymodel_amd.models import make_model
+    from impuritymodel_amd.operators import getDipoleOperators
+
+    m = make_model(n_bath=10, seed=3)
+    n, es, po, pg = _ground(m, 3)
+    es = ospec.energy_cut(tuple(es))
+    w = np.linspace(-25, 25, 2000)
+    eps = [[1, 0, 0], [0, 1, 0], [0, 0, 1]]
+    _, a, _ = spectra.xas_spectra(n, m.hOp, pg[:len(es)], es, w, 0.2, eps, m.restrictions, m.nBaths,
+                                  krylovSize=200)
+    _, ao, _ = ospec.xas(n, m.hOp, getDipoleOperators(m.nBaths, eps), po[:len(es)], es, w, 0.2, m.restrictions,
+                         krylovSize=200)
+    err = np.abs(a - ao).max() / np.abs(ao).max()
+    print("plain recurrence XAS vs oracle: %.3g of the peak" % err)
+    assert err <= 1e-4
