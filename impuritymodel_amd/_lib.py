@@ -77,6 +77,7 @@ SIGNATURES = {
     "imped_walkers_decide": [_p, _i32, _p, _p],
     "imped_walkers_state": [_p, _p, _p, _p, _p],
     "imped_walkers_served": [_p, _i64, _i64, _p],
+    "imped_walkers_set_record_cap": [_p, _i64],
     "imped_walkers_reset_counters": [_p],
     "imped_pinned_alloc": [_i64, _p],
     "imped_pinned_free": [_p],

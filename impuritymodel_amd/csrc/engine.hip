@@ -662,6 +662,7 @@ static void build(Batch& b, int mode, double weight_min, int max_layers) {
   sc.err.reserve(4);
   HIP_CHECK(hipMemsetAsync(sc.err.p, 0, 4, s));
   if (b.R) {
+    TimedLaunch tl("csr_fill", s);
     if (b.W == 1)
       hipLaunchKernelGGL(csr_fill_kernel<1>, dim3(blocks_for(b.R, 128)), dim3(128), 0, s, hash_view(b),
                          sc.rowgid.as<int32_t>(), b.row_chain.as<int32_t>(), b.R, b.pool_coloff.as<int64_t>(),
@@ -1503,8 +1504,8 @@ int imped_pinned_free(void* p) {
 int imped_mlp_predict(imped_mlp net, const float* x, int batch, const double* weights, double scale, double* mean,
                       double* disagreement) {
   API_BEGIN
-  if (batch < 1 || !weights || !(scale > 0.0)) {
-    set_error("imped_mlp_predict: batch must be >= 1, weights given and scale > 0");
+  if (!net || !x || !disagreement || batch < 1 || !weights || !(scale > 0.0)) {
+    set_error("imped_mlp_predict: network, inputs, disagreement and weights required, batch >= 1, scale > 0");
     return IMPED_ERR_INVALID;
   }
   mlp_predict(*reinterpret_cast<Mlp*>(net), x, batch, weights, scale, nullptr, mean, disagreement, nullptr, stream());
@@ -1592,6 +1593,10 @@ int imped_walkers_waiting(imped_walkers wk, int32_t* idx, double* proposals, int
 
 int imped_walkers_decide(imped_walkers wk, int32_t count, const int32_t* idx, const double* misfit) {
   API_BEGIN
+  if (!wk || count < 0 || (count > 0 && (!idx || !misfit))) {
+    set_error("imped_walkers_decide: walkers required; indices and misfits for count > 0");
+    return IMPED_ERR_INVALID;
+  }
   Walkers& w = *reinterpret_cast<Walkers*>(wk);
   for (int k = 0; k < count; ++k)
     if (idx[k] < 0 || idx[k] >= w.W || !w.waiting[idx[k]] || (k > 0 && idx[k] <= idx[k - 1])) {
@@ -1604,6 +1609,10 @@ int imped_walkers_decide(imped_walkers wk, int32_t count, const int32_t* idx, co
 
 int imped_walkers_state(imped_walkers wk, double* theta, double* cur, double* best, int64_t* counters) {
   API_BEGIN
+  if (!wk) {
+    set_error("imped_walkers_state: walkers required");
+    return IMPED_ERR_INVALID;
+  }
   const Walkers& w = *reinterpret_cast<const Walkers*>(wk);
   if (theta) std::copy(w.theta.begin(), w.theta.end(), theta);
   if (cur) std::copy(w.cur.begin(), w.cur.end(), cur);
@@ -1619,16 +1628,36 @@ int imped_walkers_state(imped_walkers wk, double* theta, double* cur, double* be
 
 int imped_walkers_served(imped_walkers wk, int64_t start, int64_t count, double* params) {
   API_BEGIN
+  if (!wk || (count > 0 && !params)) {
+    set_error("imped_walkers_served: walkers and an output array required");
+    return IMPED_ERR_INVALID;
+  }
   const Walkers& w = *reinterpret_cast<const Walkers*>(wk);
-  if (start < 0 || count < 0 || start + count > w.n_served) {
-    set_error("imped_walkers_served: rows out of range");
+  const int64_t recorded = (int64_t)(w.served.size() / (size_t)w.P);
+  if (start < 0 || count < 0 || start + count > recorded) {
+    set_error("imped_walkers_served: rows out of range (only the recorded rows can be read)");
     return IMPED_ERR_INVALID;
   }
   std::copy(w.served.begin() + start * w.P, w.served.begin() + (start + count) * w.P, params);
   API_END
 }
 
+int imped_walkers_set_record_cap(imped_walkers wk, int64_t max_rows) {
+  if (!wk || max_rows < 0) {
+    set_error("imped_walkers_set_record_cap: walkers required and max_rows >= 0");
+    return IMPED_ERR_INVALID;
+  }
+  Walkers& w = *reinterpret_cast<Walkers*>(wk);
+  w.record_cap = max_rows;
+  if ((int64_t)(w.served.size() / (size_t)w.P) > max_rows) w.served.resize((size_t)max_rows * w.P);
+  return IMPED_OK;
+}
+
 int imped_walkers_reset_counters(imped_walkers wk) {
+  if (!wk) {
+    set_error("imped_walkers_reset_counters: walkers required");
+    return IMPED_ERR_INVALID;
+  }
   Walkers& w = *reinterpret_cast<Walkers*>(wk);
   w.n_predicted = w.n_served = w.n_accepted = 0;
   w.served.clear();

@@ -296,6 +296,7 @@ int64_t scan_counts(const int32_t* counts, int64_t n, int64_t* offsets, hipStrea
 
 void sort_chain_keys(int W, int n_orb, int n_chain_bits, const uint64_t* keys, const int32_t* chain, int32_t* perm,
                      int64_t n, hipStream_t s) {
+  TimedLaunch tl("sort", s);  // the span of all passes of one BFS layer's sort
   sort_words_then(W, n_orb, n_chain_bits, keys, chain, perm, n, s);
 }
 

@@ -44,7 +44,11 @@ struct Walkers {
   // GPU predicts, and the k-th normal used is always the stream's k-th
   std::vector<double> pool;
   size_t pool_head = 0;
-  std::vector<double> served;  // parameters of every surrogate-served proposal, [n][P]
+  // parameters of the surrogate-served proposals, [n][P]: the first
+  // record_cap of them since the last counter reset (a bounded record: a long
+  // run does not grow host memory without limit)
+  std::vector<double> served;
+  int64_t record_cap = int64_t(1) << 20;
   int64_t n_predicted = 0, n_served = 0, n_accepted = 0;
   int n_waiting = 0;
   // per-step scratch

@@ -79,7 +79,8 @@ int walkers_step(Walkers& w, const double* weights, double scale, const double* 
   for (int k = 0; k < nf; ++k) {
     const int i = w.free_idx[k];
     if (w.dis[k] < w.threshold) {
-      w.served.insert(w.served.end(), w.prop.begin() + (size_t)i * P, w.prop.begin() + (size_t)(i + 1) * P);
+      if (w.n_served < w.record_cap)
+        w.served.insert(w.served.end(), w.prop.begin() + (size_t)i * P, w.prop.begin() + (size_t)(i + 1) * P);
       ++w.n_served;
     }
   }

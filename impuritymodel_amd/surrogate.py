@@ -666,10 +666,17 @@ class NativeFittingLoop:
     def finish(self):
         self.flush()
 
+    def set_record_cap(self, max_rows):
+        """Keep at most ``max_rows`` served proposals for ``served()`` (the
+        library's default is 2^20)."""
+        self._check(self._lib.imped_walkers_set_record_cap(self.h, int(max_rows)))
+        self._record_cap = int(max_rows)
+
     def served(self):
-        """(params, spectra) the surrogate produced since the last counter reset;
-        the spectra are predicted again (rows do not depend on their batch)."""
-        n = self.n_surrogate
+        """(params, spectra) the surrogate produced since the last counter reset
+        (the first record cap of them); the spectra are predicted again (rows
+        do not depend on their batch)."""
+        n = min(self.n_surrogate, getattr(self, "_record_cap", 1 << 20))
         params = np.zeros((n, len(PARAM_NAMES)))
         if n:
             self._check(self._lib.imped_walkers_served(self.h, 0, n, self._ptr(params)))

@@ -48,7 +48,8 @@ int imped_device_synchronize(void);
 /* Launch timing: when enabled, every kernel launch of the engine is bracketed
  * by HIP events on the library's stream; imped_timing_query sums the elapsed
  * time of the launches recorded under a kernel name ("apply_count",
- * "apply_fill", "eigen", "eigen_flat", "lanczos", "spmv", "solve",
+ * "apply_fill", "discover", "sort" (the span of one BFS layer's sort passes),
+ * "csr_fill", "eigen", "eigen_flat", "lanczos", "spmv", "solve",
  * "solve_combine", "cf", "mlp_gemm", "mlp_adam").  No reference counterpart. */
 int imped_timing_enable(int on);
 int imped_timing_reset(void);
@@ -294,8 +295,14 @@ int imped_walkers_decide(imped_walkers w, int32_t count, const int32_t* idx, con
 /* theta [n_walkers][n_params], cur [n_walkers], best misfit so far and the
  * counters {predicted, served, accepted, waiting} (any may be NULL). */
 int imped_walkers_state(imped_walkers w, double* theta, double* cur, double* best, int64_t* counters);
-/* Rows [start, start + count) of the served proposals' parameters. */
+/* Rows [start, start + count) of the recorded served proposals' parameters
+ * (the first record cap of them since the last counter reset). */
 int imped_walkers_served(imped_walkers w, int64_t start, int64_t count, double* params);
+/* Records at most max_rows served proposals (8 n_params bytes each) between
+ * counter resets; later ones are counted but not recorded.  The default cap
+ * is 2^20 rows (64 MiB at 8 parameters), so a long fitting run does not grow
+ * host memory without bound.  No reference counterpart. */
+int imped_walkers_set_record_cap(imped_walkers w, int64_t max_rows);
 /* Zeroes the counters and forgets the served proposals. */
 int imped_walkers_reset_counters(imped_walkers w);
 /* Page-locked host memory (hipHostMalloc) for outputs read back every call:

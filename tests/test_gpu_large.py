@@ -13,7 +13,11 @@ properties the reference's algorithm fixes:
   exactly -- nothing the BFS missed is reachable, nothing extra was added;
 * the BFS basis starts with the seeds in their order, then ascending layers;
 * grid-wide Lanczos of the large chain equals the first steps computed by the
-  workgroup kernel on the same CSR (alpha/beta 1e-10 relative).
+  workgroup kernel on the same CSR (alpha/beta 1e-10 relative);
+* the 300-step XAS, XPS and NIXS Green's functions of every chain: spectral
+  weight -Im g >= 0, the sum rule int -Im g dw = pi |T psi|^2 on a wide grid,
+  and the device continued fraction equal to the same alpha/beta's resolvent
+  evaluated on the host from the tridiagonal's eigendecomposition.
 """
 
 from math import comb
@@ -48,7 +52,7 @@ def large(gpu):
     xb = Batch(ops_h, [0])
     xb.seed_from_vec(seeds)
     xb.build(fixed_basis=False)
-    return dict(m=m, ops_h=ops_h, basis=basis, gb=gb, xb=xb, seeds=seeds, engine=engine, Batch=Batch)
+    return dict(m=m, ops_h=ops_h, basis=basis, gb=gb, xb=xb, seeds=seeds, engine=engine, Batch=Batch, ev=ev[0], W=W)
 
 
 def test_ground_basis_count_and_occupations(large):
@@ -107,3 +111,63 @@ def test_flat_lanczos_matches_workgroup_kernel_at_full_size(large):
     assert k1[0] == k2[0] == 20 and s1[0] == s2[0] == 0
     np.testing.assert_allclose(a_flat, a_wg, rtol=1e-10, atol=0)
     np.testing.assert_allclose(b_flat[:, :19], b_wg[:, :19], rtol=1e-10, atol=0)
+
+
+def _host_resolvent(alpha, beta, z):
+    """sum_k q_k[0]^2 / (z - theta_k) of the tridiagonal (alpha, beta)."""
+    K = len(alpha)
+    t = np.diag(alpha) + np.diag(beta[: K - 1], 1) + np.diag(beta[: K - 1], -1)
+    theta, q = np.linalg.eigh(t)
+    wgt = q[0] ** 2
+    return (wgt[None, :] / (z[:, None] - theta[None, :])).sum(axis=1)
+
+
+@pytest.mark.parametrize("kind", ["XAS", "XPS", "NIXS"])
+def test_full_size_spectra_properties(large, kind):
+    import os
+
+    from impuritymodel_amd.batched import POLARIZATIONS
+    from impuritymodel_amd.engine import Ops, vec_apply
+    from impuritymodel_amd.models import K_B
+    from impuritymodel_amd.operators import getDipoleOperators, getNIXSOperators, getPhotoEmissionOperators
+
+    m, gb, Batch, W = large["m"], large["gb"], large["Batch"], large["W"]
+    n = m.n_spin_orbitals
+    e = large["ev"]
+    kept = [i for i in range(3) if e[i] - e[0] < 10.0 * K_B * 300.0]
+    psis = gb.vec_from_eigen(np.zeros(len(kept), np.int32), np.array(kept, np.int32), 1e-7)
+    if kind == "XAS":
+        tops, sign = getDipoleOperators(m.nBaths, POLARIZATIONS), 1
+    elif kind == "XPS":
+        tops, sign = getPhotoEmissionOperators(m.nBaths, l=1), -1
+    else:
+        r, R = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "Ni3d.dat")).T
+        qs = [2.0 * np.ones(3) / np.sqrt(3.0), 7.0 * np.ones(3) / np.sqrt(3.0)]
+        tops, sign = getNIXSOperators(m.nBaths, qs, 2, 2, R, R, r), 1
+    ops_t = Ops(tops, [m.restrictions] * len(tops), n, W)
+    nt, ne = len(tops), len(kept)
+    out_src = np.array([i for t in range(nt) for i in range(ne)], np.int32)
+    out_model = np.array([t for t in range(nt) for i in range(ne)], np.int32)
+    seeds = vec_apply(ops_t, psis, out_src, out_model, 1e-7, normalize=True)
+    norm2 = seeds.norm2()
+    xb = Batch(large["ops_h"], np.zeros(len(out_src), np.int32))
+    xb.seed_from_vec(seeds)
+    xb.build(fixed_basis=False)
+    print("%s: %d chains, sizes %d..%d" % (kind, xb.C, xb.chain_n.min(), xb.chain_n.max()), flush=True)
+    K = 300
+    alpha, beta, kout, status = xb.lanczos(K)
+    assert np.all(status == 0) and np.all(kout == K)
+    omega = np.linspace(-300.0, 300.0, 30001)
+    delta = 0.2
+    energy = e[np.array(kept)][out_src]
+    ones = np.ones(xb.C, np.int32)
+    g, _ = xb.cf(energy, norm2, np.full(xb.C, sign, np.int32), ones, 1.0, np.arange(xb.C, dtype=np.int32), xb.C,
+                 omega, delta, want_g=True)
+    for c in range(xb.C):
+        a_c = -g[c].imag
+        assert a_c.min() >= -1e-12 * a_c.max()
+        total = np.trapezoid(a_c, omega)
+        assert abs(total - np.pi * norm2[c]) <= 3e-3 * np.pi * norm2[c], (c, total, np.pi * norm2[c])
+        z = sign * (omega + 1j * delta) + energy[c]
+        ref = sign * norm2[c] * _host_resolvent(alpha[c], beta[c], z)
+        assert np.abs(g[c] - ref).max() <= 1e-9 * np.abs(ref).max()

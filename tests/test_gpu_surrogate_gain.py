@@ -1,0 +1,78 @@
+"""What the trained surrogate gains over the fitting loop without it.
+
+A small ensemble (3 members, the bench's 8-512-512-512-1000 architecture) is
+trained by the library's kernels on exact spectra of the XAS family, its
+trust threshold calibrated on held-out exact spectra to 0.8 x the 1e-2 L1
+bar.  The fitting loop then runs the same walkers (same seeds) with and
+without it.  With the surrogate:
+
+* every spectrum it serves is within L1 1e-2 of exact ED (recomputed here);
+* the loop needs fewer exact evaluations for the same number of steps;
+* it produces more spectra per second.
+"""
+
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_surrogate_gain_over_exact_only_loop(gpu):
+    import torch
+
+    from impuritymodel_amd import surrogate as S
+
+    fam = S.XASFamily()
+    P = S.sample_params(8192, seed=5)
+    Y = np.concatenate([S.exact_spectra(fam, P[i:i + 4096])[0] for i in range(0, len(P), 4096)])
+    Pv = S.sample_params(1024, seed=6)
+    Yv = S.exact_spectra(fam, Pv)[0]
+    sur = S.Surrogate(fam, S.Ensemble(3, (8, 512, 512, 512, len(S.OMEGA)), max_batch=1024, seed=1))
+    t0 = time.perf_counter()
+    losses = sur.fit(P, Y, epochs=400, seed=1)
+    print("training: %d Adam steps in %.1f s, final loss %.3g" % (len(losses), time.perf_counter() - t0,
+                                                                  float(losses[-20:].mean())), flush=True)
+    err, dis = sur.calibrate(Pv, Yv, bar=0.8e-2)
+    trusted = dis < sur.threshold
+    print("validation: median L1 %.4f, trusted %.3f" % (float(np.median(err)), float(trusted.mean())), flush=True)
+    assert trusted.mean() > 0.2
+
+    target = S.exact_spectra(fam, S.sample_params(1, seed=99))[0][0]
+    steps, walkers = 20, 128
+
+    def run(surrogate):
+        if surrogate is None:
+            loop = S.FittingLoop(fam, target, None, n_walkers=walkers, seed=3)
+        else:
+            loop = S.NativeFittingLoop(fam, target, surrogate, n_walkers=walkers, seed=3)
+        loop.reset_counters()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            loop.step()
+        loop.finish()
+        torch.cuda.synchronize()
+        return loop, time.perf_counter() - t
+
+    run(sur)  # warm-up of the prediction path
+    loop_s, dt_s = run(sur)
+    loop_e, dt_e = run(None)
+    n_exact_s, n_exact_e = loop_s.n_exact, loop_e.n_exact
+    rate_s, rate_e = loop_s.n_spectra / dt_s, loop_e.n_spectra / dt_e
+    print("with surrogate: %d spectra (%d served, %d exact) in %.4f s = %.0f spectra/s"
+          % (loop_s.n_spectra, loop_s.n_surrogate, n_exact_s, dt_s, rate_s), flush=True)
+    print("exact only:     %d spectra (%d exact) in %.4f s = %.0f spectra/s"
+          % (loop_e.n_spectra, n_exact_e, dt_e, rate_e), flush=True)
+    # every served spectrum against exact ED
+    sp, ss = loop_s.served()
+    assert len(sp) == loop_s.n_surrogate > 0
+    exact = np.concatenate([S.exact_spectra(fam, sp[i:i + 4096])[0] for i in range(0, len(sp), 4096)])
+    l1 = S.l1_error(ss, exact)
+    print("served L1 vs exact: max %.4f, mean %.4f" % (float(l1.max()), float(l1.mean())), flush=True)
+    assert np.all(l1 <= 1e-2)
+    # the same number of steps needs fewer exact evaluations per spectrum and yields more spectra per second
+    assert n_exact_s < n_exact_e
+    assert n_exact_s / loop_s.n_spectra < n_exact_e / loop_e.n_spectra
+    assert rate_s > rate_e

@@ -111,3 +111,20 @@ def test_c_abi_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     # the Python binding declares argtypes for every function of the header
     assert names - {"imped_last_error"} <= set(_lib.SIGNATURES)
+
+
+def test_c_abi_rejects_null_arguments():
+    """Entry points check their handles and arrays before touching memory or
+    the GPU: NULL arguments return IMPED_ERR_INVALID with a message."""
+    from impuritymodel_amd._lib import IMPED_ERR_INVALID, lib
+
+    w = np.ones(4)
+    assert lib.imped_mlp_predict(None, None, 4, None, 1.0, None, None) == IMPED_ERR_INVALID
+    assert b"imped_mlp_predict" in lib.imped_last_error()
+    assert lib.imped_walkers_decide(None, 1, None, None) == IMPED_ERR_INVALID
+    assert lib.imped_walkers_state(None, None, None, None, None) == IMPED_ERR_INVALID
+    assert lib.imped_walkers_served(None, 0, 1, None) == IMPED_ERR_INVALID
+    assert lib.imped_walkers_reset_counters(None) == IMPED_ERR_INVALID
+    assert lib.imped_walkers_set_record_cap(None, 10) == IMPED_ERR_INVALID
+    assert lib.imped_walkers_step(None, w.ctypes.data, 1.0, w.ctypes.data, None) == IMPED_ERR_INVALID
+    assert lib.imped_walkers_waiting(None, None, None, None) == IMPED_ERR_INVALID
