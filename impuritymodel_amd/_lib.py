@@ -93,11 +93,28 @@ class ImpedError(RuntimeError):
         self.code = code
 
 
+def _torch_runtime_first():
+    """Let PyTorch's GPU runtime initialize before this library is loaded.
+
+    PyTorch carries its own HIP runtime.  When libimped.so is loaded first
+    and PyTorch's runtime then initializes the device, the first HIP call of
+    libimped.so finds no device (hipErrorNoDevice); with PyTorch's runtime
+    initialized first, both work.  Counting devices does not initialize
+    anything; ``torch.cuda.init()`` runs only where a device exists."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.device_count() > 0:
+        torch.cuda.init()
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             "libimped.so is not built (expected at %s); run python -m impuritymodel_amd.build_lib" % LIB_PATH
         )
+    _torch_runtime_first()
     lib = C.CDLL(LIB_PATH)
     for name, args in SIGNATURES.items():
         f = getattr(lib, name)

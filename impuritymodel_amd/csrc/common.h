@@ -118,6 +118,7 @@ struct Tuning {
   int64_t eigen_lds_vectors = -1;       // cap on the eigensolver's LDS Krylov vectors (-1: as many as fit)
   int64_t small_sort = 1;               // one-workgroup sort for small key sets (0: digit passes always)
   int64_t mlp_graph = 0;                // surrogate predictions as one replayed HIP graph (0: direct launches)
+  int64_t lanczos_lds = 1;              // reorthogonalized chains whose Krylov basis fits: basis in LDS
 };
 extern Tuning g_tuning;
 

@@ -892,6 +892,8 @@ int imped_set_tuning(const char* name, int64_t value) {
     g_tuning.small_sort = value != 0;
   } else if (n == "mlp_graph") {
     g_tuning.mlp_graph = value != 0;
+  } else if (n == "lanczos_lds") {
+    g_tuning.lanczos_lds = value != 0;
   } else if (n == "eigen_lds_vectors") {
     g_tuning.eigen_lds_vectors = value < 0 ? -1 : value;
   } else if (n == "lanczos_graph_steps") {

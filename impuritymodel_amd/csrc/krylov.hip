@@ -17,7 +17,9 @@
 //   omega grid for every chain, fused with the Boltzmann weights of
 //   thermal_average (average.py:11-45) and the sum over transition operators
 //   (spectra.py:233-238).
+#include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "krylov.h"
 
@@ -155,6 +157,136 @@ __global__ __launch_bounds__(BT) void lanczos_kernel(LanczosArgs a) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
+    a.kout[c] = kdone;
+    a.status[c] = status;
+  }
+}
+
+// ---- fully reorthogonalized Lanczos with the Krylov basis in LDS
+//
+// For a chain whose K' = min(K, n) Krylov vectors fit in LDS next to the
+// residual, one wave runs the whole recurrence on LDS: V[l] at row l of
+// stride s (the smallest s >= n with s = 1 mod 8, so both the row-parallel
+// update, lane i reading V[l][i], and the vector-parallel dot pass, lane l
+// reading V[l][i], touch 64 distinct LDS banks), then w, then the
+// Gram-Schmidt coefficients.  The dot pass gives lane l the whole product
+// <V_l, w> (a sequential sum over rows), so a step needs no per-vector
+// wave reductions.  The SpMV, alpha, beta and three-term update are those
+// of lanczos_kernel; v_{j-1} and v_j are rows j-1 and j of V.
+
+__device__ __forceinline__ int64_t lds_row_stride(int64_t n) { return (n + 6) / 8 * 8 + 1; }
+
+size_t lanczos_lds_bytes(int64_t n, int K) {
+  const int64_t kk = K < n ? K : n;
+  const int64_t s = (n + 6) / 8 * 8 + 1;
+  return (size_t)(kk * s + s + kk + 2) * sizeof(double2);
+}
+
+__global__ __launch_bounds__(64) void lanczos_lds_kernel(LanczosArgs a) {
+  extern __shared__ double2 lds[];
+  const int c = a.chain_list[blockIdx.x];
+  const int64_t n = a.chain_n[c];
+  const int64_t base = a.row_base[c];
+  double* alpha = a.alpha + (size_t)c * a.K;
+  double* beta = a.beta + (size_t)c * a.K;
+  const int lane = threadIdx.x;
+  if (n == 0) {
+    if (lane == 0) {
+      a.kout[c] = 0;
+      a.status[c] = IMPED_OK;
+    }
+    return;
+  }
+  const int K = (int)(a.K < n ? a.K : n);
+  const int64_t s = lds_row_stride(n);
+  double2* V = lds;
+  double2* w = V + (size_t)K * s;
+  double2* cl = w + s;
+  const int64_t* rowptr = a.rowptr + base;
+  // y = H x and Re<x, y> over the chain's rows (x, y in LDS)
+  auto spmv = [&](const double2* x, double2* y) {
+    double pa = 0.0;
+    for (int64_t i = lane; i < n; i += 64) {
+      const int64_t e0 = rowptr[i], e1 = rowptr[i + 1];
+      double ax = 0.0, ay = 0.0;
+      for (int64_t e = e0; e < e1; ++e) {
+        const double2 h = a.val[e];
+        const double2 v = x[a.col[e]];
+        ax += h.x * v.x - h.y * v.y;
+        ay += h.x * v.y + h.y * v.x;
+      }
+      y[i] = make_double2(ax, ay);
+      pa += ax * x[i].x + ay * x[i].y;
+    }
+    return wave_sum(pa);
+  };
+  // one classical Gram-Schmidt pass against V_0..V_{m-1}; returns |w|^2
+  auto gram_schmidt = [&](int m) {
+    for (int l = lane; l < m; l += 64) {
+      const double2* vl = V + (size_t)l * s;
+      double cx = 0.0, cy = 0.0;
+#pragma unroll 4
+      for (int64_t i = 0; i < n; ++i) {
+        const double2 v = vl[i], b = w[i];
+        cx += v.x * b.x + v.y * b.y;
+        cy += v.x * b.y - v.y * b.x;
+      }
+      cl[l] = make_double2(cx, cy);
+    }
+    __syncthreads();
+    double p = 0.0;
+    for (int64_t i = lane; i < n; i += 64) {
+      double wx = w[i].x, wy = w[i].y;
+#pragma unroll 8
+      for (int l = 0; l < m; ++l) {
+        const double2 cc = cl[l];
+        const double2 av = V[(size_t)l * s + i];
+        wx -= av.x * cc.x - av.y * cc.y;
+        wy -= av.x * cc.y + av.y * cc.x;
+      }
+      w[i] = make_double2(wx, wy);
+      p += wx * wx + wy * wy;
+    }
+    __syncthreads();
+    return wave_sum(p);
+  };
+  for (int64_t i = lane; i < n; i += 64) V[i] = a.v0[base + i];
+  __syncthreads();
+  // wp = H v0; alpha0 = Re(conj(wp) . v0); w = wp - alpha0 v0
+  double al = spmv(V, w);
+  if (lane == 0) alpha[0] = al;
+  for (int64_t i = lane; i < n; i += 64) w[i] = make_double2(w[i].x - al * V[i].x, w[i].y - al * V[i].y);
+  __syncthreads();
+  int status = IMPED_OK;
+  int kdone = K;
+  for (int j = 1; j < K; ++j) {
+    double p = 0.0;
+    for (int64_t i = lane; i < n; i += 64) p += w[i].x * w[i].x + w[i].y * w[i].y;
+    p = wave_sum(p);
+    // full reorthogonalization against V_0..V_{j-1}; a second pass when
+    // the first removed more than half of |w|^2 (as lanczos_kernel)
+    double p1 = gram_schmidt(j);
+    if (p1 < 0.5 * p) p1 = gram_schmidt(j);
+    const double b = sqrt(p1);
+    if (lane == 0) beta[j - 1] = b;
+    if (b == 0.0) {
+      status = IMPED_ERR_BREAKDOWN;
+      kdone = j;
+      break;
+    }
+    double2* vj = V + (size_t)j * s;
+    const double2* vp = vj - s;
+    for (int64_t i = lane; i < n; i += 64) vj[i] = make_double2(w[i].x / b, w[i].y / b);
+    __syncthreads();
+    al = spmv(vj, w);
+    if (lane == 0) alpha[j] = al;
+    for (int64_t i = lane; i < n; i += 64) {
+      const double2 x = vj[i], y = vp[i];
+      w[i] = make_double2(w[i].x - al * x.x - b * y.x, w[i].y - al * x.y - b * y.y);
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
     a.kout[c] = kdone;
     a.status[c] = status;
   }
@@ -1911,6 +2043,22 @@ int eigen_flat_chain(const KrylovBatch& kb, int c, int nev, double tol, int max_
 
 }  // namespace
 
+namespace {
+
+// LDS of one chain of lanczos_lds_kernel: two chains per CU (160 KiB).
+constexpr size_t LDS_CHAIN_BYTES = 80 * 1024;
+
+void launch_lanczos_lds(const LanczosArgs& a, unsigned nb, size_t bytes, hipStream_t s) {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    HIP_CHECK(hipFuncSetAttribute((const void*)lanczos_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)LDS_CHAIN_BYTES));
+  });
+  hipLaunchKernelGGL(lanczos_lds_kernel, dim3(nb), dim3(64), bytes, s, a);
+}
+
+}  // namespace
+
 void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, double* alpha, double* beta,
                  int32_t* kout, int32_t* status, hipStream_t s) {
   KScratch& sc = kscratch();
@@ -1939,13 +2087,21 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
   a.kout = kout;
   a.status = status;
   if (kb.C == 0) return;
-  // chains bucketed by dimension: one wave, 256 or 1024 threads per chain;
-  // the largest (plain recurrence) run grid-wide, one chain at a time
-  std::vector<int32_t> lists[3], flat;
+  // chains bucketed by dimension: with reorthogonalization, chains whose
+  // Krylov basis fits LDS_CHAIN_BYTES get one wave with the basis in LDS;
+  // otherwise one wave, 256 or 1024 threads per chain; the largest (plain
+  // recurrence) run grid-wide, one chain at a time
+  std::vector<int32_t> lists[4], flat;
+  size_t lds_max = 0;
   for (int c = 0; c < kb.C; ++c) {
     const int64_t n = kb.h_n[c];
     if (!reorth && n > g_tuning.flat_min_n) {
       flat.push_back(c);
+      continue;
+    }
+    if (reorth && g_tuning.lanczos_lds && lanczos_lds_bytes(n, K) <= LDS_CHAIN_BYTES) {
+      lists[3].push_back(c);
+      lds_max = std::max(lds_max, lanczos_lds_bytes(n, K));
       continue;
     }
     const int bkt = n <= g_tuning.lanczos_wave_max_n ? 0 : (n <= g_tuning.lanczos_wg256_max_n ? 1 : 2);
@@ -1957,7 +2113,7 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
   for (auto& l : lists) all.insert(all.end(), l.begin(), l.end());
   h2d(sc.clist, all.data(), all.size(), s);
   size_t off = 0;
-  for (int bkt = 0; bkt < 3; ++bkt) {
+  for (int bkt = 0; bkt < 4; ++bkt) {
     const unsigned nb = (unsigned)lists[bkt].size();
     if (!nb) continue;
     a.chain_list = sc.clist.as<int32_t>() + off;
@@ -1967,8 +2123,10 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
       hipLaunchKernelGGL(lanczos_kernel<64>, dim3(nb), dim3(64), shm, s, a);
     else if (bkt == 1)
       hipLaunchKernelGGL(lanczos_kernel<256>, dim3(nb), dim3(256), shm, s, a);
-    else
+    else if (bkt == 2)
       hipLaunchKernelGGL(lanczos_kernel<1024>, dim3(nb), dim3(1024), shm, s, a);
+    else
+      launch_lanczos_lds(a, nb, lds_max, s);
     HIP_CHECK(hipGetLastError());
   }
   // the chain list buffer must outlive the launches

@@ -4,10 +4,10 @@
 // appended in ascending key order (``basis += sorted(basis_new)``,
 // impurityModel/ed/finite.py:1749), and applyOp's insertion order decides the
 // order of a transition operator's output (finite.py:1375-1388).  Records are
-// sorted by stable LSD radix passes over the key words (least significant
-// first) and then the chain id, using rocPRIM's device radix sort.
+// sorted by this file's stable LSD radix passes over the key words (least
+// significant first) and then the chain id; counts become offsets by this
+// file's exclusive scan.
 #include <cstring>
-#include <rocprim/device/device_scan.hpp>
 
 #include "kernels.h"
 
@@ -23,23 +23,119 @@ Scratch& scratch() {
   return s;
 }
 
-__global__ void widen_kernel(const int32_t* in, int64_t* out, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = in[i];
+// ---- exclusive scan of int32 counts into int64 offsets
+//
+// A tile is SCAN_T threads x SCAN_ITEMS consecutive counts per thread: each
+// thread sums its items, each wave scans its threads' sums with lane
+// shuffles, the wave totals are scanned through LDS, and every thread writes
+// its items' offsets.  An input of one tile takes one launch; a larger one
+// takes its tile totals, a scan of those totals (the same way, one level
+// down) and the tile scans.
+
+constexpr int SCAN_T = 1024;
+constexpr int SCAN_ITEMS = 8;
+constexpr int64_t SCAN_TILE = (int64_t)SCAN_T * SCAN_ITEMS;
+
+__device__ __forceinline__ int64_t wave_inclusive_scan(int64_t x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+// Exclusive prefix of v over the workgroup (thread order); *total gets the sum.
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
+  __shared__ int64_t wsum[SCAN_T / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t inc = wave_inclusive_scan(v);
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  if (wave == 0) {
+    const int64_t t = lane < SCAN_T / 64 ? wsum[lane] : 0;
+    const int64_t ti = wave_inclusive_scan(t);
+    if (lane < SCAN_T / 64) wsum[lane] = ti - t;
+    if (lane == SCAN_T / 64 - 1) *total = ti;
+  }
+  __syncthreads();
+  const int64_t r = wsum[wave] + inc - v;
+  __syncthreads();
+  return r;
+}
+
+template <class In>
+__device__ __forceinline__ void load_items(const In* in, int64_t n, int64_t first, int64_t* it) {
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t i = first + k;
+    it[k] = i < n ? (int64_t)in[i] : 0;
+  }
+}
+
+// Sum of each tile's values.
+template <class In>
+__global__ __launch_bounds__(SCAN_T) void scan_tile_sums_kernel(const In* in, int64_t n, int64_t* sums) {
+  __shared__ int64_t total;
+  int64_t it[SCAN_ITEMS];
+  load_items(in, n, (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS, it);
+  int64_t t = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) t += it[k];
+  block_exclusive_scan(t, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Exclusive offsets of every value: the tile's prefix (sums, or 0) plus the
+// in-tile scan.  A thread reads its own items before writing them, so out
+// may be in.
+template <class In>
+__global__ __launch_bounds__(SCAN_T) void scan_tiles_kernel(const In* in, int64_t n, const int64_t* sums,
+                                                            int64_t* out) {
+  __shared__ int64_t total;
+  int64_t it[SCAN_ITEMS];
+  const int64_t first = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  load_items(in, n, first, it);
+  int64_t t = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) t += it[k];
+  int64_t run = block_exclusive_scan(t, &total) + (sums ? sums[blockIdx.x] : 0);
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (first + k < n) out[first + k] = run;
+    run += it[k];
+  }
+}
+
+// Exclusive scan of n values of in into out, one level per SCAN_TILE factor:
+// tile sums, their scan (recursively), tile scans.
+template <class In>
+void scan_level(const In* in, int64_t n, int64_t* out, int level, hipStream_t s) {
+  const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (tiles == 1) {
+    hipLaunchKernelGGL(scan_tiles_kernel<In>, dim3(1), dim3(SCAN_T), 0, s, in, n, (const int64_t*)nullptr, out);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  if (tiles > (int64_t)UINT32_MAX) throw HipError{"scan: too many values"};
+  static DevBuf levels[4];  // tile sums of each level (4 levels cover 2^52 values)
+  if (level >= 4) throw HipError{"scan: too many levels"};
+  DevBuf& sb = levels[level];
+  sb.reserve(tiles * sizeof(int64_t));
+  int64_t* sums = sb.as<int64_t>();
+  hipLaunchKernelGGL(scan_tile_sums_kernel<In>, dim3((unsigned)tiles), dim3(SCAN_T), 0, s, in, n, sums);
+  HIP_CHECK(hipGetLastError());
+  scan_level<int64_t>(sums, tiles, sums, level + 1, s);
+  hipLaunchKernelGGL(scan_tiles_kernel<In>, dim3((unsigned)tiles), dim3(SCAN_T), 0, s, in, n, (const int64_t*)sums,
+                     out);
+  HIP_CHECK(hipGetLastError());
 }
 
 // Exclusive scan of int32 counts into int64 offsets, stream-ordered.
 void scan_counts_async(const int32_t* counts, int64_t n, int64_t* offsets, hipStream_t s) {
   if (n == 0) return;
-  Scratch& sc = scratch();
-  sc.wide.reserve(n * sizeof(int64_t));
-  int64_t* wide = sc.wide.as<int64_t>();
-  hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, counts, wide, n);
-  size_t tb = 0;
-  HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, wide, offsets, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s));
-  sc.temp.reserve(tb);
-  HIP_CHECK(
-      rocprim::exclusive_scan(sc.temp.p, tb, wide, offsets, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s));
+  scan_level<int32_t>(counts, n, offsets, 0, s);
 }
 
 __global__ void gather_word_kernel(const uint64_t* keys, int W, int w, const int32_t* perm, uint64_t* out,

@@ -11,8 +11,6 @@
 // truncate_eigvecs: eigenvector dicts of eigensystem (finite.py:114-117),
 // components with |v_i|^2 >= slaterWeightMin in basis order, not renormalized.
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 #include "krylov.h"
 

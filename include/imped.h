@@ -71,7 +71,10 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * workgroup; 0: always the digit passes), "mlp_graph" (1: a surrogate
  * prediction's kernels run as one replayed HIP graph, which shortens the
  * GPU's span but costs the calling thread more than the launches it
- * replaces; 0, the default: direct launches, as with switched-on timing). */
+ * replaces; 0, the default: direct launches, as with switched-on timing),
+ * "lanczos_lds" (1, the default: a fully reorthogonalized chain whose Krylov
+ * basis fits 80 KiB runs on one wave with the basis in LDS; 0: the basis in
+ * memory). */
 int imped_set_tuning(const char* name, int64_t value);
 
 /* The engine's device radix sort (the one that orders BFS layers and

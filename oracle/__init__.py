@@ -27,7 +27,11 @@ How each part is pinned (the tests named here live in ``tests/``):
   hold, seeds first, layers ascending).
 * ``lanczos``/``spectra``: exact resolvent from dense ``eigh`` when the Krylov
   space is the whole space, Hankel moments, and the sum rule
-  ``int A dw = pi |T psi|^2``.
+  ``int A dw = pi |T psi|^2``; the compositions (``get_spectra`` with its
+  per-operator memo, the photoemission negation, thermal averages,
+  operator sums) by the reference's own golden 50-bath run: ``XAS.dat``,
+  ``NIXS.dat``, ``XPS.dat`` and ``PS.dat`` and every BFS basis size of its
+  log (``test_refout_oracle.py``).
 * ``eigen``: dense ``eigh`` brute force.
 * ``average``: closed form of the Boltzmann average.
 * ``spectra.rixs_map``: dense Kramers-Heisenberg amplitudes in the full Fock
@@ -36,9 +40,12 @@ How each part is pinned (the tests named here live in ``tests/``):
   gradients against central finite differences; the BF16 emulation mode
   against the float64 mode within BF16 rounding.
 
-The reference's own pickled data (``h0/*.pickle``, the object arrays of
-``referenceOutput/.../data.npz``) is not loaded: the safe loader
-(``numpy.load(allow_pickle=False)``) refuses it.  The end-to-end golden spectra
-of the 50-bath run therefore cannot be reproduced here; see the tests for the
-pins used instead.
+* ``basis``/``eigen``/``observables`` on the golden run too: basis 1770,
+  E0 = -903.8237, the E - E0 and expectation-value tables and the
+  eigenvectors' product-state counts of ``output.txt``.
+
+The golden run's inputs (the pickled h0 and the ``hOp``/``restrictions`` of
+``data.npz``) are read by ``impuritymodel_amd.readers``' allow-list
+unpickler, which builds nothing but numpy values, ordered dicts, frozensets
+and complex numbers, and are kept as fixtures in ``tests/golden/``.
 """
