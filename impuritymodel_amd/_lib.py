@@ -60,6 +60,7 @@ SIGNATURES = {
     "imped_batch_get_basis": [_p, _p],
     "imped_batch_get_csr": [_p, _p, _p, _p],
     "imped_batch_lanczos": [_p, _p, _i32, _i32, _p, _p, _p, _p],
+    "imped_lanczos_noalloc": [_p, _p, _p, _i64, _p, _i32, _i32, _p, _p, _p, _p, _p],
     "imped_batch_eigen": [_p, _i32, _d, _i32, _u64, _p, _p, _p],
     "imped_batch_cf": [_p, _p, _p, _p, _p, _p, _p, _i32, _p, _i32, _d, _p, _p],
     "imped_mlp_create": [_i32, _i32, _p, _i32, _p],

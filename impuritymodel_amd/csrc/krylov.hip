@@ -2429,3 +2429,119 @@ void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t
 }
 
 }  // namespace imped
+
+// ------------------------------------------------------------------ allocation-free Lanczos
+//
+// imped_lanczos_noalloc runs the Lanczos kernel of imped_batch_lanczos on one
+// chain whose CSR, start vector and outputs the caller holds in device
+// memory, with every intermediate (a one-chain header and the Lanczos vectors)
+// in the fixed device array `scratch` below instead of memory the library
+// allocates.  The array's device address comes from hipGetSymbolAddress once
+// per device; the array name itself is never used as an address in host code.
+
+namespace imped {
+namespace {
+
+constexpr size_t kScratchBytes = size_t(64) << 20;  // per device, for the life of the process
+constexpr size_t kScratchHeader = 256;              // row_base[2], chain_n[1] (int64), chain_list[1] (int32)
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array (looked up on first use).
+void* scratch_address(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// The chain table of a one-chain launch: row_base = {0, n}, chain_n = {n},
+// chain_list = {0} (written by a kernel: no host copies, nothing to clear).
+__global__ void one_chain_header_kernel(int64_t* h, int64_t n) {
+  h[0] = 0;
+  h[1] = n;
+  h[2] = n;
+  reinterpret_cast<int32_t*>(h + 3)[0] = 0;
+}
+
+}  // namespace
+}  // namespace imped
+
+using imped::HipError;
+
+extern "C" int imped_lanczos_noalloc(const int64_t* rowptr, const int32_t* col, const double* val, int64_t n,
+                                     const double* v0, int K, int reorth, double* alpha, double* beta, int32_t* kout,
+                                     int32_t* status, void* stream) {
+  using namespace imped;
+  try {
+    if (!rowptr || !col || !val || !v0 || !alpha || !beta || !kout || !status || n < 1 || K < 1) {
+      set_error("imped_lanczos_noalloc: null pointer, n < 1 or K < 1");
+      return IMPED_ERR_INVALID;
+    }
+    if (reorth && K > 3500) {
+      set_error("imped_lanczos_noalloc: K must be <= 3500 with reorthogonalization");
+      return IMPED_ERR_INVALID;
+    }
+    const int64_t kk = std::min<int64_t>(K, n);
+    const double need = (double)kScratchHeader + 48.0 * (double)n + (reorth ? 16.0 * (double)kk * (double)n : 0.0);
+    if (need > (double)kScratchBytes) {
+      set_error("imped_lanczos_noalloc: the chain needs " + std::to_string((long long)need) +
+                " bytes of scratch (48 n, plus 16 n min(K, n) with reorthogonalization); the fixed scratch holds " +
+                std::to_string((long long)kScratchBytes));
+      return IMPED_ERR_CAPACITY;
+    }
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+      set_error("imped_lanczos_noalloc: device index out of range");
+      return IMPED_ERR_INVALID;
+    }
+    unsigned char* base = static_cast<unsigned char*>(scratch_address(dev));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int64_t* head = reinterpret_cast<int64_t*>(base);
+    hipLaunchKernelGGL(one_chain_header_kernel, dim3(1), dim3(1), 0, s, head, n);
+    HIP_CHECK(hipGetLastError());
+    LanczosArgs a{};
+    a.row_base = head;
+    a.chain_n = head + 2;
+    a.chain_list = reinterpret_cast<const int32_t*>(head + 3);
+    a.rowptr = rowptr;
+    a.col = col;
+    a.val = reinterpret_cast<const double2*>(val);
+    a.v0 = reinterpret_cast<const double2*>(v0);
+    a.K = K;
+    a.reorth = reorth;
+    a.ws = reinterpret_cast<double2*>(base + kScratchHeader);
+    a.vstore = reorth ? a.ws + 3 * n : nullptr;
+    a.alpha = alpha;
+    a.beta = beta;
+    a.kout = kout;
+    a.status = status;
+    const size_t shm = reorth ? (size_t)(K + 2) * sizeof(double2) : 0;
+    // the size buckets of run_lanczos, so a chain gets the same kernel (and
+    // the same arithmetic) as through imped_batch_lanczos
+    if (reorth && g_tuning.lanczos_lds && lanczos_lds_bytes(n, K) <= LDS_CHAIN_BYTES)
+      launch_lanczos_lds(a, 1, lanczos_lds_bytes(n, K), s);
+    else if (n <= g_tuning.lanczos_wave_max_n)
+      hipLaunchKernelGGL(lanczos_kernel<64>, dim3(1), dim3(64), shm, s, a);
+    else if (n <= g_tuning.lanczos_wg256_max_n)
+      hipLaunchKernelGGL(lanczos_kernel<256>, dim3(1), dim3(256), shm, s, a);
+    else
+      hipLaunchKernelGGL(lanczos_kernel<1024>, dim3(1), dim3(1024), shm, s, a);
+    HIP_CHECK(hipGetLastError());
+  } catch (const HipError& e) {
+    set_error(e.msg);
+    return IMPED_ERR_HIP;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return IMPED_ERR_INVALID;
+  }
+  return IMPED_OK;
+}

@@ -197,6 +197,23 @@ int imped_batch_get_csr(imped_batch batch, int64_t* rowptr, int32_t* col, double
 int imped_batch_lanczos(imped_batch batch, const double* v0, int K, int reorth, double* alpha, double* beta,
                         int32_t* kout, int32_t* status);
 
+/* Allocation-free variant of imped_batch_lanczos for one chain: the same
+ * Lanczos kernel (plain recurrence, or reorth = 1) on a CSR, start vector and
+ * outputs that the caller holds in DEVICE memory -- rowptr[n+1] int64, col
+ * int32, val complex, v0[n] complex, alpha[K], beta[K], kout[1], status[1] --
+ * queued on `stream` (a hipStream_t; NULL = the null stream) without
+ * synchronizing.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  It calls no hipMalloc/hipFree: the Lanczos vectors live in
+ * a fixed 64 MiB device array per GPU, which limits a chain to
+ * 256 + 48 n (+ 16 n min(K, n) with reorthogonalization) bytes, i.e. about
+ * 1.39e6 rows plain; larger chains return IMPED_ERR_CAPACITY.  Calls that
+ * overlap on one GPU share that array and race; the array keeps the last
+ * call's contents.  Returns after queuing the work. */
+int imped_lanczos_noalloc(const int64_t* rowptr, const int32_t* col, const double* val, int64_t n, const double* v0,
+                          int K, int reorth, double* alpha, double* beta, int32_t* kout, int32_t* status,
+                          void* stream);
+
 /* eigsh(h, k=nev, which="SA", tol) of eigensystem (finite.py:95-109) for every
  * chain: evals[C][nev] ascending, evecs (may be NULL) rows*nev complex with
  * chain c's block at row_base[c]*nev and eigenvector k at offset k*n_c.

@@ -128,3 +128,5 @@ def test_c_abi_rejects_null_arguments():
     assert lib.imped_walkers_set_record_cap(None, 10) == IMPED_ERR_INVALID
     assert lib.imped_walkers_step(None, w.ctypes.data, 1.0, w.ctypes.data, None) == IMPED_ERR_INVALID
     assert lib.imped_walkers_waiting(None, None, None, None) == IMPED_ERR_INVALID
+    assert lib.imped_lanczos_noalloc(None, None, None, 10, None, 5, 0, None, None, None, None, None) == \
+        IMPED_ERR_INVALID
