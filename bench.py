@@ -21,6 +21,8 @@ the same proposals give the plain GPU ED rate.
 
 ``--workload batched128``: 128 seeded models with bath sizes 10-50 (26-66
 spin-orbitals) per step, exact path only (the batched-ED configuration).
+``--workload exact``: the fitting family's batched exact-ED call alone
+(``--models`` proposals per call), no surrogate and no training.
 
 The timed loops run as the product does, without kernel timing; each is
 then repeated with the same seeds (the same walk) with a HIP-event pair
@@ -67,6 +69,12 @@ def algorithmic_bytes_eigen(n, nnz, steps):
     few-hundred-state sectors of these workloads the vectors sit in LDS and
     L2, not in HBM (see ``gram_schmidt_bytes``)."""
     return algorithmic_bytes_lanczos(n, nnz, steps)
+
+
+def eigen_itemsize(fam):
+    """Bytes per Krylov-vector element of the family's eigensolver: 8 when
+    every Hamiltonian coefficient is real (the real kernel), else 16."""
+    return 8 if not (np.any(fam.c0.imag) or np.any(fam.cmat.imag)) else 16
 
 
 def gram_schmidt_bytes(n, steps, itemsize=16):
@@ -263,7 +271,6 @@ def run_fit(args, torch):
     from impuritymodel_amd import engine
     from impuritymodel_amd import surrogate as S
     from impuritymodel_amd.batched import plain_xas
-    from impuritymodel_amd.engine import Batch, Ops
 
     fam = S.XASFamily()
     sur, info = S.train_surrogate(fam, n_train=args.n_train, epochs=args.epochs, seed=0)
@@ -331,24 +338,20 @@ def run_fit(args, torch):
         score = dict(n=0)
 
     # algorithmic bytes of the exact-ED kernels inside the timed surrogate loop
-    ops = Ops([fam.models(target_theta[None])[0].term_table(1)], None, 26, 1)
-    gb = Batch(ops, [0])
-    gb.set_seeds([fam._bases[1]])
-    gb.build(fixed_basis=True)
-    n_g, nnz_g = int(gb.chain_n[0]), int(gb.chain_nnz[0])
+    gs_item = eigen_itemsize(fam)
+
     def ed_bytes(loop):
-        eig = sum(algorithmic_bytes_eigen([n_g] * len(r.eig_steps), [nnz_g] * len(r.eig_steps), r.eig_steps)
-                  for r in loop.exact_results)
+        eig = sum(algorithmic_bytes_eigen(r.ground_dims, r.ground_nnz, r.eig_steps) for r in loop.exact_results)
         lan = sum(algorithmic_bytes_lanczos(r.chain_dims, r.chain_nnz, r.kout) for r in loop.exact_results)
         return {"eigen": eig, "lanczos": lan}
 
     def gs_rate(loop, kern):
         # the eigensolver's Gram-Schmidt reads of LDS/L2-resident Krylov
-        # vectors (real arithmetic: 8-byte elements), per second of eigen_kernel
+        # vectors, per second of eigen_kernel
         ms, k = kern["eigen"]
         if not k:
             return None
-        b = sum(gram_schmidt_bytes([n_g] * len(r.eig_steps), r.eig_steps, 8) for r in loop.exact_results)
+        b = sum(gram_schmidt_bytes(r.ground_dims, r.eig_steps, gs_item) for r in loop.exact_results)
         return dict(kernel="eigen_kernel", gram_schmidt_gbs=round(b / (ms * 1e-3) / 1e9, 1),
                     what="classical Gram-Schmidt reads of the stored Krylov vectors (LDS/L2), one pass per step; "
                          "not HBM traffic, not in the roofline")
@@ -413,6 +416,87 @@ def run_fit(args, torch):
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+
+
+def run_exact(args, torch):
+    """The fitting family's exact path alone: ``--models`` proposals per batched
+    exact-ED call (the call that answers the surrogate's untrusted proposals),
+    no surrogate and no training."""
+    from impuritymodel_amd import engine
+    from impuritymodel_amd import surrogate as S
+
+    fam = S.XASFamily()
+    P = [S.sample_params(args.models, seed=1000 + k) for k in range(args.steps + args.warmup)]
+    for k in range(args.warmup):
+        S.exact_spectra(fam, P[k], sectors=not args.general_path)
+    torch.cuda.synchronize()
+    ROCTX.push("exact_timed")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        S.exact_spectra(fam, P[args.warmup + k], sectors=not args.general_path)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ROCTX.pop()
+    engine.timing_enable(True)
+    engine.timing_reset()
+    ROCTX.push("exact_instrumented")
+    results = [S.exact_spectra(fam, P[args.warmup + k], sectors=not args.general_path)[1] for k in range(args.steps)]
+    torch.cuda.synchronize()
+    ROCTX.pop()
+    kernels = {k: engine.timing_query(k) for k in KERNELS + ("discover", "sort", "csr_fill", "affine_fill",
+                                                              "affine_seed")}
+    engine.timing_enable(False)
+    # the same proposals through the other path (the general path, or with
+    # --general-path the sector cache)
+    nb = min(args.steps, 5)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for k in range(nb):
+        S.exact_spectra(fam, P[args.warmup + k], sectors=args.general_path)
+    torch.cuda.synchronize()
+    other_ms = 1e3 * (time.perf_counter() - t1) / nb
+    eig = sum(algorithmic_bytes_eigen(r.ground_dims, r.ground_nnz, r.eig_steps) for r in results)
+    lan = sum(algorithmic_bytes_lanczos(r.chain_dims, r.chain_nnz, r.kout) for r in results)
+    roofline = _roofline({k: kernels[k] for k in ("eigen", "lanczos")}, {"eigen": eig, "lanczos": lan})
+    if roofline is not None and roofline["kernel"] == "eigen_kernel":
+        b = sum(gram_schmidt_bytes(r.ground_dims, r.eig_steps, eigen_itemsize(fam)) for r in results)
+        roofline["gram_schmidt_gbs"] = round(b / (kernels["eigen"][0] * 1e-3) / 1e9, 1)
+    r0 = results[-1]
+    return {
+        "metric": METRIC,
+        "value": round(args.models * args.steps / dt, 3),
+        "unit": "spectra/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded parameters)",
+        "config": {
+            "workload": "exact path of the fitting loop: %d proposals of the NiO-like 10-bath family per batched "
+                        "exact-ED call (3 eigenstates, 3 polarizations, fully reorthogonalized Lanczos, 1000 "
+                        "omega points), no surrogate" % args.models,
+            "models_per_call": args.models,
+            "path": "general (every sector rebuilt per call)" if args.general_path else
+                    "family sector cache (patterns fixed, values and start vectors formed per call)",
+            "ground_chains_per_model": len(r0.ground_dims) // args.models,
+            "ground_dims": sorted(set(int(x) for x in r0.ground_dims)),
+            "ground_nnz": sorted(set(int(x) for x in r0.ground_nnz)),
+            "excited_dims": sorted(set(int(x) for x in r0.chain_dims)),
+            ("sector_cache_ms_per_call" if args.general_path else "general_path_ms_per_call"): round(other_ms, 3),
+            "chains_per_call": int(len(r0.chain_dims)),
+            "eigen_steps_range": [int(min(r.eig_steps.min() for r in results)),
+                                  int(max(r.eig_steps.max() for r in results))],
+            "kernel_timing": "HIP events around every launch in a repeat of the timed calls",
+            "kernel_ms_per_call": {k: round(v[0] / args.steps, 4) for k, v in kernels.items() if v[1]},
+            "launches_per_call": {k: v[1] // args.steps for k, v in kernels.items() if v[1]},
+        },
+        "roofline": roofline,
+        "cpu_baseline": None,
     }
 
 
@@ -745,8 +829,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=("fit", "batched128", "large", "rixs"), default="fit")
+    ap.add_argument("--workload", choices=("fit", "exact", "batched128", "large", "rixs"), default="fit")
     ap.add_argument("--models", type=int, default=128)
+    ap.add_argument("--general-path", action="store_true",
+                    help="exact workload: rebuild every sector per call instead of the family sector cache")
     ap.add_argument("--walkers", type=int, default=256)
     ap.add_argument("--n-train", type=int, default=20000)
     ap.add_argument("--epochs", type=int, default=2500)
@@ -777,7 +863,8 @@ def main():
         torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
-    run = {"fit": run_fit, "batched128": run_batched128, "large": run_large, "rixs": run_rixs}[args.workload]
+    run = {"fit": run_fit, "exact": run_exact, "batched128": run_batched128, "large": run_large,
+           "rixs": run_rixs}[args.workload]
     out = run(args, torch)
     print(json.dumps(out))
 

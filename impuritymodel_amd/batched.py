@@ -53,6 +53,8 @@ class XASResult:
     chain_nnz: np.ndarray = None
     eig_steps: np.ndarray = None
     kout: np.ndarray = None
+    ground_dims: np.ndarray = None   # per eigensolver chain
+    ground_nnz: np.ndarray = None
 
 
 def _dipole_tables(models, W):
@@ -152,7 +154,7 @@ def xas_batch_ops(ops_h, ops_t, bases, omega, delta=0.2, krylovSize=200, nPsiMax
     tim["cf"] = time.perf_counter() - t1
     tim["total"] = time.perf_counter() - t0
     return XASResult(omega, spec, [[evals[m][i] for i in kept[m]] for m in range(M)], tim, xb.chain_n.copy(),
-                     xb.chain_nnz.copy(), steps, kout)
+                     xb.chain_nnz.copy(), steps, kout, gb.chain_n.copy(), gb.chain_nnz.copy())
 
 
 def state_spectra(ops_h, ops_t, psis, energies, omega, delta, krylovSize=150, T=300.0, sign=1, reorth=0,

@@ -263,6 +263,10 @@ struct Batch {
   bool built = false;
   bool row_keys = false;  // pool_keys holds the basis in row order (built, not set_csr)
   bool real = false;      // all CSR values real
+  // start vectors formed on the device (imped_batch_seed_affine), one entry per row
+  DevBuf v0dev;
+  bool have_v0dev = false;
+  DevBuf fill_meta;  // chain table of imped_batch_set_affine
 };
 
 // Seed amplitudes are placed on the first rows of their chain: every chain
@@ -692,6 +696,112 @@ static void build(Batch& b, int mode, double weight_min, int max_layers) {
   b.real = b.ops->real;
 }
 
+// ------------------------------------------------------------------ affine CSR structures
+//
+// A set of fixed sparsity patterns (structures) whose values are affine in
+// a parameter vector: value = base + sum_p theta_p dirs_p, entry by entry.  A
+// model family whose Hamiltonian is affine in its parameters has, on a fixed
+// sector, a fixed pattern and affine values; a batch of parameter sets then
+// needs one fill kernel instead of operator application, sorting and
+// hashing.  Structures with no parameters hold fixed matrices (transition
+// operators from a ground sector into an excited one).
+
+struct AffineCsr {
+  int S = 0, P = 0;
+  std::vector<int64_t> h_n, h_nnz, h_row_off, h_nnz_off, h_maxcol;  // per structure (h_maxcol = -1: empty)
+  int64_t total_rows = 0, total_nnz = 0;
+  DevBuf srowptr, scol, base, dirs, meta;  // meta: n, row_off, nnz_off per structure (int64)
+  bool real = true;
+};
+
+// Rows, columns and values of every chain of a batch from its structure and
+// model: chain c = blockIdx.x, gridDim.y blocks per chain.  meta: row_base [C+1], chain_n [C], nnz_base
+// [C+1], chain_struct [C], chain_model [C] (int64).
+__global__ __launch_bounds__(256) void affine_fill_kernel(const int64_t* smeta, int S, const int64_t* srowptr,
+                                                          const int32_t* scol, const double2* base,
+                                                          const double2* dirs, int P, int64_t total_nnz,
+                                                          const int64_t* meta, int C, const double* theta,
+                                                          double weight_min, int64_t* rowptr, int32_t* col,
+                                                          double2* val, int64_t* row_base_out, int64_t* chain_n_out) {
+  const int c = blockIdx.x;
+  const int64_t* row_base = meta;
+  const int64_t* chain_n = meta + C + 1;
+  const int64_t* nnz_base = chain_n + C;
+  const int64_t* chain_struct = nnz_base + C + 1;
+  const int64_t* chain_model = chain_struct + C;
+  const int st = (int)chain_struct[c];
+  const int64_t n = smeta[st], roff = smeta[S + st], eoff = smeta[2 * S + st];
+  const int64_t nnz = srowptr[roff + n];
+  const double* th = theta + (size_t)chain_model[c] * P;
+  const int64_t first = (int64_t)blockIdx.y * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.y * blockDim.x;
+  for (int64_t i = first; i < n; i += stride) rowptr[row_base[c] + i] = nnz_base[c] + srowptr[roff + i];
+  for (int64_t k = first; k < nnz; k += stride) {
+    const int64_t e = eoff + k;
+    double vx = base[e].x, vy = base[e].y;
+    for (int p = 0; p < P; ++p) {
+      const double2 d = dirs[(size_t)p * total_nnz + e];
+      vx += th[p] * d.x;
+      vy += th[p] * d.y;
+    }
+    // the pruning of the Hamiltonian build, |h|^2 < weight_min
+    if (vx * vx + vy * vy < weight_min) vx = vy = 0.0;
+    col[nnz_base[c] + k] = scol[e];
+    val[nnz_base[c] + k] = make_double2(vx, vy);
+  }
+  if (blockIdx.y == 0 && threadIdx.x == 0) {
+    row_base_out[c] = row_base[c];
+    chain_n_out[c] = chain_n[c];
+    if (c == C - 1) {
+      rowptr[row_base[C]] = nnz_base[C];
+      row_base_out[C] = row_base[C];
+    }
+  }
+}
+
+// Start vectors of chain c = blockIdx.x (one wave): v = T psi over the
+// chain's rows, T = structure chain_struct[c] (rows: the chain's rows;
+// columns: rows of ground chain src[c]), psi = eigenvector vec[c] of that
+// ground chain with components |psi_i|^2 < psi_wmin dropped (the eigenvector
+// dict of eigensystem), entries |v_r|^2 < wmin dropped (applyOp's pruning),
+// then normalized; norm2[c] = |v|^2 before normalization.  Each row sums its
+// T entries in stored (source) order.
+__global__ __launch_bounds__(64) void affine_seed_kernel(const int64_t* smeta, int S, const int64_t* srowptr,
+                                                         const int32_t* scol, const double2* tval,
+                                                         const int64_t* xmeta, int C, const int64_t* g_row_base,
+                                                         const int64_t* g_chain_n, const double2* evecs, int nev,
+                                                         double psi_wmin, double wmin, double2* v0,
+                                                         double* norm2) {
+  const int c = blockIdx.x;
+  const int64_t* x_row_base = xmeta;
+  const int64_t* src = xmeta + C + 1;
+  const int64_t* vec = src + C;
+  const int64_t* tstruct = vec + C;
+  const int st = (int)tstruct[c];
+  const int64_t n = smeta[st], roff = smeta[S + st], eoff = smeta[2 * S + st];
+  const int g = (int)src[c];
+  const int64_t ng = g_chain_n[g];
+  const double2* psi = evecs + (size_t)g_row_base[g] * nev + (size_t)vec[c] * ng;
+  double2* v = v0 + x_row_base[c];
+  double p = 0.0;
+  for (int64_t r = threadIdx.x; r < n; r += 64) {
+    double ax = 0.0, ay = 0.0;
+    for (int64_t e = srowptr[roff + r]; e < srowptr[roff + r + 1]; ++e) {
+      const double2 t = tval[eoff + e];
+      const double2 y = psi[scol[eoff + e]];
+      if (y.x * y.x + y.y * y.y < psi_wmin) continue;
+      ax += t.x * y.x - t.y * y.y;
+      ay += t.x * y.y + t.y * y.x;
+    }
+    if (ax * ax + ay * ay < wmin) ax = ay = 0.0;
+    v[r] = make_double2(ax, ay);
+    p += ax * ax + ay * ay;
+  }
+  p = wave_sum(p);
+  const double inv = p > 0.0 ? 1.0 / sqrt(p) : 0.0;
+  for (int64_t r = threadIdx.x; r < n; r += 64) v[r] = make_double2(v[r].x * inv, v[r].y * inv);
+  if (threadIdx.x == 0) norm2[c] = p;
+}
+
 }  // namespace imped
 
 using namespace imped;
@@ -1103,6 +1213,7 @@ int imped_batch_set_csr(imped_batch batch, const int64_t* chain_n, const int64_t
                         const double* val) {
   API_BEGIN
   Batch* b = reinterpret_cast<Batch*>(batch);
+  b->have_v0dev = false;
   b->h_n.assign(chain_n, chain_n + b->C);
   b->h_row_base.assign(b->C + 1, 0);
   for (int c = 0; c < b->C; ++c) {
@@ -1184,6 +1295,208 @@ int imped_batch_get_csr(imped_batch batch, int64_t* rowptr, int32_t* col, double
   API_END
 }
 
+/* ------------------------------------------------------------------ affine CSR structures */
+
+int imped_affine_create(int n_struct, const int64_t* struct_n, const int64_t* rowptr, const int32_t* col,
+                        const double* base, int n_params, const double* dirs, imped_affine* out) {
+  API_BEGIN
+  if (n_struct < 1 || !struct_n || !rowptr || !col || !base || n_params < 0 || (n_params > 0 && !dirs) || !out) {
+    set_error("imped_affine_create: n_struct >= 1, sizes, rowptr, col, base (and dirs when n_params > 0)");
+    return IMPED_ERR_INVALID;
+  }
+  auto a = std::make_unique<AffineCsr>();
+  a->S = n_struct;
+  a->P = n_params;
+  a->h_n.assign(struct_n, struct_n + n_struct);
+  a->h_row_off.assign(n_struct, 0);
+  a->h_nnz_off.assign(n_struct, 0);
+  a->h_nnz.assign(n_struct, 0);
+  a->h_maxcol.assign(n_struct, -1);
+  int64_t ro = 0, eo = 0;
+  for (int st = 0; st < n_struct; ++st) {
+    if (struct_n[st] < 0) {
+      set_error("imped_affine_create: negative structure size");
+      return IMPED_ERR_INVALID;
+    }
+    a->h_row_off[st] = ro;
+    a->h_nnz_off[st] = eo;
+    const int64_t* rp = rowptr + ro;
+    if (rp[0] != 0) {
+      set_error("imped_affine_create: every structure's rowptr starts at 0");
+      return IMPED_ERR_INVALID;
+    }
+    for (int64_t i = 0; i < struct_n[st]; ++i)
+      if (rp[i + 1] < rp[i]) {
+        set_error("imped_affine_create: rowptr must be non-decreasing");
+        return IMPED_ERR_INVALID;
+      }
+    a->h_nnz[st] = rp[struct_n[st]];
+    ro += struct_n[st] + 1;
+    eo += a->h_nnz[st];
+  }
+  a->total_rows = ro;
+  a->total_nnz = eo;
+  for (int st = 0; st < n_struct; ++st)
+    for (int64_t e = a->h_nnz_off[st]; e < a->h_nnz_off[st] + a->h_nnz[st]; ++e) {
+      if (col[e] < 0) {
+        set_error("imped_affine_create: negative column index");
+        return IMPED_ERR_INVALID;
+      }
+      a->h_maxcol[st] = std::max<int64_t>(a->h_maxcol[st], col[e]);
+    }
+  for (int64_t e = 0; e < eo && a->real; ++e) a->real = base[2 * e + 1] == 0.0;
+  for (int64_t e = 0; e < eo * n_params && a->real; ++e) a->real = dirs[2 * e + 1] == 0.0;
+  std::vector<int64_t> sm(3 * (size_t)n_struct);
+  for (int st = 0; st < n_struct; ++st) {
+    sm[st] = a->h_n[st];
+    sm[n_struct + st] = a->h_row_off[st];
+    sm[2 * n_struct + st] = a->h_nnz_off[st];
+  }
+  hipStream_t s = stream();
+  h2d(a->meta, sm.data(), sm.size(), s);
+  h2d(a->srowptr, rowptr, ro, s);
+  h2d(a->scol, col, std::max<int64_t>(eo, 1), s);
+  h2d(a->base, base, std::max<int64_t>(eo, 1) * 2, s);
+  if (n_params > 0) h2d(a->dirs, dirs, std::max<int64_t>(eo * n_params, 1) * 2, s);
+  stream_sync(s);
+  *out = reinterpret_cast<imped_affine>(a.release());
+  API_END
+}
+
+int imped_affine_destroy(imped_affine aff) {
+  delete reinterpret_cast<AffineCsr*>(aff);
+  return IMPED_OK;
+}
+
+int imped_batch_set_affine(imped_batch batch, imped_affine aff, const int32_t* chain_struct,
+                           const int32_t* chain_model, int n_models, const double* theta, double weight_min) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  const AffineCsr* a = reinterpret_cast<const AffineCsr*>(aff);
+  if (!b || !a || (b->C > 0 && (!chain_struct || !chain_model)) || n_models < 0 ||
+      (a->P > 0 && n_models > 0 && !theta)) {
+    set_error("imped_batch_set_affine: batch, structures, chain_struct and chain_model (and theta) required");
+    return IMPED_ERR_INVALID;
+  }
+  const int C = b->C;
+  for (int c = 0; c < C; ++c) {
+    const int st = chain_struct[c];
+    if (st < 0 || st >= a->S || chain_model[c] < 0 || (a->P > 0 && chain_model[c] >= n_models)) {
+      set_error("imped_batch_set_affine: chain structure or model index out of range");
+      return IMPED_ERR_INVALID;
+    }
+    if (a->h_maxcol[st] >= a->h_n[st]) {
+      set_error("imped_batch_set_affine: a structure used as a Hamiltonian must be square");
+      return IMPED_ERR_INVALID;
+    }
+  }
+  std::vector<int64_t> meta(3 * (size_t)C + 2 + 2 * (size_t)C);
+  int64_t* row_base = meta.data();
+  int64_t* chain_n = row_base + C + 1;
+  int64_t* nnz_base = chain_n + C;
+  int64_t* cs = nnz_base + C + 1;
+  int64_t* cm = cs + C;
+  b->h_n.assign(C, 0);
+  b->h_nnz.assign(C, 0);
+  b->h_row_base.assign(C + 1, 0);
+  int64_t maxw = 1;
+  for (int c = 0; c < C; ++c) {
+    const int st = chain_struct[c];
+    b->h_n[c] = a->h_n[st];
+    b->h_nnz[c] = a->h_nnz[st];
+    row_base[c + 1] = row_base[c] + a->h_n[st];
+    nnz_base[c + 1] = nnz_base[c] + a->h_nnz[st];
+    chain_n[c] = a->h_n[st];
+    cs[c] = st;
+    cm[c] = chain_model[c];
+    maxw = std::max(maxw, std::max(a->h_n[st], a->h_nnz[st]));
+  }
+  for (int c = 0; c <= C; ++c) b->h_row_base[c] = row_base[c];
+  b->R = row_base[C];
+  b->NNZ = nnz_base[C];
+  hipStream_t s = stream();
+  DevBuf& dmeta = b->fill_meta;
+  h2d(dmeta, meta.data(), meta.size(), s);
+  DevBuf th;
+  if (a->P > 0 && n_models > 0) h2d(th, theta, (size_t)n_models * a->P, s);
+  b->rowptr.reserve((b->R + 1) * sizeof(int64_t));
+  b->col.reserve(std::max<int64_t>(b->NNZ, 1) * 4);
+  b->val.reserve(std::max<int64_t>(b->NNZ, 1) * 16);
+  b->row_base.reserve((C + 1) * sizeof(int64_t));
+  b->chain_n.reserve(std::max(C, 1) * sizeof(int64_t));
+  if (C == 0) {
+    const int64_t zero = 0;
+    h2d(b->rowptr, &zero, 1, s);
+    h2d(b->row_base, &zero, 1, s);
+  } else {
+    const unsigned gy = (unsigned)std::min<int64_t>((maxw + 255) / 256, 64);
+    TimedLaunch tl("affine_fill", s);
+    hipLaunchKernelGGL(affine_fill_kernel, dim3(C, gy), dim3(256), 0, s, a->meta.as<int64_t>(), a->S,
+                       a->srowptr.as<int64_t>(), a->scol.as<int32_t>(), a->base.as<double2>(),
+                       a->P > 0 ? a->dirs.as<double2>() : nullptr, a->P, a->total_nnz, dmeta.as<int64_t>(), C,
+                       a->P > 0 ? th.as<double>() : nullptr, weight_min, b->rowptr.as<int64_t>(),
+                       b->col.as<int32_t>(), b->val.as<double2>(), b->row_base.as<int64_t>(),
+                       b->chain_n.as<int64_t>());
+    HIP_CHECK(hipGetLastError());
+  }
+  // th and the chain table must outlive the launch
+  stream_sync(s);
+  b->built = true;
+  b->row_keys = false;
+  b->real = a->real;
+  b->have_v0dev = false;
+  b->have_seed_amps = false;
+  API_END
+}
+
+int imped_batch_seed_affine(imped_batch batch, imped_batch ground, imped_affine trans, const int32_t* src_chain,
+                            const int32_t* src_vec, const int32_t* chain_struct, double psi_weight_min,
+                            double weight_min, double* norm2) {
+  API_BEGIN
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  const Batch* g = reinterpret_cast<const Batch*>(ground);
+  const AffineCsr* a = reinterpret_cast<const AffineCsr*>(trans);
+  if (!b || !g || !a || !b->built || !g->built || g->nev < 1 || (b->C > 0 && (!src_chain || !src_vec ||
+      !chain_struct || !norm2))) {
+    set_error("imped_batch_seed_affine: built batches (the ground one after imped_batch_eigen), structures, "
+              "chain maps and norm2 required");
+    return IMPED_ERR_INVALID;
+  }
+  const int C = b->C;
+  std::vector<int64_t> meta(4 * (size_t)C + 1);
+  for (int c = 0; c <= C; ++c) meta[c] = b->h_row_base[c];
+  for (int c = 0; c < C; ++c) {
+    const int st = chain_struct[c];
+    if (src_chain[c] < 0 || src_chain[c] >= g->C || src_vec[c] < 0 || src_vec[c] >= g->nev || st < 0 ||
+        st >= a->S || a->h_n[st] != b->h_n[c] || a->h_maxcol[st] >= g->h_n[src_chain[c]]) {
+      set_error("imped_batch_seed_affine: chain maps out of range, or a structure's rows (columns) differ from "
+                "the chain's (exceed the ground chain's)");
+      return IMPED_ERR_INVALID;
+    }
+    meta[C + 1 + c] = src_chain[c];
+    meta[2 * C + 1 + c] = src_vec[c];
+    meta[3 * C + 1 + c] = st;
+  }
+  // structure columns index the ground chain's rows
+  hipStream_t s = stream();
+  DevBuf dmeta, dn;
+  h2d(dmeta, meta.data(), meta.size(), s);
+  b->v0dev.reserve(std::max<int64_t>(b->R, 1) * 16);
+  dn.reserve(std::max(C, 1) * sizeof(double));
+  if (C > 0) {
+    TimedLaunch tl("affine_seed", s);
+    hipLaunchKernelGGL(affine_seed_kernel, dim3(C), dim3(64), 0, s, a->meta.as<int64_t>(), a->S,
+                       a->srowptr.as<int64_t>(), a->scol.as<int32_t>(), a->base.as<double2>(), dmeta.as<int64_t>(),
+                       C, g->row_base.as<int64_t>(), g->chain_n.as<int64_t>(), g->evecs.as<double2>(), g->nev,
+                       psi_weight_min, weight_min, b->v0dev.as<double2>(), dn.as<double>());
+    HIP_CHECK(hipGetLastError());
+    d2h(norm2, dn.p, C, s);
+  }
+  stream_sync(s);
+  b->have_v0dev = true;
+  API_END
+}
+
 int imped_batch_lanczos(imped_batch batch, const double* v0_host, int K, int reorth, double* alpha, double* beta,
                         int32_t* kout, int32_t* status) {
   API_BEGIN
@@ -1197,6 +1510,8 @@ int imped_batch_lanczos(imped_batch batch, const double* v0_host, int K, int reo
   if (v0_host) {
     h2d(v0, v0_host, std::max<int64_t>(b->R, 1) * 2, stream());
     v0d = v0.as<double2>();
+  } else if (b->have_v0dev) {
+    v0d = b->v0dev.as<double2>();
   } else {
     if (!b->have_seed_amps || !seeds_fit(*b)) {
       set_error("imped_batch_lanczos: no start vector (or more seeds than rows in a chain)");

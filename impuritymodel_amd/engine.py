@@ -209,6 +209,32 @@ class Batch:
                                   np.int64)
         return self
 
+    def set_affine(self, affine, chain_struct, chain_model, theta, weight_min=1e-12):
+        """CSR of every chain from ``affine`` (an :class:`Affine`): chain c is
+        structure chain_struct[c] with parameter row chain_model[c] of theta."""
+        chain_struct = np.ascontiguousarray(chain_struct, np.int32)
+        chain_model = np.ascontiguousarray(chain_model, np.int32)
+        th = np.ascontiguousarray(np.asarray(theta, np.float64).reshape(-1, max(affine.P, 1))[:, : affine.P])
+        check(lib.imped_batch_set_affine(self.h, affine.h, ptr(_nz(chain_struct)), ptr(_nz(chain_model)), len(th),
+                                         ptr(_nz(th.reshape(-1))), float(weight_min)))
+        self.chain_n = affine.n[chain_struct]
+        self.chain_nnz = affine.nnz[chain_struct]
+        self.rows, self.nnz = int(self.chain_n.sum()), int(self.chain_nnz.sum())
+        self.row_base = np.concatenate([[0], np.cumsum(self.chain_n)]).astype(np.int64)
+        return self
+
+    def seed_affine(self, ground, trans, src_chain, src_vec, chain_struct, psi_weight_min=1e-7, weight_min=1e-7):
+        """Start vectors T psi on the device (``imped_batch_seed_affine``):
+        chain c from eigenvector src_vec[c] of chain src_chain[c] of the
+        ``ground`` batch and structure chain_struct[c] of ``trans``.  Returns
+        norm2 [C], |T psi|^2 before normalization."""
+        norm2 = np.zeros(max(self.C, 1), np.float64)
+        check(lib.imped_batch_seed_affine(self.h, ground.h, trans.h, ptr(_nz(np.ascontiguousarray(src_chain, np.int32))),
+                                          ptr(_nz(np.ascontiguousarray(src_vec, np.int32))),
+                                          ptr(_nz(np.ascontiguousarray(chain_struct, np.int32))),
+                                          float(psi_weight_min), float(weight_min), ptr(norm2)))
+        return norm2[: self.C]
+
     def basis_words(self):
         keys = np.zeros((max(self.rows, 1), self.W), np.uint64)
         check(lib.imped_batch_get_basis(self.h, ptr(keys)))
@@ -286,6 +312,33 @@ class Batch:
             ptr(_nz(np.asarray(spec_id, np.int32))), int(n_spec), ptr(omega), len(omega), float(delta),
             None if g is None else ptr(g.view(np.float64)), ptr(spec)))
         return (g[: self.C] if want_g else None), spec[:n_spec]
+
+
+class Affine:
+    """Fixed CSR structures with parameter-affine values (``imped_affine_*``).
+
+    ``structs`` is a list of (rowptr, col, base, dirs) per structure: rowptr
+    [n+1] from 0, col [nnz], base [nnz] complex, dirs [P, nnz] complex (P the
+    same for all structures; P = 0 for fixed matrices)."""
+
+    def __init__(self, structs, n_params=0):
+        self.P = int(n_params)
+        self.n = np.array([len(r) - 1 for r, _, _, _ in structs], np.int64)
+        self.nnz = np.array([int(r[-1]) for r, _, _, _ in structs], np.int64)
+        rowptr = np.concatenate([np.asarray(r, np.int64) for r, _, _, _ in structs])
+        col = np.concatenate([np.asarray(c, np.int32) for _, c, _, _ in structs])
+        base = _c64(np.concatenate([np.asarray(b, np.complex128) for _, _, b, _ in structs]))
+        dirs = None
+        if self.P:
+            dirs = _c64(np.concatenate([np.asarray(d, np.complex128).reshape(self.P, -1) for _, _, _, d in structs],
+                                       axis=1))
+        h = C.c_void_p()
+        check(lib.imped_affine_create(len(structs), ptr(self.n), ptr(rowptr), ptr(_nz(col)), ptr(_nz(base)), self.P,
+                                      ptr(None if dirs is None else _nz(dirs)), C.byref(h)))
+        self.h = h
+
+    def __del__(self, _release=_release):
+        _release("imped_affine_destroy", self)
 
 
 BREAKDOWN = IMPED_ERR_BREAKDOWN

@@ -135,6 +135,7 @@ class XASFamily:
         self._dip_tables = {}
         self._bases = {}
         self._ops = {}
+        self._sectors = {}
 
     def hop(self, theta):
         """The operator dict of one parameter set, assembled term by term."""
@@ -178,6 +179,18 @@ class XASFamily:
         ops_h, ops_t = self._ops[(cap, W)]
         ops_h.set_affine(self.c0[tab.src], self.cmat[:, tab.src], th)
         return ops_h, ops_t
+
+    def sectors(self, W=1):
+        """The family's sector cache for W words (built on first use), or None
+        when its sectors do not have the structure the cache needs."""
+        from .family_sectors import FamilySectors, SectorCacheUnavailable
+
+        if W not in self._sectors:
+            try:
+                self._sectors[W] = FamilySectors(self, W)
+            except SectorCacheUnavailable:
+                self._sectors[W] = None
+        return self._sectors[W]
 
     def models(self, thetas):
         thetas = np.asarray(thetas, np.float64)
@@ -702,14 +715,24 @@ def fitting_loop(family, target, surrogate=None, n_walkers=256, n_iter=8, step=0
                 seconds=time.perf_counter() - t0, best_l1=loop.best, sur_params=sp, sur_spectra=ss)
 
 
-def exact_spectra(family, thetas, krylovSize=200, reorth=1, **kw):
-    """Exact XAS of parameter sets by the batched GPU ED, area-normalized."""
+def exact_spectra(family, thetas, krylovSize=200, reorth=1, sectors=True, **kw):
+    """Exact XAS of parameter sets by the batched GPU ED, area-normalized.
+
+    With ``sectors`` (the default) the family's sector cache
+    (:class:`impuritymodel_amd.family_sectors.FamilySectors`) supplies every
+    sector's pattern and only the values and vectors are formed per call;
+    families whose sectors the cache cannot hold, and ``sectors=False``, run
+    the general path (every sector rebuilt per parameter set)."""
     from .batched import ground_bases, xas_batch_ops
 
     from . import opcompile
 
     thetas = np.asarray(thetas, np.float64)
     W = opcompile.n_words(ops.n_spin_orbitals(family.nBaths))
+    cache = family.sectors(W) if sectors else None
+    if cache is not None:
+        res = cache.xas(thetas, OMEGA, delta=DELTA, krylovSize=krylovSize, T=thetas[:, 7], reorth=reorth, **kw)
+        return area_normalize(res.spectra), res
     if W not in family._bases:
         family._bases[W] = ground_bases(family.models(thetas[:1]), W)[0]
     ops_h, ops_t = family.batch_ops(thetas, W)

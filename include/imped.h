@@ -38,6 +38,7 @@ extern "C" {
 typedef struct imped_ops_s* imped_ops;
 typedef struct imped_vec_s* imped_vec;
 typedef struct imped_batch_s* imped_batch;
+typedef struct imped_affine_s* imped_affine;
 
 const char* imped_last_error(void);
 int imped_abi_version(void);
@@ -49,8 +50,8 @@ int imped_device_synchronize(void);
  * by HIP events on the library's stream; imped_timing_query sums the elapsed
  * time of the launches recorded under a kernel name ("apply_count",
  * "apply_fill", "discover", "sort" (the span of one BFS layer's sort passes),
- * "csr_fill", "eigen", "eigen_flat", "lanczos", "spmv", "solve",
- * "solve_combine", "cf", "mlp_gemm", "mlp_adam").  No reference counterpart. */
+ * "csr_fill", "affine_fill", "affine_seed", "eigen", "eigen_flat", "lanczos",
+ * "spmv", "solve", "solve_combine", "cf", "mlp_gemm", "mlp_adam").  No reference counterpart. */
 int imped_timing_enable(int on);
 int imped_timing_reset(void);
 int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
@@ -188,8 +189,43 @@ int imped_batch_get_basis(imped_batch batch, uint64_t* keys);
  * mapped back: h[col, row] = conj(val). */
 int imped_batch_get_csr(imped_batch batch, int64_t* rowptr, int32_t* col, double* val);
 
+/* ---- fixed sparsity patterns with parameter-affine values ---- */
+
+/* n_struct CSR structures, structure s with struct_n[s] rows: its rowptr
+ * (struct_n[s]+1 entries, starting at 0) follows structure s-1's in rowptr,
+ * its entries follow s-1's in col, base (complex) and, per parameter p,
+ * dirs[p] (complex, [n_params][total entries]).  A structure's values for
+ * parameters theta are base + sum_p theta_p dirs[p], entry by entry.  Used
+ * for sectors whose basis and pattern do not change over a model family: the
+ * Hamiltonian of a sector of an affine family, and (n_params = 0) a fixed
+ * transition operator from a ground sector's rows (columns) into an excited
+ * sector's rows.  No reference counterpart: the reference rebuilds every
+ * sector per model (finite.py:1514-1610, 1696-1894). */
+int imped_affine_create(int n_struct, const int64_t* struct_n, const int64_t* rowptr, const int32_t* col,
+                        const double* base, int n_params, const double* dirs, imped_affine* out);
+int imped_affine_destroy(imped_affine aff);
+/* The batch's CSR from structures: chain c is square structure chain_struct[c]
+ * with the values of parameter set chain_model[c] of theta [n_models][n_params],
+ * entries with |h|^2 < weight_min set to zero (the pruning of
+ * get_hamiltonian_matrix / expand_basis_and_hamiltonian).  Replaces
+ * imped_batch_build for such sectors; formed on the device. */
+int imped_batch_set_affine(imped_batch batch, imped_affine aff, const int32_t* chain_struct,
+                           const int32_t* chain_model, int n_models, const double* theta, double weight_min);
+/* Start vectors of the batch's chains on the device, for imped_batch_lanczos
+ * with v0 = NULL: chain c gets T psi, T = structure chain_struct[c] of `trans`
+ * (rows: chain c's rows; columns: rows of chain src_chain[c] of `ground`),
+ * psi = eigenvector src_vec[c] of that chain after imped_batch_eigen with
+ * components |psi_i|^2 < psi_weight_min dropped (the eigenvector dicts of
+ * eigensystem, finite.py:114-117); entries |v|^2 < weight_min are pruned and
+ * v is normalized, norm2[c] = |v|^2 before normalization (applyOp followed
+ * by getSpectra's normalization, finite.py:1301-1511, spectra.py:786-790). */
+int imped_batch_seed_affine(imped_batch batch, imped_batch ground, imped_affine trans, const int32_t* src_chain,
+                            const int32_t* src_vec, const int32_t* chain_struct, double psi_weight_min,
+                            double weight_min, double* norm2);
+
 /* get_tridiagonal_krylov_vectors (finite.py:1897-1995) for every chain from
- * v0 (rows complex, or NULL = the seed amplitudes in the first rows of each
+ * v0 (rows complex, or NULL = the start vectors of imped_batch_seed_affine
+ * when the batch has them, else the seed amplitudes in the first rows of each
  * chain).  reorth = 0 is the reference's plain recurrence; reorth = 1 adds
  * full reorthogonalization (no reference counterpart).  alpha, beta are
  * [C][K]; kout[c] = min(K, n_c), or the step of a beta == 0 breakdown with

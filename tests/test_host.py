@@ -130,3 +130,6 @@ def test_c_abi_rejects_null_arguments():
     assert lib.imped_walkers_waiting(None, None, None, None) == IMPED_ERR_INVALID
     assert lib.imped_lanczos_noalloc(None, None, None, 10, None, 5, 0, None, None, None, None, None) == \
         IMPED_ERR_INVALID
+    assert lib.imped_affine_create(0, None, None, None, None, 0, None, None) == IMPED_ERR_INVALID
+    assert lib.imped_batch_set_affine(None, None, None, None, 0, None, 0.0) == IMPED_ERR_INVALID
+    assert lib.imped_batch_seed_affine(None, None, None, None, None, None, 0.0, 0.0, None) == IMPED_ERR_INVALID
