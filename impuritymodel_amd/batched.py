@@ -144,7 +144,7 @@ def xas_batch_ops(ops_h, ops_t, bases, omega, delta=0.2, krylovSize=200, nPsiMax
     tim["expand_build"] = time.perf_counter() - t1
     t1 = time.perf_counter()
     K = int(min(krylovSize, max(int(xb.chain_n.max()), 1)))
-    _, _, kout, status = xb.lanczos(K, reorth=reorth)
+    _, _, kout, status = xb.lanczos(K, reorth=reorth, want_coeffs=False)
     if np.any(status != 0):
         raise ValueError("Warning: beta==0, implementation absent!")
     tim["lanczos"] = time.perf_counter() - t1
@@ -186,7 +186,7 @@ def state_spectra(ops_h, ops_t, psis, energies, omega, delta, krylovSize=150, T=
     info["chain_nnz"] = xb.chain_nnz.copy()
     t0 = time.perf_counter()
     K = int(min(krylovSize, max(int(xb.chain_n.max()), 1)))
-    _, _, kout, status = xb.lanczos(K, reorth=reorth)
+    _, _, kout, status = xb.lanczos(K, reorth=reorth, want_coeffs=False)
     if np.any(status != 0):
         raise ValueError("Warning: beta==0, implementation absent!")
     info["lanczos_s"] = time.perf_counter() - t0
@@ -233,7 +233,7 @@ def plain_xas(model, omega=None, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0, 
             xb.seed_from_vec(seed_vec)
             xb.build(fixed_basis=False)
             K = int(min(krylovSize, max(int(xb.chain_n[0]), 1)))
-            xb.lanczos(K, reorth=reorth)
+            xb.lanczos(K, reorth=reorth, want_coeffs=False)
             _, spec = xb.cf([e[i]], norm2, [1], [1], K_B * T, [0], 1, omega, delta)
             acc += np.exp(-(e[i] - e0) / (K_B * T)) * spec[0]
     return acc / Z

@@ -7,7 +7,9 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -88,6 +90,26 @@ inline void h2d(DevBuf& b, const T* src, size_t n, hipStream_t s) {
   b.reserve(n * sizeof(T));
   if (n) HIP_CHECK(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
 }
+
+// Several host arrays sent in one copy: add() places an array at a 256-byte
+// aligned offset of the pack and returns that offset; upload() copies the
+// whole pack into a device buffer, after which at<T>(d, off) is the array's
+// device address.
+struct HostPack {
+  std::vector<char> bytes;
+  template <class T>
+  size_t add(const T* p, size_t n) {
+    const size_t off = (bytes.size() + 255) & ~size_t(255);
+    bytes.resize(off + std::max<size_t>(n, 1) * sizeof(T), 0);
+    if (n) std::memcpy(bytes.data() + off, p, n * sizeof(T));
+    return off;
+  }
+  void upload(DevBuf& d, hipStream_t s) const { h2d(d, bytes.data(), std::max<size_t>(bytes.size(), 1), s); }
+  template <class T>
+  static T* at(const DevBuf& d, size_t off) {
+    return reinterpret_cast<T*>(static_cast<char*>(d.p) + off);
+  }
+};
 
 // Device-to-host reads.  Small ones (<= 64 KiB) go through a pinned ring:
 // the copy is queued like a kernel and dst is filled by the next

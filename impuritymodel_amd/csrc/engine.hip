@@ -55,8 +55,8 @@ Staging& staging() {
   static Staging* st = new Staging;
   return *st;
 }
-constexpr size_t kStageMax = 64 * 1024;   // reads up to this size are staged
-constexpr size_t kRingBytes = 4 << 20;    // pinned ring
+constexpr size_t kStageMax = 4 << 20;    // reads up to this size are staged
+constexpr size_t kRingBytes = 32 << 20;  // pinned ring
 }  // namespace
 
 void d2h_bytes(void* dst, const void* src, size_t bytes, hipStream_t s) {
@@ -235,6 +235,7 @@ struct Batch {
   int C = 0, W = 1;
   std::vector<int32_t> h_chain_model;
   DevBuf chain_model;
+  bool chain_model_dev = false;
   // seeds (host offsets; device keys/amps)
   std::vector<int64_t> h_seed_off;
   DevBuf seed_keys, seed_amps, seed_off_dev;
@@ -256,7 +257,7 @@ struct Batch {
   DevBuf rowptr, col, val, row_chain, row_base, chain_n;
   // Lanczos
   int K = 0;
-  DevBuf alpha, beta, kout, status;
+  DevBuf alpha, beta, kout;  // kout holds kout[C] then status[C]
   // ground states
   int nev = 0;
   DevBuf evals, evecs;
@@ -1148,8 +1149,7 @@ int imped_batch_create(imped_ops ops, int n_chains, const int32_t* chain_model, 
   b->ops = o;
   b->C = n_chains;
   b->W = o->W;
-  b->h_chain_model.assign(chain_model, chain_model + n_chains);
-  h2d(b->chain_model, chain_model, std::max(n_chains, 1), stream());
+  b->h_chain_model.assign(chain_model, chain_model + n_chains);  // sent to the device by the first build
   b->h_seed_off.assign(n_chains + 1, 0);
   *out = reinterpret_cast<imped_batch>(b.release());
   API_END
@@ -1205,6 +1205,10 @@ int imped_batch_seed_from_vec(imped_batch batch, imped_vec vec) {
 int imped_batch_build(imped_batch batch, int fixed_basis, double weight_min, int max_layers) {
   API_BEGIN
   Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b->chain_model_dev) {
+    h2d(b->chain_model, b->h_chain_model.data(), std::max<size_t>(b->h_chain_model.size(), 1), stream());
+    b->chain_model_dev = true;
+  }
   build(*b, fixed_basis ? 1 : 0, weight_min, max_layers);
   API_END
 }
@@ -1415,10 +1419,13 @@ int imped_batch_set_affine(imped_batch batch, imped_affine aff, const int32_t* c
   b->R = row_base[C];
   b->NNZ = nnz_base[C];
   hipStream_t s = stream();
-  DevBuf& dmeta = b->fill_meta;
-  h2d(dmeta, meta.data(), meta.size(), s);
-  DevBuf th;
-  if (a->P > 0 && n_models > 0) h2d(th, theta, (size_t)n_models * a->P, s);
+  // the chain table and theta in one copy, kept with the batch
+  HostPack pk;
+  const size_t o_meta = pk.add(meta.data(), meta.size());
+  const size_t o_th = pk.add(theta, a->P > 0 ? (size_t)n_models * a->P : 0);
+  pk.upload(b->fill_meta, s);
+  const int64_t* dmeta = HostPack::at<int64_t>(b->fill_meta, o_meta);
+  const double* dth = HostPack::at<double>(b->fill_meta, o_th);
   b->rowptr.reserve((b->R + 1) * sizeof(int64_t));
   b->col.reserve(std::max<int64_t>(b->NNZ, 1) * 4);
   b->val.reserve(std::max<int64_t>(b->NNZ, 1) * 16);
@@ -1433,14 +1440,12 @@ int imped_batch_set_affine(imped_batch batch, imped_affine aff, const int32_t* c
     TimedLaunch tl("affine_fill", s);
     hipLaunchKernelGGL(affine_fill_kernel, dim3(C, gy), dim3(256), 0, s, a->meta.as<int64_t>(), a->S,
                        a->srowptr.as<int64_t>(), a->scol.as<int32_t>(), a->base.as<double2>(),
-                       a->P > 0 ? a->dirs.as<double2>() : nullptr, a->P, a->total_nnz, dmeta.as<int64_t>(), C,
-                       a->P > 0 ? th.as<double>() : nullptr, weight_min, b->rowptr.as<int64_t>(),
+                       a->P > 0 ? a->dirs.as<double2>() : nullptr, a->P, a->total_nnz, dmeta, C,
+                       a->P > 0 ? dth : nullptr, weight_min, b->rowptr.as<int64_t>(),
                        b->col.as<int32_t>(), b->val.as<double2>(), b->row_base.as<int64_t>(),
                        b->chain_n.as<int64_t>());
     HIP_CHECK(hipGetLastError());
   }
-  // th and the chain table must outlive the launch
-  stream_sync(s);
   b->built = true;
   b->row_keys = false;
   b->real = a->real;
@@ -1530,8 +1535,8 @@ int imped_batch_lanczos(imped_batch batch, const double* v0_host, int K, int reo
   b->K = K;
   b->alpha.reserve((size_t)std::max(b->C, 1) * K * 8);
   b->beta.reserve((size_t)std::max(b->C, 1) * K * 8);
-  b->kout.reserve((size_t)std::max(b->C, 1) * 4);
-  b->status.reserve((size_t)std::max(b->C, 1) * 4);
+  b->kout.reserve((size_t)std::max(b->C, 1) * 8);  // kout[C], then status[C]
+  int32_t* status_d = b->kout.as<int32_t>() + std::max(b->C, 1);
   KrylovBatch kb{b->C,
                  b->row_base.as<int64_t>(),
                  b->chain_n.as<int64_t>(),
@@ -1540,13 +1545,23 @@ int imped_batch_lanczos(imped_batch batch, const double* v0_host, int K, int reo
                  b->val.as<double2>(),
                  b->h_n,
                  b->h_nnz};
-  run_lanczos(kb, v0d, K, reorth, b->alpha.as<double>(), b->beta.as<double>(), b->kout.as<int32_t>(),
-              b->status.as<int32_t>(), stream());
+  run_lanczos(kb, v0d, K, reorth, b->alpha.as<double>(), b->beta.as<double>(), b->kout.as<int32_t>(), status_d,
+              stream());
   if (alpha) d2h(alpha, b->alpha.p, (size_t)b->C * K, stream());
   if (beta) d2h(beta, b->beta.p, (size_t)b->C * K, stream());
-  if (kout) d2h(kout, b->kout.p, b->C, stream());
-  if (status) d2h(status, b->status.p, b->C, stream());
+  std::vector<int32_t> ks;
+  if (kout && status && b->C) {
+    ks.resize(2 * (size_t)b->C);
+    d2h(ks.data(), b->kout.p, ks.size(), stream());  // kout and status in one read
+  } else {
+    if (kout) d2h(kout, b->kout.p, b->C, stream());
+    if (status) d2h(status, status_d, b->C, stream());
+  }
   stream_sync(stream());
+  if (!ks.empty()) {
+    std::copy(ks.begin(), ks.begin() + b->C, kout);
+    std::copy(ks.begin() + b->C, ks.end(), status);
+  }
   API_END
 }
 
@@ -1559,7 +1574,12 @@ int imped_batch_eigen(imped_batch batch, int nev, double tol, int max_steps, uin
     return IMPED_ERR_INVALID;
   }
   b->nev = nev;
-  b->evals.reserve((size_t)std::max(b->C, 1) * nev * 8);
+  // evals[C][nev], then steps[C] and conv[C] (int32): one read brings all back
+  const int C1 = std::max(b->C, 1);
+  const size_t ev_bytes = (size_t)C1 * nev * 8;
+  b->evals.reserve(ev_bytes + (size_t)C1 * 8);
+  int32_t* steps_d = reinterpret_cast<int32_t*>(static_cast<char*>(b->evals.p) + ev_bytes);
+  int32_t* conv_d = steps_d + C1;
   b->evecs.reserve(std::max<int64_t>(b->R, 1) * nev * 16);
   KrylovBatch kb{b->C,
                  b->row_base.as<int64_t>(),
@@ -1569,15 +1589,18 @@ int imped_batch_eigen(imped_batch batch, int nev, double tol, int max_steps, uin
                  b->val.as<double2>(),
                  b->h_n,
                  b->h_nnz};
-  DevBuf steps;
-  steps.reserve(std::max(b->C, 1) * 4);
-  int rc = run_eigen(kb, nev, tol, max_steps, seed, b->evals.as<double>(), b->evecs.as<double2>(),
-                     steps.as<int32_t>(), b->real, stream());
-  if (rc == IMPED_ERR_INVALID) return rc;
-  if (evals) d2h(evals, b->evals.p, (size_t)b->C * nev, stream());
+  int rc = run_eigen(kb, nev, tol, max_steps, seed, b->evals.as<double>(), b->evecs.as<double2>(), steps_d, conv_d,
+                     b->real, stream());
+  if (rc != IMPED_OK) return rc;
+  std::vector<char> es(ev_bytes + (size_t)C1 * 8);
+  if (b->C) d2h(es.data(), b->evals.p, es.size(), stream());
   if (evecs_host) d2h(evecs_host, b->evecs.p, (size_t)b->R * nev * 2, stream());
-  if (steps_out) d2h(steps_out, steps.p, b->C, stream());
   stream_sync(stream());
+  if (evals) std::memcpy(evals, es.data(), (size_t)b->C * nev * 8);
+  if (steps_out) std::memcpy(steps_out, es.data() + ev_bytes, (size_t)b->C * 4);
+  const int32_t* conv = reinterpret_cast<const int32_t*>(es.data() + ev_bytes + (size_t)C1 * 4);
+  for (int c = 0; c < b->C; ++c)
+    if (!conv[c]) rc = IMPED_ERR_NOCONV;
   if (rc != IMPED_OK) {
     set_error("imped_batch_eigen: not converged within max_steps");
     return rc;

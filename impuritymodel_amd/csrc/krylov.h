@@ -27,9 +27,11 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
 // Lowest nev eigenpairs of every chain.  evals [C][nev]; evecs: chain c's
 // block starts at row_base[c]*nev, eigenvector k at offset k*n_c.
 // real_values: every CSR value has zero imaginary part, so the solver may
-// run on real vectors (when the eigen_real tuning flag allows it).
+// run on real vectors (when the eigen_real tuning flag allows it).  conv
+// (device, [C]) gets 1 for a chain that met tol, else 0; nothing is read
+// back: the caller reads conv with evals and steps.
 int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_t seed, double* evals,
-              double2* evecs, int32_t* steps, bool real_values, hipStream_t s);
+              double2* evecs, int32_t* steps, int32_t* conv, bool real_values, hipStream_t s);
 
 // (z_{c,s} - H_c) x = y_c for S shifts per chain (z_host: [C][S] complex,
 // every Im z != 0), one Krylov space per chain.  y: one entry per batch row.

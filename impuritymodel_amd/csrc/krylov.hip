@@ -1695,9 +1695,9 @@ __global__ __launch_bounds__(FT) void flat_copy_kernel(const double2* src, T* ds
 
 struct KScratch {
   DevBuf ws, vstore, coef, ab, yk, tw, tpiv, voff, conv, wbuf, clist;
-  DevBuf cf_alpha_tmp, e, nrm, sg, wt, soff, sch, om, gout, sout;
+  DevBuf cf_alpha_tmp, cf_in, gout, sout;
   DevBuf fpart, fcpart, fcoef, fscal;
-  DevBuf sz, scz, sres, ssteps, skt;
+  DevBuf sz, scz, sres, ssteps, eig_in;
 };
 KScratch& kscratch() {
   static KScratch k;
@@ -2134,7 +2134,7 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
 }
 
 int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_t seed, double* evals,
-              double2* evecs, int32_t* steps, bool real_values, hipStream_t s) {
+              double2* evecs, int32_t* steps, int32_t* conv, bool real_values, hipStream_t s) {
   KScratch& sc = kscratch();
   const int C = kb.C;
   if (C == 0) return IMPED_OK;
@@ -2157,7 +2157,6 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
     voff[c + 1] = voff[c] + mm * kb.h_n[c];
     R += small ? kb.h_n[c] : 0;
   }
-  h2d(sc.voff, voff.data(), C + 1, s);
   sc.vstore.reserve(std::max<int64_t>(voff[C], 1) * 16);
   sc.wbuf.reserve(std::max<int64_t>(R, 1) * 16);
   sc.coef.reserve((size_t)C * max_steps * 16);
@@ -2165,7 +2164,6 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   sc.yk.reserve((size_t)C * nev * max_steps * 8);
   sc.tw.reserve((size_t)C * nev * 5 * max_steps * 8);
   sc.tpiv.reserve((size_t)C * nev * max_steps * 4);
-  sc.conv.reserve((size_t)C * 4);
   EigenArgs a{};
   a.row_base = kb.row_base;
   a.chain_n = kb.chain_n;
@@ -2177,7 +2175,6 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   a.tol = tol;
   a.seed = seed;
   a.vstore = sc.vstore.p;
-  a.voff = sc.voff.as<int64_t>();
   a.wbuf = sc.wbuf.p;
   a.coef = sc.coef.as<double2>();
   a.ab = sc.ab.as<double>();
@@ -2187,14 +2184,19 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   a.evals = evals;
   a.evecs = evecs;
   a.steps = steps;
-  a.conv = sc.conv.as<int32_t>();
+  a.conv = conv;
   a.dgks = (int)g_tuning.eigen_dgks;
   a.check_every = (int)std::max<int64_t>(1, g_tuning.eigen_check_every);
   // largest chains first so the longest workgroups start first; chains up to
   // eigen_wave_max_n get one wave each (reductions by shuffles, no barriers),
   // the others eigen_bt threads
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kb.h_n[x] > kb.h_n[y]; });
-  if (!order.empty()) h2d(sc.clist, order.data(), order.size(), s);
+  // vector offsets and the chain order in one copy
+  HostPack pk;
+  const size_t o_voff = pk.add(voff.data(), C + 1), o_list = pk.add(order.data(), order.size());
+  pk.upload(sc.eig_in, s);
+  a.voff = HostPack::at<int64_t>(sc.eig_in, o_voff);
+  int32_t* clist = HostPack::at<int32_t>(sc.eig_in, o_list);
   size_t n_wide = 0;
   while (n_wide < order.size() && kb.h_n[order[n_wide]] > g_tuning.eigen_wave_max_n) ++n_wide;
   // LDS layout of a launch whose largest chain is order[first]: coefficients
@@ -2272,10 +2274,10 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   if (n_wide < order.size()) {
     TimedLaunch tl("eigen", s);
     layout(n_wide, 64, order.size() - n_wide);
-    a.chain_list = sc.clist.as<int32_t>() + n_wide;
+    a.chain_list = clist + n_wide;
     launch(64, dim3((unsigned)(order.size() - n_wide)));
   }
-  a.chain_list = sc.clist.as<int32_t>();
+  a.chain_list = clist;
   if (n_wide > 0) {
     // eigen_bt 0 (automatic): a launch lasts as long as its largest chain,
     // which gets 1024 threads above 512 rows (fewer rows per thread), else 512
@@ -2285,12 +2287,7 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
     layout(0, bt, n_wide);
     launch(bt, dim3((unsigned)n_wide));
   }
-  std::vector<int32_t> conv(C, 1);
-  d2h(conv.data(), sc.conv.p, C, s);
-  stream_sync(s);
   int rc = IMPED_OK;
-  for (int c : order)
-    if (!conv[c]) rc = IMPED_ERR_NOCONV;
   for (int c : large) {
     const int64_t base = chain_base(kb, c);
     double2* X = evecs + base * nev;
@@ -2299,7 +2296,10 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
                                                   steps + c, s)
                        : eigen_flat_chain<double2>(kb, c, nev, tol, max_steps, seed, evals + (size_t)c * nev, X,
                                                    steps + c, s);
-    if (r != IMPED_OK) rc = r;
+    const int32_t ok = r == IMPED_OK;
+    HIP_CHECK(hipMemcpyAsync(conv + c, &ok, sizeof(ok), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));  // ok lives on this stack frame
+    if (r != IMPED_OK && r != IMPED_ERR_NOCONV) rc = r;
   }
   return rc;
 }
@@ -2401,18 +2401,18 @@ void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t
   std::vector<int32_t> fill(off.begin(), off.end() - 1);
   for (int c = 0; c < C; ++c)
     if (spec_id[c] >= 0 && spec_id[c] < n_spec) chains[fill[spec_id[c]]++] = c;
-  h2d(sc.e, energy, std::max(C, 1), s);
-  h2d(sc.nrm, norm2, std::max(C, 1), s);
-  h2d(sc.sg, sign, std::max(C, 1), s);
-  h2d(sc.wt, zflag, std::max(C, 1), s);
-  h2d(sc.soff, off.data(), n_spec + 1, s);
-  h2d(sc.sch, chains.data(), chains.size(), s);
-  h2d(sc.om, omega, n_w, s);
-  h2d(sc.skt, kT, std::max(n_spec, 1), s);
+  // every per-call input in one copy
+  HostPack pk;
+  const size_t o_e = pk.add(energy, C), o_n = pk.add(norm2, C), o_s = pk.add(sign, C), o_z = pk.add(zflag, C),
+               o_off = pk.add(off.data(), n_spec + 1), o_ch = pk.add(chains.data(), chains.size()),
+               o_om = pk.add(omega, n_w), o_kt = pk.add(kT, std::max(n_spec, 1));
+  pk.upload(sc.cf_in, s);
   if (g_out) sc.gout.reserve((size_t)std::max(C, 1) * n_w * 16);
   sc.sout.reserve((size_t)std::max(n_spec, 1) * n_w * 8);
-  CFArgs a{alpha, beta, kout, K, sc.e.as<double>(), sc.nrm.as<double>(), sc.sg.as<int32_t>(), sc.wt.as<int32_t>(),
-           sc.skt.as<double>(), sc.soff.as<int32_t>(), sc.sch.as<int32_t>(), sc.om.as<double>(), n_w, delta,
+  CFArgs a{alpha, beta, kout, K, HostPack::at<double>(sc.cf_in, o_e), HostPack::at<double>(sc.cf_in, o_n),
+           HostPack::at<int32_t>(sc.cf_in, o_s), HostPack::at<int32_t>(sc.cf_in, o_z),
+           HostPack::at<double>(sc.cf_in, o_kt), HostPack::at<int32_t>(sc.cf_in, o_off),
+           HostPack::at<int32_t>(sc.cf_in, o_ch), HostPack::at<double>(sc.cf_in, o_om), n_w, delta,
            g_out ? sc.gout.as<double2>() : nullptr, sc.sout.as<double>()};
   if (n_spec > 0 && n_w > 0) {
     TimedLaunch tl("cf", s);

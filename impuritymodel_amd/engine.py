@@ -247,15 +247,19 @@ class Batch:
         check(lib.imped_batch_get_csr(self.h, ptr(rowptr), ptr(col), ptr(val.view(np.float64))))
         return rowptr, col[: self.nnz], val[: self.nnz]
 
-    def lanczos(self, K, reorth=0, v0=None):
-        alpha = np.zeros((max(self.C, 1), K), np.float64)
-        beta = np.zeros((max(self.C, 1), K), np.float64)
+    def lanczos(self, K, reorth=0, v0=None, want_coeffs=True):
+        """Lanczos chains of every chain (``imped_batch_lanczos``); alpha and
+        beta stay on the device (for ``cf``) unless ``want_coeffs``."""
+        alpha = np.zeros((max(self.C, 1), K), np.float64) if want_coeffs else None
+        beta = np.zeros((max(self.C, 1), K), np.float64) if want_coeffs else None
         kout = np.zeros(max(self.C, 1), np.int32)
         status = np.zeros(max(self.C, 1), np.int32)
         v0a = None if v0 is None else _c64(v0)
         check(lib.imped_batch_lanczos(self.h, ptr(v0a), int(K), int(reorth), ptr(alpha), ptr(beta), ptr(kout),
                                       ptr(status)))
         self.K = K
+        if not want_coeffs:
+            return None, None, kout[: self.C], status[: self.C]
         return alpha[: self.C], beta[: self.C], kout[: self.C], status[: self.C]
 
     def eigen(self, nev, tol=1e-12, max_steps=600, seed=12345, want_vectors=True, allow_noconv=False):

@@ -239,7 +239,7 @@ class FamilySectors:
         tim["expand_build"] = 0.0
         t1 = time.perf_counter()
         K = int(min(krylovSize, max(int(xb.chain_n.max()) if len(model) else 1, 1)))
-        _, _, kout, status = xb.lanczos(K, reorth=reorth)
+        _, _, kout, status = xb.lanczos(K, reorth=reorth, want_coeffs=False)
         if np.any(status != 0) or np.any(norm2 == 0):
             raise ValueError("Warning: beta==0, implementation absent!")
         tim["lanczos"] = time.perf_counter() - t1

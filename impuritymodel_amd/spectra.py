@@ -53,7 +53,7 @@ def getGreen(n_spin_orbitals, e, psi, hOp, omega, delta, krylovSize, slaterWeigh
     b.set_seeds([_words(keys, n)], [amps])
     b.build(fixed_basis=False, weight_min=1e-12)
     K = int(min(krylovSize, b.chain_n[0]))
-    _, _, kout, status = b.lanczos(K, reorth=1 if reorth == "full" else 0)
+    _, _, kout, status = b.lanczos(K, reorth=1 if reorth == "full" else 0, want_coeffs=False)
     if status[0] != 0:
         raise ValueError("Warning: beta==0, implementation absent!")
     g, _ = b.cf([e], [1.0], [1], [1], 1.0, [0], 1, omega, delta, want_g=True)
@@ -132,7 +132,7 @@ def getSpectra(n_spin_orbitals, hOp, tOps, psis, es, w, delta, restrictions=None
     # Krylov size per chain is min(krylovSize, n_c) (spectra.py:672); chains
     # shorter than K stop at their dimension inside the kernel.
     K = int(min(krylovSize, max(int(b.chain_n.max()), 1)))
-    _, _, kout, status = b.lanczos(K, reorth=1 if reorth == "full" else 0)
+    _, _, kout, status = b.lanczos(K, reorth=1 if reorth == "full" else 0, want_coeffs=False)
     if np.any(status != 0):
         raise ValueError("Warning: beta==0, implementation absent!")
     energy = np.asarray(es, np.float64)[out_src[live]]
@@ -194,7 +194,7 @@ def _rixs_chains(n, hops, tin, tout, src, es, nin, nout, wIns, wLoss, delta1, de
     tim["ground_build"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     K = int(min(krylovSize, max(int(gb.chain_n.max()), 1)))
-    _, _, _, status = gb.lanczos(K, reorth=1 if reorth == "full" else 0)
+    _, _, _, status = gb.lanczos(K, reorth=1 if reorth == "full" else 0, want_coeffs=False)
     if np.any(status != 0):
         raise ValueError("Warning: beta==0, implementation absent!")
     energy = np.repeat(e_c, nwi * nout)

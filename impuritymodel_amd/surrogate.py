@@ -731,6 +731,7 @@ def exact_spectra(family, thetas, krylovSize=200, reorth=1, sectors=True, **kw):
     W = opcompile.n_words(ops.n_spin_orbitals(family.nBaths))
     cache = family.sectors(W) if sectors else None
     if cache is not None:
+        family._bases.setdefault(W, cache.ground_basis)
         res = cache.xas(thetas, OMEGA, delta=DELTA, krylovSize=krylovSize, T=thetas[:, 7], reorth=reorth, **kw)
         return area_normalize(res.spectra), res
     if W not in family._bases:
