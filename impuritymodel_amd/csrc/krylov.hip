@@ -203,10 +203,55 @@ __global__ __launch_bounds__(64) void lanczos_lds_kernel(LanczosArgs a) {
   double2* w = V + (size_t)K * s;
   double2* cl = w + s;
   const int64_t* rowptr = a.rowptr + base;
+  // The first row of every lane (row = lane) keeps up to RC of its entries
+  // in registers for all K steps, so most of the SpMV reads no memory; the
+  // rest of the row, and rows from 64 on, are read from memory each step.
+  constexpr int RC = 16;
+  double2 hv[RC];
+  int hc[RC];
+  int64_t r0e1 = 0, r0rest = 0;
+  int hn = 0;
+  if (lane < n) {
+    const int64_t e0 = rowptr[lane];
+    r0e1 = rowptr[lane + 1];
+    hn = (int)(r0e1 - e0 < RC ? r0e1 - e0 : RC);
+    r0rest = e0 + hn;
+#pragma unroll
+    for (int k = 0; k < RC; ++k) {
+      hv[k] = k < hn ? a.val[e0 + k] : make_double2(0.0, 0.0);
+      hc[k] = k < hn ? a.col[e0 + k] : 0;
+    }
+  }
   // y = H x and Re<x, y> over the chain's rows (x, y in LDS)
   auto spmv = [&](const double2* x, double2* y) {
     double pa = 0.0;
-    for (int64_t i = lane; i < n; i += 64) {
+    if (lane < n) {
+      double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+#pragma unroll
+      for (int k = 0; k < RC; k += 2) {
+        if (k < hn) {
+          const double2 v = x[hc[k]];
+          ax += hv[k].x * v.x - hv[k].y * v.y;
+          ay += hv[k].x * v.y + hv[k].y * v.x;
+        }
+        if (k + 1 < hn) {
+          const double2 v = x[hc[k + 1]];
+          bx += hv[k + 1].x * v.x - hv[k + 1].y * v.y;
+          by += hv[k + 1].x * v.y + hv[k + 1].y * v.x;
+        }
+      }
+      for (int64_t e = r0rest; e < r0e1; ++e) {
+        const double2 h = a.val[e];
+        const double2 v = x[a.col[e]];
+        ax += h.x * v.x - h.y * v.y;
+        ay += h.x * v.y + h.y * v.x;
+      }
+      ax += bx;
+      ay += by;
+      y[lane] = make_double2(ax, ay);
+      pa += ax * x[lane].x + ay * x[lane].y;
+    }
+    for (int64_t i = lane + 64; i < n; i += 64) {
       const int64_t e0 = rowptr[i], e1 = rowptr[i + 1];
       double ax = 0.0, ay = 0.0;
       for (int64_t e = e0; e < e1; ++e) {
@@ -220,30 +265,50 @@ __global__ __launch_bounds__(64) void lanczos_lds_kernel(LanczosArgs a) {
     }
     return wave_sum(pa);
   };
-  // one classical Gram-Schmidt pass against V_0..V_{m-1}; returns |w|^2
+  // one classical Gram-Schmidt pass against V_0..V_{m-1}; returns |w|^2.
+  // Two partial sums per product (even and odd terms) halve the dependent
+  // chain of additions.
   auto gram_schmidt = [&](int m) {
     for (int l = lane; l < m; l += 64) {
       const double2* vl = V + (size_t)l * s;
-      double cx = 0.0, cy = 0.0;
-#pragma unroll 4
-      for (int64_t i = 0; i < n; ++i) {
+      double cx = 0.0, cy = 0.0, dx = 0.0, dy = 0.0;
+      int64_t i = 0;
+#pragma unroll 2
+      for (; i + 1 < n; i += 2) {
+        const double2 v = vl[i], b = w[i], v1 = vl[i + 1], b1 = w[i + 1];
+        cx += v.x * b.x + v.y * b.y;
+        cy += v.x * b.y - v.y * b.x;
+        dx += v1.x * b1.x + v1.y * b1.y;
+        dy += v1.x * b1.y - v1.y * b1.x;
+      }
+      if (i < n) {
         const double2 v = vl[i], b = w[i];
         cx += v.x * b.x + v.y * b.y;
         cy += v.x * b.y - v.y * b.x;
       }
-      cl[l] = make_double2(cx, cy);
+      cl[l] = make_double2(cx + dx, cy + dy);
     }
     __syncthreads();
     double p = 0.0;
     for (int64_t i = lane; i < n; i += 64) {
-      double wx = w[i].x, wy = w[i].y;
-#pragma unroll 8
-      for (int l = 0; l < m; ++l) {
+      double ux = 0.0, uy = 0.0, tx = 0.0, ty = 0.0;
+      int l = 0;
+#pragma unroll 4
+      for (; l + 1 < m; l += 2) {
+        const double2 cc = cl[l], c1 = cl[l + 1];
+        const double2 av = V[(size_t)l * s + i], a1 = V[(size_t)(l + 1) * s + i];
+        ux += av.x * cc.x - av.y * cc.y;
+        uy += av.x * cc.y + av.y * cc.x;
+        tx += a1.x * c1.x - a1.y * c1.y;
+        ty += a1.x * c1.y + a1.y * c1.x;
+      }
+      if (l < m) {
         const double2 cc = cl[l];
         const double2 av = V[(size_t)l * s + i];
-        wx -= av.x * cc.x - av.y * cc.y;
-        wy -= av.x * cc.y + av.y * cc.x;
+        ux += av.x * cc.x - av.y * cc.y;
+        uy += av.x * cc.y + av.y * cc.x;
       }
+      const double wx = w[i].x - (ux + tx), wy = w[i].y - (uy + ty);
       w[i] = make_double2(wx, wy);
       p += wx * wx + wy * wy;
     }
@@ -1360,14 +1425,27 @@ struct CFArgs {
   double2* g_out;  // optional [C][n_w]
   double* spec_out;
   int q0;          // first spectrum of this launch (grids hold at most 65535 rows)
+  int tile;        // chains staged in LDS at a time
 };
 
+// 1/d for the continued fraction: conj(d) / |d|^2 with a Newton-refined
+// hardware reciprocal.  |d|^2 stays in range here: Im d = +-delta != 0 and
+// |d| is of the order of the spectrum's energies.
+__device__ __forceinline__ double2 cf_recip(double2 d) {
+  const double inv = rcp_nr(d.x * d.x + d.y * d.y);
+  return make_double2(d.x * inv, -d.y * inv);
+}
+
 // One thread per (spectrum, omega); the spectrum's chains are summed in order.
+// The workgroup's threads share the spectrum, so its chains' alpha and beta^2
+// are staged in LDS, a.tile chains at a time, and every level of a thread's
+// recurrence reads them there (a broadcast) instead of from memory.
 __global__ __launch_bounds__(256) void cf_kernel(CFArgs a) {
+  extern __shared__ double2 ab[];  // [a.tile][a.K]: (alpha_i, beta_i^2)
   const int q = a.q0 + blockIdx.y;
   const int iw = blockIdx.x * blockDim.x + threadIdx.x;
-  if (iw >= a.n_w) return;
-  const double om = a.omega[iw];
+  const bool act = iw < a.n_w;
+  const double om = act ? a.omega[iw] : 0.0;
   const int t0 = a.spec_chain_off[q], t1 = a.spec_chain_off[q + 1];
   // Boltzmann weights exp(-(e - e_min)/kT) / Z over the spectrum's states
   const double kT = a.kT[q];
@@ -1377,33 +1455,40 @@ __global__ __launch_bounds__(256) void cf_kernel(CFArgs a) {
   for (int t = t0; t < t1; ++t)
     if (a.zflag[a.spec_chains[t]]) Z += exp(-(a.energy[a.spec_chains[t]] - emin) / kT);
   double acc = 0.0;
-  for (int t = t0; t < t1; ++t) {
-    const int c = a.spec_chains[t];
-    const int K = a.kout[c];
-    const double sg = (double)a.sign[c];
-    // z = sg*omega + i*sg*delta + e  (photoemission uses -omega, -delta)
-    const double zr = sg * om + a.energy[c];
-    const double zi = sg * a.delta;
-    const double* al = a.alpha + (size_t)c * a.K;
-    const double* be = a.beta + (size_t)c * a.K;
-    double2 g = make_double2(0.0, 0.0);
-    for (int i = K - 1; i >= 0; --i) {
-      double2 d;
-      if (i == K - 1) {
-        d = make_double2(zr - al[i], zi);
-      } else {
-        const double b2 = be[i] * be[i];
-        d = make_double2(zr - al[i] - b2 * g.x, zi - b2 * g.y);
-      }
-      g = crecip(d);
+  for (int tb = t0; tb < t1; tb += a.tile) {
+    const int nt = min(a.tile, t1 - tb);
+    __syncthreads();  // the previous tile has been read
+    for (int idx = threadIdx.x; idx < nt * a.K; idx += blockDim.x) {
+      const int t = idx / a.K, i = idx - t * a.K;
+      const size_t c = (size_t)a.spec_chains[tb + t];
+      const double b = a.beta[c * a.K + i];
+      ab[idx] = make_double2(a.alpha[c * a.K + i], b * b);
     }
-    // gs = sign * norm2 * g ; spectrum contribution weight * (-Im gs)
-    const double f = sg * a.norm2[c];
-    const double2 gs = make_double2(f * g.x, f * g.y);
-    if (a.g_out) a.g_out[(size_t)c * a.n_w + iw] = gs;
-    acc += exp(-(a.energy[c] - emin) / kT) * (-gs.y);
+    __syncthreads();
+    if (!act) continue;
+    for (int t = 0; t < nt; ++t) {
+      const int c = a.spec_chains[tb + t];
+      const int K = a.kout[c];
+      const double sg = (double)a.sign[c];
+      // z = sg*omega + i*sg*delta + e  (photoemission uses -omega, -delta)
+      const double zr = sg * om + a.energy[c];
+      const double zi = sg * a.delta;
+      const double2* abt = ab + (size_t)t * a.K;
+      double2 g = make_double2(0.0, 0.0);
+      for (int i = K - 1; i >= 0; --i) {
+        const double2 q2 = abt[i];
+        // the last level has no tail (beta_{K-1} is not part of the chain)
+        const double b2 = i < K - 1 ? q2.y : 0.0;
+        g = cf_recip(make_double2(zr - q2.x - b2 * g.x, zi - b2 * g.y));
+      }
+      // gs = sign * norm2 * g ; spectrum contribution weight * (-Im gs)
+      const double f = sg * a.norm2[c];
+      const double2 gs = make_double2(f * g.x, f * g.y);
+      if (a.g_out) a.g_out[(size_t)c * a.n_w + iw] = gs;
+      acc += exp(-(a.energy[c] - emin) / kT) * (-gs.y);
+    }
   }
-  a.spec_out[(size_t)q * a.n_w + iw] = Z > 0.0 ? acc / Z : 0.0;
+  if (act) a.spec_out[(size_t)q * a.n_w + iw] = Z > 0.0 ? acc / Z : 0.0;
 }
 
 // ------------------------------------------------------------------ flat (multi-workgroup) Krylov
@@ -2414,12 +2499,24 @@ void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t
            HostPack::at<double>(sc.cf_in, o_kt), HostPack::at<int32_t>(sc.cf_in, o_off),
            HostPack::at<int32_t>(sc.cf_in, o_ch), HostPack::at<double>(sc.cf_in, o_om), n_w, delta,
            g_out ? sc.gout.as<double2>() : nullptr, sc.sout.as<double>()};
+  // LDS tile: as many of a spectrum's chains as 56 KiB holds, at least one
+  // (a chain longer than 3584 levels takes up to 150 KiB)
+  if ((size_t)K * sizeof(double2) > 150 * 1024)
+    throw HipError{"continued fraction: at most 9600 Lanczos levels per chain"};
+  int max_chains = 1;
+  for (int q = 0; q < n_spec; ++q) max_chains = std::max(max_chains, off[q + 1] - off[q]);
+  a.tile = (int)std::max<int64_t>(1, std::min<int64_t>(max_chains, (56 * 1024) / ((int64_t)K * 16)));
+  const size_t shm = (size_t)a.tile * K * sizeof(double2);
+  static std::once_flag big_lds;
+  std::call_once(big_lds, [] {
+    HIP_CHECK(hipFuncSetAttribute((const void*)cf_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+  });
   if (n_spec > 0 && n_w > 0) {
     TimedLaunch tl("cf", s);
     for (int q0 = 0; q0 < n_spec; q0 += 65535) {
       a.q0 = q0;
       const unsigned rows = (unsigned)std::min(65535, n_spec - q0);
-      hipLaunchKernelGGL(cf_kernel, dim3(blocks_for(n_w, 256), rows), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(cf_kernel, dim3(blocks_for(n_w, 256), rows), dim3(256), shm, s, a);
       HIP_CHECK(hipGetLastError());
     }
   }
