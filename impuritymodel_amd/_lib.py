@@ -64,6 +64,8 @@ SIGNATURES = {
     "imped_affine_destroy": [_p],
     "imped_batch_set_affine": [_p, _p, _p, _p, _i32, _p, _d],
     "imped_batch_seed_affine": [_p, _p, _p, _p, _p, _p, _d, _d, _p],
+    "imped_family_xas": [_p, _p, _p, _i32, _i32, _p, _i32, _p, _p, _i32, _i32, _d, _i32, _i32, _d, _i32, _u64, _d,
+                         _d, _d, _p, _i32, _d, _p, _p, _p, _p, _p, _p, _p],
     "imped_lanczos_noalloc": [_p, _p, _p, _i64, _p, _i32, _i32, _p, _p, _p, _p, _p],
     "imped_batch_eigen": [_p, _i32, _d, _i32, _u64, _p, _p, _p],
     "imped_batch_cf": [_p, _p, _p, _p, _p, _p, _p, _i32, _p, _i32, _d, _p, _p],

@@ -141,6 +141,7 @@ struct Tuning {
   int64_t small_sort = 1;               // one-workgroup sort for small key sets (0: digit passes always)
   int64_t mlp_graph = 0;                // surrogate predictions as one replayed HIP graph (0: direct launches)
   int64_t lanczos_lds = 1;              // reorthogonalized chains whose Krylov basis fits: basis in LDS
+  int64_t lanczos_lds_waves = 4;        // waves per chain of the LDS-resident Lanczos (1, 2 or 4)
 };
 extern Tuning g_tuning;
 

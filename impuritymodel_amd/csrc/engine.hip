@@ -268,6 +268,8 @@ struct Batch {
   DevBuf v0dev;
   bool have_v0dev = false;
   DevBuf fill_meta;  // chain table of imped_batch_set_affine
+  DevBuf seed_meta, norm2dev;  // chain table and |T psi|^2 of imped_batch_seed_affine
+  bool have_norm2dev = false;
 };
 
 // Seed amplitudes are placed on the first rows of their chain: every chain
@@ -1005,6 +1007,12 @@ int imped_set_tuning(const char* name, int64_t value) {
     g_tuning.mlp_graph = value != 0;
   } else if (n == "lanczos_lds") {
     g_tuning.lanczos_lds = value != 0;
+  } else if (n == "lanczos_lds_waves") {
+    if (value != 1 && value != 2 && value != 4) {
+      set_error("imped_set_tuning: lanczos_lds_waves must be 1, 2 or 4");
+      return IMPED_ERR_INVALID;
+    }
+    g_tuning.lanczos_lds_waves = value;
   } else if (n == "eigen_lds_vectors") {
     g_tuning.eigen_lds_vectors = value < 0 ? -1 : value;
   } else if (n == "lanczos_graph_steps") {
@@ -1209,6 +1217,8 @@ int imped_batch_build(imped_batch batch, int fixed_basis, double weight_min, int
     h2d(b->chain_model, b->h_chain_model.data(), std::max<size_t>(b->h_chain_model.size(), 1), stream());
     b->chain_model_dev = true;
   }
+  b->have_v0dev = false;
+  b->have_norm2dev = false;
   build(*b, fixed_basis ? 1 : 0, weight_min, max_layers);
   API_END
 }
@@ -1218,6 +1228,7 @@ int imped_batch_set_csr(imped_batch batch, const int64_t* chain_n, const int64_t
   API_BEGIN
   Batch* b = reinterpret_cast<Batch*>(batch);
   b->have_v0dev = false;
+  b->have_norm2dev = false;
   b->h_n.assign(chain_n, chain_n + b->C);
   b->h_row_base.assign(b->C + 1, 0);
   for (int c = 0; c < b->C; ++c) {
@@ -1450,6 +1461,7 @@ int imped_batch_set_affine(imped_batch batch, imped_affine aff, const int32_t* c
   b->row_keys = false;
   b->real = a->real;
   b->have_v0dev = false;
+  b->have_norm2dev = false;
   b->have_seed_amps = false;
   API_END
 }
@@ -1462,9 +1474,9 @@ int imped_batch_seed_affine(imped_batch batch, imped_batch ground, imped_affine 
   const Batch* g = reinterpret_cast<const Batch*>(ground);
   const AffineCsr* a = reinterpret_cast<const AffineCsr*>(trans);
   if (!b || !g || !a || !b->built || !g->built || g->nev < 1 || (b->C > 0 && (!src_chain || !src_vec ||
-      !chain_struct || !norm2))) {
-    set_error("imped_batch_seed_affine: built batches (the ground one after imped_batch_eigen), structures, "
-              "chain maps and norm2 required");
+      !chain_struct))) {
+    set_error("imped_batch_seed_affine: built batches (the ground one after imped_batch_eigen), structures and "
+              "chain maps required");
     return IMPED_ERR_INVALID;
   }
   const int C = b->C;
@@ -1484,21 +1496,139 @@ int imped_batch_seed_affine(imped_batch batch, imped_batch ground, imped_affine 
   }
   // structure columns index the ground chain's rows
   hipStream_t s = stream();
-  DevBuf dmeta, dn;
-  h2d(dmeta, meta.data(), meta.size(), s);
+  h2d(b->seed_meta, meta.data(), meta.size(), s);
   b->v0dev.reserve(std::max<int64_t>(b->R, 1) * 16);
-  dn.reserve(std::max(C, 1) * sizeof(double));
+  b->norm2dev.reserve(std::max(C, 1) * sizeof(double));
   if (C > 0) {
     TimedLaunch tl("affine_seed", s);
     hipLaunchKernelGGL(affine_seed_kernel, dim3(C), dim3(64), 0, s, a->meta.as<int64_t>(), a->S,
-                       a->srowptr.as<int64_t>(), a->scol.as<int32_t>(), a->base.as<double2>(), dmeta.as<int64_t>(),
-                       C, g->row_base.as<int64_t>(), g->chain_n.as<int64_t>(), g->evecs.as<double2>(), g->nev,
-                       psi_weight_min, weight_min, b->v0dev.as<double2>(), dn.as<double>());
+                       a->srowptr.as<int64_t>(), a->scol.as<int32_t>(), a->base.as<double2>(),
+                       b->seed_meta.as<int64_t>(), C, g->row_base.as<int64_t>(), g->chain_n.as<int64_t>(),
+                       g->evecs.as<double2>(), g->nev, psi_weight_min, weight_min, b->v0dev.as<double2>(),
+                       b->norm2dev.as<double>());
     HIP_CHECK(hipGetLastError());
-    d2h(norm2, dn.p, C, s);
+    if (norm2) d2h(norm2, b->norm2dev.p, C, s);
   }
-  stream_sync(s);
+  // without norm2 nothing is read back and nothing waits
+  if (norm2) stream_sync(s);
   b->have_v0dev = true;
+  b->have_norm2dev = true;
+  API_END
+}
+
+/* ------------------------------------------------------------------ family XAS in one call */
+
+int imped_family_xas(imped_affine ground, imped_affine sector, imped_affine trans, int n_comp, int n_pol,
+                     const int32_t* sector_index, int n_models, const double* theta, const double* kT, int nev,
+                     int n_psi_max, double energy_cut, int krylov_size, int reorth, double eig_tol,
+                     int eig_max_steps, uint64_t seed, double h_weight_min, double psi_weight_min,
+                     double weight_min, const double* omega, int n_w, double delta, double* spectra,
+                     double* energies, int32_t* n_kept, int32_t* eig_steps, int32_t* chain_struct, int32_t* kout,
+                     int32_t* info) {
+  API_BEGIN
+  const AffineCsr* ag = reinterpret_cast<const AffineCsr*>(ground);
+  const AffineCsr* as = reinterpret_cast<const AffineCsr*>(sector);
+  const AffineCsr* at = reinterpret_cast<const AffineCsr*>(trans);
+  if (!ag || !as || !at || n_comp < 1 || n_comp > ag->S || n_pol < 1 || !sector_index || n_models < 0 ||
+      (n_models > 0 && (!theta || !kT || !spectra || !energies || !n_kept)) || nev < 1 || n_psi_max < 1 ||
+      krylov_size < 1 || n_w < 0 || (n_w > 0 && !omega)) {
+    set_error("imped_family_xas: structures, n_comp <= ground structures, sector_index, theta, kT, outputs, "
+              "nev >= 1, n_psi_max >= 1 and krylov_size >= 1 required");
+    return IMPED_ERR_INVALID;
+  }
+  for (int i = 0; i < n_comp * n_pol; ++i)
+    if (sector_index[i] < 0 || sector_index[i] >= as->S || sector_index[i] >= at->S) {
+      set_error("imped_family_xas: sector index out of range");
+      return IMPED_ERR_INVALID;
+    }
+  for (int m = 0; m < n_models; ++m)
+    if (!(kT[m] > 0.0)) {
+      set_error("imped_family_xas: kT must be positive");
+      return IMPED_ERR_INVALID;
+    }
+  if (n_models == 0) return IMPED_OK;
+  auto make_batch = [](int C) {
+    auto b = std::make_unique<Batch>();
+    b->C = C;
+    b->h_seed_off.assign(C + 1, 0);
+    return b;
+  };
+  // ground chains (model, component)
+  const int G = n_models * n_comp;
+  auto gb = make_batch(G);
+  std::vector<int32_t> g_struct(G), g_model(G);
+  for (int c = 0; c < G; ++c) {
+    g_struct[c] = c % n_comp;
+    g_model[c] = c / n_comp;
+  }
+  int rc = imped_batch_set_affine(reinterpret_cast<imped_batch>(gb.get()), ground, g_struct.data(), g_model.data(),
+                                  n_models, theta, h_weight_min);
+  if (rc != IMPED_OK) return rc;
+  std::vector<double> ev((size_t)G * nev);
+  std::vector<int32_t> st(G);
+  rc = imped_batch_eigen(reinterpret_cast<imped_batch>(gb.get()), nev, eig_tol, eig_max_steps, seed, ev.data(),
+                         nullptr, st.data());
+  if (rc != IMPED_OK) return rc;
+  if (eig_steps) std::copy(st.begin(), st.end(), eig_steps);
+  // per model: the lowest n_psi_max of the components' eigenvalues (stable
+  // in component-major order), kept within energy_cut kT of the lowest
+  std::vector<int32_t> x_struct, x_model, x_src, x_vec, x_sign, x_z, x_spec;
+  std::vector<double> x_e;
+  std::vector<int> ord(n_comp * nev);
+  for (int m = 0; m < n_models; ++m) {
+    const double* e = ev.data() + (size_t)m * n_comp * nev;
+    for (int i = 0; i < n_comp * nev; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return e[x] < e[y]; });
+    const int top = std::min(n_psi_max, n_comp * nev);
+    int kept = 0;
+    while (kept < top && e[ord[kept]] - e[ord[0]] < energy_cut * kT[m]) ++kept;
+    n_kept[m] = kept;
+    for (int i = 0; i < n_psi_max; ++i) energies[(size_t)m * n_psi_max + i] = i < kept ? e[ord[i]] : NAN;
+    for (int t = 0; t < n_pol; ++t)
+      for (int i = 0; i < kept; ++i) {
+        const int k = ord[i] / nev, v = ord[i] % nev;
+        x_struct.push_back(sector_index[k * n_pol + t]);
+        x_model.push_back(m);
+        x_src.push_back(m * n_comp + k);
+        x_vec.push_back(v);
+        x_e.push_back(e[ord[i]]);
+        x_sign.push_back(1);
+        x_z.push_back(t == 0);
+        x_spec.push_back(m);
+      }
+  }
+  const int X = (int)x_struct.size();
+  auto xb = make_batch(X);
+  rc = imped_batch_set_affine(reinterpret_cast<imped_batch>(xb.get()), sector, x_struct.data(), x_model.data(),
+                              n_models, theta, h_weight_min);
+  if (rc != IMPED_OK) return rc;
+  rc = imped_batch_seed_affine(reinterpret_cast<imped_batch>(xb.get()), reinterpret_cast<imped_batch>(gb.get()),
+                               trans, x_src.data(), x_vec.data(), x_struct.data(), psi_weight_min, weight_min, nullptr);
+  if (rc != IMPED_OK) return rc;
+  int64_t nmax = 1;
+  for (int c = 0; c < X; ++c) nmax = std::max(nmax, xb->h_n[c]);
+  const int K = (int)std::min<int64_t>(krylov_size, nmax);
+  rc = imped_batch_lanczos(reinterpret_cast<imped_batch>(xb.get()), nullptr, K, reorth, nullptr, nullptr, nullptr,
+                           nullptr);
+  if (rc != IMPED_OK) return rc;
+  // kout and status come back with the spectra (one wait, in run_cf)
+  std::vector<int32_t> ks(2 * (size_t)X);
+  d2h(ks.data(), xb->kout.p, ks.size(), stream());
+  run_cf(X, K, xb->alpha.as<double>(), xb->beta.as<double>(), xb->kout.as<int32_t>(), x_e.data(), nullptr,
+         x_sign.data(), x_z.data(), kT, x_spec.data(), n_models, omega, n_w, delta, nullptr, spectra, stream(),
+         xb->norm2dev.as<double>());
+  if (chain_struct) std::copy(x_struct.begin(), x_struct.end(), chain_struct);
+  if (kout) std::copy(ks.begin(), ks.begin() + X, kout);
+  if (info) {
+    info[0] = X;
+    info[1] = K;
+    info[2] = info[3] = 0;
+  }
+  for (int c = 0; c < X; ++c)
+    if (ks[X + c] != IMPED_OK) {
+      set_error("imped_family_xas: Lanczos breakdown (beta == 0) in an excited chain");
+      return IMPED_ERR_BREAKDOWN;
+    }
   API_END
 }
 
@@ -1549,6 +1679,7 @@ int imped_batch_lanczos(imped_batch batch, const double* v0_host, int K, int reo
               stream());
   if (alpha) d2h(alpha, b->alpha.p, (size_t)b->C * K, stream());
   if (beta) d2h(beta, b->beta.p, (size_t)b->C * K, stream());
+  if (!alpha && !beta && !kout && !status) return IMPED_OK;  // results stay on the device; nothing waits
   std::vector<int32_t> ks;
   if (kout && status && b->C) {
     ks.resize(2 * (size_t)b->C);
@@ -1667,8 +1798,12 @@ int imped_batch_cf(imped_batch batch, const double* energy, const double* norm2,
       set_error("imped_batch_cf: kT must be positive");
       return IMPED_ERR_INVALID;
     }
+  if (!norm2 && !b->have_norm2dev) {
+    set_error("imped_batch_cf: norm2 is NULL and the batch has no device norms (imped_batch_seed_affine)");
+    return IMPED_ERR_INVALID;
+  }
   run_cf(b->C, b->K, b->alpha.as<double>(), b->beta.as<double>(), b->kout.as<int32_t>(), energy, norm2, sign, zflag,
-         kT, spec_id, n_spec, omega, n_w, delta, g_out, spec_out, stream());
+         kT, spec_id, n_spec, omega, n_w, delta, g_out, spec_out, stream(), b->norm2dev.as<double>());
   API_END
 }
 

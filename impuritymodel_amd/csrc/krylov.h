@@ -42,10 +42,12 @@ int run_solve(const KrylovBatch& kb, const double2* y, int S, const double* z_ho
               double2* X, int32_t* steps_host, double* resid_host, hipStream_t s);
 
 // Continued fractions of all chains on the omega grid plus the Boltzmann
-// average over each spectrum's chains (host arrays are uploaded).
+// average over each spectrum's chains (host arrays are uploaded; with norm2
+// NULL the chains' norms are read from norm2_dev on the device).
 void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t* kout, const double* energy,
             const double* norm2, const int32_t* sign, const int32_t* zflag, const double* kT, const int32_t* spec_id,
-            int n_spec, const double* omega, int n_w, double delta, double* g_out, double* spec_out, hipStream_t s);
+            int n_spec, const double* omega, int n_w, double delta, double* g_out, double* spec_out, hipStream_t s,
+            const double* norm2_dev = nullptr);
 
 // out vector o = op[out_model[o]] applied to source vector out_src[o].
 void vector_apply(OpsView ops, int W, const std::vector<int64_t>& src_off, const uint64_t* src_keys,

@@ -357,6 +357,236 @@ __global__ __launch_bounds__(64) void lanczos_lds_kernel(LanczosArgs a) {
   }
 }
 
+// ---- the same recurrence with NW waves per chain
+//
+// lanczos_lds_kernel leaves each step's Gram-Schmidt products as one long
+// sequential sum per lane (n terms for a dot, m for an update) on a single
+// wave.  Here NW waves share a chain: for the dots, wave g sums rows
+// [g n/NW, (g+1) n/NW) of every <V_l, w> and the partials are added in LDS;
+// for the update, wave g sums vectors [g m/NW, (g+1) m/NW) of every row and
+// those partials are added in LDS.  Each dependent chain is NW times shorter
+// and the chain's CU runs NW times as many waves.  Row t of the SpMV (t <
+// 64 NW) keeps up to RC entries of H in registers.  LDS: V (K rows of
+// stride s), w, the coefficients, and NW x max(K, s) partials (dots, then
+// updates).
+
+size_t lanczos_ldsw_bytes(int64_t n, int K, int nw) {
+  const int64_t kk = K < n ? K : n;
+  const int64_t s = (n + 6) / 8 * 8 + 1;
+  return (size_t)(kk * s + s + kk + 2 + nw * (kk > s ? kk : s)) * sizeof(double2);
+}
+
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
+  constexpr int T = 64 * NW;
+  constexpr int RC = 16;
+  extern __shared__ double2 lds[];
+  __shared__ double red[2][NW];
+  const int c = a.chain_list[blockIdx.x];
+  const int64_t n = a.chain_n[c];
+  const int64_t base = a.row_base[c];
+  double* alpha = a.alpha + (size_t)c * a.K;
+  double* beta = a.beta + (size_t)c * a.K;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (n == 0) {
+    if (tid == 0) {
+      a.kout[c] = 0;
+      a.status[c] = IMPED_OK;
+    }
+    return;
+  }
+  const int K = (int)(a.K < n ? a.K : n);
+  const int64_t s = lds_row_stride(n);
+  double2* V = lds;
+  double2* w = V + (size_t)K * s;
+  double2* cl = w + s;
+  // dot partials [NW][K] and update partials [NW][s] share one region: the
+  // two phases are separated by barriers
+  double2* part = cl + (K + 2);
+  double2* upd = part;
+  const int64_t* rowptr = a.rowptr + base;
+  // workgroup sum: every thread gets the same value (the waves' partials
+  // added in wave order); red alternates between two rows so one barrier
+  // per sum suffices
+  int rsel = 0;
+  auto bsum = [&](double x) {
+    x = wave_sum(x);
+    if (lane == 0) red[rsel][wv] = x;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int g = 0; g < NW; ++g) t += red[rsel][g];
+    rsel ^= 1;
+    return t;
+  };
+  double2 hv[RC];
+  int hc[RC];
+  int64_t r_end = 0, r_rest = 0;
+  int hn = 0;
+  if (tid < n) {
+    const int64_t e0 = rowptr[tid];
+    r_end = rowptr[tid + 1];
+    hn = (int)(r_end - e0 < RC ? r_end - e0 : RC);
+    r_rest = e0 + hn;
+#pragma unroll
+    for (int k = 0; k < RC; ++k) {
+      hv[k] = k < hn ? a.val[e0 + k] : make_double2(0.0, 0.0);
+      hc[k] = k < hn ? a.col[e0 + k] : 0;
+    }
+  }
+  // y = H x (rows by their owner threads) and Re<x, y>; ends with a barrier
+  auto spmv = [&](const double2* x, double2* y) {
+    double pa = 0.0;
+    if (tid < n) {
+      double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+#pragma unroll
+      for (int k = 0; k < RC; k += 2) {
+        if (k < hn) {
+          const double2 v = x[hc[k]];
+          ax += hv[k].x * v.x - hv[k].y * v.y;
+          ay += hv[k].x * v.y + hv[k].y * v.x;
+        }
+        if (k + 1 < hn) {
+          const double2 v = x[hc[k + 1]];
+          bx += hv[k + 1].x * v.x - hv[k + 1].y * v.y;
+          by += hv[k + 1].x * v.y + hv[k + 1].y * v.x;
+        }
+      }
+      for (int64_t e = r_rest; e < r_end; ++e) {
+        const double2 h = a.val[e];
+        const double2 v = x[a.col[e]];
+        ax += h.x * v.x - h.y * v.y;
+        ay += h.x * v.y + h.y * v.x;
+      }
+      ax += bx;
+      ay += by;
+      y[tid] = make_double2(ax, ay);
+      pa += ax * x[tid].x + ay * x[tid].y;
+    }
+    for (int64_t i = tid + T; i < n; i += T) {
+      const int64_t e0 = rowptr[i], e1 = rowptr[i + 1];
+      double ax = 0.0, ay = 0.0;
+      for (int64_t e = e0; e < e1; ++e) {
+        const double2 h = a.val[e];
+        const double2 v = x[a.col[e]];
+        ax += h.x * v.x - h.y * v.y;
+        ay += h.x * v.y + h.y * v.x;
+      }
+      y[i] = make_double2(ax, ay);
+      pa += ax * x[i].x + ay * x[i].y;
+    }
+    return bsum(pa);
+  };
+  // one classical Gram-Schmidt pass against V_0..V_{m-1}; returns |w|^2
+  auto gram_schmidt = [&](int m) {
+    const int64_t r0 = n * wv / NW, r1 = n * (wv + 1) / NW;
+    for (int l = lane; l < m; l += 64) {
+      const double2* vl = V + (size_t)l * s;
+      double cx = 0.0, cy = 0.0, dx = 0.0, dy = 0.0;
+      int64_t i = r0;
+#pragma unroll 4
+      for (; i + 1 < r1; i += 2) {
+        const double2 v = vl[i], b = w[i], v1 = vl[i + 1], b1 = w[i + 1];
+        cx += v.x * b.x + v.y * b.y;
+        cy += v.x * b.y - v.y * b.x;
+        dx += v1.x * b1.x + v1.y * b1.y;
+        dy += v1.x * b1.y - v1.y * b1.x;
+      }
+      if (i < r1) {
+        const double2 v = vl[i], b = w[i];
+        cx += v.x * b.x + v.y * b.y;
+        cy += v.x * b.y - v.y * b.x;
+      }
+      part[(size_t)wv * K + l] = make_double2(cx + dx, cy + dy);
+    }
+    __syncthreads();
+    for (int l = tid; l < m; l += T) {
+      double2 t = part[l];
+#pragma unroll
+      for (int g = 1; g < NW; ++g) {
+        const double2 u = part[(size_t)g * K + l];
+        t.x += u.x;
+        t.y += u.y;
+      }
+      cl[l] = t;
+    }
+    __syncthreads();
+    const int l0 = m * wv / NW, l1 = m * (wv + 1) / NW;
+    for (int64_t i = lane; i < n; i += 64) {
+      double ux = 0.0, uy = 0.0, tx = 0.0, ty = 0.0;
+      int l = l0;
+#pragma unroll 4
+      for (; l + 1 < l1; l += 2) {
+        const double2 cc = cl[l], c1 = cl[l + 1];
+        const double2 av = V[(size_t)l * s + i], a1 = V[(size_t)(l + 1) * s + i];
+        ux += av.x * cc.x - av.y * cc.y;
+        uy += av.x * cc.y + av.y * cc.x;
+        tx += a1.x * c1.x - a1.y * c1.y;
+        ty += a1.x * c1.y + a1.y * c1.x;
+      }
+      if (l < l1) {
+        const double2 cc = cl[l];
+        const double2 av = V[(size_t)l * s + i];
+        ux += av.x * cc.x - av.y * cc.y;
+        uy += av.x * cc.y + av.y * cc.x;
+      }
+      upd[(size_t)wv * s + i] = make_double2(ux + tx, uy + ty);
+    }
+    __syncthreads();
+    double p = 0.0;
+    for (int64_t i = tid; i < n; i += T) {
+      double ux = upd[i].x, uy = upd[i].y;
+#pragma unroll
+      for (int g = 1; g < NW; ++g) {
+        ux += upd[(size_t)g * s + i].x;
+        uy += upd[(size_t)g * s + i].y;
+      }
+      const double wx = w[i].x - ux, wy = w[i].y - uy;
+      w[i] = make_double2(wx, wy);
+      p += wx * wx + wy * wy;
+    }
+    return bsum(p);
+  };
+  for (int64_t i = tid; i < n; i += T) V[i] = a.v0[base + i];
+  __syncthreads();
+  // wp = H v0; alpha0 = Re(conj(wp) . v0); w = wp - alpha0 v0
+  double al = spmv(V, w);
+  if (tid == 0) alpha[0] = al;
+  for (int64_t i = tid; i < n; i += T) w[i] = make_double2(w[i].x - al * V[i].x, w[i].y - al * V[i].y);
+  int status = IMPED_OK;
+  int kdone = K;
+  for (int j = 1; j < K; ++j) {
+    double p = 0.0;
+    for (int64_t i = tid; i < n; i += T) p += w[i].x * w[i].x + w[i].y * w[i].y;
+    p = bsum(p);
+    // full reorthogonalization against V_0..V_{j-1}; a second pass when
+    // the first removed more than half of |w|^2 (as lanczos_kernel)
+    double p1 = gram_schmidt(j);
+    if (p1 < 0.5 * p) p1 = gram_schmidt(j);
+    const double b = sqrt(p1);
+    if (tid == 0) beta[j - 1] = b;
+    if (b == 0.0) {
+      status = IMPED_ERR_BREAKDOWN;
+      kdone = j;
+      break;
+    }
+    double2* vj = V + (size_t)j * s;
+    const double2* vp = vj - s;
+    for (int64_t i = tid; i < n; i += T) vj[i] = make_double2(w[i].x / b, w[i].y / b);
+    __syncthreads();
+    al = spmv(vj, w);
+    if (tid == 0) alpha[j] = al;
+    for (int64_t i = tid; i < n; i += T) {
+      const double2 x = vj[i], y = vp[i];
+      w[i] = make_double2(w[i].x - al * x.x - b * y.x, w[i].y - al * x.y - b * y.y);
+    }
+  }
+  if (tid == 0) {
+    a.kout[c] = kdone;
+    a.status[c] = status;
+  }
+}
+
 // ------------------------------------------------------------------ eigen
 
 __device__ __forceinline__ uint64_t splitmix(uint64_t x) {
@@ -2133,13 +2363,41 @@ namespace {
 // LDS of one chain of lanczos_lds_kernel: two chains per CU (160 KiB).
 constexpr size_t LDS_CHAIN_BYTES = 80 * 1024;
 
+int lds_waves() {
+  const int64_t w = g_tuning.lanczos_lds_waves;
+  return w == 2 || w == 4 ? (int)w : 1;
+}
+
+// LDS of one chain of the LDS-resident Lanczos for the current wave count
+size_t lds_chain_bytes(int64_t n, int K) {
+  const int nw = lds_waves();
+  return nw == 1 ? lanczos_lds_bytes(n, K) : lanczos_ldsw_bytes(n, K, nw);
+}
+
+template <class F>
+void set_lds_limit(F* kernel) {
+  HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)LDS_CHAIN_BYTES));
+}
+
+// nb chains (a.chain_list) on the LDS-resident Lanczos, `bytes` of LDS each
 void launch_lanczos_lds(const LanczosArgs& a, unsigned nb, size_t bytes, hipStream_t s) {
   static std::once_flag once;
   std::call_once(once, [] {
-    HIP_CHECK(hipFuncSetAttribute((const void*)lanczos_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)LDS_CHAIN_BYTES));
+    set_lds_limit(lanczos_lds_kernel);
+    set_lds_limit(lanczos_ldsw_kernel<2>);
+    set_lds_limit(lanczos_ldsw_kernel<4>);
   });
-  hipLaunchKernelGGL(lanczos_lds_kernel, dim3(nb), dim3(64), bytes, s, a);
+  switch (lds_waves()) {
+    case 4:
+      hipLaunchKernelGGL(lanczos_ldsw_kernel<4>, dim3(nb), dim3(256), bytes, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL(lanczos_ldsw_kernel<2>, dim3(nb), dim3(128), bytes, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL(lanczos_lds_kernel, dim3(nb), dim3(64), bytes, s, a);
+  }
 }
 
 }  // namespace
@@ -2184,9 +2442,9 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
       flat.push_back(c);
       continue;
     }
-    if (reorth && g_tuning.lanczos_lds && lanczos_lds_bytes(n, K) <= LDS_CHAIN_BYTES) {
+    if (reorth && g_tuning.lanczos_lds && lds_chain_bytes(n, K) <= LDS_CHAIN_BYTES) {
       lists[3].push_back(c);
-      lds_max = std::max(lds_max, lanczos_lds_bytes(n, K));
+      lds_max = std::max(lds_max, lds_chain_bytes(n, K));
       continue;
     }
     const int bkt = n <= g_tuning.lanczos_wave_max_n ? 0 : (n <= g_tuning.lanczos_wg256_max_n ? 1 : 2);
@@ -2214,8 +2472,8 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
       launch_lanczos_lds(a, nb, lds_max, s);
     HIP_CHECK(hipGetLastError());
   }
-  // the chain list buffer must outlive the launches
-  stream_sync(s);
+  // no wait: the chain list stays in the scratch buffer, and later copies
+  // into it are queued behind these launches on the same stream
 }
 
 int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_t seed, double* evals,
@@ -2475,7 +2733,8 @@ int run_solve(const KrylovBatch& kb, const double2* y, int S, const double* z_ho
 
 void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t* kout, const double* energy,
             const double* norm2, const int32_t* sign, const int32_t* zflag, const double* kT, const int32_t* spec_id,
-            int n_spec, const double* omega, int n_w, double delta, double* g_out, double* spec_out, hipStream_t s) {
+            int n_spec, const double* omega, int n_w, double delta, double* g_out, double* spec_out, hipStream_t s,
+            const double* norm2_dev) {
   KScratch& sc = kscratch();
   // chains grouped by spectrum, in chain order
   std::vector<int32_t> off(n_spec + 1, 0), chains;
@@ -2488,13 +2747,15 @@ void run_cf(int C, int K, const double* alpha, const double* beta, const int32_t
     if (spec_id[c] >= 0 && spec_id[c] < n_spec) chains[fill[spec_id[c]]++] = c;
   // every per-call input in one copy
   HostPack pk;
-  const size_t o_e = pk.add(energy, C), o_n = pk.add(norm2, C), o_s = pk.add(sign, C), o_z = pk.add(zflag, C),
+  const size_t o_e = pk.add(energy, C), o_n = pk.add(norm2, norm2 ? C : 0), o_s = pk.add(sign, C),
+               o_z = pk.add(zflag, C),
                o_off = pk.add(off.data(), n_spec + 1), o_ch = pk.add(chains.data(), chains.size()),
                o_om = pk.add(omega, n_w), o_kt = pk.add(kT, std::max(n_spec, 1));
   pk.upload(sc.cf_in, s);
   if (g_out) sc.gout.reserve((size_t)std::max(C, 1) * n_w * 16);
   sc.sout.reserve((size_t)std::max(n_spec, 1) * n_w * 8);
-  CFArgs a{alpha, beta, kout, K, HostPack::at<double>(sc.cf_in, o_e), HostPack::at<double>(sc.cf_in, o_n),
+  CFArgs a{alpha, beta, kout, K, HostPack::at<double>(sc.cf_in, o_e),
+           norm2 ? HostPack::at<double>(sc.cf_in, o_n) : norm2_dev,
            HostPack::at<int32_t>(sc.cf_in, o_s), HostPack::at<int32_t>(sc.cf_in, o_z),
            HostPack::at<double>(sc.cf_in, o_kt), HostPack::at<int32_t>(sc.cf_in, o_off),
            HostPack::at<int32_t>(sc.cf_in, o_ch), HostPack::at<double>(sc.cf_in, o_om), n_w, delta,
@@ -2624,8 +2885,8 @@ extern "C" int imped_lanczos_noalloc(const int64_t* rowptr, const int32_t* col, 
     const size_t shm = reorth ? (size_t)(K + 2) * sizeof(double2) : 0;
     // the size buckets of run_lanczos, so a chain gets the same kernel (and
     // the same arithmetic) as through imped_batch_lanczos
-    if (reorth && g_tuning.lanczos_lds && lanczos_lds_bytes(n, K) <= LDS_CHAIN_BYTES)
-      launch_lanczos_lds(a, 1, lanczos_lds_bytes(n, K), s);
+    if (reorth && g_tuning.lanczos_lds && lds_chain_bytes(n, K) <= LDS_CHAIN_BYTES)
+      launch_lanczos_lds(a, 1, lds_chain_bytes(n, K), s);
     else if (n <= g_tuning.lanczos_wave_max_n)
       hipLaunchKernelGGL(lanczos_kernel<64>, dim3(1), dim3(64), shm, s, a);
     else if (n <= g_tuning.lanczos_wg256_max_n)

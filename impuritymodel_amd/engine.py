@@ -345,6 +345,40 @@ class Affine:
         _release("imped_affine_destroy", self)
 
 
+def family_xas(ground, sector, trans, sector_index, theta, kT, omega, delta, nev=3, n_psi_max=3, energy_cut=10.0,
+               krylov_size=200, reorth=1, eig_tol=1e-12, eig_max_steps=600, seed=12345, h_weight_min=1e-12,
+               psi_weight_min=1e-7, weight_min=1e-7):
+    """``imped_family_xas``: thermal, polarization-summed XAS of parameter
+    sets from an affine family's sector structures (:class:`Affine` handles
+    ``ground``, ``sector``, ``trans``; ``sector_index`` [n_comp, n_pol]) in
+    one call.  Returns a dict with ``spectra`` [M, n_w], ``energies`` (list
+    of kept eigenvalues per model), ``eig_steps`` [M * n_comp],
+    ``chain_struct`` and ``kout`` of the excited chains, and ``K``."""
+    sector_index = np.ascontiguousarray(sector_index, np.int32)
+    n_comp, n_pol = sector_index.shape
+    th = np.ascontiguousarray(np.asarray(theta, np.float64).reshape(len(theta), -1)[:, : ground.P])
+    M = len(th)
+    kT = np.ascontiguousarray(np.broadcast_to(np.asarray(kT, np.float64), (M,)))
+    omega = np.ascontiguousarray(omega, np.float64)
+    spectra = np.zeros((max(M, 1), len(omega)), np.float64)
+    energies = np.zeros((max(M, 1), n_psi_max), np.float64)
+    n_kept = np.zeros(max(M, 1), np.int32)
+    eig_steps = np.zeros(max(M * n_comp, 1), np.int32)
+    cap = max(M * n_pol * n_psi_max, 1)
+    chain_struct = np.zeros(cap, np.int32)
+    kout = np.zeros(cap, np.int32)
+    info = np.zeros(4, np.int32)
+    check(lib.imped_family_xas(ground.h, sector.h, trans.h, n_comp, n_pol, ptr(sector_index), M,
+                               ptr(_nz(th.reshape(-1))), ptr(_nz(kT)), int(nev), int(n_psi_max), float(energy_cut),
+                               int(krylov_size), int(reorth), float(eig_tol), int(eig_max_steps), int(seed),
+                               float(h_weight_min), float(psi_weight_min), float(weight_min), ptr(_nz(omega)),
+                               len(omega), float(delta), ptr(spectra), ptr(energies), ptr(n_kept), ptr(eig_steps),
+                               ptr(chain_struct), ptr(kout), ptr(info)))
+    X = int(info[0])
+    return dict(spectra=spectra[:M], energies=[list(energies[m, : n_kept[m]]) for m in range(M)],
+                eig_steps=eig_steps[: M * n_comp], chain_struct=chain_struct[:X], kout=kout[:X], K=int(info[1]))
+
+
 BREAKDOWN = IMPED_ERR_BREAKDOWN
 
 

@@ -40,7 +40,7 @@ import scipy.sparse
 import scipy.sparse.csgraph
 
 from .batched import POLARIZATIONS, XASResult, _dipole_tables, ground_bases
-from .engine import Affine, Batch, Ops, vec_apply, vec_from_host
+from .engine import Affine, Batch, Ops, family_xas, vec_apply, vec_from_host
 from .models import K_B
 from .operators import n_spin_orbitals
 
@@ -199,7 +199,29 @@ class FamilySectors:
     def xas(self, thetas, omega, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0, energy_cut=10.0, reorth=1,
             eig_tol=1e-12, eig_max_steps=600, seed=12345):
         """Thermal, polarization-summed XAS of parameter sets [M, >= P]
-        (the spectra of ``batched.xas_batch_ops`` on the family's operators)."""
+        (the spectra of ``batched.xas_batch_ops`` on the family's operators),
+        all steps in one library call (``imped_family_xas``)."""
+        t0 = time.perf_counter()
+        th = np.ascontiguousarray(np.asarray(thetas, np.float64).reshape(len(thetas), -1)[:, : self.P])
+        M = len(th)
+        omega = np.asarray(omega, np.float64)
+        T_m = np.broadcast_to(np.asarray(T, np.float64), (M,))
+        r = family_xas(self.ground, self.sector, self.trans, self.sector_index, th, K_B * T_m, omega, delta,
+                       nev=nPsiMax, n_psi_max=nPsiMax, energy_cut=energy_cut, krylov_size=krylovSize, reorth=reorth,
+                       eig_tol=eig_tol, eig_max_steps=eig_max_steps, seed=seed)
+        tim = {"total": time.perf_counter() - t0, "expand_build": 0.0}
+        cs = r["chain_struct"]
+        gdims = np.tile(self.comp_n, M)
+        gnnz = np.tile(self.ground.nnz, M)
+        return XASResult(omega, r["spectra"], r["energies"], tim, self.sector.n[cs], self.sector.nnz[cs],
+                         r["eig_steps"], r["kout"], gdims, gnnz)
+
+    def xas_steps(self, thetas, omega, delta=0.2, krylovSize=200, nPsiMax=3, T=300.0, energy_cut=10.0, reorth=1,
+                  eig_tol=1e-12, eig_max_steps=600, seed=12345):
+        """:meth:`xas` as a sequence of the batch entry points
+        (``imped_batch_set_affine``, ``imped_batch_eigen``,
+        ``imped_batch_seed_affine``, ``imped_batch_lanczos``,
+        ``imped_batch_cf``) with the state selection in Python."""
         t0 = time.perf_counter()
         tim = {}
         th = np.ascontiguousarray(np.asarray(thetas, np.float64).reshape(len(thetas), -1)[:, : self.P])

@@ -74,8 +74,9 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * GPU's span but costs the calling thread more than the launches it
  * replaces; 0, the default: direct launches, as with switched-on timing),
  * "lanczos_lds" (1, the default: a fully reorthogonalized chain whose Krylov
- * basis fits 80 KiB runs on one wave with the basis in LDS; 0: the basis in
- * memory). */
+ * basis fits 80 KiB runs on one workgroup with the basis in LDS; 0: the
+ * basis in memory), "lanczos_lds_waves" (waves of that workgroup: 1, 2 or
+ * 4, the default; the Gram-Schmidt sums are split over the waves). */
 int imped_set_tuning(const char* name, int64_t value);
 
 /* The engine's device radix sort (the one that orders BFS layers and
@@ -218,10 +219,47 @@ int imped_batch_set_affine(imped_batch batch, imped_affine aff, const int32_t* c
  * components |psi_i|^2 < psi_weight_min dropped (the eigenvector dicts of
  * eigensystem, finite.py:114-117); entries |v|^2 < weight_min are pruned and
  * v is normalized, norm2[c] = |v|^2 before normalization (applyOp followed
- * by getSpectra's normalization, finite.py:1301-1511, spectra.py:786-790). */
+ * by getSpectra's normalization, finite.py:1301-1511, spectra.py:786-790).
+ * norm2 may be NULL: the norms then stay on the device (for imped_batch_cf
+ * with norm2 NULL) and the call returns without waiting for the GPU. */
 int imped_batch_seed_affine(imped_batch batch, imped_batch ground, imped_affine trans, const int32_t* src_chain,
                             const int32_t* src_vec, const int32_t* chain_struct, double psi_weight_min,
                             double weight_min, double* norm2);
+
+/* The thermal, polarization-summed XAS of n_models parameter sets (theta
+ * [n_models][n_params], kT [n_models] > 0) of an affine family from its
+ * sector structures, in one call: the batched composition of eigensystem
+ * (finite.py:58-118), the energy cut (get_spectra.py:302-303), applyOp of the
+ * dipole operators, expand_basis_and_hamiltonian, Lanczos and getSpectra
+ * with thermal_average (spectra.py:222-238, 743-806; average.py:11-45).
+ *  - ground chain (m, k), k < n_comp: structure k of `ground`, theta row m,
+ *    values pruned at h_weight_min; lowest nev eigenpairs (imped_batch_eigen
+ *    with eig_tol, eig_max_steps, seed);
+ *  - per model, the lowest n_psi_max of its components' eigenvalues in
+ *    ascending order (ties in component order), kept while within
+ *    energy_cut * kT[m] of the lowest;
+ *  - per kept state (component k) and polarization t < n_pol, in (model,
+ *    polarization, state) order, an excited chain: structure
+ *    sector_index[k][t] of `sector` (theta row m), started from T psi with
+ *    structure sector_index[k][t] of `trans` (imped_batch_seed_affine with
+ *    psi_weight_min, weight_min);
+ *  - Lanczos with K = min(krylov_size, largest excited chain) steps (reorth
+ *    as imped_batch_lanczos), continued fractions on omega[n_w] with
+ *    broadening delta, Boltzmann weights over each model's kept states.
+ * Outputs: spectra [n_models][n_w], energies [n_models][n_psi_max] (the kept
+ * ones, then NaN), n_kept [n_models]; may be NULL: eig_steps [n_models *
+ * n_comp], chain_struct and kout [n_models * n_pol * n_psi_max],
+ * info[4] = {excited chains, K, 0, 0}.  Two waits for the GPU per call (the
+ * eigenvalues, then the spectra).  Returns IMPED_ERR_NOCONV or
+ * IMPED_ERR_BREAKDOWN as imped_batch_eigen and imped_batch_lanczos report.
+ * No reference counterpart as one call. */
+int imped_family_xas(imped_affine ground, imped_affine sector, imped_affine trans, int n_comp, int n_pol,
+                     const int32_t* sector_index, int n_models, const double* theta, const double* kT, int nev,
+                     int n_psi_max, double energy_cut, int krylov_size, int reorth, double eig_tol,
+                     int eig_max_steps, uint64_t seed, double h_weight_min, double psi_weight_min,
+                     double weight_min, const double* omega, int n_w, double delta, double* spectra,
+                     double* energies, int32_t* n_kept, int32_t* eig_steps, int32_t* chain_struct, int32_t* kout,
+                     int32_t* info);
 
 /* get_tridiagonal_krylov_vectors (finite.py:1897-1995) for every chain from
  * v0 (rows complex, or NULL = the start vectors of imped_batch_seed_affine
@@ -229,7 +267,9 @@ int imped_batch_seed_affine(imped_batch batch, imped_batch ground, imped_affine 
  * chain).  reorth = 0 is the reference's plain recurrence; reorth = 1 adds
  * full reorthogonalization (no reference counterpart).  alpha, beta are
  * [C][K]; kout[c] = min(K, n_c), or the step of a beta == 0 breakdown with
- * status[c] = IMPED_ERR_BREAKDOWN. */
+ * status[c] = IMPED_ERR_BREAKDOWN.  With alpha, beta, kout and status all
+ * NULL the results stay on the device (for imped_batch_cf) and the call
+ * returns without waiting for the GPU. */
 int imped_batch_lanczos(imped_batch batch, const double* v0, int K, int reorth, double* alpha, double* beta,
                         int32_t* kout, int32_t* status);
 
@@ -280,7 +320,8 @@ int imped_batch_solve(imped_batch batch, int n_shift, const double* shifts, doub
  * g_out[C][n_w] (complex, may be NULL); and spectra
  *   spec[q][w] = sum_{c in q} p_c * (-Im g_c(w)),
  *   p_c = exp(-(e_c - min e)/kT[q]) / Z, Z summed over chains with zflag = 1
- * (thermal_average, average.py:11-45, summed over operators). */
+ * (thermal_average, average.py:11-45, summed over operators).  norm2 NULL:
+ * the norms imped_batch_seed_affine left on the device. */
 int imped_batch_cf(imped_batch batch, const double* energy, const double* norm2, const int32_t* sign,
                    const int32_t* zflag, const double* kT, const int32_t* spec_id, int n_spec, const double* omega, int n_w,
                    double delta, double* g_out, double* spec_out);

@@ -164,11 +164,13 @@ def test_plain_recurrence_xas_vs_oracle(gpu):
     assert err <= 1e-4
 
 
-def test_lds_lanczos_kernel_matches_memory_kernel(gpu):
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_lds_lanczos_kernel_matches_memory_kernel(gpu, waves):
     """Fully reorthogonalized chains whose Krylov basis fits LDS run on the
-    LDS kernel; with the lanczos_lds knob off the same chains run with the
-    basis in memory.  Same recurrence, Gram-Schmidt sums in another order:
-    alpha/beta agree to 1e-10 relative."""
+    LDS kernel (one workgroup of 1, 2 or 4 waves per chain); with the
+    lanczos_lds knob off the same chains run with the basis in memory.  Same
+    recurrence, Gram-Schmidt sums in another order: alpha/beta agree to 1e-10
+    relative."""
     import scipy.sparse
 
     from impuritymodel_amd import engine
@@ -192,12 +194,14 @@ def test_lds_lanczos_kernel_matches_memory_kernel(gpu):
     b.set_csr(sizes, np.array(rowptr, np.int64), np.concatenate(cols).astype(np.int32), np.concatenate(vals))
     v0 = np.concatenate([rng.standard_normal(n) + 1j * rng.standard_normal(n) for n in sizes])
     K = max(sizes)
-    a1, b1, k1, s1 = b.lanczos(K, reorth=1, v0=v0)
     try:
+        engine.set_tuning("lanczos_lds_waves", waves)
+        a1, b1, k1, s1 = b.lanczos(K, reorth=1, v0=v0)
         engine.set_tuning("lanczos_lds", 0)
         a2, b2, k2, s2 = b.lanczos(K, reorth=1, v0=v0)
     finally:
         engine.set_tuning("lanczos_lds", 1)
+        engine.set_tuning("lanczos_lds_waves", 4)
     assert np.array_equal(k1, k2) and np.array_equal(s1, s2)
     for c, n in enumerate(sizes):
         kk = int(k1[c])

@@ -77,6 +77,27 @@ def test_cache_path_equals_general_path(family, T):
     assert err <= 1e-10
 
 
+def test_one_call_equals_entry_point_sequence(family):
+    """imped_family_xas (one call, state selection in C++) against the same
+    composition from the batch entry points with the selection in Python:
+    same kernels on the same inputs, so the same spectra."""
+    from impuritymodel_amd import surrogate as S
+
+    cache = family.sectors(1)
+    for T in (300.0, 10.0):
+        th = _thetas(12, 9, T)
+        one = cache.xas(th, S.OMEGA, delta=S.DELTA, T=th[:, 7])
+        seq = cache.xas_steps(th, S.OMEGA, delta=S.DELTA, T=th[:, 7])
+        for m in range(len(th)):
+            assert one.energies[m] == seq.energies[m]
+        assert np.array_equal(one.chain_dims, seq.chain_dims)
+        assert np.array_equal(one.kout, seq.kout)
+        assert np.array_equal(one.eig_steps, seq.eig_steps)
+        err = np.abs(one.spectra - seq.spectra).max() / np.abs(seq.spectra).max()
+        print("one call vs entry-point sequence (T=%g): %.3g of the peak" % (T, err))
+        assert err <= 1e-14
+
+
 def test_cache_path_vs_oracle(family):
     """exact_spectra (cache path) against the oracle at 1e-8 of the peak."""
     from oracle import basis as obasis

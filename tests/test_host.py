@@ -133,3 +133,5 @@ def test_c_abi_rejects_null_arguments():
     assert lib.imped_affine_create(0, None, None, None, None, 0, None, None) == IMPED_ERR_INVALID
     assert lib.imped_batch_set_affine(None, None, None, None, 0, None, 0.0) == IMPED_ERR_INVALID
     assert lib.imped_batch_seed_affine(None, None, None, None, None, None, 0.0, 0.0, None) == IMPED_ERR_INVALID
+    assert lib.imped_family_xas(None, None, None, 1, 1, None, 0, None, None, 1, 1, 1.0, 1, 1, 1e-12, 10, 0, 0.0, 0.0,
+                                0.0, None, 0, 0.1, None, None, None, None, None, None, None) == IMPED_ERR_INVALID
