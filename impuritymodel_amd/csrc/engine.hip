@@ -1021,6 +1021,12 @@ int imped_set_tuning(const char* name, int64_t value) {
       return IMPED_ERR_INVALID;
     }
     g_tuning.lanczos_graph_steps = value;
+  } else if (n == "eigen_check_start") {
+    if (value < 0) {
+      set_error("imped_set_tuning: eigen_check_start must be >= 0");
+      return IMPED_ERR_INVALID;
+    }
+    g_tuning.eigen_check_start = value;
   } else if (n == "eigen_check_every") {
     if (value < 1) {
       set_error("imped_set_tuning: eigen_check_every must be >= 1");

@@ -42,6 +42,24 @@ __device__ __forceinline__ double block_sum(double x, double* lds) {
   return r;
 }
 
+// block_sum with one barrier: lds holds two rows of BT/64 partials used in
+// turn (parity flips on every call), so a call's writes never meet the
+// previous call's reads; every thread of the workgroup makes the same
+// sequence of calls.
+template <int BT>
+__device__ __forceinline__ double block_sum1(double x, double* lds, int& parity) {
+  x = wave_sum(x);
+  if (BT == 64) return x;
+  double* row = lds + parity * (BT / 64);
+  parity ^= 1;
+  if ((threadIdx.x & 63) == 0) row[threadIdx.x >> 6] = x;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int k = 0; k < BT / 64; ++k) r += row[k];
+  return r;
+}
+
 // y = H x for the rows of one chain; returns the partial Re<x, y>.
 template <int BT>
 __device__ __forceinline__ double spmv_rows(const int64_t* rowptr, const int32_t* col, const double2* val,
@@ -780,6 +798,7 @@ struct EigenArgs {
   const int32_t* chain_list;
   int dgks;          // 1: second Gram-Schmidt pass only when needed
   int check_every;   // Ritz check interval (steps)
+  int check_start;   // no check before this many steps (except at the last step)
   int lstride;       // LDS arrays: Gram-Schmidt coefficients (lstride + 2), then, when
   int lds_ab;        //   lds_ab = lds_work = 1, alpha/beta (2 lstride) and per eigenvalue y
   int lds_work;      //   (lstride), LU work (4 lstride) and pivots (lstride ints); lstride >= every
@@ -1117,6 +1136,104 @@ __device__ __forceinline__ double three_term(const EigenArgs& a, int64_t base, i
   return block_sum<BT>(p0, red);
 }
 
+// The rows of H kept in registers: thread t < n holds up to HR entries of
+// row t (the rest of the row, and rows from BT on, are read from memory).
+// HR shrinks with the workgroup size, whose waves have fewer registers each.
+template <int BT>
+constexpr int hrow_len() {
+  return BT <= 256 ? 16 : (BT <= 512 ? 8 : 1);
+}
+template <int HR>
+struct HRow {
+  double2 v[HR];
+  int c[HR];
+  int cnt = 0;
+  int64_t rest = 0, end = 0;
+};
+
+template <int HR>
+__device__ __forceinline__ void load_hrow(HRow<HR>& h, const int64_t* rowptr, const int32_t* col, const double2* val,
+                                          int64_t base, int64_t n) {
+  if ((int64_t)threadIdx.x < n) {
+    const int64_t e0 = rowptr[base + threadIdx.x];
+    h.end = rowptr[base + threadIdx.x + 1];
+    h.cnt = (int)(h.end - e0 < HR ? h.end - e0 : HR);
+    h.rest = e0 + h.cnt;
+#pragma unroll
+    for (int k = 0; k < HR; ++k) {
+      h.v[k] = k < h.cnt ? val[e0 + k] : make_double2(0.0, 0.0);
+      h.c[k] = k < h.cnt ? col[e0 + k] : 0;
+    }
+  }
+}
+
+// spmv_t with row threadIdx.x taken from registers
+template <int BT, int HR, class T>
+__device__ __forceinline__ double spmv_cached(const HRow<HR>& h, const int64_t* rowptr, const int32_t* col,
+                                              const double2* val, int64_t base, int64_t n, const T* x, T* y) {
+  double pa = 0.0;
+  if ((int64_t)threadIdx.x < n) {
+    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+#pragma unroll
+    for (int k = 0; k < HR; k += 2) {
+      if (k < h.cnt) {
+        const T v = x[h.c[k]];
+        ax += h.v[k].x * re_(v) - h.v[k].y * im_(v);
+        if constexpr (!is_real_v<T>) ay += h.v[k].x * im_(v) + h.v[k].y * re_(v);
+      }
+      if (k + 1 < HR && k + 1 < h.cnt) {
+        const T v = x[h.c[k + 1]];
+        bx += h.v[k + 1].x * re_(v) - h.v[k + 1].y * im_(v);
+        if constexpr (!is_real_v<T>) by += h.v[k + 1].x * im_(v) + h.v[k + 1].y * re_(v);
+      }
+    }
+    for (int64_t e = h.rest; e < h.end; ++e) {
+      const double2 hh = val[e];
+      const T v = x[col[e]];
+      ax += hh.x * re_(v) - hh.y * im_(v);
+      if constexpr (!is_real_v<T>) ay += hh.x * im_(v) + hh.y * re_(v);
+    }
+    ax += bx;
+    ay += by;
+    const int64_t i = threadIdx.x;
+    y[i] = mk<T>(ax, ay);
+    pa += ax * re_(x[i]) + ay * im_(x[i]);
+  }
+  for (int64_t i = threadIdx.x + BT; i < n; i += BT) {
+    const int64_t e0 = rowptr[base + i], e1 = rowptr[base + i + 1];
+    double ax = 0.0, ay = 0.0;
+    for (int64_t e = e0; e < e1; ++e) {
+      const double2 hh = val[e];
+      const T v = x[col[e]];
+      ax += hh.x * re_(v) - hh.y * im_(v);
+      if constexpr (!is_real_v<T>) ay += hh.x * im_(v) + hh.y * re_(v);
+    }
+    y[i] = mk<T>(ax, ay);
+    pa += ax * re_(x[i]) + ay * im_(x[i]);
+  }
+  return pa;
+}
+
+template <int BT, int HR, class T>
+__device__ __forceinline__ double three_term_cached(const HRow<HR>& h, const EigenArgs& a, int64_t base, int64_t n,
+                                                    const T* vj, const T* vp, bool has_prev, double bprev, T* w,
+                                                    double* red2, int& parity, double* al_out) {
+  const double al = block_sum1<BT>(spmv_cached<BT, HR, T>(h, a.rowptr, a.col, a.val, base, n, vj, w), red2, parity);
+  double p0 = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
+    double xr = re_(w[i]) - al * re_(vj[i]);
+    double xi = im_(w[i]) - al * im_(vj[i]);
+    if (has_prev) {
+      xr -= bprev * re_(vp[i]);
+      xi -= bprev * im_(vp[i]);
+    }
+    w[i] = mk<T>(xr, xi);
+    p0 += xr * xr + xi * xi;
+  }
+  *al_out = al;
+  return block_sum1<BT>(p0, red2, parity);
+}
+
 // cl[l] = <V_l, w> for l in [l0, l0 + U) ∩ [l0, lend), one wave (l0 and
 // lend wave-uniform).  Every lane loads all U vectors' entries (indices
 // past lend clamped to lend - 1, their sums discarded), so the loads issue
@@ -1215,13 +1332,13 @@ __device__ __forceinline__ double update_split(const T* VL, int kl, const T* VG,
 
 // x = sum_l y[l] V_l over l < m of the split basis, for one row i.
 template <class T>
-__device__ __forceinline__ void combine_split(const T* VL, int kl, const T* VG, int m, int64_t n, int64_t i,
-                                              const double* y, double* xr, double* xi) {
+__device__ __forceinline__ void combine_split(const T* VL, int64_t sL, int kl, const T* VG, int m, int64_t n,
+                                              int64_t i, const double* y, double* xr, double* xi) {
   const int ma = m < kl ? m : kl;
   double r = 0.0, q = 0.0;
 #pragma unroll 8
   for (int l = 0; l < ma; ++l) {
-    const T v = VL[(size_t)l * n + i];
+    const T v = VL[(size_t)l * sL + i];
     r += y[l] * re_(v);
     q += y[l] * im_(v);
   }
@@ -1235,6 +1352,105 @@ __device__ __forceinline__ void combine_split(const T* VL, int kl, const T* VG, 
   *xi = q;
 }
 
+// Gram-Schmidt over the split basis with the work shared by the waves
+// through partial sums in LDS (part: BT/64 rows of n), no per-vector wave
+// reductions.  Dots: wave g sums rows [g n/NW, (g+1) n/NW) of <V_l, w>
+// for every l (lane l), the NW partials of each l are then added.  Update:
+// wave g sums sum_l c_l V_l[i] over vectors [g m/NW, (g+1) m/NW) for every
+// row i (lane), the NW partials of each row are then subtracted from w.
+// The LDS prefix (l < kl) has row stride sL, the rest of the basis n.
+template <class T>
+__device__ __forceinline__ const T* basis_row(const T* VL, int64_t sL, int kl, const T* VG, int64_t n, int l) {
+  return l < kl ? VL + (size_t)l * sL : VG + (size_t)l * n;
+}
+
+template <int BT, class T>
+__device__ __forceinline__ void dots_part(const T* VL, int64_t sL, int kl, const T* VG, int m, int64_t n, const T* w,
+                                          double2* part, double2* cl) {
+  constexpr int NW = BT / 64;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t r0 = n * wv / NW, r1 = n * (wv + 1) / NW;
+  for (int l = lane; l < m; l += 64) {
+    const T* vl = basis_row(VL, sL, kl, VG, n, l);
+    double cx = 0.0, cy = 0.0, dx = 0.0, dy = 0.0;
+    int64_t i = r0;
+#pragma unroll 4
+    for (; i + 1 < r1; i += 2) {
+      const T v = vl[i], b = w[i], v1 = vl[i + 1], b1 = w[i + 1];
+      cx += re_(v) * re_(b) + im_(v) * im_(b);
+      dx += re_(v1) * re_(b1) + im_(v1) * im_(b1);
+      if constexpr (!is_real_v<T>) {
+        cy += re_(v) * im_(b) - im_(v) * re_(b);
+        dy += re_(v1) * im_(b1) - im_(v1) * re_(b1);
+      }
+    }
+    if (i < r1) {
+      const T v = vl[i], b = w[i];
+      cx += re_(v) * re_(b) + im_(v) * im_(b);
+      if constexpr (!is_real_v<T>) cy += re_(v) * im_(b) - im_(v) * re_(b);
+    }
+    part[(size_t)wv * n + l] = make_double2(cx + dx, cy + dy);
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < m; l += BT) {
+    double2 t = part[l];
+#pragma unroll
+    for (int g = 1; g < NW; ++g) {
+      const double2 u = part[(size_t)g * n + l];
+      t.x += u.x;
+      t.y += u.y;
+    }
+    cl[l] = t;
+  }
+  __syncthreads();
+}
+
+template <int BT, class T>
+__device__ __forceinline__ double update_part(const T* VL, int64_t sL, int kl, const T* VG, int m, int64_t n, T* w,
+                                              const double2* cl, double2* part, double* red2, int& parity) {
+  constexpr int NW = BT / 64;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l0 = m * wv / NW, l1 = m * (wv + 1) / NW;
+  for (int64_t i = lane; i < n; i += 64) {
+    double ux = 0.0, uy = 0.0, tx = 0.0, ty = 0.0;
+    int l = l0;
+#pragma unroll 4
+    for (; l + 1 < l1; l += 2) {
+      const double2 c0 = cl[l], c1 = cl[l + 1];
+      const T v0 = basis_row(VL, sL, kl, VG, n, l)[i], v1 = basis_row(VL, sL, kl, VG, n, l + 1)[i];
+      ux += re_(v0) * c0.x - im_(v0) * c0.y;
+      tx += re_(v1) * c1.x - im_(v1) * c1.y;
+      if constexpr (!is_real_v<T>) {
+        uy += re_(v0) * c0.y + im_(v0) * c0.x;
+        ty += re_(v1) * c1.y + im_(v1) * c1.x;
+      }
+    }
+    if (l < l1) {
+      const double2 c0 = cl[l];
+      const T v0 = basis_row(VL, sL, kl, VG, n, l)[i];
+      ux += re_(v0) * c0.x - im_(v0) * c0.y;
+      if constexpr (!is_real_v<T>) uy += re_(v0) * c0.y + im_(v0) * c0.x;
+    }
+    part[(size_t)wv * n + i] = make_double2(ux + tx, uy + ty);
+  }
+  __syncthreads();
+  double p = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += BT) {
+    double ux = part[i].x, uy = part[i].y;
+#pragma unroll
+    for (int g = 1; g < NW; ++g) {
+      ux += part[(size_t)g * n + i].x;
+      uy += part[(size_t)g * n + i].y;
+    }
+    const double wx = re_(w[i]) - ux, wy = im_(w[i]) - uy;
+    w[i] = mk<T>(wx, wy);
+    p += wx * wx + wy * wy;
+  }
+  return block_sum1<BT>(p, red2, parity);
+}
+
 // LW: the tridiagonal and the Ritz work arrays live in LDS (a compile-time
 // choice, so every access to them is an LDS instruction), else in memory.
 // LV (with LW): the residual w and the first a.kl Krylov vectors live in LDS
@@ -1242,6 +1458,8 @@ __device__ __forceinline__ void combine_split(const T* VL, int kl, const T* VG, 
 template <int BT, class T, bool LW, bool LV>
 __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   __shared__ double red[BT / 64];
+  __shared__ double red2[2 * (BT / 64)];  // block_sum1's alternating rows
+  int parity = 0;
   __shared__ int s_done;
   __shared__ double s_lam[MAX_NEV], s_lo[MAX_NEV], s_hi[MAX_NEV];
   __shared__ int s_first[MAX_NEV];
@@ -1294,16 +1512,30 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   T* VL;
   T* w;
   int kl;
+  int64_t sL = n;           // row stride of the LDS prefix
+  double2* part = nullptr;  // with LV: Gram-Schmidt partial sums, BT/64 rows of n
   if constexpr (LV) {
     const size_t off = ((size_t)((char*)(pivb + (size_t)nev * L) - (char*)cl) + 15) & ~(size_t)15;
     w = reinterpret_cast<T*>(reinterpret_cast<char*>(cl) + off);
     VL = w + n;
     kl = a.kl;
+    sL = lds_row_stride(n);
+    part = reinterpret_cast<double2*>((reinterpret_cast<uintptr_t>(VL + (size_t)kl * sL) + 15) & ~(uintptr_t)15);
   } else {
     w = reinterpret_cast<T*>(a.wbuf) + base;
     VL = VG;
     kl = 0;
   }
+  // with the LDS layout (small chains), each thread's row of H in registers
+  HRow<hrow_len<BT>()> hrow;
+  if constexpr (LV) load_hrow(hrow, a.rowptr, a.col, a.val, base, n);
+  auto step3 = [&](const T* vj, const T* vp, bool has_prev, double bprev, double* al) {
+    if constexpr (LV)
+      return three_term_cached<BT, hrow_len<BT>(), T>(hrow, a, base, n, vj, vp, has_prev, bprev, w, red2, parity,
+                                                      al);
+    else
+      return three_term<BT, T>(a, base, n, vj, vp, has_prev, bprev, w, red, al);
+  };
   // seeded random start vector (real when T is real)
   {
     T* V0 = kl > 0 ? VL : VG;
@@ -1320,6 +1552,8 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   __syncthreads();
   int m = 0;
   int converged = 0;
+  // no Ritz check before min(check_start, n/2) steps (the last step is always checked)
+  const int cstart = (int)(a.check_start < n / 2 ? a.check_start : n / 2);
   for (int j = 0; j < mmax; ++j) {
     // three-term step: w = H v_j - alpha_j v_j - beta_{j-1} v_{j-1}, with the
     // big cancellation done before any Gram-Schmidt coefficient is formed
@@ -1328,21 +1562,29 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
     const double bprev = j > 0 ? sb[j - 1] : 0.0;
     double al, p0;
     if (j + 1 <= kl)
-      p0 = three_term<BT, T>(a, base, n, VL + (size_t)j * n, VL + (size_t)(j > 0 ? j - 1 : 0) * n, j > 0, bprev, w,
-                             red, &al);
+      p0 = step3(VL + (size_t)j * sL, VL + (size_t)(j > 0 ? j - 1 : 0) * sL, j > 0, bprev, &al);
     else if (j == kl && j > 0)
-      p0 = three_term<BT, T>(a, base, n, VG + (size_t)j * n, VL + (size_t)(j - 1) * n, true, bprev, w, red, &al);
+      p0 = step3(VG + (size_t)j * n, VL + (size_t)(j - 1) * sL, true, bprev, &al);
     else
-      p0 = three_term<BT, T>(a, base, n, VG + (size_t)j * n, VG + (size_t)(j > 0 ? j - 1 : 0) * n, j > 0, bprev,
-                             w, red, &al);
+      p0 = step3(VG + (size_t)j * n, VG + (size_t)(j > 0 ? j - 1 : 0) * n, j > 0, bprev, &al);
     // full reorthogonalization against V_0..V_j; a second pass when the
     // first removed more than half of |w|^2 (Daniel-Gragg-Kaufman-Stewart),
     // always when the criterion is switched off
-    dots_split<BT, T>(VL, kl, VG, j + 1, n, w, cl);
-    double p1 = update_split<BT, T>(VL, kl, VG, j + 1, n, w, cl, red);
-    if (!a.dgks || p1 < 0.5 * p0) {
+    double p1;
+    if constexpr (LV) {
+      dots_part<BT, T>(VL, sL, kl, VG, j + 1, n, w, part, cl);
+      p1 = update_part<BT, T>(VL, sL, kl, VG, j + 1, n, w, cl, part, red2, parity);
+      if (!a.dgks || p1 < 0.5 * p0) {
+        dots_part<BT, T>(VL, sL, kl, VG, j + 1, n, w, part, cl);
+        p1 = update_part<BT, T>(VL, sL, kl, VG, j + 1, n, w, cl, part, red2, parity);
+      }
+    } else {
       dots_split<BT, T>(VL, kl, VG, j + 1, n, w, cl);
       p1 = update_split<BT, T>(VL, kl, VG, j + 1, n, w, cl, red);
+      if (!a.dgks || p1 < 0.5 * p0) {
+        dots_split<BT, T>(VL, kl, VG, j + 1, n, w, cl);
+        p1 = update_split<BT, T>(VL, kl, VG, j + 1, n, w, cl, red);
+      }
     }
     if (threadIdx.x == 0) {
       alpha[j] = al;
@@ -1356,7 +1598,7 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
       sb[j] = b;
     }
     __syncthreads();
-    if (last || (m >= nev && m % a.check_every == 0)) {
+    if (last || (m >= nev && m >= cstart && m % a.check_every == 0)) {
       // Ritz pairs of T_m: eigenvalues by workgroup multisection, then
       // thread k refines eigenvector k by inverse iteration
       const int ne = nev < m ? nev : m;
@@ -1400,12 +1642,13 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
       }
     }
     if (last) break;
+    const double rb = 1.0 / b;
     if (j + 1 < kl) {
-      T* vn = VL + (size_t)(j + 1) * n;
-      for (int64_t i = threadIdx.x; i < n; i += BT) vn[i] = mk<T>(re_(w[i]) / b, im_(w[i]) / b);
+      T* vn = VL + (size_t)(j + 1) * sL;
+      for (int64_t i = threadIdx.x; i < n; i += BT) vn[i] = mk<T>(re_(w[i]) * rb, im_(w[i]) * rb);
     } else {
       T* vn = VG + (size_t)(j + 1) * n;
-      for (int64_t i = threadIdx.x; i < n; i += BT) vn[i] = mk<T>(re_(w[i]) / b, im_(w[i]) / b);
+      for (int64_t i = threadIdx.x; i < n; i += BT) vn[i] = mk<T>(re_(w[i]) * rb, im_(w[i]) * rb);
     }
     __syncthreads();
   }
@@ -1417,7 +1660,7 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
     double pn = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += BT) {
       double xr, xi;
-      combine_split<T>(VL, kl, VG, m, n, i, y, &xr, &xi);
+      combine_split<T>(VL, sL, kl, VG, m, n, i, y, &xr, &xi);
       X[(size_t)k * n + i] = make_double2(xr, xi);
       pn += xr * xr + xi * xi;
     }
@@ -2530,6 +2773,7 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   a.conv = conv;
   a.dgks = (int)g_tuning.eigen_dgks;
   a.check_every = (int)std::max<int64_t>(1, g_tuning.eigen_check_every);
+  a.check_start = (int)std::max<int64_t>(0, g_tuning.eigen_check_start);
   // largest chains first so the longest workgroups start first; chains up to
   // eigen_wave_max_n get one wave each (reductions by shuffles, no barriers),
   // the others eigen_bt threads
@@ -2568,14 +2812,18 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
     lv = false;
     a.kl = 0;
     if (a.lds_work && share > woff + static_lds) {
-      int64_t kl = std::min<int64_t>((int64_t)((share - static_lds - woff) / (esz * nmax)) - 1,
-                                     std::min<int64_t>(max_steps, nmax));
+      // w, then kl vectors of row stride sl, then the Gram-Schmidt partial
+      // sums (bt/64 rows of nmax complex)
+      const int64_t sl = (nmax + 6) / 8 * 8 + 1;
+      const int64_t part_bytes = (int64_t)(bt / 64) * nmax * 16 + 16;
+      const int64_t avail = (int64_t)share - (int64_t)static_lds - (int64_t)woff - nmax * (int64_t)esz - part_bytes;
+      int64_t kl = avail > 0 ? std::min<int64_t>(avail / ((int64_t)esz * sl), std::min<int64_t>(max_steps, nmax)) : 0;
       const int64_t cap = g_tuning.eigen_lds_vectors;
       if (cap >= 0) kl = std::min(kl, cap);
       if (kl >= 8 || (cap >= 0 && kl >= 1)) {
         lv = true;
         a.kl = (int)kl;
-        shm = woff + (size_t)(kl + 1) * nmax * esz;
+        shm = woff + (size_t)(nmax + kl * sl) * esz + (size_t)part_bytes;
       }
     }
   };
