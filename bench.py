@@ -197,23 +197,24 @@ def measured_peaks(torch):
 BF16_MFMA_PEAK_TFLOPS = 2500.0  # dense bf16 matrix-core peak of one MI355X
 
 
-def _pmc_traffic(kernel):
+def _pmc_traffic(kernel, workload=None):
     """HBM bytes per dispatch of ``kernel`` measured by the rocprofv3 --pmc
     runs (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size)
-    whose summaries are kept in profiles/, or None."""
+    whose summaries are kept in profiles/, or None.  A line tagged with a
+    workload (``workload=...``) counts only for that workload."""
     import glob
     import re
 
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc_traffic.md"))):
         with open(path) as f:
             for line in f:
-                m = re.match(r"traffic_bytes_per_dispatch kernel=(\S+) bytes=(\d+)", line.strip())
-                if m and m.group(1) == kernel:
+                m = re.match(r"traffic_bytes_per_dispatch kernel=(\S+) bytes=(\d+)(?: workload=(\S+))?", line.strip())
+                if m and m.group(1) == kernel and (m.group(3) is None or m.group(3) == workload):
                     return int(m.group(2))
     return None
 
 
-def _roofline(kernels, bytes_by_kernel, flops_by_kernel=None):
+def _roofline(kernels, bytes_by_kernel, flops_by_kernel=None, workload=None):
     """Roofline object of the kernel with the largest timed total: HBM-bound
     kernels priced by algorithmic bytes against 8 TB/s, matrix-core kernels
     by algorithmic flops against the dense bf16 peak."""
@@ -228,12 +229,12 @@ def _roofline(kernels, bytes_by_kernel, flops_by_kernel=None):
         per_launch = flops_by_kernel[dom] / launches
         achieved = per_launch / (avg_ms * 1e-3) / 1e12
         return dict(bound="mfma", kernel=dom + "_kernel", achieved=round(achieved, 3), peak=BF16_MFMA_PEAK_TFLOPS,
-                    unit="TFLOP/s", frac=round(achieved / BF16_MFMA_PEAK_TFLOPS, 5), traffic=_pmc_traffic(dom + "_kernel"),
+                    unit="TFLOP/s", frac=round(achieved / BF16_MFMA_PEAK_TFLOPS, 5), traffic=_pmc_traffic(dom + "_kernel", workload),
                     avg_launch_ms=round(avg_ms, 4), launches=launches, algorithmic_flops_per_launch=per_launch)
     per_launch = bytes_by_kernel[dom] / launches
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     return dict(bound="hbm", kernel=dom + "_kernel", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=_pmc_traffic(dom + "_kernel"), avg_launch_ms=round(avg_ms, 4),
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=_pmc_traffic(dom + "_kernel", workload), avg_launch_ms=round(avg_ms, 4),
                 launches=launches, algorithmic_bytes_per_launch=per_launch)
 
 
@@ -458,7 +459,8 @@ def run_exact(args, torch):
     other_ms = 1e3 * (time.perf_counter() - t1) / nb
     eig = sum(algorithmic_bytes_eigen(r.ground_dims, r.ground_nnz, r.eig_steps) for r in results)
     lan = sum(algorithmic_bytes_lanczos(r.chain_dims, r.chain_nnz, r.kout) for r in results)
-    roofline = _roofline({k: kernels[k] for k in ("eigen", "lanczos")}, {"eigen": eig, "lanczos": lan})
+    roofline = _roofline({k: kernels[k] for k in ("eigen", "lanczos")}, {"eigen": eig, "lanczos": lan},
+                         workload="exact%d" % args.models)
     if roofline is not None and roofline["kernel"] == "eigen_kernel":
         b = sum(gram_schmidt_bytes(r.ground_dims, r.eig_steps, eigen_itemsize(fam)) for r in results)
         roofline["gram_schmidt_gbs"] = round(b / (kernels["eigen"][0] * 1e-3) / 1e9, 1)
