@@ -214,6 +214,15 @@ def _pmc_traffic(kernel, workload=None):
     return None
 
 
+def _tuning_value(args, name, default):
+    """The value ``--tuning name=value`` set for this run, or the engine's default."""
+    for kv in getattr(args, "tuning", []) or []:
+        k, v = kv.split("=", 1)
+        if k == name:
+            return int(v)
+    return default
+
+
 def _roofline(kernels, bytes_by_kernel, flops_by_kernel=None, workload=None):
     """Roofline object of the kernel with the largest timed total: HBM-bound
     kernels priced by algorithmic bytes against 8 TB/s, matrix-core kernels
@@ -534,20 +543,22 @@ def run_large(args, torch):
         psis = gb.vec_from_eigen(np.zeros(len(kept), np.int32), np.array(kept, np.int32), 1e-7)
         return gb, e[kept], psis, t_build, t_eig, int(steps[0])
 
-    def xas_step():
+    def xas_step(label):
         t0 = time.perf_counter()
         gb, es, psis, t_build, t_eig, steps = ground()
+        print("large %s: ground state done (%.2f s)" % (label, time.perf_counter() - t0), file=sys.stderr, flush=True)
         spec, info = state_spectra(ops_h, ops_xas, psis, es, omega, 0.2, K, T=300.0)
         torch.cuda.synchronize()
+        print("large %s: XAS done (%.2f s)" % (label, time.perf_counter() - t0), file=sys.stderr, flush=True)
         return time.perf_counter() - t0, gb, es, psis, t_build, t_eig, steps, spec, info
 
     for _ in range(args.warmup):
-        xas_step()
+        xas_step("warm-up")
     # timed steps: launch timing off, so the Lanczos steps replay as HIP graphs
     torch.cuda.synchronize()
     ROCTX.push("large_timed")
     t0 = time.perf_counter()
-    runs = [xas_step() for _ in range(args.steps)]
+    runs = [xas_step("timed") for _ in range(args.steps)]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ROCTX.pop()
@@ -557,7 +568,7 @@ def run_large(args, torch):
     engine.timing_enable(True)
     engine.timing_reset()
     ROCTX.push("large_instrumented")
-    _, gb_i, _, _, _, _, eig_steps_i, _, info_i = xas_step()
+    _, gb_i, _, _, _, _, eig_steps_i, _, info_i = xas_step("instrumented")
     ROCTX.pop()
     kernels = {k: engine.timing_query(k)
                for k in ("spmv", "eigen_flat", "apply_count", "apply_fill", "discover", "sort", "csr_fill", "cf")}
@@ -576,14 +587,20 @@ def run_large(args, torch):
                  states_per_s=round((gb_i.rows + float(np.sum(info_i["chain_n"]))) / (build_ms * 1e-3), 1)
                  if build_ms else None)
     # algorithmic bytes of every flat SpMV launch in the instrumented step: the
-    # Lanczos chains (K steps each) and the ground-state eigensolver
-    # (eig_steps SpMVs); 20 nnz + 8 (n+1) + 32 n per launch (complex128 values,
-    # int32 columns, int64 row pointers, x read once, y written)
-    spmv_bytes = lambda nn, zz: 20.0 * zz + 8.0 * (nn + 1) + 32.0 * nn
-    per_step = sum(int(k) * spmv_bytes(nn, zz)
+    # Lanczos chains (K steps each, complex vectors) and the ground-state
+    # eigensolver (eig_steps SpMVs, real vectors when H is real); per launch
+    # (value + 4) nnz + 8 (n+1) + 2 n (vector element) bytes: int32 columns,
+    # int64 row pointers, x read once, y written.  A real H is read as 8-byte
+    # values (the real_values knob, on by default), else as complex128.
+    h_real = not any(complex(v).imag for v in m.hOp.values())
+    vb = 8.0 if h_real and _tuning_value(args, "real_values", 1) else 16.0
+    eb = 8.0 if h_real and _tuning_value(args, "eigen_real", 1) else 16.0
+    spmv_bytes = lambda nn, zz, xb: (vb + 4.0) * zz + 8.0 * (nn + 1) + 2.0 * xb * nn
+    per_step = sum(int(k) * spmv_bytes(nn, zz, 16.0)
                    for nn, zz, k in zip(info_i["chain_n"], info_i["chain_nnz"], info_i["kout"]))
-    per_step += eig_steps_i * spmv_bytes(gb_i.rows, gb_i.nnz)
-    roofline = _roofline({"spmv": kernels["spmv"]}, {"spmv": per_step})
+    per_step += eig_steps_i * spmv_bytes(gb_i.rows, gb_i.nnz, eb)
+    roofline = _roofline({"spmv": kernels["spmv"]}, {"spmv": per_step},
+                         workload="large_real" if vb == 8.0 else "large_complex")
     roofline["measured_on"] = "one instrumented step after the timed ones (launch timing forces direct launches)"
 
     # XPS (2p removal, photoemission convention) and NIXS (|q| = 2, 7 along (111), 3d -> 3d: k = 2, 4)
@@ -844,6 +861,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="large workload: skip the XPS/NIXS extras")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="copy this many MiB device-to-device first (counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--tuning", action="append", default=[], metavar="NAME=VALUE",
+                    help="set an engine launch-shape knob (imped_set_tuning) before the run; repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the engine runs on a single MI355X; "
@@ -865,9 +884,17 @@ def main():
         torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
+    if args.tuning:
+        from impuritymodel_amd import engine
+
+        for kv in args.tuning:
+            name, value = kv.split("=", 1)
+            engine.set_tuning(name, int(value))
     run = {"fit": run_fit, "exact": run_exact, "batched128": run_batched128, "large": run_large,
            "rixs": run_rixs}[args.workload]
     out = run(args, torch)
+    if args.tuning:
+        out["config"]["tuning"] = dict(kv.split("=", 1) for kv in args.tuning)
     print(json.dumps(out))
 
 

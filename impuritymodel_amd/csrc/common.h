@@ -137,6 +137,7 @@ struct Tuning {
   int64_t eigen_check_start = 48;       // no Ritz check before min(this, n/2) steps
   int64_t flat_min_n = 32768;           // chains above this size use the grid-wide (flat) kernels
   int64_t spmv_lpr = 8;                 // lanes per CSR row in the flat SpMV (4..64)
+  int64_t real_values = 1;              // grid-wide SpMV on a real CSR reads 8-byte real values
   int64_t lanczos_graph_steps = 16;     // flat Lanczos steps per replayed HIP graph (0: direct launches)
   int64_t eigen_lds_vectors = -1;       // cap on the eigensolver's LDS Krylov vectors (-1: as many as fit)
   int64_t small_sort = 1;               // one-workgroup sort for small key sets (0: digit passes always)

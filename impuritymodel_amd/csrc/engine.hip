@@ -270,6 +270,8 @@ struct Batch {
   DevBuf fill_meta;  // chain table of imped_batch_set_affine
   DevBuf seed_meta, norm2dev;  // chain table and |T psi|^2 of imped_batch_seed_affine
   bool have_norm2dev = false;
+  DevBuf val_re;  // real parts of val for the grid-wide kernels (real CSR with a chain above flat_min_n)
+  bool have_val_re = false;
 };
 
 // Seed amplitudes are placed on the first rows of their chain: every chain
@@ -805,6 +807,41 @@ __global__ __launch_bounds__(64) void affine_seed_kernel(const int64_t* smeta, i
   if (threadIdx.x == 0) norm2[c] = p;
 }
 
+__global__ void real_part_kernel(const double2* v, int64_t n, double* out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = v[i].x;
+}
+
+// The Krylov view of a batch; for a real CSR with a chain run by the
+// grid-wide kernels, the real parts of the values (made once per CSR) ride
+// along so those kernels read 8 bytes per value.
+KrylovBatch krylov_view(Batch* b) {
+  KrylovBatch kb{b->C,
+                 b->row_base.as<int64_t>(),
+                 b->chain_n.as<int64_t>(),
+                 b->rowptr.as<int64_t>(),
+                 b->col.as<int32_t>(),
+                 b->val.as<double2>(),
+                 b->h_n,
+                 b->h_nnz};
+  bool flat = false;
+  for (int c = 0; c < b->C; ++c) flat = flat || b->h_n[c] > g_tuning.flat_min_n;
+  if (b->real && flat && g_tuning.real_values) {
+    if (!b->have_val_re) {
+      b->val_re.reserve(std::max<int64_t>(b->NNZ, 1) * sizeof(double));
+      if (b->NNZ > 0) {
+        const unsigned nb = (unsigned)std::min<int64_t>((b->NNZ + 255) / 256, 8192);
+        hipLaunchKernelGGL(real_part_kernel, dim3(nb), dim3(256), 0, stream(), b->val.as<double2>(), b->NNZ,
+                           b->val_re.as<double>());
+        HIP_CHECK(hipGetLastError());
+      }
+      b->have_val_re = true;
+    }
+    kb.val_re = b->val_re.as<double>();
+  }
+  return kb;
+}
+
 }  // namespace imped
 
 using namespace imped;
@@ -1021,6 +1058,8 @@ int imped_set_tuning(const char* name, int64_t value) {
       return IMPED_ERR_INVALID;
     }
     g_tuning.lanczos_graph_steps = value;
+  } else if (n == "real_values") {
+    g_tuning.real_values = value != 0;
   } else if (n == "eigen_check_start") {
     if (value < 0) {
       set_error("imped_set_tuning: eigen_check_start must be >= 0");
@@ -1225,6 +1264,7 @@ int imped_batch_build(imped_batch batch, int fixed_basis, double weight_min, int
   }
   b->have_v0dev = false;
   b->have_norm2dev = false;
+  b->have_val_re = false;
   build(*b, fixed_basis ? 1 : 0, weight_min, max_layers);
   API_END
 }
@@ -1235,6 +1275,7 @@ int imped_batch_set_csr(imped_batch batch, const int64_t* chain_n, const int64_t
   Batch* b = reinterpret_cast<Batch*>(batch);
   b->have_v0dev = false;
   b->have_norm2dev = false;
+  b->have_val_re = false;
   b->h_n.assign(chain_n, chain_n + b->C);
   b->h_row_base.assign(b->C + 1, 0);
   for (int c = 0; c < b->C; ++c) {
@@ -1468,6 +1509,7 @@ int imped_batch_set_affine(imped_batch batch, imped_affine aff, const int32_t* c
   b->real = a->real;
   b->have_v0dev = false;
   b->have_norm2dev = false;
+  b->have_val_re = false;
   b->have_seed_amps = false;
   API_END
 }
@@ -1673,14 +1715,7 @@ int imped_batch_lanczos(imped_batch batch, const double* v0_host, int K, int reo
   b->beta.reserve((size_t)std::max(b->C, 1) * K * 8);
   b->kout.reserve((size_t)std::max(b->C, 1) * 8);  // kout[C], then status[C]
   int32_t* status_d = b->kout.as<int32_t>() + std::max(b->C, 1);
-  KrylovBatch kb{b->C,
-                 b->row_base.as<int64_t>(),
-                 b->chain_n.as<int64_t>(),
-                 b->rowptr.as<int64_t>(),
-                 b->col.as<int32_t>(),
-                 b->val.as<double2>(),
-                 b->h_n,
-                 b->h_nnz};
+  const KrylovBatch kb = krylov_view(b);
   run_lanczos(kb, v0d, K, reorth, b->alpha.as<double>(), b->beta.as<double>(), b->kout.as<int32_t>(), status_d,
               stream());
   if (alpha) d2h(alpha, b->alpha.p, (size_t)b->C * K, stream());
@@ -1718,14 +1753,7 @@ int imped_batch_eigen(imped_batch batch, int nev, double tol, int max_steps, uin
   int32_t* steps_d = reinterpret_cast<int32_t*>(static_cast<char*>(b->evals.p) + ev_bytes);
   int32_t* conv_d = steps_d + C1;
   b->evecs.reserve(std::max<int64_t>(b->R, 1) * nev * 16);
-  KrylovBatch kb{b->C,
-                 b->row_base.as<int64_t>(),
-                 b->chain_n.as<int64_t>(),
-                 b->rowptr.as<int64_t>(),
-                 b->col.as<int32_t>(),
-                 b->val.as<double2>(),
-                 b->h_n,
-                 b->h_nnz};
+  const KrylovBatch kb = krylov_view(b);
   int rc = run_eigen(kb, nev, tol, max_steps, seed, b->evals.as<double>(), b->evecs.as<double2>(), steps_d, conv_d,
                      b->real, stream());
   if (rc != IMPED_OK) return rc;

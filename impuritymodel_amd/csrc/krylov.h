@@ -16,6 +16,9 @@ struct KrylovBatch {
   const int32_t* col;
   const double2* val;
   std::vector<int64_t> h_n, h_nnz;
+  // the real parts of val when every value is real (NULL otherwise): the
+  // grid-wide SpMV then reads 8 bytes per value instead of 16
+  const double* val_re = nullptr;
 };
 
 // Plain (reorth = 0) or fully reorthogonalized (reorth = 1) Lanczos from v0

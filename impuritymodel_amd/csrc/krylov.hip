@@ -1976,9 +1976,11 @@ __global__ __launch_bounds__(256) void cf_kernel(CFArgs a) {
 
 constexpr int FT = 256;  // threads per flat block
 
-template <int LPR, class T>
+// VR: the values are real and read from vre (8 bytes each) instead of val.
+template <int LPR, class T, bool VR = false>
 __device__ __forceinline__ void spmv_rows(const int64_t* rowptr, const int32_t* col, const double2* val, int64_t base,
-                                          int64_t n, const T* x, T* y, double* partial, double* red) {
+                                          int64_t n, const T* x, T* y, double* partial, double* red,
+                                          const double* vre = nullptr) {
   const int g = threadIdx.x / LPR, r = threadIdx.x % LPR;
   double pa = 0.0;
   // grid-stride over groups of FT/LPR rows, so the partial sums number at
@@ -1989,13 +1991,19 @@ __device__ __forceinline__ void spmv_rows(const int64_t* rowptr, const int32_t* 
     if (row < n) {
       const int64_t e0 = rowptr[base + row], e1 = rowptr[base + row + 1];
       for (int64_t e = e0 + r; e < e1; e += LPR) {
-        const double2 h = val[e];
         const T v = x[col[e]];
-        if constexpr (is_real_v<T>) {
-          ax += h.x * re_(v);
+        if constexpr (VR) {
+          const double h = vre[e];
+          ax += h * re_(v);
+          if constexpr (!is_real_v<T>) ay += h * im_(v);
         } else {
-          ax += h.x * re_(v) - h.y * im_(v);
-          ay += h.x * im_(v) + h.y * re_(v);
+          const double2 h = val[e];
+          if constexpr (is_real_v<T>) {
+            ax += h.x * re_(v);
+          } else {
+            ax += h.x * re_(v) - h.y * im_(v);
+            ay += h.x * im_(v) + h.y * re_(v);
+          }
         }
       }
     }
@@ -2013,11 +2021,12 @@ __device__ __forceinline__ void spmv_rows(const int64_t* rowptr, const int32_t* 
   if (threadIdx.x == 0) partial[blockIdx.x] = pa;
 }
 
-template <int LPR, class T>
+template <int LPR, class T, bool VR>
 __global__ __launch_bounds__(FT) void flat_spmv_kernel(const int64_t* rowptr, const int32_t* col, const double2* val,
-                                                       int64_t base, int64_t n, const T* x, T* y, double* partial) {
+                                                       const double* vre, int64_t base, int64_t n, const T* x, T* y,
+                                                       double* partial) {
   __shared__ double red[FT / 64];
-  spmv_rows<LPR, T>(rowptr, col, val, base, n, x, y, partial, red);
+  spmv_rows<LPR, T, VR>(rowptr, col, val, base, n, x, y, partial, red, vre);
 }
 
 // out[k] = sum of partial[k*stride .. k*stride + count) in a fixed order;
@@ -2276,8 +2285,13 @@ void launch_spmv(int lpr, const KrylovBatch& kb, int64_t base, int64_t n, const 
   const unsigned nb = (unsigned)std::min<int64_t>(std::max<int64_t>((n + FT / lpr - 1) / (FT / lpr), 1), 4096);
   *nb_out = nb;
   TimedLaunch tl("spmv", s);
-#define IMPED_SPMV(L) \
-  hipLaunchKernelGGL((flat_spmv_kernel<L, T>), dim3(nb), dim3(FT), 0, s, kb.rowptr, kb.col, kb.val, base, n, x, y, partial)
+#define IMPED_SPMV(L)                                                                                           \
+  if (kb.val_re)                                                                                               \
+    hipLaunchKernelGGL((flat_spmv_kernel<L, T, true>), dim3(nb), dim3(FT), 0, s, kb.rowptr, kb.col, kb.val,     \
+                       kb.val_re, base, n, x, y, partial);                                                      \
+  else                                                                                                         \
+    hipLaunchKernelGGL((flat_spmv_kernel<L, T, false>), dim3(nb), dim3(FT), 0, s, kb.rowptr, kb.col, kb.val,    \
+                       kb.val_re, base, n, x, y, partial)
   switch (lpr) {
     case 4: IMPED_SPMV(4); break;
     case 8: IMPED_SPMV(8); break;
@@ -2315,6 +2329,7 @@ struct StepArgs {
   const int64_t* rowptr;
   const int32_t* col;
   const double2* val;
+  const double* val_re;  // real values (NULL: complex val)
   int64_t base, n;
   int32_t jc;  // index of the step opened last (-1 before the first)
   int32_t kc;  // steps of this chain
@@ -2363,11 +2378,12 @@ __global__ __launch_bounds__(FT) void step_scale_kernel(StepArgs* a) {
     dst[i] = make_double2(src[i].x / b, src[i].y / b);
 }
 
-template <int LPR>
+template <int LPR, bool VR>
 __global__ __launch_bounds__(FT) void step_spmv_kernel(StepArgs* a) {
   __shared__ double red[FT / 64];
   const int j = a->jc;
-  spmv_rows<LPR, double2>(a->rowptr, a->col, a->val, a->base, a->n, a->buf[(j + 1) % 2], a->buf[2], a->part, red);
+  spmv_rows<LPR, double2, VR>(a->rowptr, a->col, a->val, a->base, a->n, a->buf[(j + 1) % 2], a->buf[2], a->part, red,
+                              a->val_re);
 }
 
 __global__ __launch_bounds__(FT) void step_alpha_kernel(StepArgs* a, int64_t count) {
@@ -2392,18 +2408,24 @@ unsigned pow2_blocks(int64_t need, unsigned cap) {
   return b;
 }
 
-void launch_lanczos_step(StepArgs* a, unsigned nbv, unsigned nbs, int lpr, hipStream_t s) {
+void launch_lanczos_step(StepArgs* a, unsigned nbv, unsigned nbs, int lpr, bool real_vals, hipStream_t s) {
   hipLaunchKernelGGL(step_beta_kernel, dim3(1), dim3(FT), 0, s, a, (int64_t)nbv);
   hipLaunchKernelGGL(step_scale_kernel, dim3(nbv), dim3(FT), 0, s, a);
   {
     TimedLaunch tl("spmv", s);
+#define IMPED_STEP_SPMV(L)                                                                  \
+  if (real_vals)                                                                          \
+    hipLaunchKernelGGL((step_spmv_kernel<L, true>), dim3(nbs), dim3(FT), 0, s, a);          \
+  else                                                                                    \
+    hipLaunchKernelGGL((step_spmv_kernel<L, false>), dim3(nbs), dim3(FT), 0, s, a)
     switch (lpr) {
-      case 4: hipLaunchKernelGGL((step_spmv_kernel<4>), dim3(nbs), dim3(FT), 0, s, a); break;
-      case 8: hipLaunchKernelGGL((step_spmv_kernel<8>), dim3(nbs), dim3(FT), 0, s, a); break;
-      case 16: hipLaunchKernelGGL((step_spmv_kernel<16>), dim3(nbs), dim3(FT), 0, s, a); break;
-      case 32: hipLaunchKernelGGL((step_spmv_kernel<32>), dim3(nbs), dim3(FT), 0, s, a); break;
-      default: hipLaunchKernelGGL((step_spmv_kernel<64>), dim3(nbs), dim3(FT), 0, s, a); break;
+      case 4: IMPED_STEP_SPMV(4); break;
+      case 8: IMPED_STEP_SPMV(8); break;
+      case 16: IMPED_STEP_SPMV(16); break;
+      case 32: IMPED_STEP_SPMV(32); break;
+      default: IMPED_STEP_SPMV(64); break;
     }
+#undef IMPED_STEP_SPMV
   }
   hipLaunchKernelGGL(step_alpha_kernel, dim3(1), dim3(FT), 0, s, a, (int64_t)nbs);
   hipLaunchKernelGGL(step_update_kernel, dim3(nbv), dim3(FT), 0, s, a);
@@ -2421,21 +2443,23 @@ StepArgs* step_args_block() {
 struct StepGraph {
   unsigned nbv, nbs;
   int lpr, G;
+  bool real_vals;
   hipGraphExec_t exec;
 };
 
 // Instantiated G-step graph for a grid shape (captured on a private stream;
 // at most 32 shapes are kept, the oldest destroyed after `s` has drained).
-hipGraphExec_t lanczos_step_graph(StepArgs* a, unsigned nbv, unsigned nbs, int lpr, int G, hipStream_t s) {
+hipGraphExec_t lanczos_step_graph(StepArgs* a, unsigned nbv, unsigned nbs, int lpr, int G, bool real_vals,
+                                  hipStream_t s) {
   static std::vector<StepGraph> cache;
   static hipStream_t cap = nullptr;
   for (const auto& g : cache)
-    if (g.nbv == nbv && g.nbs == nbs && g.lpr == lpr && g.G == G) return g.exec;
+    if (g.nbv == nbv && g.nbs == nbs && g.lpr == lpr && g.G == G && g.real_vals == real_vals) return g.exec;
   if (!cap) HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
   HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
   hipGraph_t graph = nullptr;
   try {
-    for (int g = 0; g < G; ++g) launch_lanczos_step(a, nbv, nbs, lpr, cap);
+    for (int g = 0; g < G; ++g) launch_lanczos_step(a, nbv, nbs, lpr, real_vals, cap);
   } catch (...) {
     (void)hipStreamEndCapture(cap, &graph);
     if (graph) (void)hipGraphDestroy(graph);
@@ -2451,7 +2475,7 @@ hipGraphExec_t lanczos_step_graph(StepArgs* a, unsigned nbv, unsigned nbs, int l
     HIP_CHECK(hipGraphExecDestroy(cache.front().exec));
     cache.erase(cache.begin());
   }
-  cache.push_back(StepGraph{nbv, nbs, lpr, G, exec});
+  cache.push_back(StepGraph{nbv, nbs, lpr, G, real_vals, exec});
   return exec;
 }
 
@@ -2476,6 +2500,7 @@ void lanczos_flat_chain(const KrylovBatch& kb, int c, const double2* v0, int K, 
   v.rowptr = kb.rowptr;
   v.col = kb.col;
   v.val = kb.val;
+  v.val_re = kb.val_re;
   v.base = base;
   v.n = n;
   v.kc = Kc;
@@ -2485,10 +2510,10 @@ void lanczos_flat_chain(const KrylovBatch& kb, int c, const double2* v0, int K, 
   const int G = (int)g_tuning.lanczos_graph_steps;
   int j = 0;
   if (G > 0 && !timing_enabled() && Kc >= G) {
-    hipGraphExec_t exec = lanczos_step_graph(a, nbv, nbs, lpr, G, s);
+    hipGraphExec_t exec = lanczos_step_graph(a, nbv, nbs, lpr, G, kb.val_re != nullptr, s);
     for (; j + G <= Kc; j += G) HIP_CHECK(hipGraphLaunch(exec, s));
   }
-  for (; j < Kc; ++j) launch_lanczos_step(a, nbv, nbs, lpr, s);
+  for (; j < Kc; ++j) launch_lanczos_step(a, nbv, nbs, lpr, kb.val_re != nullptr, s);
 }
 
 // Lowest eigenpairs of one large chain: Lanczos with full reorthogonalization

@@ -66,6 +66,9 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * min(this, n/2) steps of an n-row chain, 48 by default; the last step is
  * always checked), "flat_min_n" (chains above it use the grid-wide
  * kernels), "spmv_lpr" (lanes per row of the grid-wide SpMV: 4..64),
+ * "real_values" (1, the default: when every value of a batch's CSR is real,
+ * the grid-wide kernels read an 8-byte copy of the real parts, made once per
+ * CSR; 0: the complex values),
  * "lanczos_graph_steps" (grid-wide Lanczos steps per replayed HIP graph,
  * 0..256; 0 launches every step directly, as does switched-on timing),
  * "eigen_lds_vectors" (cap on the Krylov vectors the eigensolver keeps in
