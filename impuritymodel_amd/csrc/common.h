@@ -179,6 +179,13 @@ __device__ __forceinline__ double dpp_move(double x) {
 // active: DPP moves inside registers (quad swaps, half-row and row mirrors,
 // then row broadcasts 15 and 31 collect the four rows in lane 63), read back
 // from lane 63.  Fixed order, so runs are reproducible.
+// Maximum over the wave, the same in every lane (wave-uniform).
+__device__ __forceinline__ int wave_max_int(int x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = max(x, __shfl_xor(x, off));
+  return __builtin_amdgcn_readfirstlane(x);
+}
+
 __device__ __forceinline__ double wave_sum(double x) {
   x += dpp_move<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
   x += dpp_move<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]

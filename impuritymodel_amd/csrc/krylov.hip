@@ -380,18 +380,23 @@ __global__ __launch_bounds__(64) void lanczos_lds_kernel(LanczosArgs a) {
 // lanczos_lds_kernel leaves each step's Gram-Schmidt products as one long
 // sequential sum per lane (n terms for a dot, m for an update) on a single
 // wave.  Here NW waves share a chain: for the dots, wave g sums rows
-// [g n/NW, (g+1) n/NW) of every <V_l, w> and the partials are added in LDS;
-// for the update, wave g sums vectors [g m/NW, (g+1) m/NW) of every row and
-// those partials are added in LDS.  Each dependent chain is NW times shorter
-// and the chain's CU runs NW times as many waves.  Row t of the SpMV (t <
-// 64 NW) keeps up to RC entries of H in registers.  LDS: V (K rows of
-// stride s), w, the coefficients, and NW x max(K, s) partials (dots, then
-// updates).
+// [g n/NW, (g+1) n/NW) of every <V_l, w> (lane l); for the update, wave g
+// sums vectors [g m/NW, (g+1) m/NW) of every row (lane), each lane adding
+// the NW dot partials of the coefficients it needs itself; the NW update
+// partials of each row are then subtracted from w.  Each dependent chain is
+// NW times shorter and the chain's CU runs NW times as many waves.  Row t
+// of the SpMV (t < 64 NW) keeps up to RC entries of H in registers.
+//
+// A step takes five workgroup barriers: the SpMV reads w scaled by 1/beta
+// (so no pass writes v_j before it) into a second residual buffer, and
+// |w|^2 is posted by the three-term update and summed after the next
+// barrier.  LDS: V (K rows of stride s), w, w2, NW x K dot partials and
+// NW x s update partials.
 
 size_t lanczos_ldsw_bytes(int64_t n, int K, int nw) {
   const int64_t kk = K < n ? K : n;
   const int64_t s = (n + 6) / 8 * 8 + 1;
-  return (size_t)(kk * s + s + kk + 2 + nw * (kk > s ? kk : s)) * sizeof(double2);
+  return (size_t)(kk * s + 2 * s + nw * kk + nw * s) * sizeof(double2);
 }
 
 template <int NW>
@@ -417,24 +422,26 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
   const int64_t s = lds_row_stride(n);
   double2* V = lds;
   double2* w = V + (size_t)K * s;
-  double2* cl = w + s;
-  // dot partials [NW][K] and update partials [NW][s] share one region: the
-  // two phases are separated by barriers
-  double2* part = cl + (K + 2);
-  double2* upd = part;
+  double2* w2 = w + s;
+  double2* dotp = w2 + s;                // [NW][K]
+  double2* updp = dotp + (size_t)NW * K;  // [NW][s]
   const int64_t* rowptr = a.rowptr + base;
-  // workgroup sum: every thread gets the same value (the waves' partials
-  // added in wave order); red alternates between two rows so one barrier
-  // per sum suffices
+  // workgroup sums: post() leaves each wave's partial in a row of red
+  // (rows used in turn), collect() adds them after a barrier; a row is
+  // written again only two posts later, past a barrier that follows every
+  // collect of it
   int rsel = 0;
-  auto bsum = [&](double x) {
+  auto post = [&](double x) {
     x = wave_sum(x);
     if (lane == 0) red[rsel][wv] = x;
-    __syncthreads();
+    const int r = rsel;
+    rsel ^= 1;
+    return r;
+  };
+  auto collect = [&](int r) {
     double t = 0.0;
 #pragma unroll
-    for (int g = 0; g < NW; ++g) t += red[rsel][g];
-    rsel ^= 1;
+    for (int g = 0; g < NW; ++g) t += red[r][g];
     return t;
   };
   double2 hv[RC];
@@ -452,22 +459,29 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
       hc[k] = k < hn ? a.col[e0 + k] : 0;
     }
   }
-  // y = H x (rows by their owner threads) and Re<x, y>; ends with a barrier
-  auto spmv = [&](const double2* x, double2* y) {
+  // the wave's longest register row: groups of four entries past it are
+  // skipped (a wave-uniform branch); inside a group every load is issued
+  // unconditionally (entries past a row's end hold 0 at column 0)
+  const int hn_wave = wave_max_int(hn);
+  // y = H (sc x) over the rows of this thread, and its part of Re<sc x, y>;
+  // xj (when not null) receives sc x for those rows
+  auto spmv = [&](const double2* x, double sc, double2* y, double2* xj) {
     double pa = 0.0;
     if (tid < n) {
       double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
 #pragma unroll
-      for (int k = 0; k < RC; k += 2) {
-        if (k < hn) {
-          const double2 v = x[hc[k]];
-          ax += hv[k].x * v.x - hv[k].y * v.y;
-          ay += hv[k].x * v.y + hv[k].y * v.x;
-        }
-        if (k + 1 < hn) {
-          const double2 v = x[hc[k + 1]];
-          bx += hv[k + 1].x * v.x - hv[k + 1].y * v.y;
-          by += hv[k + 1].x * v.y + hv[k + 1].y * v.x;
+      for (int k0 = 0; k0 < RC; k0 += 4) {
+        if (k0 < hn_wave) {
+          double2 v[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) v[u] = x[hc[k0 + u]];
+#pragma unroll
+          for (int u = 0; u < 4; u += 2) {
+            ax += hv[k0 + u].x * v[u].x - hv[k0 + u].y * v[u].y;
+            ay += hv[k0 + u].x * v[u].y + hv[k0 + u].y * v[u].x;
+            bx += hv[k0 + u + 1].x * v[u + 1].x - hv[k0 + u + 1].y * v[u + 1].y;
+            by += hv[k0 + u + 1].x * v[u + 1].y + hv[k0 + u + 1].y * v[u + 1].x;
+          }
         }
       }
       for (int64_t e = r_rest; e < r_end; ++e) {
@@ -476,10 +490,12 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
         ax += h.x * v.x - h.y * v.y;
         ay += h.x * v.y + h.y * v.x;
       }
-      ax += bx;
-      ay += by;
+      ax = (ax + bx) * sc;
+      ay = (ay + by) * sc;
       y[tid] = make_double2(ax, ay);
-      pa += ax * x[tid].x + ay * x[tid].y;
+      const double2 xi = make_double2(x[tid].x * sc, x[tid].y * sc);
+      if (xj) xj[tid] = xi;
+      pa += ax * xi.x + ay * xi.y;
     }
     for (int64_t i = tid + T; i < n; i += T) {
       const int64_t e0 = rowptr[i], e1 = rowptr[i + 1];
@@ -490,12 +506,17 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
         ax += h.x * v.x - h.y * v.y;
         ay += h.x * v.y + h.y * v.x;
       }
+      ax *= sc;
+      ay *= sc;
       y[i] = make_double2(ax, ay);
-      pa += ax * x[i].x + ay * x[i].y;
+      const double2 xi = make_double2(x[i].x * sc, x[i].y * sc);
+      if (xj) xj[i] = xi;
+      pa += ax * xi.x + ay * xi.y;
     }
-    return bsum(pa);
+    return pa;
   };
-  // one classical Gram-Schmidt pass against V_0..V_{m-1}; returns |w|^2
+  // one classical Gram-Schmidt pass of w against V_0..V_{m-1} (w complete
+  // on entry); returns |w|^2 and leaves w complete
   auto gram_schmidt = [&](int m) {
     const int64_t r0 = n * wv / NW, r1 = n * (wv + 1) / NW;
     for (int l = lane; l < m; l += 64) {
@@ -515,27 +536,27 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
         cx += v.x * b.x + v.y * b.y;
         cy += v.x * b.y - v.y * b.x;
       }
-      part[(size_t)wv * K + l] = make_double2(cx + dx, cy + dy);
+      dotp[(size_t)wv * K + l] = make_double2(cx + dx, cy + dy);
     }
     __syncthreads();
-    for (int l = tid; l < m; l += T) {
-      double2 t = part[l];
+    // coefficient l = the NW dot partials of l, added by each lane that needs it
+    auto coef = [&](int l) {
+      double2 t = dotp[l];
 #pragma unroll
       for (int g = 1; g < NW; ++g) {
-        const double2 u = part[(size_t)g * K + l];
+        const double2 u = dotp[(size_t)g * K + l];
         t.x += u.x;
         t.y += u.y;
       }
-      cl[l] = t;
-    }
-    __syncthreads();
+      return t;
+    };
     const int l0 = m * wv / NW, l1 = m * (wv + 1) / NW;
     for (int64_t i = lane; i < n; i += 64) {
       double ux = 0.0, uy = 0.0, tx = 0.0, ty = 0.0;
       int l = l0;
-#pragma unroll 4
+#pragma unroll 2
       for (; l + 1 < l1; l += 2) {
-        const double2 cc = cl[l], c1 = cl[l + 1];
+        const double2 cc = coef(l), c1 = coef(l + 1);
         const double2 av = V[(size_t)l * s + i], a1 = V[(size_t)(l + 1) * s + i];
         ux += av.x * cc.x - av.y * cc.y;
         uy += av.x * cc.y + av.y * cc.x;
@@ -543,44 +564,53 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
         ty += a1.x * c1.y + a1.y * c1.x;
       }
       if (l < l1) {
-        const double2 cc = cl[l];
+        const double2 cc = coef(l);
         const double2 av = V[(size_t)l * s + i];
         ux += av.x * cc.x - av.y * cc.y;
         uy += av.x * cc.y + av.y * cc.x;
       }
-      upd[(size_t)wv * s + i] = make_double2(ux + tx, uy + ty);
+      updp[(size_t)wv * s + i] = make_double2(ux + tx, uy + ty);
     }
     __syncthreads();
     double p = 0.0;
     for (int64_t i = tid; i < n; i += T) {
-      double ux = upd[i].x, uy = upd[i].y;
+      double ux = updp[i].x, uy = updp[i].y;
 #pragma unroll
       for (int g = 1; g < NW; ++g) {
-        ux += upd[(size_t)g * s + i].x;
-        uy += upd[(size_t)g * s + i].y;
+        ux += updp[(size_t)g * s + i].x;
+        uy += updp[(size_t)g * s + i].y;
       }
       const double wx = w[i].x - ux, wy = w[i].y - uy;
       w[i] = make_double2(wx, wy);
       p += wx * wx + wy * wy;
     }
-    return bsum(p);
+    const int r = post(p);
+    __syncthreads();
+    return collect(r);
   };
   for (int64_t i = tid; i < n; i += T) V[i] = a.v0[base + i];
   __syncthreads();
-  // wp = H v0; alpha0 = Re(conj(wp) . v0); w = wp - alpha0 v0
-  double al = spmv(V, w);
+  // w2 = H v0; alpha0 = Re(conj(w2) . v0); w = w2 - alpha0 v0
+  int r = post(spmv(V, 1.0, w2, nullptr));
+  __syncthreads();
+  double al = collect(r);
   if (tid == 0) alpha[0] = al;
-  for (int64_t i = tid; i < n; i += T) w[i] = make_double2(w[i].x - al * V[i].x, w[i].y - al * V[i].y);
+  double p = 0.0;
+  for (int64_t i = tid; i < n; i += T) {
+    const double wx = w2[i].x - al * V[i].x, wy = w2[i].y - al * V[i].y;
+    w[i] = make_double2(wx, wy);
+    p += wx * wx + wy * wy;
+  }
+  int rp = post(p);
+  __syncthreads();
   int status = IMPED_OK;
   int kdone = K;
   for (int j = 1; j < K; ++j) {
-    double p = 0.0;
-    for (int64_t i = tid; i < n; i += T) p += w[i].x * w[i].x + w[i].y * w[i].y;
-    p = bsum(p);
+    const double pw = collect(rp);
     // full reorthogonalization against V_0..V_{j-1}; a second pass when
     // the first removed more than half of |w|^2 (as lanczos_kernel)
     double p1 = gram_schmidt(j);
-    if (p1 < 0.5 * p) p1 = gram_schmidt(j);
+    if (p1 < 0.5 * pw) p1 = gram_schmidt(j);
     const double b = sqrt(p1);
     if (tid == 0) beta[j - 1] = b;
     if (b == 0.0) {
@@ -590,14 +620,20 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
     }
     double2* vj = V + (size_t)j * s;
     const double2* vp = vj - s;
-    for (int64_t i = tid; i < n; i += T) vj[i] = make_double2(w[i].x / b, w[i].y / b);
+    // v_j = w / beta is formed inside the SpMV (rows by their owners)
+    r = post(spmv(w, 1.0 / b, w2, vj));
     __syncthreads();
-    al = spmv(vj, w);
+    al = collect(r);
     if (tid == 0) alpha[j] = al;
+    p = 0.0;
     for (int64_t i = tid; i < n; i += T) {
       const double2 x = vj[i], y = vp[i];
-      w[i] = make_double2(w[i].x - al * x.x - b * y.x, w[i].y - al * x.y - b * y.y);
+      const double wx = w2[i].x - al * x.x - b * y.x, wy = w2[i].y - al * x.y - b * y.y;
+      w[i] = make_double2(wx, wy);
+      p += wx * wx + wy * wy;
     }
+    rp = post(p);
+    __syncthreads();
   }
   if (tid == 0) {
     a.kout[c] = kdone;
@@ -1141,13 +1177,14 @@ __device__ __forceinline__ double three_term(const EigenArgs& a, int64_t base, i
 // HR shrinks with the workgroup size, whose waves have fewer registers each.
 template <int BT>
 constexpr int hrow_len() {
-  return BT <= 256 ? 16 : (BT <= 512 ? 8 : 1);
+  return BT <= 256 ? 16 : (BT <= 512 ? 8 : 4);
 }
 template <int HR>
 struct HRow {
+  static_assert(HR % 4 == 0, "register rows are read in groups of four");
   double2 v[HR];
   int c[HR];
-  int cnt = 0;
+  int cnt = 0, cnt_wave = 0;  // entries held; the wave's largest cnt
   int64_t rest = 0, end = 0;
 };
 
@@ -1165,6 +1202,7 @@ __device__ __forceinline__ void load_hrow(HRow<HR>& h, const int64_t* rowptr, co
       h.c[k] = k < h.cnt ? col[e0 + k] : 0;
     }
   }
+  h.cnt_wave = wave_max_int(h.cnt);
 }
 
 // spmv_t with row threadIdx.x taken from registers
@@ -1174,17 +1212,25 @@ __device__ __forceinline__ double spmv_cached(const HRow<HR>& h, const int64_t* 
   double pa = 0.0;
   if ((int64_t)threadIdx.x < n) {
     double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+    // groups of four entries up to the wave's longest register row (a
+    // wave-uniform branch), every load of a group issued together (entries
+    // past a row's end hold 0 at column 0)
 #pragma unroll
-    for (int k = 0; k < HR; k += 2) {
-      if (k < h.cnt) {
-        const T v = x[h.c[k]];
-        ax += h.v[k].x * re_(v) - h.v[k].y * im_(v);
-        if constexpr (!is_real_v<T>) ay += h.v[k].x * im_(v) + h.v[k].y * re_(v);
-      }
-      if (k + 1 < HR && k + 1 < h.cnt) {
-        const T v = x[h.c[k + 1]];
-        bx += h.v[k + 1].x * re_(v) - h.v[k + 1].y * im_(v);
-        if constexpr (!is_real_v<T>) by += h.v[k + 1].x * im_(v) + h.v[k + 1].y * re_(v);
+    for (int k0 = 0; k0 < HR; k0 += 4) {
+      if (k0 < h.cnt_wave) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = x[h.c[k0 + u]];
+#pragma unroll
+        for (int u = 0; u < 4; u += 2) {
+          const double2 h0 = h.v[k0 + u], h1 = h.v[k0 + u + 1];
+          ax += h0.x * re_(v[u]) - h0.y * im_(v[u]);
+          bx += h1.x * re_(v[u + 1]) - h1.y * im_(v[u + 1]);
+          if constexpr (!is_real_v<T>) {
+            ay += h0.x * im_(v[u]) + h0.y * re_(v[u]);
+            by += h1.x * im_(v[u + 1]) + h1.y * re_(v[u + 1]);
+          }
+        }
       }
     }
     for (int64_t e = h.rest; e < h.end; ++e) {
