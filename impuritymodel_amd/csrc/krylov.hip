@@ -652,20 +652,6 @@ __device__ __forceinline__ uint64_t splitmix(uint64_t x) {
 
 __device__ __forceinline__ double u01(uint64_t x) { return (double)(splitmix(x) >> 11) * (1.0 / 9007199254740992.0); }
 
-// Number of eigenvalues of the tridiagonal (a, b) below x (Sturm sequence).
-__device__ int sturm_count(const double* a, const double* b, int m, double x, double pivmin) {
-  int cnt = 0;
-  double q = a[0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  if (q < 0) ++cnt;
-  for (int i = 1; i < m; ++i) {
-    q = a[i] - x - b[i - 1] * b[i - 1] / q;
-    if (fabs(q) < pivmin) q = -pivmin;
-    if (q < 0) ++cnt;
-  }
-  return cnt;
-}
-
 // 1/x to within an ulp: the hardware reciprocal estimate refined by two
 // Newton steps (a short dependent chain, where a correctly rounded division
 // is a long one).
@@ -677,47 +663,49 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return fma(r, e, r);
 }
 
-// Sturm counts at P shifts: P independent recurrences interleaved, so the
-// latency of one step is shared by all of them.  Same recurrence and pivot
-// guard as sturm_count, with b^2/q formed as b^2 * rcp_nr(q).
+// Sturm counts at P shifts from the leading principal minors of T - x:
+// d_0 = 1, d_1 = a_0 - x, d_{i+1} = (a_i - x) d_i - b_{i-1}^2 d_{i-1}; the
+// number of eigenvalues of the m x m tridiagonal (a, b) below x is the
+// number of sign changes in d_0, ..., d_m.  There is no division, so the
+// dependent chain of a step is one FMA (the quotient form d_i / d_{i-1}
+// needs a reciprocal per step); the P recurrences are interleaved.  Every 8
+// steps the pair (d_{i-1}, d_i) is scaled by the power of two that brings
+// the larger to [0.5, 1): exact, so no sign or ratio changes, and within 8
+// steps the minors grow or shrink by at most (1 + |a - x| + b^2)^8 or b^16,
+// far from the exponent range.  A minor that is exactly zero counts by its
+// sign bit: its neighbours have opposite signs (b != 0 inside the chain), so
+// the two changes around it count one either way.
 template <int P>
-__device__ __forceinline__ void sturm_counts(const double* a, const double* b, int m, const double* x, double pivmin,
-                                             int* cnt) {
-  double q[P];
+__device__ __forceinline__ void sturm_counts(const double* a, const double* b, int m, const double* x, int* cnt) {
+  double d0[P], d1[P];  // d_{i-1}, d_i
 #pragma unroll
   for (int p = 0; p < P; ++p) {
-    q[p] = a[0] - x[p];
-    if (fabs(q[p]) < pivmin) q[p] = -pivmin;
-    cnt[p] = q[p] < 0 ? 1 : 0;
+    d0[p] = 1.0;
+    d1[p] = a[0] - x[p];
+    cnt[p] = (int)((unsigned)__double2hiint(d1[p]) >> 31);
   }
-  for (int i = 1; i < m; ++i) {
+  auto step = [&](int i) {
     const double ai = a[i], bb = b[i - 1] * b[i - 1];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-      q[p] = (ai - x[p]) - bb * rcp_nr(q[p]);
-      if (fabs(q[p]) < pivmin) q[p] = -pivmin;
-      cnt[p] += q[p] < 0 ? 1 : 0;
+      const double dn = fma(ai - x[p], d1[p], -bb * d0[p]);
+      cnt[p] += (int)((unsigned)(__double2hiint(dn) ^ __double2hiint(d1[p])) >> 31);
+      d0[p] = d1[p];
+      d1[p] = dn;
+    }
+  };
+  int i = 1;
+  for (; i + 8 <= m; i += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) step(i + u);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int e = max(__builtin_amdgcn_frexp_exp(d0[p]), __builtin_amdgcn_frexp_exp(d1[p]));
+      d0[p] = __builtin_amdgcn_ldexp(d0[p], -e);
+      d1[p] = __builtin_amdgcn_ldexp(d1[p], -e);
     }
   }
-}
-
-__device__ double kth_eigenvalue(const double* a, const double* b, int m, int k) {
-  double lo = a[0], hi = a[0], scale = 0.0;
-  for (int i = 0; i < m; ++i) {
-    const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < m - 1 ? fabs(b[i]) : 0.0);
-    lo = fmin(lo, a[i] - r);
-    hi = fmax(hi, a[i] + r);
-    scale = fmax(scale, fabs(a[i]) + r);
-  }
-  const double pivmin = 1e-300 + 2.2e-16 * 2.2e-16 * scale * scale;
-  lo -= 1e-14 * scale + 1e-300;
-  hi += 1e-14 * scale + 1e-300;
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid <= lo || mid >= hi) break;
-    if (sturm_count(a, b, m, mid, pivmin) > k) hi = mid; else lo = mid;
-  }
-  return 0.5 * (lo + hi);
+  for (; i < m; ++i) step(i);
 }
 
 // Inverse iteration for the eigenvector of (a, b) at lambda; LU with partial
@@ -1028,7 +1016,7 @@ template <int BT>
 __device__ __forceinline__ void ritz_values(const double* a, const double* b, int m, int nev, const double* hint, double* s_lam,
                             double* s_lo, double* s_hi, int* s_first, int* s_flag) {
   __shared__ double s_part[3][BT / 64];
-  __shared__ double s_bounds[4];
+  __shared__ double s_bounds[3];
   {
     double lo = INFINITY, hi = -INFINITY, sc = 0.0;
     for (int i = threadIdx.x; i < m; i += BT) {
@@ -1056,15 +1044,13 @@ __device__ __forceinline__ void ritz_values(const double* a, const double* b, in
       }
       s_bounds[0] = lo - 1e-14 * sc - 1e-300;
       s_bounds[1] = hi + 1e-14 * sc + 1e-300;
-      s_bounds[2] = 1e-300 + 2.2e-16 * 2.2e-16 * sc * sc;
-      s_bounds[3] = sc;
+      s_bounds[2] = sc;
     }
   }
   __syncthreads();
   const int G = BT / nev;
   const int g = threadIdx.x / G, r = threadIdx.x % G;
-  const double pivmin = s_bounds[2];
-  const double scale = s_bounds[3];
+  const double scale = s_bounds[2];
   if ((int)threadIdx.x < nev) {
     s_lo[threadIdx.x] = s_bounds[0];
     s_hi[threadIdx.x] = s_bounds[1];
@@ -1078,7 +1064,7 @@ __device__ __forceinline__ void ritz_values(const double* a, const double* b, in
     if (g < nev) {
       double x = hint_point(g, r);
       int cnt;
-      sturm_counts<1>(a, b, m, &x, pivmin, &cnt);
+      sturm_counts<1>(a, b, m, &x, &cnt);
       if (cnt > g) atomicMin(&s_first[g], r);
     }
     __syncthreads();
@@ -1099,7 +1085,7 @@ __device__ __forceinline__ void ritz_values(const double* a, const double* b, in
       const double lo = s_lo[g], hi = s_hi[g];
       double x = lo + (hi - lo) * (double)(r + 1) / (double)(G + 1);
       int cnt;
-      sturm_counts<1>(a, b, m, &x, pivmin, &cnt);
+      sturm_counts<1>(a, b, m, &x, &cnt);
       if (cnt > g) atomicMin(&s_first[g], r);
     }
     __syncthreads();
