@@ -1390,10 +1390,60 @@ __device__ __forceinline__ void combine_split(const T* VL, int64_t sL, int kl, c
 // for every l (lane l), the NW partials of each l are then added.  Update:
 // wave g sums sum_l c_l V_l[i] over vectors [g m/NW, (g+1) m/NW) for every
 // row i (lane), the NW partials of each row are then subtracted from w.
-// The LDS prefix (l < kl) has row stride sL, the rest of the basis n.
+// The LDS prefix (l < kl) has row stride sL, the rest of the basis n.  The
+// prefix and the rest are read in separate loops, each through a pointer
+// whose address space the compiler knows, so the prefix is read with LDS
+// instructions (a pointer that may point to either would be read with flat
+// loads, which wait on both the memory and the LDS counters).
+
+// <v, w> over rows [r0, r1); v and w each in one address space
 template <class T>
-__device__ __forceinline__ const T* basis_row(const T* VL, int64_t sL, int kl, const T* VG, int64_t n, int l) {
-  return l < kl ? VL + (size_t)l * sL : VG + (size_t)l * n;
+__device__ __forceinline__ double2 dot_rows(const T* __restrict__ v, const T* __restrict__ w, int64_t r0, int64_t r1) {
+  double cx = 0.0, cy = 0.0, dx = 0.0, dy = 0.0;
+  int64_t i = r0;
+#pragma unroll 4
+  for (; i + 1 < r1; i += 2) {
+    const T a = v[i], b = w[i], a1 = v[i + 1], b1 = w[i + 1];
+    cx += re_(a) * re_(b) + im_(a) * im_(b);
+    dx += re_(a1) * re_(b1) + im_(a1) * im_(b1);
+    if constexpr (!is_real_v<T>) {
+      cy += re_(a) * im_(b) - im_(a) * re_(b);
+      dy += re_(a1) * im_(b1) - im_(a1) * re_(b1);
+    }
+  }
+  if (i < r1) {
+    const T a = v[i], b = w[i];
+    cx += re_(a) * re_(b) + im_(a) * im_(b);
+    if constexpr (!is_real_v<T>) cy += re_(a) * im_(b) - im_(a) * re_(b);
+  }
+  return make_double2(cx + dx, cy + dy);
+}
+
+// sum_l c_l V_l[i] over l in [l0, l1), V_l = V + l * stride
+template <class T>
+__device__ __forceinline__ void comb_rows(const T* __restrict__ V, int64_t stride, const double2* __restrict__ cl, int l0,
+                                          int l1, int64_t i, double& ux, double& uy) {
+  double tx = 0.0, ty = 0.0;
+  int l = l0;
+#pragma unroll 4
+  for (; l + 1 < l1; l += 2) {
+    const double2 c0 = cl[l], c1 = cl[l + 1];
+    const T v0 = V[(size_t)l * stride + i], v1 = V[(size_t)(l + 1) * stride + i];
+    ux += re_(v0) * c0.x - im_(v0) * c0.y;
+    tx += re_(v1) * c1.x - im_(v1) * c1.y;
+    if constexpr (!is_real_v<T>) {
+      uy += re_(v0) * c0.y + im_(v0) * c0.x;
+      ty += re_(v1) * c1.y + im_(v1) * c1.x;
+    }
+  }
+  if (l < l1) {
+    const double2 c0 = cl[l];
+    const T v0 = V[(size_t)l * stride + i];
+    ux += re_(v0) * c0.x - im_(v0) * c0.y;
+    if constexpr (!is_real_v<T>) uy += re_(v0) * c0.y + im_(v0) * c0.x;
+  }
+  ux += tx;
+  uy += ty;
 }
 
 template <int BT, class T>
@@ -1404,25 +1454,12 @@ __device__ __forceinline__ void dots_part(const T* VL, int64_t sL, int kl, const
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t r0 = n * wv / NW, r1 = n * (wv + 1) / NW;
   for (int l = lane; l < m; l += 64) {
-    const T* vl = basis_row(VL, sL, kl, VG, n, l);
-    double cx = 0.0, cy = 0.0, dx = 0.0, dy = 0.0;
-    int64_t i = r0;
-#pragma unroll 4
-    for (; i + 1 < r1; i += 2) {
-      const T v = vl[i], b = w[i], v1 = vl[i + 1], b1 = w[i + 1];
-      cx += re_(v) * re_(b) + im_(v) * im_(b);
-      dx += re_(v1) * re_(b1) + im_(v1) * im_(b1);
-      if constexpr (!is_real_v<T>) {
-        cy += re_(v) * im_(b) - im_(v) * re_(b);
-        dy += re_(v1) * im_(b1) - im_(v1) * re_(b1);
-      }
-    }
-    if (i < r1) {
-      const T v = vl[i], b = w[i];
-      cx += re_(v) * re_(b) + im_(v) * im_(b);
-      if constexpr (!is_real_v<T>) cy += re_(v) * im_(b) - im_(v) * re_(b);
-    }
-    part[(size_t)wv * n + l] = make_double2(cx + dx, cy + dy);
+    double2 d;
+    if (l < kl)
+      d = dot_rows(VL + (size_t)l * sL, w, r0, r1);
+    else
+      d = dot_rows(VG + (size_t)l * n, w, r0, r1);
+    part[(size_t)wv * n + l] = d;
   }
   __syncthreads();
   for (int l = threadIdx.x; l < m; l += BT) {
@@ -1445,27 +1482,13 @@ __device__ __forceinline__ double update_part(const T* VL, int64_t sL, int kl, c
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int l0 = m * wv / NW, l1 = m * (wv + 1) / NW;
+  const int lm = l1 < kl ? l1 : kl;  // [l0, lm) in the LDS prefix, [max(l0, kl), l1) in memory
+  const int lg = l0 > kl ? l0 : kl;
   for (int64_t i = lane; i < n; i += 64) {
-    double ux = 0.0, uy = 0.0, tx = 0.0, ty = 0.0;
-    int l = l0;
-#pragma unroll 4
-    for (; l + 1 < l1; l += 2) {
-      const double2 c0 = cl[l], c1 = cl[l + 1];
-      const T v0 = basis_row(VL, sL, kl, VG, n, l)[i], v1 = basis_row(VL, sL, kl, VG, n, l + 1)[i];
-      ux += re_(v0) * c0.x - im_(v0) * c0.y;
-      tx += re_(v1) * c1.x - im_(v1) * c1.y;
-      if constexpr (!is_real_v<T>) {
-        uy += re_(v0) * c0.y + im_(v0) * c0.x;
-        ty += re_(v1) * c1.y + im_(v1) * c1.x;
-      }
-    }
-    if (l < l1) {
-      const double2 c0 = cl[l];
-      const T v0 = basis_row(VL, sL, kl, VG, n, l)[i];
-      ux += re_(v0) * c0.x - im_(v0) * c0.y;
-      if constexpr (!is_real_v<T>) uy += re_(v0) * c0.y + im_(v0) * c0.x;
-    }
-    part[(size_t)wv * n + i] = make_double2(ux + tx, uy + ty);
+    double ux = 0.0, uy = 0.0;
+    if (l0 < lm) comb_rows(VL, sL, cl, l0, lm, i, ux, uy);
+    if (lg < l1) comb_rows(VG, n, cl, lg, l1, i, ux, uy);
+    part[(size_t)wv * n + i] = make_double2(ux, uy);
   }
   __syncthreads();
   double p = 0.0;
@@ -1552,7 +1575,10 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
     VL = w + n;
     kl = a.kl;
     sL = lds_row_stride(n);
-    part = reinterpret_cast<double2*>((reinterpret_cast<uintptr_t>(VL + (size_t)kl * sL) + 15) & ~(uintptr_t)15);
+    // byte offset from the LDS base (an integer round trip of the address
+    // would hide from the compiler that part is in LDS)
+    const size_t poff = (off + ((size_t)n + (size_t)kl * sL) * sizeof(T) + 15) & ~(size_t)15;
+    part = reinterpret_cast<double2*>(reinterpret_cast<char*>(cl) + poff);
   } else {
     w = reinterpret_cast<T*>(a.wbuf) + base;
     VL = VG;
@@ -1570,16 +1596,21 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   };
   // seeded random start vector (real when T is real)
   {
-    T* V0 = kl > 0 ? VL : VG;
-    double p = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += BT) {
-      const uint64_t h = a.seed * 0x100000001B3ull + (uint64_t)c * 0x9E3779B97F4A7C15ull + 2 * (uint64_t)i;
-      const double xr = u01(h) - 0.5, xi = is_real_v<T> ? 0.0 : u01(h + 1) - 0.5;
-      V0[i] = mk<T>(xr, xi);
-      p += xr * xr + xi * xi;
-    }
-    const double nrm = sqrt(block_sum<BT>(p, red));
-    for (int64_t i = threadIdx.x; i < n; i += BT) V0[i] = mk<T>(re_(V0[i]) / nrm, im_(V0[i]) / nrm);
+    auto start = [&](T* V0) {
+      double p = 0.0;
+      for (int64_t i = threadIdx.x; i < n; i += BT) {
+        const uint64_t h = a.seed * 0x100000001B3ull + (uint64_t)c * 0x9E3779B97F4A7C15ull + 2 * (uint64_t)i;
+        const double xr = u01(h) - 0.5, xi = is_real_v<T> ? 0.0 : u01(h + 1) - 0.5;
+        V0[i] = mk<T>(xr, xi);
+        p += xr * xr + xi * xi;
+      }
+      const double nrm = sqrt(block_sum<BT>(p, red));
+      for (int64_t i = threadIdx.x; i < n; i += BT) V0[i] = mk<T>(re_(V0[i]) / nrm, im_(V0[i]) / nrm);
+    };
+    if (kl > 0)
+      start(VL);
+    else
+      start(VG);
   }
   __syncthreads();
   int m = 0;
