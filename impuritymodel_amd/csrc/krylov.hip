@@ -1191,81 +1191,6 @@ __device__ __forceinline__ void load_hrow(HRow<HR>& h, const int64_t* rowptr, co
   h.cnt_wave = wave_max_int(h.cnt);
 }
 
-// spmv_t with row threadIdx.x taken from registers
-template <int BT, int HR, class T>
-__device__ __forceinline__ double spmv_cached(const HRow<HR>& h, const int64_t* rowptr, const int32_t* col,
-                                              const double2* val, int64_t base, int64_t n, const T* x, T* y) {
-  double pa = 0.0;
-  if ((int64_t)threadIdx.x < n) {
-    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
-    // groups of four entries up to the wave's longest register row (a
-    // wave-uniform branch), every load of a group issued together (entries
-    // past a row's end hold 0 at column 0)
-#pragma unroll
-    for (int k0 = 0; k0 < HR; k0 += 4) {
-      if (k0 < h.cnt_wave) {
-        T v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = x[h.c[k0 + u]];
-#pragma unroll
-        for (int u = 0; u < 4; u += 2) {
-          const double2 h0 = h.v[k0 + u], h1 = h.v[k0 + u + 1];
-          ax += h0.x * re_(v[u]) - h0.y * im_(v[u]);
-          bx += h1.x * re_(v[u + 1]) - h1.y * im_(v[u + 1]);
-          if constexpr (!is_real_v<T>) {
-            ay += h0.x * im_(v[u]) + h0.y * re_(v[u]);
-            by += h1.x * im_(v[u + 1]) + h1.y * re_(v[u + 1]);
-          }
-        }
-      }
-    }
-    for (int64_t e = h.rest; e < h.end; ++e) {
-      const double2 hh = val[e];
-      const T v = x[col[e]];
-      ax += hh.x * re_(v) - hh.y * im_(v);
-      if constexpr (!is_real_v<T>) ay += hh.x * im_(v) + hh.y * re_(v);
-    }
-    ax += bx;
-    ay += by;
-    const int64_t i = threadIdx.x;
-    y[i] = mk<T>(ax, ay);
-    pa += ax * re_(x[i]) + ay * im_(x[i]);
-  }
-  for (int64_t i = threadIdx.x + BT; i < n; i += BT) {
-    const int64_t e0 = rowptr[base + i], e1 = rowptr[base + i + 1];
-    double ax = 0.0, ay = 0.0;
-    for (int64_t e = e0; e < e1; ++e) {
-      const double2 hh = val[e];
-      const T v = x[col[e]];
-      ax += hh.x * re_(v) - hh.y * im_(v);
-      if constexpr (!is_real_v<T>) ay += hh.x * im_(v) + hh.y * re_(v);
-    }
-    y[i] = mk<T>(ax, ay);
-    pa += ax * re_(x[i]) + ay * im_(x[i]);
-  }
-  return pa;
-}
-
-template <int BT, int HR, class T>
-__device__ __forceinline__ double three_term_cached(const HRow<HR>& h, const EigenArgs& a, int64_t base, int64_t n,
-                                                    const T* vj, const T* vp, bool has_prev, double bprev, T* w,
-                                                    double* red2, int& parity, double* al_out) {
-  const double al = block_sum1<BT>(spmv_cached<BT, HR, T>(h, a.rowptr, a.col, a.val, base, n, vj, w), red2, parity);
-  double p0 = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += BT) {
-    double xr = re_(w[i]) - al * re_(vj[i]);
-    double xi = im_(w[i]) - al * im_(vj[i]);
-    if (has_prev) {
-      xr -= bprev * re_(vp[i]);
-      xi -= bprev * im_(vp[i]);
-    }
-    w[i] = mk<T>(xr, xi);
-    p0 += xr * xr + xi * xi;
-  }
-  *al_out = al;
-  return block_sum1<BT>(p0, red2, parity);
-}
-
 // cl[l] = <V_l, w> for l in [l0, l0 + U) ∩ [l0, lend), one wave (l0 and
 // lend wave-uniform).  Every lane loads all U vectors' entries (indices
 // past lend clamped to lend - 1, their sums discarded), so the loads issue
@@ -1384,13 +1309,13 @@ __device__ __forceinline__ void combine_split(const T* VL, int64_t sL, int kl, c
   *xi = q;
 }
 
-// Gram-Schmidt over the split basis with the work shared by the waves
-// through partial sums in LDS (part: BT/64 rows of n), no per-vector wave
-// reductions.  Dots: wave g sums rows [g n/NW, (g+1) n/NW) of <V_l, w>
-// for every l (lane l), the NW partials of each l are then added.  Update:
-// wave g sums sum_l c_l V_l[i] over vectors [g m/NW, (g+1) m/NW) for every
-// row i (lane), the NW partials of each row are then subtracted from w.
-// The LDS prefix (l < kl) has row stride sL, the rest of the basis n.  The
+// Dots of the Gram-Schmidt pass over the split basis with the work shared
+// by the waves through partial sums in LDS (part: BT/64 rows of n), no
+// per-vector wave reductions: wave g sums rows [g n/NW, (g+1) n/NW) of
+// <V_l, w> for every l (lane l), the NW partials of each l are then added
+// into cl.  The update (update_rows) is owned by rows, so it needs no
+// partial sums.  The LDS prefix (l < kl) has row stride sL, the rest of the
+// basis n.  The
 // prefix and the rest are read in separate loops, each through a pointer
 // whose address space the compiler knows, so the prefix is read with LDS
 // instructions (a pointer that may point to either would be read with flat
@@ -1417,33 +1342,6 @@ __device__ __forceinline__ double2 dot_rows(const T* __restrict__ v, const T* __
     if constexpr (!is_real_v<T>) cy += re_(a) * im_(b) - im_(a) * re_(b);
   }
   return make_double2(cx + dx, cy + dy);
-}
-
-// sum_l c_l V_l[i] over l in [l0, l1), V_l = V + l * stride
-template <class T>
-__device__ __forceinline__ void comb_rows(const T* __restrict__ V, int64_t stride, const double2* __restrict__ cl, int l0,
-                                          int l1, int64_t i, double& ux, double& uy) {
-  double tx = 0.0, ty = 0.0;
-  int l = l0;
-#pragma unroll 4
-  for (; l + 1 < l1; l += 2) {
-    const double2 c0 = cl[l], c1 = cl[l + 1];
-    const T v0 = V[(size_t)l * stride + i], v1 = V[(size_t)(l + 1) * stride + i];
-    ux += re_(v0) * c0.x - im_(v0) * c0.y;
-    tx += re_(v1) * c1.x - im_(v1) * c1.y;
-    if constexpr (!is_real_v<T>) {
-      uy += re_(v0) * c0.y + im_(v0) * c0.x;
-      ty += re_(v1) * c1.y + im_(v1) * c1.x;
-    }
-  }
-  if (l < l1) {
-    const double2 c0 = cl[l];
-    const T v0 = V[(size_t)l * stride + i];
-    ux += re_(v0) * c0.x - im_(v0) * c0.y;
-    if constexpr (!is_real_v<T>) uy += re_(v0) * c0.y + im_(v0) * c0.x;
-  }
-  ux += tx;
-  uy += ty;
 }
 
 template <int BT, class T>
@@ -1475,35 +1373,129 @@ __device__ __forceinline__ void dots_part(const T* VL, int64_t sL, int kl, const
   __syncthreads();
 }
 
+// Update of the Gram-Schmidt pass by rows: wave g owns rows [g n/NW,
+// (g+1) n/NW) of w, in groups of 16 rows, each row on a quad of lanes that
+// splits the vectors (lane q of the quad takes l = q, q + 4, ...); the quad
+// adds its four sums by DPP and its lane 0 writes the row.  No partial sums
+// cross waves, so the pass ends with the barrier of the norm's sum.
 template <int BT, class T>
-__device__ __forceinline__ double update_part(const T* VL, int64_t sL, int kl, const T* VG, int m, int64_t n, T* w,
-                                              const double2* cl, double2* part, double* red2, int& parity) {
+__device__ __forceinline__ double update_rows(const T* VL, int64_t sL, int kl, const T* VG, int m, int64_t n, T* w,
+                                              const double2* cl, double* red2, int& parity) {
   constexpr int NW = BT / 64;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int l0 = m * wv / NW, l1 = m * (wv + 1) / NW;
-  const int lm = l1 < kl ? l1 : kl;  // [l0, lm) in the LDS prefix, [max(l0, kl), l1) in memory
-  const int lg = l0 > kl ? l0 : kl;
-  for (int64_t i = lane; i < n; i += 64) {
-    double ux = 0.0, uy = 0.0;
-    if (l0 < lm) comb_rows(VL, sL, cl, l0, lm, i, ux, uy);
-    if (lg < l1) comb_rows(VG, n, cl, lg, l1, i, ux, uy);
-    part[(size_t)wv * n + i] = make_double2(ux, uy);
-  }
-  __syncthreads();
+  const int64_t r0 = n * wv / NW, r1 = n * (wv + 1) / NW;
+  const int q = lane & 3;
+  const int lm = m < kl ? m : kl;           // vectors [0, lm) in the LDS prefix
+  const int lg = kl + ((q - kl) & 3);       // first l >= kl with l = q (mod 4)
   double p = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += BT) {
-    double ux = part[i].x, uy = part[i].y;
-#pragma unroll
-    for (int g = 1; g < NW; ++g) {
-      ux += part[(size_t)g * n + i].x;
-      uy += part[(size_t)g * n + i].y;
+  for (int64_t rb = r0; rb < r1; rb += 16) {
+    const int64_t i = rb + (lane >> 2);
+    const bool on = i < r1;
+    const int64_t ii = on ? i : r0;  // idle lanes read a valid row, their sums are dropped
+    double ux = 0.0, uy = 0.0, tx = 0.0, ty = 0.0;
+    int l = q;
+#pragma unroll 2
+    for (; l + 4 < lm; l += 8) {
+      const double2 c0 = cl[l], c1 = cl[l + 4];
+      const T v0 = VL[(size_t)l * sL + ii], v1 = VL[(size_t)(l + 4) * sL + ii];
+      ux += re_(v0) * c0.x - im_(v0) * c0.y;
+      tx += re_(v1) * c1.x - im_(v1) * c1.y;
+      if constexpr (!is_real_v<T>) {
+        uy += re_(v0) * c0.y + im_(v0) * c0.x;
+        ty += re_(v1) * c1.y + im_(v1) * c1.x;
+      }
     }
-    const double wx = re_(w[i]) - ux, wy = im_(w[i]) - uy;
-    w[i] = mk<T>(wx, wy);
-    p += wx * wx + wy * wy;
+    if (l < lm) {
+      const double2 c0 = cl[l];
+      const T v0 = VL[(size_t)l * sL + ii];
+      ux += re_(v0) * c0.x - im_(v0) * c0.y;
+      if constexpr (!is_real_v<T>) uy += re_(v0) * c0.y + im_(v0) * c0.x;
+    }
+    for (int lv = lg; lv < m; lv += 4) {
+      const double2 c0 = cl[lv];
+      const T v0 = VG[(size_t)lv * n + ii];
+      ux += re_(v0) * c0.x - im_(v0) * c0.y;
+      if constexpr (!is_real_v<T>) uy += re_(v0) * c0.y + im_(v0) * c0.x;
+    }
+    ux += tx;
+    uy += ty;
+    ux += dpp_move<0xB1, 0xF>(ux);  // quad_perm [1,0,3,2]
+    ux += dpp_move<0x4E, 0xF>(ux);  // quad_perm [2,3,0,1]
+    if constexpr (!is_real_v<T>) {
+      uy += dpp_move<0xB1, 0xF>(uy);
+      uy += dpp_move<0x4E, 0xF>(uy);
+    }
+    if (on && q == 0) {
+      const double wx = re_(w[i]) - ux, wy = im_(w[i]) - uy;
+      w[i] = mk<T>(wx, wy);
+      p += wx * wx + wy * wy;
+    }
   }
   return block_sum1<BT>(p, red2, parity);
+}
+
+// First half of a Lanczos step on the LDS layout: y = H v_j into y (thread
+// t's row of H from registers, entries in groups of four up to the wave's
+// longest register row, the rest of the row and rows from BT on from
+// memory), with v_j = x * s read from x = w (s =
+// 1/beta_{j-1}; the start vector itself at j = 0, s = 1), each thread also
+// storing its rows of v_j at vdst (null at j = 0).  Returns the partial
+// Re<v_j, y>.
+template <int BT, int HR, class T, class VP>
+__device__ __forceinline__ double spmv_scaled(const HRow<HR>& h, const int64_t* rowptr, const int32_t* col,
+                                              const double2* val, int64_t base, int64_t n, const T* x, double sc,
+                                              VP* vdst, T* y) {
+  double pa = 0.0;
+  auto row_out = [&](int64_t i, double ax, double ay) {
+    const T xi = x[i];
+    const double vr = re_(xi) * sc, vi = im_(xi) * sc;
+    if (vdst) vdst[i] = mk<T>(vr, vi);
+    ax *= sc;
+    ay *= sc;
+    y[i] = mk<T>(ax, ay);
+    pa += ax * vr + ay * vi;
+  };
+  if ((int64_t)threadIdx.x < n) {
+    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+#pragma unroll
+    for (int k0 = 0; k0 < HR; k0 += 4) {
+      if (k0 < h.cnt_wave) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = x[h.c[k0 + u]];
+#pragma unroll
+        for (int u = 0; u < 4; u += 2) {
+          const double2 h0 = h.v[k0 + u], h1 = h.v[k0 + u + 1];
+          ax += h0.x * re_(v[u]) - h0.y * im_(v[u]);
+          bx += h1.x * re_(v[u + 1]) - h1.y * im_(v[u + 1]);
+          if constexpr (!is_real_v<T>) {
+            ay += h0.x * im_(v[u]) + h0.y * re_(v[u]);
+            by += h1.x * im_(v[u + 1]) + h1.y * re_(v[u + 1]);
+          }
+        }
+      }
+    }
+    for (int64_t e = h.rest; e < h.end; ++e) {
+      const double2 hh = val[e];
+      const T v = x[col[e]];
+      ax += hh.x * re_(v) - hh.y * im_(v);
+      if constexpr (!is_real_v<T>) ay += hh.x * im_(v) + hh.y * re_(v);
+    }
+    row_out(threadIdx.x, ax + bx, ay + by);
+  }
+  for (int64_t i = threadIdx.x + BT; i < n; i += BT) {
+    const int64_t e0 = rowptr[base + i], e1 = rowptr[base + i + 1];
+    double ax = 0.0, ay = 0.0;
+    for (int64_t e = e0; e < e1; ++e) {
+      const double2 hh = val[e];
+      const T v = x[col[e]];
+      ax += hh.x * re_(v) - hh.y * im_(v);
+      if constexpr (!is_real_v<T>) ay += hh.x * im_(v) + hh.y * re_(v);
+    }
+    row_out(i, ax, ay);
+  }
+  return pa;
 }
 
 // LW: the tridiagonal and the Ritz work arrays live in LDS (a compile-time
@@ -1572,12 +1564,12 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   if constexpr (LV) {
     const size_t off = ((size_t)((char*)(pivb + (size_t)nev * L) - (char*)cl) + 15) & ~(size_t)15;
     w = reinterpret_cast<T*>(reinterpret_cast<char*>(cl) + off);
-    VL = w + n;
+    VL = w + 2 * n;  // after w and H v_j
     kl = a.kl;
     sL = lds_row_stride(n);
     // byte offset from the LDS base (an integer round trip of the address
     // would hide from the compiler that part is in LDS)
-    const size_t poff = (off + ((size_t)n + (size_t)kl * sL) * sizeof(T) + 15) & ~(size_t)15;
+    const size_t poff = (off + ((size_t)2 * n + (size_t)kl * sL) * sizeof(T) + 15) & ~(size_t)15;
     part = reinterpret_cast<double2*>(reinterpret_cast<char*>(cl) + poff);
   } else {
     w = reinterpret_cast<T*>(a.wbuf) + base;
@@ -1587,13 +1579,6 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   // with the LDS layout (small chains), each thread's row of H in registers
   HRow<hrow_len<BT>()> hrow;
   if constexpr (LV) load_hrow(hrow, a.rowptr, a.col, a.val, base, n);
-  auto step3 = [&](const T* vj, const T* vp, bool has_prev, double bprev, double* al) {
-    if constexpr (LV)
-      return three_term_cached<BT, hrow_len<BT>(), T>(hrow, a, base, n, vj, vp, has_prev, bprev, w, red2, parity,
-                                                      al);
-    else
-      return three_term<BT, T>(a, base, n, vj, vp, has_prev, bprev, w, red, al);
-  };
   // seeded random start vector (real when T is real)
   {
     auto start = [&](T* V0) {
@@ -1617,103 +1602,165 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   int converged = 0;
   // no Ritz check before min(check_start, n/2) steps (the last step is always checked)
   const int cstart = (int)(a.check_start < n / 2 ? a.check_start : n / 2);
-  for (int j = 0; j < mmax; ++j) {
-    // three-term step: w = H v_j - alpha_j v_j - beta_{j-1} v_{j-1}, with the
-    // big cancellation done before any Gram-Schmidt coefficient is formed
-    // (coefficients taken from H v_j itself would carry errors of order
-    // eps*|H v_j| and lose orthogonality geometrically)
-    const double bprev = j > 0 ? sb[j - 1] : 0.0;
-    double al, p0;
-    if (j + 1 <= kl)
-      p0 = step3(VL + (size_t)j * sL, VL + (size_t)(j > 0 ? j - 1 : 0) * sL, j > 0, bprev, &al);
-    else if (j == kl && j > 0)
-      p0 = step3(VG + (size_t)j * n, VL + (size_t)(j - 1) * sL, true, bprev, &al);
-    else
-      p0 = step3(VG + (size_t)j * n, VG + (size_t)(j > 0 ? j - 1 : 0) * n, j > 0, bprev, &al);
-    // full reorthogonalization against V_0..V_j; a second pass when the
-    // first removed more than half of |w|^2 (Daniel-Gragg-Kaufman-Stewart),
-    // always when the criterion is switched off
-    double p1;
-    if constexpr (LV) {
+  // Ritz pairs of T_m with m steps done and beta_m = b: eigenvalues by
+  // workgroup multisection, then thread k refines eigenvector k by inverse
+  // iteration; true when the chain is done (converged, or the last step)
+  auto ritz_check = [&](bool last, double b) -> bool {
+    // Ritz pairs of T_m: eigenvalues by workgroup multisection, then
+    // thread k refines eigenvector k by inverse iteration
+    const int ne = nev < m ? nev : m;
+    const bool have_prev = ne == hint_ne;
+    ritz_values<BT>(sa, sb, m, ne, have_prev ? s_hint : nullptr, s_lam, s_lo, s_hi, s_first, &s_flag);
+    // The eigenvector work is skipped while some Ritz value still moved by
+    // more than tol max(1, |lam|) since the previous check: its residual
+    // cannot be below tol yet (the Ritz value error is of the order of the
+    // squared residual over the gap), so the pair is checked next time.
+    if (threadIdx.x == 0) s_full = (last || have_prev) ? 1 : 0;
+    __syncthreads();
+    if (!last && have_prev && (int)threadIdx.x < ne &&
+        !(fabs(s_lam[threadIdx.x] - s_hint[threadIdx.x]) <= a.tol * fmax(1.0, fabs(s_lam[threadIdx.x]))))
+      atomicAnd(&s_full, 0);
+    __syncthreads();
+    if ((int)threadIdx.x < ne) s_hint[threadIdx.x] = s_lam[threadIdx.x];
+    hint_ne = ne;
+    if (threadIdx.x == 0) s_done = 1;
+    __syncthreads();
+    if (s_full) {
+      if ((int)threadIdx.x < nev && (int)threadIdx.x < m) {
+        const int k = threadIdx.x;
+        const double lam = s_lam[k];
+        double* y = Y + (size_t)k * ys;
+        double* tw = twb + (size_t)k * (LW ? 4 * ys : 5 * ys);
+        int* piv = pivb + (size_t)k * ys;
+        tri_inverse_iteration(sa, sb, m, lam, y, tw, tw + ys, tw + 2 * ys, tw + 3 * ys, piv,
+                              a.seed + 7919 * (uint64_t)k);
+        a.evals[(size_t)c * nev + k] = lam;
+        const double res = fabs(b * y[m - 1]);
+        if (!(res <= a.tol * fmax(1.0, fabs(lam)))) atomicAnd(&s_done, 0);
+      }
+      __syncthreads();
+      // orthogonalize inverse-iteration vectors of clustered eigenvalues
+      ortho_clustered(Y, ys, m, ne, s_lam);
+      __syncthreads();
+      if (s_done || last) {
+        converged = s_done;
+        return true;
+      }
+    }
+    return false;
+  };
+  auto check_due = [&](bool last) { return last || (m >= nev && m >= cstart && m % a.check_every == 0); };
+  if constexpr (LV) {
+    // Lanczos step on the LDS layout with five workgroup barriers: the sum
+    // of alpha after the SpMV, which also forms and stores v_j = w /
+    // beta_{j-1} (the start vector at j = 0); the sum of |w|^2 after the
+    // three-term update of each thread's own rows; the dots of the
+    // Gram-Schmidt pass and their partial sums; the sum of |w|^2 after its
+    // row-owned update.  Every thread has alpha and beta from the sums; the
+    // tridiagonal goes to LDS for the Ritz check, which synchronizes first.
+    constexpr int HR = hrow_len<BT>();
+    T* w2 = w + n;  // H v_j
+    double bprev = 0.0;
+    for (int j = 0; j < mmax; ++j) {
+      // three-term step: w = H v_j - alpha_j v_j - beta_{j-1} v_{j-1}, with
+      // the big cancellation done before any Gram-Schmidt coefficient is
+      // formed (coefficients taken from H v_j itself would carry errors of
+      // order eps*|H v_j| and lose orthogonality geometrically)
+      double pa;
+      if (j == 0)
+        pa = kl > 0 ? spmv_scaled<BT, HR, T, T>(hrow, a.rowptr, a.col, a.val, base, n, VL, 1.0, nullptr, w2)
+                    : spmv_scaled<BT, HR, T, T>(hrow, a.rowptr, a.col, a.val, base, n, VG, 1.0, nullptr, w2);
+      else if (j < kl)
+        pa = spmv_scaled<BT, HR, T, T>(hrow, a.rowptr, a.col, a.val, base, n, w, 1.0 / bprev, VL + (size_t)j * sL, w2);
+      else
+        pa = spmv_scaled<BT, HR, T, T>(hrow, a.rowptr, a.col, a.val, base, n, w, 1.0 / bprev, VG + (size_t)j * n, w2);
+      const double al = block_sum1<BT>(pa, red2, parity);
+      double p0 = 0.0;
+      auto three = [&](const T* vj, const T* vp, bool has_prev) {
+        for (int64_t i = threadIdx.x; i < n; i += BT) {
+          const T y = w2[i], v = vj[i];
+          double xr = re_(y) - al * re_(v), xi = im_(y) - al * im_(v);
+          if (has_prev) {
+            const T u = vp[i];
+            xr -= bprev * re_(u);
+            xi -= bprev * im_(u);
+          }
+          w[i] = mk<T>(xr, xi);
+          p0 += xr * xr + xi * xi;
+        }
+      };
+      if (j == 0) {
+        if (kl > 0)
+          three(VL, VL, false);
+        else
+          three(VG, VG, false);
+      } else if (j < kl) {
+        three(VL + (size_t)j * sL, VL + (size_t)(j - 1) * sL, true);
+      } else if (j == kl) {
+        three(VG + (size_t)j * n, VL + (size_t)(j - 1) * sL, true);
+      } else {
+        three(VG + (size_t)j * n, VG + (size_t)(j - 1) * n, true);
+      }
+      p0 = block_sum1<BT>(p0, red2, parity);
+      // full reorthogonalization against V_0..V_j; a second pass when the
+      // first removed more than half of |w|^2 (Daniel-Gragg-Kaufman-Stewart),
+      // always when the criterion is switched off
       dots_part<BT, T>(VL, sL, kl, VG, j + 1, n, w, part, cl);
-      p1 = update_part<BT, T>(VL, sL, kl, VG, j + 1, n, w, cl, part, red2, parity);
+      double p1 = update_rows<BT, T>(VL, sL, kl, VG, j + 1, n, w, cl, red2, parity);
       if (!a.dgks || p1 < 0.5 * p0) {
         dots_part<BT, T>(VL, sL, kl, VG, j + 1, n, w, part, cl);
-        p1 = update_part<BT, T>(VL, sL, kl, VG, j + 1, n, w, cl, part, red2, parity);
+        p1 = update_rows<BT, T>(VL, sL, kl, VG, j + 1, n, w, cl, red2, parity);
       }
-    } else {
+      const double b = sqrt(p1);
+      m = j + 1;
+      const bool last = (m == mmax) || b <= 1e-13 * (fabs(al) + bprev + 1e-300);
+      if (threadIdx.x == 0) {
+        alpha[j] = al;
+        sa[j] = al;
+        beta[j] = b;
+        sb[j] = b;
+      }
+      if (check_due(last)) {
+        __syncthreads();
+        if (ritz_check(last, b)) break;
+      }
+      if (last) break;
+      bprev = b;
+    }
+  } else {
+    for (int j = 0; j < mmax; ++j) {
+      // three-term step (as above), then full reorthogonalization
+      const double bprev = j > 0 ? sb[j - 1] : 0.0;
+      double al, p0;
+      if (j == 0)
+        p0 = three_term<BT, T>(a, base, n, VG, VG, false, 0.0, w, red, &al);
+      else
+        p0 = three_term<BT, T>(a, base, n, VG + (size_t)j * n, VG + (size_t)(j - 1) * n, true, bprev, w, red, &al);
       dots_split<BT, T>(VL, kl, VG, j + 1, n, w, cl);
-      p1 = update_split<BT, T>(VL, kl, VG, j + 1, n, w, cl, red);
+      double p1 = update_split<BT, T>(VL, kl, VG, j + 1, n, w, cl, red);
       if (!a.dgks || p1 < 0.5 * p0) {
         dots_split<BT, T>(VL, kl, VG, j + 1, n, w, cl);
         p1 = update_split<BT, T>(VL, kl, VG, j + 1, n, w, cl, red);
       }
-    }
-    if (threadIdx.x == 0) {
-      alpha[j] = al;
-      sa[j] = al;
-    }
-    const double b = sqrt(p1);
-    m = j + 1;
-    const bool last = (m == mmax) || b <= 1e-13 * (fabs(al) + bprev + 1e-300);
-    if (threadIdx.x == 0) {
-      beta[j] = b;
-      sb[j] = b;
-    }
-    __syncthreads();
-    if (last || (m >= nev && m >= cstart && m % a.check_every == 0)) {
-      // Ritz pairs of T_m: eigenvalues by workgroup multisection, then
-      // thread k refines eigenvector k by inverse iteration
-      const int ne = nev < m ? nev : m;
-      const bool have_prev = ne == hint_ne;
-      ritz_values<BT>(sa, sb, m, ne, have_prev ? s_hint : nullptr, s_lam, s_lo, s_hi, s_first, &s_flag);
-      // The eigenvector work is skipped while some Ritz value still moved by
-      // more than tol max(1, |lam|) since the previous check: its residual
-      // cannot be below tol yet (the Ritz value error is of the order of the
-      // squared residual over the gap), so the pair is checked next time.
-      if (threadIdx.x == 0) s_full = (last || have_prev) ? 1 : 0;
-      __syncthreads();
-      if (!last && have_prev && (int)threadIdx.x < ne &&
-          !(fabs(s_lam[threadIdx.x] - s_hint[threadIdx.x]) <= a.tol * fmax(1.0, fabs(s_lam[threadIdx.x]))))
-        atomicAnd(&s_full, 0);
-      __syncthreads();
-      if ((int)threadIdx.x < ne) s_hint[threadIdx.x] = s_lam[threadIdx.x];
-      hint_ne = ne;
-      if (threadIdx.x == 0) s_done = 1;
-      __syncthreads();
-      if (s_full) {
-        if ((int)threadIdx.x < nev && (int)threadIdx.x < m) {
-          const int k = threadIdx.x;
-          const double lam = s_lam[k];
-          double* y = Y + (size_t)k * ys;
-          double* tw = twb + (size_t)k * (LW ? 4 * ys : 5 * ys);
-          int* piv = pivb + (size_t)k * ys;
-          tri_inverse_iteration(sa, sb, m, lam, y, tw, tw + ys, tw + 2 * ys, tw + 3 * ys, piv,
-                                a.seed + 7919 * (uint64_t)k);
-          a.evals[(size_t)c * nev + k] = lam;
-          const double res = fabs(b * y[m - 1]);
-          if (!(res <= a.tol * fmax(1.0, fabs(lam)))) atomicAnd(&s_done, 0);
-        }
-        __syncthreads();
-        // orthogonalize inverse-iteration vectors of clustered eigenvalues
-        ortho_clustered(Y, ys, m, ne, s_lam);
-        __syncthreads();
-        if (s_done || last) {
-          converged = s_done;
-          break;
-        }
+      if (threadIdx.x == 0) {
+        alpha[j] = al;
+        sa[j] = al;
       }
-    }
-    if (last) break;
-    const double rb = 1.0 / b;
-    if (j + 1 < kl) {
-      T* vn = VL + (size_t)(j + 1) * sL;
-      for (int64_t i = threadIdx.x; i < n; i += BT) vn[i] = mk<T>(re_(w[i]) * rb, im_(w[i]) * rb);
-    } else {
+      const double b = sqrt(p1);
+      m = j + 1;
+      const bool last = (m == mmax) || b <= 1e-13 * (fabs(al) + bprev + 1e-300);
+      if (threadIdx.x == 0) {
+        beta[j] = b;
+        sb[j] = b;
+      }
+      __syncthreads();
+      if (check_due(last) && ritz_check(last, b)) break;
+      if (last) break;
+      const double rb = 1.0 / b;
       T* vn = VG + (size_t)(j + 1) * n;
       for (int64_t i = threadIdx.x; i < n; i += BT) vn[i] = mk<T>(re_(w[i]) * rb, im_(w[i]) * rb);
+      __syncthreads();
     }
-    __syncthreads();
   }
   // Ritz vectors x_k = sum_l y_k[l] V_l, normalized, stored complex
   double2* X = a.evecs + base * nev;
@@ -2900,18 +2947,18 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
     lv = false;
     a.kl = 0;
     if (a.lds_work && share > woff + static_lds) {
-      // w, then kl vectors of row stride sl, then the Gram-Schmidt partial
+      // w and H v_j, then kl vectors of row stride sl, then the Gram-Schmidt partial
       // sums (bt/64 rows of nmax complex)
       const int64_t sl = (nmax + 6) / 8 * 8 + 1;
       const int64_t part_bytes = (int64_t)(bt / 64) * nmax * 16 + 16;
-      const int64_t avail = (int64_t)share - (int64_t)static_lds - (int64_t)woff - nmax * (int64_t)esz - part_bytes;
+      const int64_t avail = (int64_t)share - (int64_t)static_lds - (int64_t)woff - 2 * nmax * (int64_t)esz - part_bytes;
       int64_t kl = avail > 0 ? std::min<int64_t>(avail / ((int64_t)esz * sl), std::min<int64_t>(max_steps, nmax)) : 0;
       const int64_t cap = g_tuning.eigen_lds_vectors;
       if (cap >= 0) kl = std::min(kl, cap);
       if (kl >= 8 || (cap >= 0 && kl >= 1)) {
         lv = true;
         a.kl = (int)kl;
-        shm = woff + (size_t)(nmax + kl * sl) * esz + (size_t)part_bytes;
+        shm = woff + (size_t)(2 * nmax + kl * sl) * esz + (size_t)part_bytes;
       }
     }
   };
