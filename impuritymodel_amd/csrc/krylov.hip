@@ -387,16 +387,21 @@ __global__ __launch_bounds__(64) void lanczos_lds_kernel(LanczosArgs a) {
 // NW times shorter and the chain's CU runs NW times as many waves.  Row t
 // of the SpMV (t < 64 NW) keeps up to RC entries of H in registers.
 //
-// A step takes five workgroup barriers: the SpMV reads w scaled by 1/beta
-// (so no pass writes v_j before it) into a second residual buffer, and
-// |w|^2 is posted by the three-term update and summed after the next
-// barrier.  LDS: V (K rows of stride s), w, w2, NW x K dot partials and
-// NW x s update partials.
+// The update is owned by rows: wave g updates rows [g n/NW, (g+1) n/NW) of
+// w (the rows whose dot partials it formed), each row on a quad of lanes
+// that splits the vectors (lane q takes l = q, q + 4, ...; DPP adds the
+// quad), with the coefficients added from the NW dot partials once per wave
+// into a row of its own.  A step takes four workgroup barriers: the SpMV
+// reads w scaled by 1/beta (so no pass writes v_j before it) into a second
+// residual buffer, |w|^2 is posted by the three-term update and summed
+// after the next barrier, and a Gram-Schmidt pass needs one barrier after
+// its dots and one for the sum of |w|^2.  LDS: V (K rows of stride s), w,
+// w2, NW x K dot partials and NW x K coefficients.
 
 size_t lanczos_ldsw_bytes(int64_t n, int K, int nw) {
   const int64_t kk = K < n ? K : n;
   const int64_t s = (n + 6) / 8 * 8 + 1;
-  return (size_t)(kk * s + 2 * s + nw * kk + nw * s) * sizeof(double2);
+  return (size_t)(kk * s + 2 * s + 2 * nw * kk) * sizeof(double2);
 }
 
 template <int NW>
@@ -423,8 +428,8 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
   double2* V = lds;
   double2* w = V + (size_t)K * s;
   double2* w2 = w + s;
-  double2* dotp = w2 + s;                // [NW][K]
-  double2* updp = dotp + (size_t)NW * K;  // [NW][s]
+  double2* dotp = w2 + s;                 // [NW][K]
+  double2* coefw = dotp + (size_t)NW * K;  // [NW][K], a row per wave
   const int64_t* rowptr = a.rowptr + base;
   // workgroup sums: post() leaves each wave's partial in a row of red
   // (rows used in turn), collect() adds them after a barrier; a row is
@@ -539,8 +544,10 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
       dotp[(size_t)wv * K + l] = make_double2(cx + dx, cy + dy);
     }
     __syncthreads();
-    // coefficient l = the NW dot partials of l, added by each lane that needs it
-    auto coef = [&](int l) {
+    // the coefficients, added from the NW dot partials by each wave into its
+    // own row (read back by the same wave only, after its own writes)
+    double2* cw = coefw + (size_t)wv * K;
+    for (int l = lane; l < m; l += 64) {
       double2 t = dotp[l];
 #pragma unroll
       for (int g = 1; g < NW; ++g) {
@@ -548,41 +555,43 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
         t.x += u.x;
         t.y += u.y;
       }
-      return t;
-    };
-    const int l0 = m * wv / NW, l1 = m * (wv + 1) / NW;
-    for (int64_t i = lane; i < n; i += 64) {
-      double ux = 0.0, uy = 0.0, tx = 0.0, ty = 0.0;
-      int l = l0;
-#pragma unroll 2
-      for (; l + 1 < l1; l += 2) {
-        const double2 cc = coef(l), c1 = coef(l + 1);
-        const double2 av = V[(size_t)l * s + i], a1 = V[(size_t)(l + 1) * s + i];
-        ux += av.x * cc.x - av.y * cc.y;
-        uy += av.x * cc.y + av.y * cc.x;
-        tx += a1.x * c1.x - a1.y * c1.y;
-        ty += a1.x * c1.y + a1.y * c1.x;
-      }
-      if (l < l1) {
-        const double2 cc = coef(l);
-        const double2 av = V[(size_t)l * s + i];
-        ux += av.x * cc.x - av.y * cc.y;
-        uy += av.x * cc.y + av.y * cc.x;
-      }
-      updp[(size_t)wv * s + i] = make_double2(ux + tx, uy + ty);
+      cw[l] = t;
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int q = lane & 3;
     double p = 0.0;
-    for (int64_t i = tid; i < n; i += T) {
-      double ux = updp[i].x, uy = updp[i].y;
-#pragma unroll
-      for (int g = 1; g < NW; ++g) {
-        ux += updp[(size_t)g * s + i].x;
-        uy += updp[(size_t)g * s + i].y;
+    for (int64_t rb = r0; rb < r1; rb += 16) {
+      const int64_t i = rb + (lane >> 2);
+      const bool on = i < r1;
+      const int64_t ii = on ? i : r0;  // idle lanes read a valid row, their sums are dropped
+      double ux = 0.0, uy = 0.0, tx = 0.0, ty = 0.0;
+      int l = q;
+#pragma unroll 2
+      for (; l + 4 < m; l += 8) {
+        const double2 c0 = cw[l], c1 = cw[l + 4];
+        const double2 v0 = V[(size_t)l * s + ii], v1 = V[(size_t)(l + 4) * s + ii];
+        ux += v0.x * c0.x - v0.y * c0.y;
+        uy += v0.x * c0.y + v0.y * c0.x;
+        tx += v1.x * c1.x - v1.y * c1.y;
+        ty += v1.x * c1.y + v1.y * c1.x;
       }
-      const double wx = w[i].x - ux, wy = w[i].y - uy;
-      w[i] = make_double2(wx, wy);
-      p += wx * wx + wy * wy;
+      if (l < m) {
+        const double2 c0 = cw[l];
+        const double2 v0 = V[(size_t)l * s + ii];
+        ux += v0.x * c0.x - v0.y * c0.y;
+        uy += v0.x * c0.y + v0.y * c0.x;
+      }
+      ux += tx;
+      uy += ty;
+      ux += dpp_move<0xB1, 0xF>(ux);  // quad_perm [1,0,3,2]
+      uy += dpp_move<0xB1, 0xF>(uy);
+      ux += dpp_move<0x4E, 0xF>(ux);  // quad_perm [2,3,0,1]
+      uy += dpp_move<0x4E, 0xF>(uy);
+      if (on && q == 0) {
+        const double wx = w[i].x - ux, wy = w[i].y - uy;
+        w[i] = make_double2(wx, wy);
+        p += wx * wx + wy * wy;
+      }
     }
     const int r = post(p);
     __syncthreads();
