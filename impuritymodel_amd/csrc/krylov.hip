@@ -60,6 +60,14 @@ __device__ __forceinline__ double block_sum1(double x, double* lds, int& parity)
   return r;
 }
 
+// Sum over the tpr lanes of a row (consecutive lanes of a quad), the same in
+// each of them; every lane of the wave active.
+__device__ __forceinline__ double row_lanes_sum(double x, int tpr) {
+  if (tpr > 1) x += dpp_move<0xB1, 0xF>(x);  // quad_perm [1,0,3,2]
+  if (tpr > 2) x += dpp_move<0x4E, 0xF>(x);  // quad_perm [2,3,0,1]
+  return x;
+}
+
 // y = H x for the rows of one chain; returns the partial Re<x, y>.
 template <int BT>
 __device__ __forceinline__ double spmv_rows(const int64_t* rowptr, const int32_t* col, const double2* val,
@@ -449,22 +457,28 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
     for (int g = 0; g < NW; ++g) t += red[r][g];
     return t;
   };
+  // a row of H on tpr = 4, 2 or 1 threads (as many as n rows leave room
+  // for): thread tid takes row tid / tpr and holds up to RC of its entries
+  // tid % tpr, tid % tpr + tpr, ... in registers; DPP adds the row's lanes
+  const int tpr = 4 * n <= T ? 4 : (2 * n <= T ? 2 : 1);
+  const int64_t hrow = tid / tpr;
   double2 hv[RC];
   int hc[RC];
   int64_t r_end = 0, r_rest = 0;
   int hn = 0;
-  if (tid < n) {
-    const int64_t e0 = rowptr[tid];
-    r_end = rowptr[tid + 1];
-    hn = (int)(r_end - e0 < RC ? r_end - e0 : RC);
-    r_rest = e0 + hn;
+  if (hrow < n) {
+    const int64_t e0 = rowptr[hrow] + tid % tpr;
+    r_end = rowptr[hrow + 1];
+    const int64_t mine = r_end > e0 ? (r_end - e0 + tpr - 1) / tpr : 0;
+    hn = (int)(mine < RC ? mine : RC);
+    r_rest = e0 + (int64_t)hn * tpr;
 #pragma unroll
     for (int k = 0; k < RC; ++k) {
-      hv[k] = k < hn ? a.val[e0 + k] : make_double2(0.0, 0.0);
-      hc[k] = k < hn ? a.col[e0 + k] : 0;
+      hv[k] = k < hn ? a.val[e0 + (int64_t)k * tpr] : make_double2(0.0, 0.0);
+      hc[k] = k < hn ? a.col[e0 + (int64_t)k * tpr] : 0;
     }
   }
-  // the wave's longest register row: groups of four entries past it are
+  // the wave's longest register share: groups of four entries past it are
   // skipped (a wave-uniform branch); inside a group every load is issued
   // unconditionally (entries past a row's end hold 0 at column 0)
   const int hn_wave = wave_max_int(hn);
@@ -472,8 +486,8 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
   // xj (when not null) receives sc x for those rows
   auto spmv = [&](const double2* x, double sc, double2* y, double2* xj) {
     double pa = 0.0;
-    if (tid < n) {
-      double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+    if (hrow < n) {
 #pragma unroll
       for (int k0 = 0; k0 < RC; k0 += 4) {
         if (k0 < hn_wave) {
@@ -489,17 +503,19 @@ __global__ __launch_bounds__(64 * NW) void lanczos_ldsw_kernel(LanczosArgs a) {
           }
         }
       }
-      for (int64_t e = r_rest; e < r_end; ++e) {
+      for (int64_t e = r_rest; e < r_end; e += tpr) {
         const double2 h = a.val[e];
         const double2 v = x[a.col[e]];
         ax += h.x * v.x - h.y * v.y;
         ay += h.x * v.y + h.y * v.x;
       }
-      ax = (ax + bx) * sc;
-      ay = (ay + by) * sc;
-      y[tid] = make_double2(ax, ay);
-      const double2 xi = make_double2(x[tid].x * sc, x[tid].y * sc);
-      if (xj) xj[tid] = xi;
+    }
+    ax = row_lanes_sum(ax + bx, tpr) * sc;
+    ay = row_lanes_sum(ay + by, tpr) * sc;
+    if (hrow < n && tid % tpr == 0) {
+      y[hrow] = make_double2(ax, ay);
+      const double2 xi = make_double2(x[hrow].x * sc, x[hrow].y * sc);
+      if (xj) xj[hrow] = xi;
       pa += ax * xi.x + ay * xi.y;
     }
     for (int64_t i = tid + T; i < n; i += T) {
@@ -1167,9 +1183,11 @@ __device__ __forceinline__ double three_term(const EigenArgs& a, int64_t base, i
   return block_sum<BT>(p0, red);
 }
 
-// The rows of H kept in registers: thread t < n holds up to HR entries of
-// row t (the rest of the row, and rows from BT on, are read from memory).
-// HR shrinks with the workgroup size, whose waves have fewer registers each.
+// The rows of H kept in registers.  A row gets tpr = 4, 2 or 1 threads (as
+// many as n rows leave room for in the workgroup); thread t takes row
+// t / tpr and holds up to HR of its entries t % tpr, t % tpr + tpr, ...
+// (the rest of the row, and rows from BT on when tpr = 1, are read from
+// memory).  The tpr partial products of a row are added by DPP.
 template <int BT>
 constexpr int hrow_len() {
   return BT <= 256 ? 16 : (BT <= 512 ? 8 : 4);
@@ -1180,21 +1198,31 @@ struct HRow {
   double2 v[HR];
   int c[HR];
   int cnt = 0, cnt_wave = 0;  // entries held; the wave's largest cnt
-  int64_t rest = 0, end = 0;
+  int64_t rest = 0, end = 0;  // this thread's next entry past the registers; the row's end
+  int tpr = 1;
 };
 
-template <int HR>
+template <int BT>
+__device__ __forceinline__ int threads_per_row(int64_t n) {
+  return 4 * n <= BT ? 4 : (2 * n <= BT ? 2 : 1);
+}
+
+template <int BT, int HR>
 __device__ __forceinline__ void load_hrow(HRow<HR>& h, const int64_t* rowptr, const int32_t* col, const double2* val,
                                           int64_t base, int64_t n) {
-  if ((int64_t)threadIdx.x < n) {
-    const int64_t e0 = rowptr[base + threadIdx.x];
-    h.end = rowptr[base + threadIdx.x + 1];
-    h.cnt = (int)(h.end - e0 < HR ? h.end - e0 : HR);
-    h.rest = e0 + h.cnt;
+  h.tpr = threads_per_row<BT>(n);
+  const int64_t r = threadIdx.x / h.tpr;
+  const int sub = threadIdx.x % h.tpr;
+  if (r < n) {
+    const int64_t e0 = rowptr[base + r] + sub;
+    h.end = rowptr[base + r + 1];
+    const int64_t mine = h.end > e0 ? (h.end - e0 + h.tpr - 1) / h.tpr : 0;
+    h.cnt = (int)(mine < HR ? mine : HR);
+    h.rest = e0 + (int64_t)h.cnt * h.tpr;
 #pragma unroll
     for (int k = 0; k < HR; ++k) {
-      h.v[k] = k < h.cnt ? val[e0 + k] : make_double2(0.0, 0.0);
-      h.c[k] = k < h.cnt ? col[e0 + k] : 0;
+      h.v[k] = k < h.cnt ? val[e0 + (int64_t)k * h.tpr] : make_double2(0.0, 0.0);
+      h.c[k] = k < h.cnt ? col[e0 + (int64_t)k * h.tpr] : 0;
     }
   }
   h.cnt_wave = wave_max_int(h.cnt);
@@ -1444,13 +1472,13 @@ __device__ __forceinline__ double update_rows(const T* VL, int64_t sL, int kl, c
   return block_sum1<BT>(p, red2, parity);
 }
 
-// First half of a Lanczos step on the LDS layout: y = H v_j into y (thread
-// t's row of H from registers, entries in groups of four up to the wave's
-// longest register row, the rest of the row and rows from BT on from
-// memory), with v_j = x * s read from x = w (s =
-// 1/beta_{j-1}; the start vector itself at j = 0, s = 1), each thread also
-// storing its rows of v_j at vdst (null at j = 0).  Returns the partial
-// Re<v_j, y>.
+// First half of a Lanczos step on the LDS layout: y = H v_j into y (the
+// rows of H from registers, HRow; entries in groups of four up to the
+// wave's longest register share, every load of a group issued together,
+// entries past a row's end holding 0 at column 0), with v_j = x * s read
+// from x = w (s = 1/beta_{j-1}; the start vector itself at j = 0, s = 1),
+// the first thread of each row also storing the row of v_j at vdst (null
+// at j = 0).  Returns the partial Re<v_j, y>.
 template <int BT, int HR, class T, class VP>
 __device__ __forceinline__ double spmv_scaled(const HRow<HR>& h, const int64_t* rowptr, const int32_t* col,
                                               const double2* val, int64_t base, int64_t n, const T* x, double sc,
@@ -1465,8 +1493,9 @@ __device__ __forceinline__ double spmv_scaled(const HRow<HR>& h, const int64_t* 
     y[i] = mk<T>(ax, ay);
     pa += ax * vr + ay * vi;
   };
-  if ((int64_t)threadIdx.x < n) {
-    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+  const int64_t r = threadIdx.x / h.tpr;
+  double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+  if (r < n) {
 #pragma unroll
     for (int k0 = 0; k0 < HR; k0 += 4) {
       if (k0 < h.cnt_wave) {
@@ -1485,24 +1514,26 @@ __device__ __forceinline__ double spmv_scaled(const HRow<HR>& h, const int64_t* 
         }
       }
     }
-    for (int64_t e = h.rest; e < h.end; ++e) {
+    for (int64_t e = h.rest; e < h.end; e += h.tpr) {
       const double2 hh = val[e];
       const T v = x[col[e]];
       ax += hh.x * re_(v) - hh.y * im_(v);
       if constexpr (!is_real_v<T>) ay += hh.x * im_(v) + hh.y * re_(v);
     }
-    row_out(threadIdx.x, ax + bx, ay + by);
   }
+  ax = row_lanes_sum(ax + bx, h.tpr);
+  if constexpr (!is_real_v<T>) ay = row_lanes_sum(ay + by, h.tpr);
+  if (r < n && (int)threadIdx.x % h.tpr == 0) row_out(r, ax, ay);
   for (int64_t i = threadIdx.x + BT; i < n; i += BT) {
     const int64_t e0 = rowptr[base + i], e1 = rowptr[base + i + 1];
-    double ax = 0.0, ay = 0.0;
+    double cx = 0.0, cy = 0.0;
     for (int64_t e = e0; e < e1; ++e) {
       const double2 hh = val[e];
       const T v = x[col[e]];
-      ax += hh.x * re_(v) - hh.y * im_(v);
-      if constexpr (!is_real_v<T>) ay += hh.x * im_(v) + hh.y * re_(v);
+      cx += hh.x * re_(v) - hh.y * im_(v);
+      if constexpr (!is_real_v<T>) cy += hh.x * im_(v) + hh.y * re_(v);
     }
-    row_out(i, ax, ay);
+    row_out(i, cx, cy);
   }
   return pa;
 }
@@ -1587,7 +1618,7 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   }
   // with the LDS layout (small chains), each thread's row of H in registers
   HRow<hrow_len<BT>()> hrow;
-  if constexpr (LV) load_hrow(hrow, a.rowptr, a.col, a.val, base, n);
+  if constexpr (LV) load_hrow<BT>(hrow, a.rowptr, a.col, a.val, base, n);
   // seeded random start vector (real when T is real)
   {
     auto start = [&](T* V0) {
