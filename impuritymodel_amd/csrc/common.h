@@ -85,10 +85,18 @@ struct DevBuf {
   }
 };
 
+// Host-to-device copies.  Up to 1 MiB go through a page-locked ring: the
+// bytes are copied into the ring at once and the device copy is queued
+// without the host waiting for it (a pageable source makes the runtime
+// stage and wait); a ring slot is reused only after stream_sync of its
+// stream.  Larger copies, copies while the stream is being captured into a
+// graph, and copies that find the ring full go straight from src.
+void h2d_bytes(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 template <class T>
 inline void h2d(DevBuf& b, const T* src, size_t n, hipStream_t s) {
   b.reserve(n * sizeof(T));
-  if (n) HIP_CHECK(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+  if (n) h2d_bytes(b.p, src, n * sizeof(T), s);
 }
 
 // Several host arrays sent in one copy: add() places an array at a 256-byte

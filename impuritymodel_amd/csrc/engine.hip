@@ -87,8 +87,56 @@ void d2h_bytes(void* dst, const void* src, size_t bytes, hipStream_t s) {
   st.pending.push_back(PendingRead{dst, slot, bytes, s});
 }
 
+// ------------------------------------------------------------------ staged small uploads
+
+namespace {
+struct Uploads {
+  std::mutex mu;
+  char* ring = nullptr;
+  size_t cap = 0, pos = 0;
+  std::vector<hipStream_t> users;  // streams with queued copies from the ring
+};
+Uploads& uploads() {
+  static Uploads* u = new Uploads;
+  return *u;
+}
+constexpr size_t kUpMax = 1 << 20;    // uploads up to this size are staged
+constexpr size_t kUpRing = 16 << 20;  // pinned ring
+}  // namespace
+
+void h2d_bytes(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (!bytes) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_CHECK(hipStreamIsCapturing(s, &cs));
+  if (bytes <= kUpMax && cs == hipStreamCaptureStatusNone) {
+    Uploads& u = uploads();
+    std::lock_guard<std::mutex> lk(u.mu);
+    if (!u.ring) {
+      HIP_CHECK(hipHostMalloc((void**)&u.ring, kUpRing, hipHostMallocDefault));
+      u.cap = kUpRing;
+    }
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (u.pos + need <= u.cap) {
+      char* slot = u.ring + u.pos;
+      u.pos += need;
+      std::memcpy(slot, src, bytes);
+      if (std::find(u.users.begin(), u.users.end(), s) == u.users.end()) u.users.push_back(s);
+      HIP_CHECK(hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, s));
+      return;
+    }
+  }
+  HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+}
+
 void stream_sync(hipStream_t s) {
   HIP_CHECK(hipStreamSynchronize(s));
+  {
+    // the ring's copies on s are done; the ring is free once no stream holds it
+    Uploads& u = uploads();
+    std::lock_guard<std::mutex> lk(u.mu);
+    u.users.erase(std::remove(u.users.begin(), u.users.end(), s), u.users.end());
+    if (u.users.empty()) u.pos = 0;
+  }
   Staging& st = staging();
   std::lock_guard<std::mutex> lk(st.mu);
   size_t keep = 0;
