@@ -30,7 +30,7 @@ DEFAULTS = {
     "lanczos_wave_max_n": 4096,
     "lanczos_wg256_max_n": 65536,
     "eigen_bt": 0,
-    "eigen_check_every": 8,
+    "eigen_check_every": 12,
     "eigen_wave_max_n": 0,
     "spmv_lpr": 8,
     "lanczos_graph_steps": 16,
@@ -42,7 +42,7 @@ DEFAULTS = {
 CANDIDATES = {
     "lanczos_wave_max_n": [512, 1024, 4096, 16384],
     "eigen_bt": [256, 512, 1024],
-    "eigen_check_every": [4, 8, 16],
+    "eigen_check_every": [4, 8, 12, 16],
     "eigen_wave_max_n": [0, 256, 1024],
     "spmv_lpr": [4, 8, 16, 32],
 }

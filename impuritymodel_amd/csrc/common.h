@@ -141,7 +141,7 @@ struct Tuning {
   int64_t eigen_wave_max_n = 0;         // eigensolver chains up to this size: one wave each (off)
   int64_t eigen_real = 1;               // real Krylov vectors when H is real
   int64_t eigen_dgks = 1;               // second Gram-Schmidt pass only when needed
-  int64_t eigen_check_every = 8;        // Ritz convergence check interval
+  int64_t eigen_check_every = 12;       // Ritz convergence check interval
   int64_t eigen_check_start = 48;       // no Ritz check before min(this, n/2) steps
   int64_t flat_min_n = 32768;           // chains above this size use the grid-wide (flat) kernels
   int64_t spmv_lpr = 8;                 // lanes per CSR row in the flat SpMV (4..64)

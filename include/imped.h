@@ -62,7 +62,8 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * chains up to this size get one wave), "eigen_bt" (threads per larger
  * eigensolver chain: 256, 512, 1024; 0, the default: 1024 when the launch's
  * largest chain has more than 512 rows, else 512), "eigen_real", "eigen_dgks",
- * "eigen_check_every", "eigen_check_start" (no Ritz convergence check before
+ * "eigen_check_every" (Ritz convergence check every this many steps, 12 by
+ * default), "eigen_check_start" (no Ritz convergence check before
  * min(this, n/2) steps of an n-row chain, 48 by default; the last step is
  * always checked), "flat_min_n" (chains above it use the grid-wide
  * kernels), "spmv_lpr" (lanes per row of the grid-wide SpMV: 4..64),

@@ -3,8 +3,9 @@
 A small ensemble (3 members, the bench's 8-512-512-512-1000 architecture,
 16k spectra, 2500 epochs) is trained by the library's kernels on exact
 spectra of the XAS family, its
-trust threshold calibrated on held-out exact spectra to 0.8 x the 1e-2 L1
-bar.  The fitting loop then runs the same walkers (same seeds) with and
+trust threshold calibrated on held-out exact spectra to 0.7 x the 1e-2 L1
+bar (the bench's 5-member ensemble uses 0.8; three members disagree less
+reliably, and at 0.8 a served spectrum near the bar has crossed it).  The fitting loop then runs the same walkers (same seeds) with and
 without it.  With the surrogate:
 
 * every spectrum it serves is within L1 1e-2 of exact ED (recomputed here);
@@ -35,7 +36,7 @@ def test_surrogate_gain_over_exact_only_loop(gpu):
     losses = sur.fit(P, Y, epochs=2500, seed=1)
     print("training: %d Adam steps in %.1f s, final loss %.3g" % (len(losses), time.perf_counter() - t0,
                                                                   float(losses[-20:].mean())), flush=True)
-    err, dis = sur.calibrate(Pv, Yv, bar=0.8e-2)
+    err, dis = sur.calibrate(Pv, Yv, bar=0.7e-2)
     trusted = dis < sur.threshold
     print("validation: median L1 %.4f, trusted %.3f" % (float(np.median(err)), float(trusted.mean())), flush=True)
     assert trusted.mean() > 0.2
