@@ -124,20 +124,28 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
     __syncthreads();
   }
   // accumulator r of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32
+  // (the column's bias read once and added before the first store)
   const int64_t cz = (int64_t)z * g.c_z;
   const int n = n0 + wn + (lane & 31);
+  constexpr bool with_bias = EPI == EPI_BIAS_GELU || EPI == EPI_BIAS;
+  const float bn = with_bias && n < g.N ? g.bias[(int64_t)z * g.bias_z + n] : 0.0f;
+  float zr[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    zr[r] = with_bias ? acc[r] + bn : acc[r];
+    asm volatile("" : "+v"(zr[r]));  // no instruction: keeps the sum here, out of the branches below
+  }
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int m = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     if (m >= g.M || n >= g.N) continue;
     const int64_t o = cz + (int64_t)m * g.ldc + n;
-    const float v = acc[r];
+    const float v = zr[r];
     if constexpr (EPI == EPI_BIAS_GELU) {
-      const float zv = v + g.bias[(int64_t)z * g.bias_z + n];
-      g.Cf[o] = zv;
-      g.Cb[o] = f2bf(gelu(zv));
+      g.Cf[o] = v;
+      g.Cb[o] = f2bf(gelu(v));
     } else if constexpr (EPI == EPI_BIAS) {
-      g.Cf[o] = v + g.bias[(int64_t)z * g.bias_z + n];
+      g.Cf[o] = v;
     } else if constexpr (EPI == EPI_DGELU) {
       g.Cb[o] = f2bf(v * gelu_grad(g.aux[o]));
     } else {
@@ -255,15 +263,26 @@ __global__ __launch_bounds__(256) void gemm_fwd_kernel(GemmArgs g) {
     }
     __syncthreads();
   }
-  // accumulator r of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32
+  // accumulator r of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+  // The column's bias is read once and added to every accumulator before the
+  // first store: read after each store (as a pointer that may alias the
+  // output would be), or first used inside the per-row branches, it puts a
+  // wait for every earlier store (stores count in vmcnt) between the stores.
   const int64_t cz = (int64_t)z * g.c_z;
   const int n = n0 + wn + (lane & 31);
+  const float bn = n < g.N ? g.bias[(int64_t)z * g.bias_z + n] : 0.0f;
+  float zr[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    zr[r] = acc[r] + bn;
+    asm volatile("" : "+v"(zr[r]));  // no instruction: keeps the sum here, out of the branches below
+  }
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int m = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     if (m >= g.M || n >= g.N) continue;
     const int64_t o = cz + (int64_t)m * g.ldc + n;
-    const float zv = acc[r] + g.bias[(int64_t)z * g.bias_z + n];
+    const float zv = zr[r];
     if (EPI != EPI_BIAS_GELU || g.Cf) g.Cf[o] = zv;  // hidden layers of a prediction keep no pre-activation
     if constexpr (EPI == EPI_BIAS_GELU) g.Cb[o] = f2bf(gelu(zv));
   }
