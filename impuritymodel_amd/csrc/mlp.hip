@@ -35,12 +35,32 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// GELU(x) = x Phi(x) and its derivative Phi(x) + x phi(x)
+// GELU(x) = x Phi(x) and its derivative Phi(x) + x phi(x), Phi(x) = (1 +
+// erf(x / sqrt 2)) / 2 (training: full-precision erf)
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float x) {
   const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
   const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
   return cdf + x * pdf;
+}
+
+// erf by the rational approximation of Abramowitz and Stegun, Handbook of
+// Mathematical Functions, eq. 7.1.26 (after Hastings): erf(x) = 1 - (a1 t +
+// a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2), t = 1 / (1 + p x) for x >= 0,
+// odd in x, absolute error at most 1.5e-7 (5.3e-7 in FP32 arithmetic) -- far
+// below the BF16 rounding (2^-8 relative) of the activations it feeds, at a
+// fraction of the instructions of the full-precision erf.  Inference only
+// (the epilogue of a hidden layer evaluates sixteen per lane); training keeps
+// gelu above, so a trained network does not depend on it.
+__device__ __forceinline__ float erf_as(float x) {
+  const float ax = fabsf(x);
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
+  const float poly =
+      t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+  return copysignf(1.0f - poly * __expf(-ax * ax), x);
+}
+__device__ __forceinline__ float gelu_inference(float x) {
+  return 0.5f * x * (1.0f + erf_as(x * 0.70710678118654752f));
 }
 
 enum { EPI_BIAS_GELU = 0, EPI_BIAS = 1, EPI_DGELU = 2, EPI_F32 = 3 };
@@ -284,7 +304,7 @@ __global__ __launch_bounds__(256) void gemm_fwd_kernel(GemmArgs g) {
     const int64_t o = cz + (int64_t)m * g.ldc + n;
     const float zv = zr[r];
     if (EPI != EPI_BIAS_GELU || g.Cf) g.Cf[o] = zv;  // hidden layers of a prediction keep no pre-activation
-    if constexpr (EPI == EPI_BIAS_GELU) g.Cb[o] = f2bf(gelu(zv));
+    if constexpr (EPI == EPI_BIAS_GELU) g.Cb[o] = f2bf(g.Cf ? gelu(zv) : gelu_inference(zv));
   }
 }
 
@@ -785,7 +805,7 @@ void mlp_set_params(Mlp& net, const float* params, hipStream_t s) {
 void mlp_forward(Mlp& net, const float* x_host, int B, float* y_host, hipStream_t s) {
   alloc_acts(net, B);
   upload_batch_input(net, x_host, B, s);
-  forward(net, B, s);
+  forward(net, B, s, nullptr, true);  // the network's outputs as predictions form them
   d2h(y_host, net.out.p, (size_t)net.E * B * net.dims[net.L], s);
   stream_sync(s);
 }

@@ -348,7 +348,10 @@ int imped_mlp_param_count(imped_mlp net, int64_t* per_member);
 /* Loads all members' parameters [n_members][P] and resets the Adam state. */
 int imped_mlp_set_params(imped_mlp net, const float* params);
 int imped_mlp_get_params(imped_mlp net, float* params);
-/* y[e][b][dims[L]] = network e applied to x[b][dims[0]] (inputs rounded to BF16). */
+/* y[e][b][dims[L]] = network e applied to x[b][dims[0]] (inputs rounded to BF16),
+ * as predictions form it: hidden-layer GELU with erf by Abramowitz and Stegun
+ * 7.1.26 (absolute error <= 1.5e-7, below the BF16 rounding of the
+ * activations); training uses the full-precision erf. */
 int imped_mlp_forward(imped_mlp net, const float* x, int batch, float* y);
 /* Forward pass plus the ensemble statistics of each sample b, on the device:
  * member spectra m_e = max(y_e / scale, 0) / sum_d weights[d] max(y_e / scale, 0),
