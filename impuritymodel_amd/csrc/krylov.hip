@@ -752,27 +752,21 @@ __device__ __forceinline__ void tri_inverse_iteration(const double* __restrict__
     const double sub = b[i];
     const double dn = a[i + 1] - lambda;
     const double un = i + 1 < m - 1 ? b[i + 1] : 0.0;
-    if (fabs(d) >= fabs(sub)) {
-      if (d == 0.0) d = tiny;
-      const double mult = sub * rcp_nr(d);
-      dd[i] = d;
-      uu[i] = u;
-      ww[i] = 0.0;
-      ll[i] = mult;
-      piv[i] = 0;
-      d = dn - mult * u;
-      u = un;
-    } else {
-      // rows i and i+1 interchanged
-      const double mult = d * rcp_nr(sub);
-      dd[i] = sub;
-      uu[i] = dn;
-      ww[i] = un;
-      ll[i] = mult;
-      piv[i] = 1;
-      d = u - mult * dn;
-      u = -mult * un;
-    }
+    // rows i and i+1 interchanged when |d| < |sub|; both cases by selects
+    // (the lanes of different eigenvalues choose differently, and branches
+    // would run both cases one after the other), one reciprocal
+    const bool sw = !(fabs(d) >= fabs(sub));
+    const double pv = sw ? sub : (d == 0.0 ? tiny : d);
+    const double mult = (sw ? d : sub) * rcp_nr(pv);
+    dd[i] = pv;
+    uu[i] = sw ? dn : u;
+    ww[i] = sw ? un : 0.0;
+    ll[i] = mult;
+    piv[i] = sw ? 1 : 0;
+    const double d_keep = dn - mult * u, d_swap = u - mult * dn;
+    const double u_swap = -mult * un;
+    d = sw ? d_swap : d_keep;
+    u = sw ? u_swap : un;
   }
   if (d == 0.0) d = tiny;
   dd[m - 1] = d;
@@ -792,15 +786,11 @@ __device__ __forceinline__ void tri_inverse_iteration(const double* __restrict__
     double yi = y[0];
 #pragma unroll 8
     for (int i = 0; i < m - 1; ++i) {
-      double yn = y[i + 1];
-      if (piv[i]) {
-        const double t = yi;
-        yi = yn;
-        yn = t;
-      }
-      yn -= ll[i] * yi;
-      y[i] = yi;
-      yi = yn;
+      const double y1 = y[i + 1];
+      const bool sw = piv[i] != 0;
+      const double top = sw ? y1 : yi, bot = sw ? yi : y1;
+      y[i] = top;
+      yi = bot - ll[i] * top;
     }
     y[m - 1] = yi;
     // back substitution with U (bandwidth 3); y1, y2 = y[i+1], y[i+2]
