@@ -134,12 +134,14 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
     stage<BK>(B, g.ldb, g.N, g.K, n0, k0, Bs, t);
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < TK; kk += 8) {
-      // lane l supplies row l%32 and k = kk + 4*(l/32) .. +3 of each operand
-      const int kq = kk + 4 * (lane >> 5);
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(&As[wm + (lane & 31)][kq]);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(&Bs[wn + (lane & 31)][kq]);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    for (int kk = 0; kk < TK; kk += 16) {
+      // 16-deep steps (gfx950): lane l supplies row l%32 and k = kk + 8*(l/32) .. +7
+      // of each operand (rows of LK = 40 elements keep every read 16-byte aligned)
+      const int kq = kk + 8 * (lane >> 5);
+      const uint4 a = *reinterpret_cast<const uint4*>(&As[wm + (lane & 31)][kq]);
+      const uint4 b = *reinterpret_cast<const uint4*>(&Bs[wn + (lane & 31)][kq]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                    0, 0, 0);
     }
     __syncthreads();
   }
