@@ -748,7 +748,8 @@ def run_rixs(args, torch):
                         "complex128",
             "eigenstates": n_kept,
             "out_going_chains": int(n_kept * 9 * len(w_in)),
-            "stage_s": {k: round(v, 4) for k, v in tim.items()},
+            "stage_s": {k: round(float(v), 4) for k, v in tim.items() if np.ndim(v) == 0},
+            "chain_n_range": {k: [int(np.min(v)), int(np.max(v))] for k, v in tim.items() if np.ndim(v) > 0 and len(v)},
             "plain_per_energy_s": round(t_plain, 3),
             "speedup_vs_plain": round(t_plain / (dt / args.steps), 2),
             "map_rel_diff_vs_plain": rel,
