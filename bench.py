@@ -769,6 +769,21 @@ def run_batched128(args, torch):
     params = batched_params(args.models, seed=0)
     models = [model_from_params(p) for p in params]
     omega = np.linspace(-25.0, 25.0, 2000)
+    n_orb = max(m.n_spin_orbitals for m in models)
+    gb_ops = Ops([m.hOp for m in models], [m.restrictions for m in models], n_orb)
+    gb = Batch(gb_ops, np.arange(len(models), dtype=np.int32))
+    gb.set_seeds(ground_bases(models, gb_ops.W))
+    gb.build(fixed_basis=True)
+    tuned = None
+    if not args.no_autotune:
+        # untimed: the eigensolver's launch shapes (threads per chain, Ritz
+        # check interval, one-wave bucket) timed on this workload's own ground
+        # chains; the timed steps run with the fastest of each
+        from impuritymodel_amd import autotune
+
+        t_tune = time.perf_counter()
+        table = autotune.autotune(eigen_batch=gb, nev=3, repeats=2)
+        tuned = {"seconds": round(time.perf_counter() - t_tune, 3), "table": table}
     for _ in range(args.warmup):
         res = xas_batch(models, omega=omega, krylovSize=200)
     torch.cuda.synchronize()
@@ -790,11 +805,6 @@ def run_batched128(args, torch):
     ROCTX.pop()
     kernels = {k: engine.timing_query(k) for k in KERNELS}
     engine.timing_enable(False)
-    n_orb = max(m.n_spin_orbitals for m in models)
-    gb_ops = Ops([m.hOp for m in models], [m.restrictions for m in models], n_orb)
-    gb = Batch(gb_ops, np.arange(len(models), dtype=np.int32))
-    gb.set_seeds(ground_bases(models, gb_ops.W))
-    gb.build(fixed_basis=True)
     roofline = _roofline(kernels, {
         "eigen": args.steps * algorithmic_bytes_eigen(gb.chain_n, gb.chain_nnz, res.eig_steps),
         "lanczos": args.steps * algorithmic_bytes_lanczos(res.chain_dims, res.chain_nnz, res.kout)})
@@ -838,6 +848,7 @@ def run_batched128(args, torch):
             "plain_vs_batched_max_rel_diff": plain_diff,
             "kernel_timing": "HIP events around every launch in a repeat of the timed steps",
             "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kernels.items() if v[1]},
+            "autotune": tuned if tuned is not None else "off (--no-autotune): default launch shapes",
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -858,6 +869,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=2500)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched128: time with the default launch shapes instead of the autotuned ones")
     ap.add_argument("--cpu-workers", type=int, default=0)
     ap.add_argument("--no-extras", action="store_true", help="large workload: skip the XPS/NIXS extras")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
