@@ -212,10 +212,29 @@ __global__ __launch_bounds__(256) void gemm_fwd_kernel(GemmArgs g) {
   extern __shared__ __align__(16) uint16_t fsm[];
   uint16_t(*As)[FLK] = reinterpret_cast<uint16_t(*)[FLK]>(fsm);
   uint16_t(*Bs)[FLK] = reinterpret_cast<uint16_t(*)[FLK]>(fsm + TM * FLK);
-  const int z = blockIdx.z;
+  // Tile order for the eight XCDs, which take the workgroups round-robin in
+  // launch order (x fastest) and each have their own L2: with the launch
+  // order, XCD i got output-column tile i of every row tile and member, so
+  // each XCD fetched every activation tile (eight fetches per tile).
+  // Renumbered so that each XCD takes one contiguous run of tiles, ordered
+  // member, column tile, row tile (rows fastest): an XCD's run covers whole
+  // columns of one or two members, every weight tile is fetched by one XCD
+  // and every activation tile by about two.  Same arithmetic per tile.
+  int z = blockIdx.z, bx = blockIdx.x, by = blockIdx.y;
+  {
+    const int gx = gridDim.x, gy = gridDim.y, tiles = gx * gy * gridDim.z;
+    if ((tiles & 7) == 0) {
+      const int l = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+      const int q = (l & 7) * (tiles >> 3) + (l >> 3);  // a bijection of [0, tiles) when 8 | tiles
+      z = q / (gx * gy);
+      const int r = q - z * gx * gy;
+      bx = r / gy;
+      by = r - bx * gy;
+    }
+  }
   const uint16_t* A = AF ? nullptr : g.A + (int64_t)z * g.a_z;
   const uint16_t* B = g.B + (int64_t)z * g.b_z;
-  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int m0 = by * TM, n0 = bx * TN;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   f32x16 acc;
