@@ -143,6 +143,7 @@ struct Tuning {
   int64_t eigen_dgks = 1;               // second Gram-Schmidt pass only when needed
   int64_t eigen_check_every = 12;       // Ritz convergence check interval
   int64_t eigen_check_start = 48;       // no Ritz check before min(this, n/2) steps
+  int64_t eigen_dense_max_n = 0;        // eigensolver chains up to this size (<= 112): dense solver (0: off)
   int64_t flat_min_n = 32768;           // chains above this size use the grid-wide (flat) kernels
   int64_t spmv_lpr = 8;                 // lanes per CSR row in the flat SpMV (4..64)
   int64_t real_values = 1;              // grid-wide SpMV on a real CSR reads 8-byte real values

@@ -1819,6 +1819,232 @@ __global__ __launch_bounds__(BT) void eigen_kernel(EigenArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ dense eigensolver
+//
+// Lowest nev eigenpairs of a small chain without a Krylov iteration.  H goes
+// into LDS as its packed lower triangle and is reduced to a real symmetric
+// tridiagonal T = Q^H H Q by n - 1 Householder reflectors H_k = I - tau_k
+// v_k v_k^H (v_k[0] = 1, acting on rows k+1..n-1), each mapping column k
+// below the diagonal to beta_k e_1 with beta_k real (beta = -sign(Re x_0)
+// |x|, tau = (beta - x_0) / beta, v = x / (x_0 - beta)).  Each reflector
+// updates the trailing block as A <- A - v w^H - w v^H with y = A v and
+// w = tau y - |tau|^2 (v^H y) v / 2 (v^H y is real for Hermitian A).  The
+// eigenvalues of T come from the Sturm multisection and its eigenvectors z
+// from the inverse iteration of the Krylov eigensolver's Ritz pairs; the
+// eigenvectors of H are x = H_0 H_1 ... H_{n-2} z.  The cost is fixed
+// (n - 1 reflectors, no convergence test), and degenerate eigenvalues come
+// out exactly, with orthogonal vectors.
+constexpr int kDenseMaxN = 112;
+
+// Offset of (i, j), i >= j, in the column-major packed lower triangle.
+__device__ __forceinline__ int dense_packed(int i, int j, int n) { return j * (2 * n - j + 1) / 2 + (i - j); }
+
+template <int BT>
+__global__ __launch_bounds__(BT) void dense_eigen_kernel(EigenArgs a) {
+  __shared__ double red2[2 * (BT / 64)];
+  int parity = 0;
+  __shared__ double s_lam[MAX_NEV], s_lo[MAX_NEV], s_hi[MAX_NEV];
+  __shared__ int s_first[MAX_NEV];
+  __shared__ int s_flag;
+  extern __shared__ double2 dsm[];
+  const int c = a.chain_list[blockIdx.x];
+  const int n = (int)a.chain_n[c];
+  const int64_t base = a.row_base[c];
+  const int nev = a.nev;
+  if (n == 0) {
+    if (threadIdx.x == 0) {
+      a.steps[c] = 0;
+      a.conv[c] = 1;
+    }
+    return;
+  }
+  const int np = n * (n + 1) / 2;
+  double2* A = dsm;                                                  // packed lower triangle
+  double2* yv = A + np;                                              // A v
+  double2* wv = yv + n;                                              // w
+  double2* tau = wv + n;                                             // tau_k
+  double2* X = tau + n;                                              // nev vectors of H
+  double* d = reinterpret_cast<double*>(X + (size_t)nev * n);        // diagonal of T
+  double* e = d + n;                                                 // off-diagonal of T
+  double* Y = e + n;                                                 // nev vectors of T
+  double* tw = Y + (size_t)nev * n;                                  // inverse iteration work, 4n each
+  int* piv = reinterpret_cast<int*>(tw + (size_t)nev * 4 * n);       // pivots, n each
+  for (int q = threadIdx.x; q < np; q += BT) A[q] = make_double2(0.0, 0.0);
+  __syncthreads();
+  // the lower triangle from each row's own entries (one thread per row, so
+  // no two threads write one element)
+  for (int i = threadIdx.x; i < n; i += BT) {
+    const int64_t e0 = a.rowptr[base + i], e1 = a.rowptr[base + i + 1];
+    for (int64_t q = e0; q < e1; ++q) {
+      const int j = a.col[q];
+      if (j >= 0 && j <= i) {
+        double2& t = A[dense_packed(i, j, n)];
+        t.x += a.val[q].x;
+        t.y += a.val[q].y;
+      }
+    }
+  }
+  __syncthreads();
+  constexpr int TPR = BT >= 512 ? 4 : 2;  // threads per row of A v (BT / TPR >= kDenseMaxN rows)
+  for (int k = 0; k + 1 < n; ++k) {
+    const int m = n - k - 1;
+    double2* col = A + dense_packed(k + 1, k, n);  // x = column k below the diagonal, then v
+    const double2 x0 = col[0];
+    const double dk = A[dense_packed(k, k, n)].x;
+    double p = 0.0;
+    for (int i = 1 + threadIdx.x; i < m; i += BT) p += col[i].x * col[i].x + col[i].y * col[i].y;
+    const double xn2 = block_sum1<BT>(p, red2, parity);  // its barrier also orders the reads of x0 before v
+    const bool refl = !(xn2 == 0.0 && x0.y == 0.0);
+    double beta = x0.x;
+    double2 tk = make_double2(0.0, 0.0), sc = make_double2(0.0, 0.0);
+    if (refl) {
+      const double nr = sqrt(x0.x * x0.x + x0.y * x0.y + xn2);
+      beta = x0.x >= 0.0 ? -nr : nr;
+      tk = make_double2((beta - x0.x) / beta, -x0.y / beta);
+      const double ur = x0.x - beta, ui = x0.y, den = ur * ur + ui * ui;
+      sc = make_double2(ur / den, -ui / den);
+    }
+    if (threadIdx.x == 0) {
+      d[k] = dk;
+      e[k] = beta;
+      tau[k] = tk;
+    }
+    if (!refl) continue;  // the same in every thread
+    for (int i = threadIdx.x; i < m; i += BT) {
+      const double2 x = col[i];
+      col[i] = i == 0 ? make_double2(1.0, 0.0) : make_double2(x.x * sc.x - x.y * sc.y, x.x * sc.y + x.y * sc.x);
+    }
+    __syncthreads();
+    {
+      // y = A v over the trailing block, TPR threads per row
+      const int r = threadIdx.x / TPR, q = threadIdx.x % TPR;
+      double yr = 0.0, yi = 0.0;
+      if (r < m) {
+        const int gr = k + 1 + r;
+        // columns up to the diagonal: row gr of the lower triangle
+#pragma unroll 4
+        for (int s = q; s <= r; s += TPR) {
+          const double2 h = A[dense_packed(gr, k + 1 + s, n)], v = col[s];
+          yr += h.x * v.x - h.y * v.y;
+          yi += h.x * v.y + h.y * v.x;
+        }
+        // columns past it: column gr below the diagonal, conjugated
+        const int cg = dense_packed(gr, gr, n) - r;  // A[cg + s] = A[k+1+s][gr] for s > r
+#pragma unroll 4
+        for (int s = r + 1 + q; s < m; s += TPR) {
+          const double2 h = A[cg + s], v = col[s];
+          yr += h.x * v.x + h.y * v.y;
+          yi += h.x * v.y - h.y * v.x;
+        }
+      }
+      yr = row_lanes_sum(yr, TPR);
+      yi = row_lanes_sum(yi, TPR);
+      if (r < m && q == 0) yv[r] = make_double2(yr, yi);
+    }
+    __syncthreads();
+    double pv = 0.0;
+    for (int i = threadIdx.x; i < m; i += BT) {
+      const double2 v = col[i], y = yv[i];
+      pv += v.x * y.x + v.y * y.y;
+    }
+    const double hv = 0.5 * (tk.x * tk.x + tk.y * tk.y) * block_sum1<BT>(pv, red2, parity);
+    for (int i = threadIdx.x; i < m; i += BT) {
+      const double2 y = yv[i], v = col[i];
+      wv[i] = make_double2(tk.x * y.x - tk.y * y.y - hv * v.x, tk.x * y.y + tk.y * y.x - hv * v.y);
+    }
+    __syncthreads();
+    // A -= v w^H + w v^H on the trailing lower triangle: rows r and m-1-r
+    // (r+1 and m-r elements) to one group of TU threads, v_r and w_r in
+    // registers
+    {
+      constexpr int TU = BT / 64;  // 64 row pairs cover kDenseMaxN rows
+      const int pr = threadIdx.x / TU, q = threadIdx.x % TU;
+      if (pr < (m + 1) / 2) {
+        for (int half = 0; half < 2; ++half) {
+          const int r = half == 0 ? pr : m - 1 - pr;
+          if (half == 1 && r == pr) break;  // the middle row of an odd m
+          const double2 vr = col[r], wr = wv[r];
+          const int gr = k + 1 + r;
+#pragma unroll 4
+          for (int s = q; s <= r; s += TU) {
+            const double2 vs = col[s], ws = wv[s];
+            double2& t = A[dense_packed(gr, k + 1 + s, n)];
+            t.x -= vr.x * ws.x + vr.y * ws.y + wr.x * vs.x + wr.y * vs.y;
+            t.y -= vr.y * ws.x - vr.x * ws.y + wr.y * vs.x - wr.x * vs.y;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d[n - 1] = A[dense_packed(n - 1, n - 1, n)].x;
+  __syncthreads();
+  const int ne = nev < n ? nev : n;
+  ritz_values<BT>(d, e, n, ne, nullptr, s_lam, s_lo, s_hi, s_first, &s_flag);
+  __syncthreads();
+  if ((int)threadIdx.x < ne) {
+    const int k = threadIdx.x;
+    double* t = tw + (size_t)k * 4 * n;
+    tri_inverse_iteration(d, e, n, s_lam[k], Y + (size_t)k * n, t, t + n, t + 2 * n, t + 3 * n, piv + (size_t)k * n,
+                          a.seed + 7919 * (uint64_t)k);
+    a.evals[(size_t)c * nev + k] = s_lam[k];
+  }
+  __syncthreads();
+  ortho_clustered(Y, n, n, ne, s_lam);
+  __syncthreads();
+  // x = H_0 H_1 ... H_{n-2} z, one wave per vector (wave sums, no barriers)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double2* out = a.evecs + base * nev;
+  for (int kv = wave; kv < ne; kv += BT / 64) {
+    double2* x = X + (size_t)kv * n;
+    const double* z = Y + (size_t)kv * n;
+    for (int i = lane; i < n; i += 64) x[i] = make_double2(z[i], 0.0);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int k = n - 2; k >= 0; --k) {
+      const double2 t = tau[k];
+      if (t.x == 0.0 && t.y == 0.0) continue;
+      const int m = n - k - 1;
+      const double2* v = A + dense_packed(k + 1, k, n);
+      double2* xs = x + k + 1;
+      double pr = 0.0, pi = 0.0;  // v^H x
+      for (int i = lane; i < m; i += 64) {
+        const double2 vi = v[i], xi = xs[i];
+        pr += vi.x * xi.x + vi.y * xi.y;
+        pi += vi.x * xi.y - vi.y * xi.x;
+      }
+      pr = wave_sum(pr);
+      pi = wave_sum(pi);
+      const double gr = t.x * pr - t.y * pi, gi = t.x * pi + t.y * pr;  // tau v^H x
+      for (int i = lane; i < m; i += 64) {
+        const double2 vi = v[i];
+        double2 xi = xs[i];
+        xi.x -= vi.x * gr - vi.y * gi;
+        xi.y -= vi.x * gi + vi.y * gr;
+        xs[i] = xi;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    double nn = 0.0;
+    for (int i = lane; i < n; i += 64) nn += x[i].x * x[i].x + x[i].y * x[i].y;
+    const double inv = 1.0 / sqrt(wave_sum(nn));
+    for (int i = lane; i < n; i += 64) out[(size_t)kv * n + i] = make_double2(x[i].x * inv, x[i].y * inv);
+  }
+  for (int k = ne; k < nev; ++k) {
+    for (int i = threadIdx.x; i < n; i += BT) out[(size_t)k * n + i] = make_double2(0.0, 0.0);
+    if (threadIdx.x == 0) a.evals[(size_t)c * nev + k] = NAN;
+  }
+  if (threadIdx.x == 0) {
+    a.steps[c] = n;
+    a.conv[c] = 1;
+  }
+}
+
+// LDS bytes of dense_eigen_kernel for an n-row chain (layout as in the kernel).
+size_t dense_eigen_bytes(int64_t n, int nev) {
+  const size_t np = (size_t)n * (size_t)(n + 1) / 2, nn = (size_t)n, ne = (size_t)nev;
+  return 16 * (np + 3 * nn + ne * nn) + 8 * (2 * nn + 5 * ne * nn) + 4 * ne * nn;
+}
+
 // ------------------------------------------------------------------ shifted solves
 //
 // (z_s - H) x_s = y for S complex shifts z_s of one chain (the incoming
@@ -2900,6 +3126,16 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   // one-workgroup-per-chain launch of the others
   std::vector<int32_t> order, large;
   for (int c = 0; c < C; ++c) (kb.h_n[c] > g_tuning.flat_min_n ? large : order).push_back(c);
+  // chains up to eigen_dense_max_n rows (at most kDenseMaxN, and the LDS
+  // layout within one CU's share) go to the dense solver
+  std::vector<int32_t> dense;
+  if (g_tuning.eigen_dense_max_n > 0) {
+    const int64_t lim = std::min<int64_t>(g_tuning.eigen_dense_max_n, kDenseMaxN);
+    std::vector<int32_t> rest;
+    for (int32_t c : order)
+      (kb.h_n[c] <= lim && dense_eigen_bytes(kb.h_n[c], nev) <= (size_t)150 * 1024 ? dense : rest).push_back(c);
+    order.swap(rest);
+  }
   std::vector<int64_t> voff(C + 1, 0);
   int64_t R = 0;
   for (int c = 0; c < C; ++c) {
@@ -2946,9 +3182,25 @@ int run_eigen(const KrylovBatch& kb, int nev, double tol, int max_steps, uint64_
   // vector offsets and the chain order in one copy
   HostPack pk;
   const size_t o_voff = pk.add(voff.data(), C + 1), o_list = pk.add(order.data(), order.size());
+  const size_t o_dense = pk.add(dense.data(), dense.size());
   pk.upload(sc.eig_in, s);
   a.voff = HostPack::at<int64_t>(sc.eig_in, o_voff);
   int32_t* clist = HostPack::at<int32_t>(sc.eig_in, o_list);
+  if (!dense.empty()) {
+    int64_t nmax = 0;
+    for (int32_t c : dense) nmax = std::max(nmax, kb.h_n[c]);
+    static bool attr = false;
+    if (!attr) {
+      HIP_CHECK(hipFuncSetAttribute((const void*)dense_eigen_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    150 * 1024));
+      attr = true;
+    }
+    TimedLaunch tl("eigen", s);
+    a.chain_list = HostPack::at<int32_t>(sc.eig_in, o_dense);
+    hipLaunchKernelGGL(dense_eigen_kernel<512>, dim3((unsigned)dense.size()), dim3(512),
+                       dense_eigen_bytes(nmax, nev), s, a);
+    HIP_CHECK(hipGetLastError());
+  }
   size_t n_wide = 0;
   while (n_wide < order.size() && kb.h_n[order[n_wide]] > g_tuning.eigen_wave_max_n) ++n_wide;
   // LDS layout of a launch whose largest chain is order[first]: coefficients

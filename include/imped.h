@@ -65,7 +65,10 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * "eigen_check_every" (Ritz convergence check every this many steps, 12 by
  * default), "eigen_check_start" (no Ritz convergence check before
  * min(this, n/2) steps of an n-row chain, 48 by default; the last step is
- * always checked), "flat_min_n" (chains above it use the grid-wide
+ * always checked), "eigen_dense_max_n" (eigensolver chains of up to this
+ * many rows, at most 112, are solved densely: Householder tridiagonalization
+ * in LDS, exact for degenerate eigenvalues, steps reported as n; 0, the
+ * default: off), "flat_min_n" (chains above it use the grid-wide
  * kernels), "spmv_lpr" (lanes per row of the grid-wide SpMV: 4..64),
  * "real_values" (1, the default: when every value of a batch's CSR is real,
  * the grid-wide kernels read an 8-byte copy of the real parts, made once per
