@@ -153,6 +153,7 @@ struct Tuning {
   int64_t mlp_graph = 0;                // surrogate predictions as one replayed HIP graph (0: direct launches)
   int64_t lanczos_lds = 1;              // reorthogonalized chains whose Krylov basis fits: basis in LDS
   int64_t lanczos_lds_waves = 4;        // waves per chain of the LDS-resident Lanczos (1, 2 or 4)
+  int64_t lanczos_dense_max_n = 0;      // reorthogonalized chains up to this size (<= 112): Householder reduction (0: off)
 };
 extern Tuning g_tuning;
 

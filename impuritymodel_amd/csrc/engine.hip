@@ -1076,6 +1076,8 @@ int imped_set_tuning(const char* name, int64_t value) {
     g_tuning.eigen_wave_max_n = value;
   else if (n == "eigen_dense_max_n")
     g_tuning.eigen_dense_max_n = value;
+  else if (n == "lanczos_dense_max_n")
+    g_tuning.lanczos_dense_max_n = value;
   else if (n == "eigen_real")
     g_tuning.eigen_real = value != 0;
   else if (n == "eigen_dgks")

@@ -2045,6 +2045,175 @@ size_t dense_eigen_bytes(int64_t n, int nev) {
   return 16 * (np + 3 * nn + ne * nn) + 8 * (2 * nn + 5 * ne * nn) + 4 * ne * nn;
 }
 
+// ---- fully reorthogonalized Lanczos of a small chain as a Householder reduction
+//
+// In exact arithmetic, K steps of Lanczos with full reorthogonalization from
+// a unit vector q give the leading K x K block of T = Q^H H Q for any unitary
+// Q whose first column is q (the implicit Q theorem), up to the signs of the
+// off-diagonal entries.  The kernel first applies the reflector that maps q
+// to a multiple of e_1 to both sides of H (packed lower triangle in LDS, as
+// in dense_eigen_kernel), then reduces the result to tridiagonal form by
+// the reflectors of columns 0..K-2: alpha_j = T_jj, beta_j = |T_{j+1,j}|.
+// No Krylov basis is stored (n(n+1)/2 + 2n complex values of LDS), so
+// several chains share a CU (five of 60 rows).  A zero off-diagonal entry
+// ends the chain as a zero beta ends the recurrence (kout,
+// IMPED_ERR_BREAKDOWN).
+template <int BT>
+__global__ __launch_bounds__(BT) void house_lanczos_kernel(LanczosArgs a) {
+  __shared__ double red2[2 * (BT / 64)];
+  int parity = 0;
+  extern __shared__ double2 dsm[];
+  const int c = a.chain_list[blockIdx.x];
+  const int n = (int)a.chain_n[c];
+  const int64_t base = a.row_base[c];
+  double* alpha = a.alpha + (size_t)c * a.K;
+  double* beta = a.beta + (size_t)c * a.K;
+  if (n == 0) {
+    if (threadIdx.x == 0) {
+      a.kout[c] = 0;
+      a.status[c] = IMPED_OK;
+    }
+    return;
+  }
+  const int K = a.K < n ? a.K : n;
+  const int np = n * (n + 1) / 2;
+  double2* A = dsm;      // packed lower triangle
+  double2* u = A + np;   // the start vector, then its reflector
+  double2* yv = u + n;   // A v, then w in place (w_i needs only y_i and v_i)
+  for (int q = threadIdx.x; q < np; q += BT) A[q] = make_double2(0.0, 0.0);
+  for (int i = threadIdx.x; i < n; i += BT) u[i] = a.v0[base + i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += BT) {
+    const int64_t e0 = a.rowptr[base + i], e1 = a.rowptr[base + i + 1];
+    for (int64_t q = e0; q < e1; ++q) {
+      const int j = a.col[q];
+      if (j >= 0 && j <= i) {
+        double2& t = A[dense_packed(i, j, n)];
+        t.x += a.val[q].x;
+        t.y += a.val[q].y;
+      }
+    }
+  }
+  __syncthreads();
+  constexpr int TPR = BT >= 512 ? 4 : 2;  // threads per row of A v (BT / TPR >= kDenseMaxN rows)
+  int kdone = K, status = IMPED_OK;
+  bool stopped = false;  // a zero beta or a zero start vector ended the chain
+  // reflector k acts on rows k+1..n-1; k = -1 is the start vector's
+  for (int k = -1; k + 1 < K; ++k) {
+    const int off = k + 1, m = n - off;
+    double2* col = k < 0 ? u : A + dense_packed(k + 1, k, n);
+    const double2 x0 = col[0];
+    const double dk = k >= 0 ? A[dense_packed(k, k, n)].x : 0.0;
+    double p = 0.0;
+    for (int i = 1 + threadIdx.x; i < m; i += BT) p += col[i].x * col[i].x + col[i].y * col[i].y;
+    const double xn2 = block_sum1<BT>(p, red2, parity);  // its barrier also orders the reads of x0 before v
+    const bool refl = !(xn2 == 0.0 && x0.y == 0.0);
+    double bk = x0.x;
+    double2 tk = make_double2(0.0, 0.0), sc = make_double2(0.0, 0.0);
+    if (refl) {
+      const double nr = sqrt(x0.x * x0.x + x0.y * x0.y + xn2);
+      bk = x0.x >= 0.0 ? -nr : nr;
+      tk = make_double2((bk - x0.x) / bk, -x0.y / bk);
+      const double ur = x0.x - bk, ui = x0.y, den = ur * ur + ui * ui;
+      sc = make_double2(ur / den, -ui / den);
+    }
+    if (k >= 0) {
+      if (threadIdx.x == 0) {
+        alpha[k] = dk;
+        beta[k] = fabs(bk);
+      }
+      if (bk == 0.0) {  // the same in every thread
+        kdone = k + 1;
+        status = IMPED_ERR_BREAKDOWN;
+        stopped = true;
+        break;
+      }
+    } else if (xn2 == 0.0 && x0.x == 0.0 && x0.y == 0.0) {
+      // a zero start vector: the recurrence gives alpha_0 = 0, then beta_0 = 0
+      if (threadIdx.x == 0) {
+        alpha[0] = 0.0;
+        if (K > 1) beta[0] = 0.0;
+      }
+      kdone = 1;
+      status = K > 1 ? IMPED_ERR_BREAKDOWN : IMPED_OK;
+      stopped = true;
+      break;
+    }
+    if (!refl) continue;  // the same in every thread
+    for (int i = threadIdx.x; i < m; i += BT) {
+      const double2 x = col[i];
+      col[i] = i == 0 ? make_double2(1.0, 0.0) : make_double2(x.x * sc.x - x.y * sc.y, x.x * sc.y + x.y * sc.x);
+    }
+    __syncthreads();
+    {
+      // y = A v over the trailing block, TPR threads per row
+      const int r = threadIdx.x / TPR, q = threadIdx.x % TPR;
+      double yr = 0.0, yi = 0.0;
+      if (r < m) {
+        const int gr = off + r;
+#pragma unroll 4
+        for (int s = q; s <= r; s += TPR) {
+          const double2 h = A[dense_packed(gr, off + s, n)], v = col[s];
+          yr += h.x * v.x - h.y * v.y;
+          yi += h.x * v.y + h.y * v.x;
+        }
+        const int cg = dense_packed(gr, gr, n) - r;  // A[cg + s] = A[off+s][gr] for s > r
+#pragma unroll 4
+        for (int s = r + 1 + q; s < m; s += TPR) {
+          const double2 h = A[cg + s], v = col[s];
+          yr += h.x * v.x + h.y * v.y;
+          yi += h.x * v.y - h.y * v.x;
+        }
+      }
+      yr = row_lanes_sum(yr, TPR);
+      yi = row_lanes_sum(yi, TPR);
+      if (r < m && q == 0) yv[r] = make_double2(yr, yi);
+    }
+    __syncthreads();
+    double pv = 0.0;
+    for (int i = threadIdx.x; i < m; i += BT) {
+      const double2 v = col[i], y = yv[i];
+      pv += v.x * y.x + v.y * y.y;
+    }
+    const double hv = 0.5 * (tk.x * tk.x + tk.y * tk.y) * block_sum1<BT>(pv, red2, parity);
+    for (int i = threadIdx.x; i < m; i += BT) {
+      const double2 y = yv[i], v = col[i];
+      yv[i] = make_double2(tk.x * y.x - tk.y * y.y - hv * v.x, tk.x * y.y + tk.y * y.x - hv * v.y);
+    }
+    __syncthreads();
+    {
+      // A -= v w^H + w v^H on the trailing lower triangle, rows r and m-1-r
+      // to one group of TU threads
+      constexpr int TU = BT / 64;
+      const int pr = threadIdx.x / TU, q = threadIdx.x % TU;
+      if (pr < (m + 1) / 2) {
+        for (int half = 0; half < 2; ++half) {
+          const int r = half == 0 ? pr : m - 1 - pr;
+          if (half == 1 && r == pr) break;
+          const double2 vr = col[r], wr = yv[r];
+          const int gr = off + r;
+#pragma unroll 4
+          for (int s = q; s <= r; s += TU) {
+            const double2 vs = col[s], ws = yv[s];
+            double2& t = A[dense_packed(gr, off + s, n)];
+            t.x -= vr.x * ws.x + vr.y * ws.y + wr.x * vs.x + wr.y * vs.y;
+            t.y -= vr.y * ws.x - vr.x * ws.y + wr.y * vs.x - wr.x * vs.y;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (!stopped) alpha[K - 1] = A[dense_packed(K - 1, K - 1, n)].x;
+    a.kout[c] = kdone;
+    a.status[c] = status;
+  }
+}
+
+// LDS bytes of house_lanczos_kernel for an n-row chain.
+size_t house_lanczos_bytes(int64_t n) { return 16 * ((size_t)n * (size_t)(n + 1) / 2 + 2 * (size_t)n); }
+
 // ------------------------------------------------------------------ shifted solves
 //
 // (z_s - H) x_s = y for S complex shifts z_s of one chain (the incoming
@@ -3068,12 +3237,18 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
   // Krylov basis fits LDS_CHAIN_BYTES get one wave with the basis in LDS;
   // otherwise one wave, 256 or 1024 threads per chain; the largest (plain
   // recurrence) run grid-wide, one chain at a time
-  std::vector<int32_t> lists[4], flat;
-  size_t lds_max = 0;
+  std::vector<int32_t> lists[5], flat;
+  size_t lds_max = 0, house_max = 0;
+  const int64_t house_lim = std::min<int64_t>(g_tuning.lanczos_dense_max_n, kDenseMaxN);
   for (int c = 0; c < kb.C; ++c) {
     const int64_t n = kb.h_n[c];
     if (!reorth && n > g_tuning.flat_min_n) {
       flat.push_back(c);
+      continue;
+    }
+    if (reorth && n <= house_lim) {
+      lists[4].push_back(c);
+      house_max = std::max(house_max, house_lanczos_bytes(n));
       continue;
     }
     if (reorth && g_tuning.lanczos_lds && lds_chain_bytes(n, K) <= LDS_CHAIN_BYTES) {
@@ -3090,7 +3265,7 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
   for (auto& l : lists) all.insert(all.end(), l.begin(), l.end());
   h2d(sc.clist, all.data(), all.size(), s);
   size_t off = 0;
-  for (int bkt = 0; bkt < 4; ++bkt) {
+  for (int bkt = 0; bkt < 5; ++bkt) {
     const unsigned nb = (unsigned)lists[bkt].size();
     if (!nb) continue;
     a.chain_list = sc.clist.as<int32_t>() + off;
@@ -3102,8 +3277,16 @@ void run_lanczos(const KrylovBatch& kb, const double2* v0, int K, int reorth, do
       hipLaunchKernelGGL(lanczos_kernel<256>, dim3(nb), dim3(256), shm, s, a);
     else if (bkt == 2)
       hipLaunchKernelGGL(lanczos_kernel<1024>, dim3(nb), dim3(1024), shm, s, a);
-    else
+    else if (bkt == 3)
       launch_lanczos_lds(a, nb, lds_max, s);
+    else {
+      static std::once_flag once;
+      std::call_once(once, [] {
+        HIP_CHECK(hipFuncSetAttribute((const void*)house_lanczos_kernel<256>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+      });
+      hipLaunchKernelGGL(house_lanczos_kernel<256>, dim3(nb), dim3(256), house_max, s, a);
+    }
     HIP_CHECK(hipGetLastError());
   }
   // no wait: the chain list stays in the scratch buffer, and later copies

@@ -68,7 +68,11 @@ int imped_timing_query(const char* name, double* total_ms, int64_t* launches);
  * always checked), "eigen_dense_max_n" (eigensolver chains of up to this
  * many rows, at most 112, are solved densely: Householder tridiagonalization
  * in LDS, exact for degenerate eigenvalues, steps reported as n; 0, the
- * default: off), "flat_min_n" (chains above it use the grid-wide
+ * default: off), "lanczos_dense_max_n" (fully reorthogonalized Lanczos
+ * chains of up to this many rows, at most 112, from unit start vectors, are
+ * computed as a Householder reduction started by the reflector of the start
+ * vector: the same alpha and beta, no Krylov basis in LDS, five 60-row
+ * chains per CU; 0, the default: off), "flat_min_n" (chains above it use the grid-wide
  * kernels), "spmv_lpr" (lanes per row of the grid-wide SpMV: 4..64),
  * "real_values" (1, the default: when every value of a batch's CSR is real,
  * the grid-wide kernels read an 8-byte copy of the real parts, made once per
