@@ -468,8 +468,14 @@ def run_exact(args, torch):
     other_ms = 1e3 * (time.perf_counter() - t1) / nb
     eig = sum(algorithmic_bytes_eigen(r.ground_dims, r.ground_nnz, r.eig_steps) for r in results)
     lan = sum(algorithmic_bytes_lanczos(r.chain_dims, r.chain_nnz, r.kout) for r in results)
-    roofline = _roofline({k: kernels[k] for k in ("eigen", "lanczos")}, {"eigen": eig, "lanczos": lan},
-                         workload="exact%d" % args.models)
+    # the per-step byte models describe the Krylov kernels; a chain solved
+    # by a Householder reduction (eigen_dense_max_n, lanczos_dense_max_n)
+    # has no such model, so its kernel is left out of the roofline
+    tuned = dict((kv.split("=", 1)[0], int(kv.split("=", 1)[1])) for kv in args.tuning)
+    krylov = [k for k, knob in (("eigen", "eigen_dense_max_n"), ("lanczos", "lanczos_dense_max_n"))
+              if tuned.get(knob, 0) <= 0]
+    roofline = _roofline({k: kernels[k] for k in krylov}, {"eigen": eig, "lanczos": lan},
+                         workload="exact%d" % args.models) if krylov else None
     if roofline is not None and roofline["kernel"] == "eigen_kernel":
         b = sum(gram_schmidt_bytes(r.ground_dims, r.eig_steps, eigen_itemsize(fam)) for r in results)
         roofline["gram_schmidt_gbs"] = round(b / (kernels["eigen"][0] * 1e-3) / 1e9, 1)
